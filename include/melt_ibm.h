@@ -1,0 +1,208 @@
+/*
+ * melt_ibm.h -- C ABI of libmeltgpu.so, the MI355X (gfx950) implementation of the
+ * melt_IBM fluid / immersed-boundary RK3 substep.
+ *
+ * The reference (kevzhong/melt_IBM) is an MPI Fortran program whose hot-path
+ * subroutines take no arguments and communicate through module globals
+ * (param.f90:4-274).  Here those globals live in a device-resident context
+ * (melt_ctx); every drop-in entry point below operates on that context exactly as
+ * the reference subroutine operates on its module state.  The Fortran binding
+ * (melt_ibm_amd/fortran/melt_ibm_f.f90) wraps these with the reference's names.
+ *
+ * Conventions
+ *  - All entry points return 0 (MELT_OK) on success, a MELT_ERR_* code otherwise;
+ *    melt_last_error() gives the message.  Work is enqueued on the context's HIP
+ *    stream; melt_sync() waits and reports errors raised by device code (the
+ *    reference's fatal paths: singular MLS moment matrix, small body volume).
+ *  - Field arrays are N1*N2*N3 doubles, i fastest (Fortran order), without the
+ *    reference's padding/ghost planes; periodicity is index arithmetic.  The
+ *    layout argument of melt_upload_field/melt_download_field lets a caller pass a
+ *    Fortran module array with its (1:n1,1:n2,kstart-1:kend+1) shape directly.
+ *  - Mesh connectivity is passed 1-based, as read from a GTS file.
+ *  - One host thread drives one context.  Independent contexts are independent.
+ */
+#ifndef MELT_IBM_H
+#define MELT_IBM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- errors */
+enum melt_status {
+  MELT_OK = 0,
+  MELT_ERR_ARG = 1,             /* bad argument / unsupported size              */
+  MELT_ERR_HIP = 2,             /* HIP runtime error                            */
+  MELT_ERR_SINGULAR_MLS = 3,    /* dgetrf info /= 0: mlsWeight.f90:121-124      */
+  MELT_ERR_SMALL_VOLUME = 4,    /* Volume < V_thresh: particle.f90:218-224      */
+  MELT_ERR_PENCIL_OVERFLOW = 5, /* > BUFF_SIZE faces in a pencil: pencil_rayTag.f90:126-130 */
+  MELT_ERR_REMESH = 6,          /* remeshing requested (IREMESH=1): not on this path */
+  MELT_ERR_RAY_RECURSION = 7,   /* parallel ray recursion did not terminate: rayAux.f90:37-44 */
+  MELT_ERR_NO_GEOMETRY = 8,     /* immersed-boundary call before melt_set_geometry */
+  MELT_ERR_WINDOW = 9           /* body outgrew the device window sized at melt_set_geometry */
+};
+
+/* ------------------------------------------------------------ field ids */
+enum melt_field_id {
+  MELT_VX = 0, MELT_VY, MELT_VZ, MELT_TEMP, MELT_PR,       /* local_arrays, param.f90:87-96 */
+  MELT_RU1, MELT_RU2, MELT_RU3, MELT_RUT,                  /* RK3 explicit-term history   */
+  MELT_FORCX, MELT_FORCY, MELT_FORCZ,                      /* HIT forcing (input)         */
+  MELT_VOFP, MELT_VOFX, MELT_VOFY, MELT_VOFZ, MELT_SDF,    /* param.f90:31-36             */
+  MELT_DQ, MELT_DPH, MELT_QCAP, MELT_HTEMP, MELT_RHS,      /* hdnl* outputs, Poisson, rhs */
+  MELT_FOR_XC, MELT_FOR_YC, MELT_FOR_ZC, MELT_FOR_TEMP,    /* mls_local, param.f90:265-268 (ghost-folded) */
+  MELT_NFIELDS
+};
+
+/* ------------------------------------------------- Lagrangian array ids */
+enum melt_lag_id {
+  /* per vertex (nv) */
+  MELT_L_XYZV = 0,     /* 3*nv  xyzv(3,nv)                 */
+  MELT_L_VERT_NOR,     /* 3*nv  vert_nor                   */
+  MELT_L_AVERT,        /*   nv  Avert                      */
+  MELT_L_QW_OVERT,     /*   nv  qw_oVert                   */
+  MELT_L_QW_IVERT,     /*   nv  qw_iVert                   */
+  MELT_L_VMELT,        /* 3*nv  vmelt                      */
+  MELT_L_VMELT_M1,     /* 3*nv  vmelt_m1                   */
+  MELT_L_DXYZV_S,      /* 3*nv  dxyzv_s                    */
+  /* per face (nf) */
+  MELT_L_TRI_BAR,      /* 3*nf  tri_bar                    */
+  MELT_L_TRI_NOR,      /* 3*nf  tri_nor                    */
+  MELT_L_SUR,          /*   nf  sur                        */
+  MELT_L_VEL_TRI,      /* 3*nf  vel_tri                    */
+  MELT_L_CFAC,         /*   nf  cfac                       */
+  MELT_L_SKEWNESS,     /*   nf  skewness                   */
+  MELT_L_QW_O,         /*   nf  qw_o                       */
+  MELT_L_QW_I,         /*   nf  qw_i                       */
+  MELT_L_PTXAB_Q1,     /* 27*nf ptxAB_q1(27,nf)            */
+  MELT_L_PTXAB_Q2,     /* 27*nf                            */
+  MELT_L_PTXAB_Q3,     /* 27*nf                            */
+  MELT_L_PTXAB_TEMP,   /* 27*nf                            */
+  MELT_L_DXYZ_S,       /* 3*nf  dxyz_s                     */
+  MELT_L_PRESS_N_TRI,  /* 3*nf  press_n_tri                */
+  MELT_L_TAU_N,        /* 3*nf  (tau_n1,tau_n2,tau_n3) per face */
+  /* per edge (ne) */
+  MELT_L_ELENGTHS,     /*   ne  eLengths                   */
+  MELT_L_NLAG
+};
+
+/* -------------------------------------------------------------- config */
+/* Input-deck values (bou.in, part.in, HITForcing.in as read at papero.f90:31-71). */
+typedef struct melt_config {
+  int32_t n1m, n2m, n3m;   /* cells per side; the reference requires n1m == n2m   */
+  int32_t precision;       /* 0: FP64 throughout; 1: mixed (FP32 bulk, FP64 refinement) */
+  double xlen, ylen, zlen;
+  double ren, prandtl, betagz;
+  double tmelt, tliq, tsol, latheat, cpliquid;
+  double dens_ratio;       /* part.in rhop/rhof                                    */
+  int32_t imlsfor, imlsstr, imelt, iremesh;
+  int32_t tagtype;         /* 1: ray tagging at init + narrow band (genSDF.f90)    */
+  int32_t forcing;         /* 1: forcx/forcy/forcz are applied (hdnl1.f90:74)      */
+  double perc_ethresh, v_on_ve_perc;
+} melt_config;
+
+/* Rigid-body / Lagrangian scalars (mls_param, param.f90:160-262). */
+typedef struct melt_body {
+  double pos_cm[3], vel_cm[3], omega_c[3], quat[4];
+  double inert_tensor[9];                       /* InertTensor(3,3), column-major */
+  double volume, surface;
+  double int_prn_dA[3], int_tau_dA[3], int_r_x_prn_dA[3], int_r_x_tau_dA[3];
+  double int_prn_dA_m1[3], int_tau_dA_m1[3], int_r_x_prn_dA_m1[3], int_r_x_tau_dA_m1[3];
+  double fp[3], ftau[3], torq_p[3], torq_tau[3]; /* tsch.f90:20-23 accumulators  */
+  double h_eulerian, celvol, e_thresh, v_thresh; /* initmls.f90:15-24             */
+  int32_t rm_flag, initial_tag;
+} melt_body;
+
+/* Host-array layout for field copies: element (i,j,k) (1-based, k in 1..n3m) is
+ * at host[(i-1) + ld1*((j-1) + ld2*(k-1+kghost))].  kghost = 1 means the array
+ * has the reference's ghost planes k=0 and k=n3m+1; melt_download_field then
+ * fills them with the periodic copies update_both_ghosts would hold
+ * (mpi_routines.f90:139-210 on one rank).  ld1 >= n1m, ld2 >= n2m. */
+typedef struct melt_layout {
+  int32_t ld1, ld2, kghost, pad;
+} melt_layout;
+
+typedef struct melt_ctx melt_ctx;
+
+/* ------------------------------------------------------------ lifecycle */
+int  melt_ctx_create(const melt_config* cfg, melt_ctx** out);
+void melt_ctx_destroy(melt_ctx* ctx);
+const char* melt_last_error(const melt_ctx* ctx);
+int  melt_sync(melt_ctx* ctx);
+int  melt_device_ok(void);   /* 1 if a HIP device is present */
+
+/* ------------------------------------------------------- data movement */
+int melt_upload_field(melt_ctx* ctx, int field, const double* host, const melt_layout* lay);
+int melt_download_field(melt_ctx* ctx, int field, double* host, const melt_layout* lay);
+int melt_fill_field(melt_ctx* ctx, int field, double value);
+
+/* Mesh import: set_connectivity + setup_particles + init_geomCoords + the
+ * geometry set-up of tri_geo + init_mlsForce (partaux.f90:160-267,
+ * create_geo.f90:1-70, tri_geo.f90:1-50, initmls.f90:1-27).  xyz0 is the
+ * unit mesh as read from the GTS file (3*nv, xyz interleaved), scaled by rad_p and
+ * centred in the box; vert_of_edge (2*ne) and edge_of_face (3*nf) are 1-based. */
+int melt_set_geometry(melt_ctx* ctx, int32_t nv, int32_t ne, int32_t nf,
+                      const double* xyz0, const int32_t* vert_of_edge,
+                      const int32_t* edge_of_face, double rad_p);
+int melt_get_body(melt_ctx* ctx, melt_body* out);
+int melt_set_body(melt_ctx* ctx, const melt_body* in);
+int melt_get_lag(melt_ctx* ctx, int id, double* host);
+int melt_set_lag(melt_ctx* ctx, int id, const double* host);
+int melt_get_pind(melt_ctx* ctx, int32_t* host);           /* pind(6,nf)             */
+int melt_get_vert_of_face(melt_ctx* ctx, int32_t* host);   /* 3*nf, 1-based          */
+
+/* --------------------------------------------------------- time control */
+int melt_set_dt(melt_ctx* ctx, double dt);       /* param dt                      */
+int melt_set_stage(melt_ctx* ctx, int32_t ns);   /* al,ga,ro = alm,gam,rom(ns), beta = dt/ren*0.5
+                                                    (papero.f90:116-137, tsch.f90:18,33-35) */
+
+/* ------------------------------------------------------ drop-in entries */
+/* Fused RK3 step: 3 substeps of tsch.f90:32-91. */
+int melt_tschem(melt_ctx* ctx);
+/* One fused substep ns (1..3) of tsch.f90:33-90. */
+int melt_substep(melt_ctx* ctx, int32_t ns);
+
+int melt_tagcells(melt_ctx* ctx);     /* tagCells.f90:4-27 (tagType 1)              */
+int melt_interp_vof(melt_ctx* ctx);   /* tagCells.f90:85-111                        */
+int melt_hdnl1(melt_ctx* ctx);        /* hdnl1.f90:1-80  -> MELT_DQ                 */
+int melt_hdnl2(melt_ctx* ctx);        /* hdnl2.f90:1-82  -> MELT_DPH                */
+int melt_hdnl3(melt_ctx* ctx);        /* hdnl3.f90:1-91  -> MELT_QCAP               */
+int melt_hdnlte(melt_ctx* ctx);       /* hdnlte.f90:1-160 -> MELT_HTEMP             */
+int melt_invtr1(melt_ctx* ctx);       /* invtr1.f90:1-77 (incl. solxi/solxj/solxk)  */
+int melt_invtr2(melt_ctx* ctx);       /* invtr2.f90:1-71                            */
+int melt_invtr3(melt_ctx* ctx);       /* invtr3.f90:1-65                            */
+int melt_invtrte(melt_ctx* ctx);      /* invtrte.f90:1-67                           */
+int melt_solxi(melt_ctx* ctx, double betadx);              /* solxi.f90:1-52 on MELT_RHS */
+int melt_solxj(melt_ctx* ctx, double betadx);              /* solxj.f90:1-40 on MELT_RHS */
+int melt_solxk(melt_ctx* ctx, int field, double betadx);   /* solxk.f90:1-60: q += solve(rhs) */
+int melt_divg(melt_ctx* ctx);         /* divg.f90:1-22   -> MELT_DPH                */
+int melt_phcalc(melt_ctx* ctx);       /* phcalc.f90:11-97 in place on MELT_DPH      */
+int melt_updvp(melt_ctx* ctx);        /* updvp.f90:8-35                             */
+int melt_prcalc(melt_ctx* ctx);       /* prcalc.f90:5-35                            */
+int melt_find_centroid_indices(melt_ctx* ctx); /* findCentroidIndices.f90:6-53      */
+int melt_mlsweight(melt_ctx* ctx);    /* mlsWeight.f90:1-45 (wght1/2/3/temp)        */
+int melt_mlsforce(melt_ctx* ctx);     /* mlsForce.f90:1-36 (forc1/2/3/temp), zeroes for_* first */
+int melt_velforce(melt_ctx* ctx);     /* velforce.f90:1-59                          */
+int melt_tempforce(melt_ctx* ctx);    /* tempforce.f90:1-46                         */
+int melt_particle(melt_ctx* ctx);     /* particle.f90:1-229 (IREMESH=0)             */
+
+/* Diagnostics used by the driver around tschem. */
+int melt_cfl(melt_ctx* ctx, double* cflm);     /* cfl.f90:1-30 (max over cells)     */
+int melt_divgck(melt_ctx* ctx, double* qmax);  /* divg.f90:26-56                    */
+int melt_inqpr(melt_ctx* ctx);                 /* inqpr.f90:3-44                    */
+
+/* Phase timing with HIP events around the four TIMED buckets of tsch.f90:37-84
+ * (tagging, Eulerian solve, MLS/particle, pressure) and around each fused-path
+ * kernel.  melt_timing_read returns the accumulated milliseconds per kernel slot
+ * and the number of samples; names from melt_timing_name. */
+int melt_timing_enable(melt_ctx* ctx, int32_t on);
+int melt_timing_reset(melt_ctx* ctx);
+int melt_timing_count(void);
+const char* melt_timing_name(int32_t slot);
+int melt_timing_read(melt_ctx* ctx, double* ms, int64_t* samples);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MELT_IBM_H */

@@ -1,0 +1,554 @@
+// C ABI of libmeltgpu (include/melt_ibm.h): context lifecycle, host <-> device
+// transfers, the drop-in entry points with the reference's subroutine semantics,
+// and the fused RK3 substep.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "common.h"
+
+int melt_fail(melt_ctx* ctx, int code, const char* what, const char* detail) {
+  if (ctx) ctx->err = std::string(what) + ": " + (detail ? detail : "");
+  return code;
+}
+
+namespace {
+
+const char* slot_names[SLOT_COUNT] = {
+    "tag_narrowband", "explicit_rhs", "solve_x", "solve_y", "solve_z_update", "mls_force", "lagrangian",
+    "divg_fft_x", "fft_y", "fft_z_solve", "ifft_y", "ifft_x", "project",
+    "phase_tagging", "phase_eulerian", "phase_mls", "phase_pressure", "substep"};
+
+const char* err_text(int code) {
+  switch (code) {
+    case MELT_ERR_SINGULAR_MLS: return "Error in dgetrf (singular MLS moment matrix)";
+    case MELT_ERR_SMALL_VOLUME: return "Geometry V/VE is smaller than threshold";
+    case MELT_ERR_PENCIL_OVERFLOW: return "pencil triangle count has exceeded buffer size";
+    case MELT_ERR_RAY_RECURSION: return "parallel-ray recursion did not terminate";
+    case MELT_ERR_WINDOW: return "immersed body outgrew the device window";
+    case MELT_ERR_ARG: return "invalid argument in device code";
+    default: return "device error";
+  }
+}
+
+bool size_ok(int n) {  // 2^a * 3^b, b <= 1 (FFT and circulant PCR closed forms)
+  if (n < 8) return false;
+  int m = n;
+  while (m % 2 == 0) m /= 2;
+  return m == 1 || m == 3;
+}
+
+__global__ void k_reset_loads(melt_body* b) {  // tsch.f90:19-23
+  for (int d = 0; d < 3; ++d) { b->fp[d] = 0.0; b->ftau[d] = 0.0; b->torq_p[d] = 0.0; b->torq_tau[d] = 0.0; }
+}
+
+__global__ void k_fill_d(double* __restrict__ p, double v, size_t n) {
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) p[c] = v;
+}
+
+int check_ctx(melt_ctx* ctx) { return ctx ? 0 : MELT_ERR_ARG; }
+
+}  // namespace
+
+// --------------------------------------------------------------- timing
+void timing_begin(melt_ctx* ctx, int slot, hipEvent_t* ev) {
+  (void)slot;
+  *ev = nullptr;
+  if (!ctx->timing) return;
+  if (ctx->ev_next >= ctx->ev_pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    ctx->ev_pool.push_back(e);
+  }
+  *ev = ctx->ev_pool[ctx->ev_next++];
+  hipEventRecord(*ev, ctx->stream);
+}
+
+void timing_end(melt_ctx* ctx, int slot, hipEvent_t ev) {
+  if (!ctx->timing || !ev) return;
+  if (ctx->ev_next >= ctx->ev_pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    ctx->ev_pool.push_back(e);
+  }
+  hipEvent_t e2 = ctx->ev_pool[ctx->ev_next++];
+  hipEventRecord(e2, ctx->stream);
+  ctx->ev_pending.push_back({slot, {ev, e2}});
+}
+
+static void timing_flush(melt_ctx* ctx) {
+  if (ctx->ev_pending.empty()) return;
+  hipStreamSynchronize(ctx->stream);
+  for (auto& p : ctx->ev_pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess) {
+      ctx->t_ms[p.first] += ms;
+      ctx->t_n[p.first] += 1;
+    }
+  }
+  ctx->ev_pending.clear();
+  ctx->ev_next = 0;
+}
+
+// ------------------------------------------------------------ lifecycle
+extern "C" int melt_device_ok(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n > 0 ? 1 : 0;
+}
+
+extern "C" void melt_ctx_destroy(melt_ctx* ctx) {
+  if (!ctx) return;
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  for (int i = 0; i < MELT_NFIELDS; ++i) if (ctx->f[i]) hipFree(ctx->f[i]);
+  for (int i = 1; i < 4; ++i) if (ctx->rhs4[i]) hipFree(ctx->rhs4[i]);
+  void* ps[] = {ctx->spec, ctx->ak1, ctx->ak2, ctx->ak3, ctx->tw1, ctx->tw2, ctx->tw3, ctx->d_body, ctx->d_vof, ctx->d_eof, ctx->d_voe,
+                ctx->d_v2f_ptr, ctx->d_v2f, ctx->d_gv, ctx->d_ge, ctx->d_gf, ctx->d_pind, ctx->d_probe, ctx->d_rxp, ctx->d_rxt, ctx->d_red,
+                ctx->d_vbox, ctx->d_acc, ctx->d_fmean, ctx->d_tkey, ctx->d_tface, ctx->d_win, ctx->d_pcount, ctx->d_plist, ctx->d_err};
+  for (void* p : ps) if (p) hipFree(p);
+  for (int i = 0; i < MELT_L_NLAG; ++i) if (ctx->d_lag[i]) hipFree(ctx->d_lag[i]);
+  if (ctx->h_err) hipHostFree(ctx->h_err);
+  for (auto e : ctx->ev_pool) hipEventDestroy(e);
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+extern "C" int melt_ctx_create(const melt_config* cfg, melt_ctx** out) {
+  if (!out) return MELT_ERR_ARG;
+  *out = nullptr;
+  if (!cfg || cfg->n1m != cfg->n2m || !size_ok(cfg->n1m) || !size_ok(cfg->n2m) || !size_ok(cfg->n3m) || cfg->n1m > 1024 || cfg->n3m > 1024)
+    return MELT_ERR_ARG;
+  if (cfg->precision != 0) return MELT_ERR_ARG;
+  melt_ctx* ctx = new melt_ctx();
+  ctx->cfg = *cfg;
+  ctx->n1 = cfg->n1m; ctx->n2 = cfg->n2m; ctx->n3 = cfg->n3m;
+  ctx->nc = (size_t)ctx->n1 * ctx->n2 * ctx->n3;
+  ctx->n1h = ctx->n1 / 2 + 1;
+  ctx->n1hp = (ctx->n1h + 7) & ~7;
+  ctx->pi = 2.0 * asin(1.0);
+  ctx->dx1 = (double)ctx->n1 / cfg->xlen; ctx->dx2 = (double)ctx->n2 / cfg->ylen; ctx->dx3 = (double)ctx->n3 / cfg->zlen;
+  ctx->dx1q = ctx->dx1 * ctx->dx1; ctx->dx2q = ctx->dx2 * ctx->dx2; ctx->dx3q = ctx->dx3 * ctx->dx3;
+  ctx->gam[0] = 8.0 / 15.0; ctx->gam[1] = 5.0 / 12.0; ctx->gam[2] = 3.0 / 4.0;
+  ctx->rom[0] = 0.0; ctx->rom[1] = -17.0 / 60.0; ctx->rom[2] = -5.0 / 12.0;
+  for (int i = 0; i < 3; ++i) ctx->alm[i] = ctx->gam[i] + ctx->rom[i];
+  int rc = 0;
+  do {
+    if (hipGetDevice(&ctx->device) != hipSuccess) { rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "no HIP device"); break; }
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { rc = MELT_ERR_HIP; break; }
+    for (int i = 0; i < MELT_NFIELDS && !rc; ++i) {
+      if (hipMalloc(&ctx->f[i], sizeof(double) * ctx->nc) != hipSuccess || hipMemset(ctx->f[i], 0, sizeof(double) * ctx->nc) != hipSuccess)
+        rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "field allocation failed");
+    }
+    if (rc) break;
+    ctx->rhs4[0] = ctx->f[MELT_RHS];
+    for (int i = 1; i < 4 && !rc; ++i)
+      if (hipMalloc(&ctx->rhs4[i], sizeof(double) * ctx->nc) != hipSuccess) rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "rhs allocation");
+    if (rc) break;
+    if (hipMalloc(&ctx->d_body, sizeof(melt_body)) != hipSuccess || hipMemset(ctx->d_body, 0, sizeof(melt_body)) != hipSuccess ||
+        hipMalloc(&ctx->d_red, sizeof(double) * 16384) != hipSuccess || hipMalloc(&ctx->d_fmean, sizeof(double) * 4) != hipSuccess ||
+        hipMemset(ctx->d_fmean, 0, sizeof(double) * 4) != hipSuccess || hipMalloc(&ctx->d_err, sizeof(int)) != hipSuccess ||
+        hipMemset(ctx->d_err, 0, sizeof(int)) != hipSuccess || hipHostMalloc(&ctx->h_err, sizeof(int)) != hipSuccess ||
+        hipMalloc(&ctx->d_win, sizeof(melt_window) * 2) != hipSuccess || hipMemset(ctx->d_win, 0, sizeof(melt_window) * 2) != hipSuccess) {
+      rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "scratch allocation failed");
+      break;
+    }
+    rc = poisson_init(ctx);
+    if (rc) break;
+    // the body state the reference starts from (module zeros) plus the MLS constants
+    melt_body& b = ctx->h_body;
+    memset(&b, 0, sizeof b);
+    b.h_eulerian = (1.0 / ctx->dx1 + 1.0 / ctx->dx2 + 1.0 / ctx->dx3) / 3.0;
+    b.celvol = 1.0 / (ctx->dx1 * ctx->dx2 * ctx->dx3);
+    b.e_thresh = cfg->perc_ethresh * b.h_eulerian;
+    b.v_thresh = cfg->v_on_ve_perc * b.celvol;
+    if (hipMemcpy(ctx->d_body, &b, sizeof b, hipMemcpyHostToDevice) != hipSuccess) rc = MELT_ERR_HIP;
+  } while (0);
+  if (rc) {
+    melt_ctx_destroy(ctx);
+    return rc;
+  }
+  *out = ctx;
+  return MELT_OK;
+}
+
+extern "C" const char* melt_last_error(const melt_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" int melt_sync(melt_ctx* ctx) {
+  if (check_ctx(ctx)) return MELT_ERR_ARG;
+  MELT_HIP(hipMemcpyAsync(ctx->h_err, ctx->d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  MELT_HIP(hipStreamSynchronize(ctx->stream));
+  int code = *ctx->h_err;
+  if (code) {
+    *ctx->h_err = 0;
+    MELT_HIP(hipMemset(ctx->d_err, 0, sizeof(int)));
+    return melt_fail(ctx, code, "device", err_text(code));
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------- data movement
+extern "C" int melt_upload_field(melt_ctx* ctx, int field, const double* host, const melt_layout* lay) {
+  if (check_ctx(ctx) || field < 0 || field >= MELT_NFIELDS || !host) return MELT_ERR_ARG;
+  melt_layout L = lay ? *lay : melt_layout{ctx->n1, ctx->n2, 0, 0};
+  if (L.ld1 < ctx->n1 || L.ld2 < ctx->n2) return melt_fail(ctx, MELT_ERR_ARG, "upload", "leading dimension too small");
+  MELT_HIP(hipStreamSynchronize(ctx->stream));
+  if (L.ld1 == ctx->n1 && L.ld2 == ctx->n2 && L.kghost == 0) {
+    MELT_HIP(hipMemcpy(ctx->f[field], host, sizeof(double) * ctx->nc, hipMemcpyHostToDevice));
+    return 0;
+  }
+  std::vector<double> tmp(ctx->nc);
+  for (int k = 0; k < ctx->n3; ++k)
+    for (int j = 0; j < ctx->n2; ++j)
+      memcpy(&tmp[(size_t)ctx->n1 * (j + (size_t)ctx->n2 * k)], &host[(size_t)L.ld1 * (j + (size_t)L.ld2 * (k + L.kghost))],
+             sizeof(double) * ctx->n1);
+  MELT_HIP(hipMemcpy(ctx->f[field], tmp.data(), sizeof(double) * ctx->nc, hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int melt_download_field(melt_ctx* ctx, int field, double* host, const melt_layout* lay) {
+  if (check_ctx(ctx) || field < 0 || field >= MELT_NFIELDS || !host) return MELT_ERR_ARG;
+  melt_layout L = lay ? *lay : melt_layout{ctx->n1, ctx->n2, 0, 0};
+  if (L.ld1 < ctx->n1 || L.ld2 < ctx->n2) return melt_fail(ctx, MELT_ERR_ARG, "download", "leading dimension too small");
+  MELT_HIP(hipStreamSynchronize(ctx->stream));
+  if (L.ld1 == ctx->n1 && L.ld2 == ctx->n2 && L.kghost == 0) {
+    MELT_HIP(hipMemcpy(host, ctx->f[field], sizeof(double) * ctx->nc, hipMemcpyDeviceToHost));
+    return 0;
+  }
+  std::vector<double> tmp(ctx->nc);
+  MELT_HIP(hipMemcpy(tmp.data(), ctx->f[field], sizeof(double) * ctx->nc, hipMemcpyDeviceToHost));
+  int k0 = L.kghost ? -1 : 0, k1 = L.kghost ? ctx->n3 + 1 : ctx->n3;
+  for (int k = k0; k < k1; ++k) {
+    int ks = pmod(k, ctx->n3);
+    for (int j = 0; j < ctx->n2; ++j)
+      memcpy(&host[(size_t)L.ld1 * (j + (size_t)L.ld2 * (k + L.kghost))], &tmp[(size_t)ctx->n1 * (j + (size_t)ctx->n2 * ks)],
+             sizeof(double) * ctx->n1);
+  }
+  return 0;
+}
+
+extern "C" int melt_fill_field(melt_ctx* ctx, int field, double value) {
+  if (check_ctx(ctx) || field < 0 || field >= MELT_NFIELDS) return MELT_ERR_ARG;
+  hipLaunchKernelGGL(k_fill_d, dim3(1024), dim3(256), 0, ctx->stream, ctx->f[field], value, ctx->nc);
+  MELT_HIP(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------- geometry / body
+extern "C" int melt_set_geometry(melt_ctx* ctx, int32_t nv, int32_t ne, int32_t nf, const double* xyz0, const int32_t* vert_of_edge,
+                                 const int32_t* edge_of_face, double rad_p) {
+  if (check_ctx(ctx) || nv <= 0 || ne <= 0 || nf <= 0 || !xyz0 || !vert_of_edge || !edge_of_face) return MELT_ERR_ARG;
+  if (ctx->has_geom) return melt_fail(ctx, MELT_ERR_ARG, "set_geometry", "geometry already set for this context");
+  ctx->nv = nv; ctx->ne = ne; ctx->nf = nf;
+  ctx->rad_p = rad_p;
+  MELT_HIP(hipStreamSynchronize(ctx->stream));
+  int rc = lagr_alloc(ctx, xyz0, vert_of_edge, edge_of_face, nullptr, rad_p);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_fill_d, dim3(1024), dim3(256), 0, ctx->stream, ctx->f[MELT_SDF], 999.0, ctx->nc);
+  MELT_HIP(hipGetLastError());
+  ctx->has_geom = 1;
+  MELT_HIP(hipMemcpyAsync(&ctx->h_body, ctx->d_body, sizeof(melt_body), hipMemcpyDeviceToHost, ctx->stream));
+  MELT_HIP(hipStreamSynchronize(ctx->stream));
+  return melt_sync(ctx);
+}
+
+extern "C" int melt_get_body(melt_ctx* ctx, melt_body* out) {
+  if (check_ctx(ctx) || !out) return MELT_ERR_ARG;
+  MELT_HIP(hipMemcpyAsync(&ctx->h_body, ctx->d_body, sizeof(melt_body), hipMemcpyDeviceToHost, ctx->stream));
+  MELT_HIP(hipStreamSynchronize(ctx->stream));
+  *out = ctx->h_body;
+  return 0;
+}
+
+extern "C" int melt_set_body(melt_ctx* ctx, const melt_body* in) {
+  if (check_ctx(ctx) || !in) return MELT_ERR_ARG;
+  ctx->h_body = *in;
+  MELT_HIP(hipMemcpyAsync(ctx->d_body, &ctx->h_body, sizeof(melt_body), hipMemcpyHostToDevice, ctx->stream));
+  MELT_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+static size_t lag_count(melt_ctx* ctx, int id) {
+  static const int len[MELT_L_NLAG] = {3, 3, 1, 1, 1, 3, 3, 3, 3, 3, 1, 3, 1, 1, 1, 1, 27, 27, 27, 27, 3, 3, 3, 1};
+  size_t n = id <= MELT_L_DXYZV_S ? ctx->nv : id <= MELT_L_TAU_N ? ctx->nf : ctx->ne;
+  return (size_t)len[id] * n;
+}
+
+extern "C" int melt_get_lag(melt_ctx* ctx, int id, double* host) {
+  if (check_ctx(ctx) || id < 0 || id >= MELT_L_NLAG || !host || !ctx->has_geom) return MELT_ERR_ARG;
+  MELT_HIP(hipStreamSynchronize(ctx->stream));
+  MELT_HIP(hipMemcpy(host, ctx->d_lag[id], sizeof(double) * lag_count(ctx, id), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int melt_set_lag(melt_ctx* ctx, int id, const double* host) {
+  if (check_ctx(ctx) || id < 0 || id >= MELT_L_NLAG || !host || !ctx->has_geom) return MELT_ERR_ARG;
+  MELT_HIP(hipStreamSynchronize(ctx->stream));
+  MELT_HIP(hipMemcpy(ctx->d_lag[id], host, sizeof(double) * lag_count(ctx, id), hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int melt_get_pind(melt_ctx* ctx, int32_t* host) {
+  if (check_ctx(ctx) || !host || !ctx->has_geom) return MELT_ERR_ARG;
+  MELT_HIP(hipStreamSynchronize(ctx->stream));
+  MELT_HIP(hipMemcpy(host, ctx->d_pind, sizeof(int) * 6 * (size_t)ctx->nf, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int melt_get_vert_of_face(melt_ctx* ctx, int32_t* host) {
+  if (check_ctx(ctx) || !host || !ctx->has_geom) return MELT_ERR_ARG;
+  MELT_HIP(hipStreamSynchronize(ctx->stream));
+  MELT_HIP(hipMemcpy(host, ctx->d_vof, sizeof(int) * 3 * (size_t)ctx->nf, hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < 3 * (size_t)ctx->nf; ++i) host[i] += 1;
+  return 0;
+}
+
+// ------------------------------------------------------------ time control
+extern "C" int melt_set_dt(melt_ctx* ctx, double dt) {
+  if (check_ctx(ctx) || !(dt > 0.0)) return MELT_ERR_ARG;
+  ctx->dt = dt;
+  return 0;
+}
+
+extern "C" int melt_set_stage(melt_ctx* ctx, int32_t ns) {
+  if (check_ctx(ctx) || ns < 1 || ns > 3) return MELT_ERR_ARG;
+  ctx->stage = ns;
+  ctx->beta = ctx->dt / ctx->cfg.ren * 0.5;  // tsch.f90:18
+  ctx->al = ctx->alm[ns - 1]; ctx->ga = ctx->gam[ns - 1]; ctx->ro = ctx->rom[ns - 1];
+  return 0;
+}
+
+static int need_stage(melt_ctx* ctx) {
+  if (ctx->stage < 1 || !(ctx->dt > 0.0)) return melt_fail(ctx, MELT_ERR_ARG, "stage", "call melt_set_dt and melt_set_stage first");
+  return 0;
+}
+
+// ---------------------------------------------------- per-routine entries
+extern "C" int melt_tagcells(melt_ctx* ctx) {
+  if (check_ctx(ctx)) return MELT_ERR_ARG;
+  int rc = ibm_tag(ctx);
+  if (rc) return rc;
+  return euler_launch_interp_vof(ctx);
+}
+extern "C" int melt_interp_vof(melt_ctx* ctx) { return check_ctx(ctx) ? MELT_ERR_ARG : euler_launch_interp_vof(ctx); }
+
+static int hdnl(melt_ctx* ctx, int which) {
+  if (check_ctx(ctx)) return MELT_ERR_ARG;
+  return euler_launch_hdnl(ctx, which);
+}
+extern "C" int melt_hdnl1(melt_ctx* ctx) { return hdnl(ctx, 0); }
+extern "C" int melt_hdnl2(melt_ctx* ctx) { return hdnl(ctx, 1); }
+extern "C" int melt_hdnl3(melt_ctx* ctx) { return hdnl(ctx, 2); }
+extern "C" int melt_hdnlte(melt_ctx* ctx) { return hdnl(ctx, 3); }
+
+static double betadx_q(melt_ctx* ctx, int which, int dir) {  // invtr1.f90:66-68, invtrte.f90:61-63
+  double dq = dir == 0 ? ctx->dx1q : dir == 1 ? ctx->dx2q : ctx->dx3q;
+  if (which < 3) return ctx->beta * ctx->al * dq;
+  double pec = ctx->cfg.ren * ctx->cfg.prandtl;
+  return ctx->dt / pec * 0.5 * ctx->al * dq;
+}
+
+static int invtr(melt_ctx* ctx, int which) {
+  if (check_ctx(ctx)) return MELT_ERR_ARG;
+  int rc = need_stage(ctx);
+  if (rc) return rc;
+  static const int q_id[4] = {MELT_VX, MELT_VY, MELT_VZ, MELT_TEMP};
+  rc = euler_launch_invtr_rhs(ctx, which);
+  if (rc) return rc;
+  double* r[1] = {ctx->f[MELT_RHS]};
+  double b0 = betadx_q(ctx, which, 0), b1 = betadx_q(ctx, which, 1), b2 = betadx_q(ctx, which, 2);
+  if ((rc = euler_launch_solve(ctx, 0, r, 1, &b0, nullptr, 1))) return rc;
+  if ((rc = euler_launch_solve(ctx, 1, r, 1, &b1, nullptr, 1))) return rc;
+  double* q[1] = {ctx->f[q_id[which]]};
+  return euler_launch_solve(ctx, 2, r, 1, &b2, q, 1);
+}
+extern "C" int melt_invtr1(melt_ctx* ctx) { return invtr(ctx, 0); }
+extern "C" int melt_invtr2(melt_ctx* ctx) { return invtr(ctx, 1); }
+extern "C" int melt_invtr3(melt_ctx* ctx) { return invtr(ctx, 2); }
+extern "C" int melt_invtrte(melt_ctx* ctx) { return invtr(ctx, 3); }
+
+extern "C" int melt_solxi(melt_ctx* ctx, double betadx) {
+  if (check_ctx(ctx)) return MELT_ERR_ARG;
+  double* r[1] = {ctx->f[MELT_RHS]};
+  return euler_launch_solve(ctx, 0, r, 1, &betadx, nullptr, 1);
+}
+extern "C" int melt_solxj(melt_ctx* ctx, double betadx) {
+  if (check_ctx(ctx)) return MELT_ERR_ARG;
+  double* r[1] = {ctx->f[MELT_RHS]};
+  return euler_launch_solve(ctx, 1, r, 1, &betadx, nullptr, 1);
+}
+extern "C" int melt_solxk(melt_ctx* ctx, int field, double betadx) {
+  if (check_ctx(ctx) || field < 0 || field >= MELT_NFIELDS || field == MELT_RHS) return MELT_ERR_ARG;
+  double* r[1] = {ctx->f[MELT_RHS]};
+  double* q[1] = {ctx->f[field]};
+  return euler_launch_solve(ctx, 2, r, 1, &betadx, q, 1);
+}
+
+extern "C" int melt_divg(melt_ctx* ctx) {
+  if (check_ctx(ctx)) return MELT_ERR_ARG;
+  int rc = need_stage(ctx);
+  return rc ? rc : euler_launch_divg(ctx);
+}
+extern "C" int melt_phcalc(melt_ctx* ctx) { return check_ctx(ctx) ? MELT_ERR_ARG : poisson_solve(ctx, 0); }
+extern "C" int melt_updvp(melt_ctx* ctx) {
+  if (check_ctx(ctx)) return MELT_ERR_ARG;
+  int rc = need_stage(ctx);
+  return rc ? rc : euler_launch_project(ctx, 1, 0, 0);
+}
+extern "C" int melt_prcalc(melt_ctx* ctx) {
+  if (check_ctx(ctx)) return MELT_ERR_ARG;
+  int rc = need_stage(ctx);
+  return rc ? rc : euler_launch_project(ctx, 0, 1, 0);
+}
+
+static int need_geom(melt_ctx* ctx) {
+  return ctx->has_geom ? 0 : melt_fail(ctx, MELT_ERR_NO_GEOMETRY, "immersed boundary", "melt_set_geometry has not been called");
+}
+extern "C" int melt_find_centroid_indices(melt_ctx* ctx) {
+  if (check_ctx(ctx)) return MELT_ERR_ARG;
+  int rc = need_geom(ctx);
+  return rc ? rc : ibm_find_centroid_indices(ctx);
+}
+extern "C" int melt_mlsweight(melt_ctx* ctx) {
+  if (check_ctx(ctx)) return MELT_ERR_ARG;
+  int rc = need_geom(ctx);
+  return rc ? rc : ibm_weights(ctx);
+}
+extern "C" int melt_mlsforce(melt_ctx* ctx) {
+  if (check_ctx(ctx)) return MELT_ERR_ARG;
+  int rc = need_geom(ctx);
+  if (rc || (rc = ibm_force_scatter(ctx))) return rc;
+  return ibm_force_drain(ctx, 0);
+}
+extern "C" int melt_velforce(melt_ctx* ctx) { return check_ctx(ctx) ? MELT_ERR_ARG : ibm_velforce(ctx, 0); }
+extern "C" int melt_tempforce(melt_ctx* ctx) { return check_ctx(ctx) ? MELT_ERR_ARG : ibm_tempforce(ctx); }
+
+// particle.f90:38-193 (IREMESH = 0).  defer: leave the uniform mean-force shift
+// to the projection (fused substep) instead of applying it here.
+static int particle_impl(melt_ctx* ctx, int defer) {
+  if (ctx->cfg.imlsfor != 1) return 0;
+  int rc;
+  if ((rc = ibm_find_centroid_indices(ctx))) return rc;
+  if ((rc = ibm_weights(ctx))) return rc;
+  if ((rc = ibm_force_scatter(ctx))) return rc;
+  if ((rc = ibm_probes(ctx))) return rc;      // heat flux and loads read the pre-force fields
+  if ((rc = ibm_force_drain(ctx, 1))) return rc;
+  if (!defer && (rc = ibm_sub_mean(ctx))) return rc;
+  return lagr_after_force(ctx);
+}
+
+extern "C" int melt_particle(melt_ctx* ctx) {
+  if (check_ctx(ctx)) return MELT_ERR_ARG;
+  int rc = need_geom(ctx);
+  if (rc || (rc = need_stage(ctx))) return rc;
+  return particle_impl(ctx, 0);
+}
+
+// ------------------------------------------------------------ fused path
+extern "C" int melt_substep(melt_ctx* ctx, int32_t ns) {
+  if (check_ctx(ctx)) return MELT_ERR_ARG;
+  int rc = melt_set_stage(ctx, ns);
+  if (rc || (rc = need_stage(ctx))) return rc;
+  const melt_config& c = ctx->cfg;
+  bool ibm = c.imlsfor == 1 && ctx->has_geom;
+  bool moving = ibm && (c.imlsstr == 1 || c.imelt == 1);  // papero.f90:95-100
+  hipEvent_t e_sub = nullptr, e_ph = nullptr, e = nullptr;
+  timing_begin(ctx, SLOT_SUBSTEP, &e_sub);
+
+  timing_begin(ctx, SLOT_PH_TAG, &e_ph);
+  if (moving) {
+    timing_begin(ctx, SLOT_TAG, &e);
+    if ((rc = ibm_tag(ctx))) return rc;
+    timing_end(ctx, SLOT_TAG, e);
+  }
+  timing_end(ctx, SLOT_PH_TAG, e_ph);
+
+  timing_begin(ctx, SLOT_PH_EULER, &e_ph);
+  timing_begin(ctx, SLOT_EXPLICIT, &e);
+  if ((rc = euler_launch_explicit_fused(ctx))) return rc;
+  timing_end(ctx, SLOT_EXPLICIT, e);
+  double* r[4] = {ctx->rhs4[RHS_U], ctx->rhs4[RHS_V], ctx->rhs4[RHS_W], ctx->rhs4[RHS_T]};
+  double* q[4] = {ctx->f[MELT_VX], ctx->f[MELT_VY], ctx->f[MELT_VZ], ctx->f[MELT_TEMP]};
+  static const int comp[4] = {0, 1, 2, 3};
+  for (int dir = 0; dir < 3; ++dir) {
+    double b[4];
+    for (int m = 0; m < 4; ++m) b[m] = betadx_q(ctx, comp[m], dir);
+    int slot = dir == 0 ? SLOT_SOLVE_X : dir == 1 ? SLOT_SOLVE_Y : SLOT_SOLVE_Z;
+    timing_begin(ctx, slot, &e);
+    if ((rc = euler_launch_solve(ctx, dir, r, 4, b, dir == 2 ? q : nullptr, 0))) return rc;
+    timing_end(ctx, slot, e);
+  }
+  timing_end(ctx, SLOT_PH_EULER, e_ph);
+
+  timing_begin(ctx, SLOT_PH_MLS, &e_ph);
+  if (ibm) {
+    timing_begin(ctx, SLOT_IBM, &e);
+    if ((rc = ibm_find_centroid_indices(ctx)) || (rc = ibm_weights(ctx)) || (rc = ibm_force_scatter(ctx)) || (rc = ibm_probes(ctx)) ||
+        (rc = ibm_force_drain(ctx, 1)))
+      return rc;
+    timing_end(ctx, SLOT_IBM, e);
+    timing_begin(ctx, SLOT_LAGR, &e);
+    if ((rc = lagr_after_force(ctx))) return rc;
+    timing_end(ctx, SLOT_LAGR, e);
+  }
+  timing_end(ctx, SLOT_PH_MLS, e_ph);
+
+  timing_begin(ctx, SLOT_PH_PRESSURE, &e_ph);
+  if ((rc = poisson_solve(ctx, 1))) return rc;
+  timing_begin(ctx, SLOT_PROJECT, &e);
+  if ((rc = euler_launch_project(ctx, 1, 1, ibm ? 1 : 0))) return rc;
+  timing_end(ctx, SLOT_PROJECT, e);
+  timing_end(ctx, SLOT_PH_PRESSURE, e_ph);
+  timing_end(ctx, SLOT_SUBSTEP, e_sub);
+  if (ctx->timing && ctx->ev_pending.size() > 4096) timing_flush(ctx);
+  return 0;
+}
+
+extern "C" int melt_tschem(melt_ctx* ctx) {
+  if (check_ctx(ctx)) return MELT_ERR_ARG;
+  if (!(ctx->dt > 0.0)) return melt_fail(ctx, MELT_ERR_ARG, "tschem", "dt not set");
+  if (ctx->has_geom) {
+    hipLaunchKernelGGL(k_reset_loads, dim3(1), dim3(1), 0, ctx->stream, ctx->d_body);
+    MELT_HIP(hipGetLastError());
+  }
+  for (int ns = 1; ns <= 3; ++ns) {
+    int rc = melt_substep(ctx, ns);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------ diagnostics
+extern "C" int melt_cfl(melt_ctx* ctx, double* cflm) {
+  if (check_ctx(ctx) || !cflm) return MELT_ERR_ARG;
+  return euler_reduce_max(ctx, 0, cflm);
+}
+extern "C" int melt_divgck(melt_ctx* ctx, double* qmax) {
+  if (check_ctx(ctx) || !qmax) return MELT_ERR_ARG;
+  return euler_reduce_max(ctx, 1, qmax);
+}
+extern "C" int melt_inqpr(melt_ctx* ctx) {
+  if (check_ctx(ctx)) return MELT_ERR_ARG;
+  return euler_launch_inqpr(ctx);
+}
+
+// ------------------------------------------------------------ timing API
+extern "C" int melt_timing_enable(melt_ctx* ctx, int32_t on) {
+  if (check_ctx(ctx)) return MELT_ERR_ARG;
+  if (!on) timing_flush(ctx);
+  ctx->timing = on ? 1 : 0;
+  return 0;
+}
+extern "C" int melt_timing_reset(melt_ctx* ctx) {
+  if (check_ctx(ctx)) return MELT_ERR_ARG;
+  timing_flush(ctx);
+  for (int i = 0; i < SLOT_COUNT; ++i) { ctx->t_ms[i] = 0; ctx->t_n[i] = 0; }
+  return 0;
+}
+extern "C" int melt_timing_count(void) { return SLOT_COUNT; }
+extern "C" const char* melt_timing_name(int32_t slot) { return (slot >= 0 && slot < SLOT_COUNT) ? slot_names[slot] : ""; }
+extern "C" int melt_timing_read(melt_ctx* ctx, double* ms, int64_t* samples) {
+  if (check_ctx(ctx) || !ms || !samples) return MELT_ERR_ARG;
+  timing_flush(ctx);
+  for (int i = 0; i < SLOT_COUNT; ++i) { ms[i] = ctx->t_ms[i]; samples[i] = ctx->t_n[i]; }
+  return 0;
+}

@@ -1,0 +1,579 @@
+// Eulerian part of the RK3 substep on gfx950: explicit advection/diffusion
+// stencils (hdnl1-3, hdnlte, invtr1-3, invtrte right-hand sides), the
+// Crank-Nicolson periodic line solves (solxi/solxj/solxk) as circulant parallel
+// cyclic reduction in LDS, the divergence, and the projection (updvp, prcalc).
+//
+// All of it is HBM-bound (about 1 flop per byte), so the kernels are written for
+// coalesced unit-stride access and to read each input field once per pass.
+#include <algorithm>
+#include <cstring>
+
+#include "common.h"
+
+namespace {
+
+// Wrapped neighbour coordinates of one cell (0-based).
+struct Cell {
+  int i[3], j[3], k[3];  // [minus, centre, plus]
+  int n1, n2;
+  __device__ size_t at(int a, int b, int c) const {
+    return (size_t)i[a + 1] + (size_t)n1 * ((size_t)j[b + 1] + (size_t)n2 * (size_t)k[c + 1]);
+  }
+};
+
+__device__ inline Cell make_cell(const Grid& g, size_t c) {
+  Cell s;
+  s.n1 = g.n1; s.n2 = g.n2;
+  int i = (int)(c % g.n1);
+  size_t r = c / g.n1;
+  int j = (int)(r % g.n2);
+  int k = (int)(r / g.n2);
+  s.i[1] = i; s.i[0] = i == 0 ? g.n1 - 1 : i - 1; s.i[2] = i == g.n1 - 1 ? 0 : i + 1;
+  s.j[1] = j; s.j[0] = j == 0 ? g.n2 - 1 : j - 1; s.j[2] = j == g.n2 - 1 ? 0 : j + 1;
+  s.k[1] = k; s.k[0] = k == 0 ? g.n3 - 1 : k - 1; s.k[2] = k == g.n3 - 1 ? 0 : k + 1;
+  return s;
+}
+
+struct EulerArgs {
+  const double* __restrict__ vx;
+  const double* __restrict__ vy;
+  const double* __restrict__ vz;
+  const double* __restrict__ temp;
+  const double* __restrict__ pr;
+  const double* __restrict__ vofp;
+  const double* __restrict__ forcx;
+  const double* __restrict__ forcy;
+  const double* __restrict__ forcz;
+  const melt_body* __restrict__ body;
+  Grid g;
+  double betagz, tliq;
+};
+
+// hdnl1.f90:32-74
+__device__ inline double h1_val(const EulerArgs& A, const Cell& s, double vofx) {
+  const double* vx = A.vx; const double* vy = A.vy; const double* vz = A.vz;
+  double udx1 = A.g.dx1 * 0.25, udx2 = A.g.dx2 * 0.25, udx3 = A.g.dx3 * 0.25;
+  double h11 = ((vx[s.at(1, 0, 0)] + vx[s.at(0, 0, 0)]) * (vx[s.at(1, 0, 0)] + vx[s.at(0, 0, 0)]) -
+                (vx[s.at(-1, 0, 0)] + vx[s.at(0, 0, 0)]) * (vx[s.at(-1, 0, 0)] + vx[s.at(0, 0, 0)])) * udx1;
+  double h12 = ((vy[s.at(0, 1, 0)] + vy[s.at(-1, 1, 0)]) * (vx[s.at(0, 1, 0)] + vx[s.at(0, 0, 0)]) -
+                (vy[s.at(0, 0, 0)] + vy[s.at(-1, 0, 0)]) * (vx[s.at(0, 0, 0)] + vx[s.at(0, -1, 0)])) * udx2;
+  double h13 = ((vz[s.at(0, 0, 1)] + vz[s.at(-1, 0, 1)]) * (vx[s.at(0, 0, 1)] + vx[s.at(0, 0, 0)]) -
+                (vz[s.at(0, 0, 0)] + vz[s.at(-1, 0, 0)]) * (vx[s.at(0, 0, 0)] + vx[s.at(0, 0, -1)])) * udx3;
+  return -(h11 + h12 + h13) + vofx * A.forcx[s.at(0, 0, 0)] / A.g.xlen;
+}
+
+// hdnl2.f90:33-75
+__device__ inline double h2_val(const EulerArgs& A, const Cell& s, double vofy) {
+  const double* vx = A.vx; const double* vy = A.vy; const double* vz = A.vz;
+  double udx1 = A.g.dx1 * 0.25, udx2 = A.g.dx2 * 0.25, udx3 = A.g.dx3 * 0.25;
+  double h21 = ((vy[s.at(1, 0, 0)] + vy[s.at(0, 0, 0)]) * (vx[s.at(1, 0, 0)] + vx[s.at(1, -1, 0)]) -
+                (vy[s.at(0, 0, 0)] + vy[s.at(-1, 0, 0)]) * (vx[s.at(0, 0, 0)] + vx[s.at(0, -1, 0)])) * udx1;
+  double h22 = ((vy[s.at(0, 1, 0)] + vy[s.at(0, 0, 0)]) * (vy[s.at(0, 1, 0)] + vy[s.at(0, 0, 0)]) -
+                (vy[s.at(0, -1, 0)] + vy[s.at(0, 0, 0)]) * (vy[s.at(0, -1, 0)] + vy[s.at(0, 0, 0)])) * udx2;
+  double h23 = ((vz[s.at(0, 0, 1)] + vz[s.at(0, -1, 1)]) * (vy[s.at(0, 0, 1)] + vy[s.at(0, 0, 0)]) -
+                (vz[s.at(0, 0, 0)] + vz[s.at(0, -1, 0)]) * (vy[s.at(0, 0, 0)] + vy[s.at(0, 0, -1)])) * udx3;
+  return -(h21 + h22 + h23) + vofy * A.forcy[s.at(0, 0, 0)] / A.g.ylen;
+}
+
+// hdnl3.f90:38-82
+__device__ inline double h3_val(const EulerArgs& A, const Cell& s, double vofz) {
+  const double* vx = A.vx; const double* vy = A.vy; const double* vz = A.vz;
+  double udx1 = A.g.dx1 * 0.25, udx2 = A.g.dx2 * 0.25, udx3 = A.g.dx3 * 0.25;
+  double h31 = (((vx[s.at(1, 0, 0)] + vx[s.at(1, 0, -1)]) * (vz[s.at(1, 0, 0)] + vz[s.at(0, 0, 0)])) -
+                ((vx[s.at(0, 0, 0)] + vx[s.at(0, 0, -1)]) * (vz[s.at(0, 0, 0)] + vz[s.at(-1, 0, 0)]))) * udx1;
+  double h32 = (((vy[s.at(0, 1, 0)] + vy[s.at(0, 1, -1)]) * (vz[s.at(0, 1, 0)] + vz[s.at(0, 0, 0)])) -
+                ((vy[s.at(0, 0, 0)] + vy[s.at(0, 0, -1)]) * (vz[s.at(0, 0, 0)] + vz[s.at(0, -1, 0)]))) * udx2;
+  double h33 = (((vz[s.at(0, 0, 1)] + vz[s.at(0, 0, 0)]) * (vz[s.at(0, 0, 1)] + vz[s.at(0, 0, 0)])) -
+                ((vz[s.at(0, 0, 0)] + vz[s.at(0, 0, -1)]) * (vz[s.at(0, 0, 0)] + vz[s.at(0, 0, -1)]))) * udx3;
+  double T_interp = (A.temp[s.at(0, 0, 0)] + A.temp[s.at(0, 0, -1)]) * 0.5;
+  double fbz = A.betagz * (T_interp - A.tliq);
+  return (-(h31 + h32 + h33) + fbz) + vofz * A.forcz[s.at(0, 0, 0)] / A.g.zlen;
+}
+
+// hdnlte.f90:45-158 (Nparticle = 1)
+__device__ inline double ht_val(const EulerArgs& A, const Cell& s, double vofp) {
+  const double* vx = A.vx; const double* vy = A.vy; const double* vz = A.vz; const double* t = A.temp;
+  const Grid& g = A.g;
+  double udx1 = g.dx1 * 0.5, udx2 = g.dx2 * 0.5, udx3 = g.dx3 * 0.5;
+  double T = t[s.at(0, 0, 0)];
+  double h31 = (vx[s.at(1, 0, 0)] * (t[s.at(1, 0, 0)] + T) - vx[s.at(0, 0, 0)] * (T + t[s.at(-1, 0, 0)])) * udx1;
+  double h32 = (vy[s.at(0, 1, 0)] * (t[s.at(0, 1, 0)] + T) - vy[s.at(0, 0, 0)] * (T + t[s.at(0, -1, 0)])) * udx2;
+  double h33 = (vz[s.at(0, 0, 1)] * (t[s.at(0, 0, 1)] + T) - vz[s.at(0, 0, 0)] * (T + t[s.at(0, 0, -1)])) * udx3;
+  double ht = -(h31 + h32 + h33) * vofp;
+  const double* pc = A.body->pos_cm;
+  const double* vc = A.body->vel_cm;
+  const double* om = A.body->omega_c;
+  int ic = s.i[1] + 1, jc = s.j[1] + 1, kc = s.k[1] + 1;
+  double r0 = gxc(g, ic) - pc[0], r1 = gym(g, jc) - pc[1], r2 = gzm(g, kc) - pc[2];
+  r0 = r0 - g.xlen * (double)fnint(r0 / g.xlen);
+  r1 = r1 - g.ylen * (double)fnint(r1 / g.ylen);
+  r2 = r2 - g.zlen * (double)fnint(r2 / g.zlen);
+  double u_imh = vc[0] + om[1] * r2 - om[2] * r1;
+  h31 = u_imh * (t[s.at(1, 0, 0)] - t[s.at(-1, 0, 0)]) * udx1;
+  r0 = gxm(g, ic) - pc[0]; r1 = gyc(g, jc) - pc[1]; r2 = gzm(g, kc) - pc[2];
+  r0 = r0 - g.xlen * (double)fnint(r0 / g.xlen);
+  r1 = r1 - g.ylen * (double)fnint(r1 / g.ylen);
+  r2 = r2 - g.zlen * (double)fnint(r2 / g.zlen);
+  double v_jmh = vc[1] + om[2] * r0 - om[0] * r2;
+  h32 = v_jmh * (t[s.at(0, 1, 0)] - t[s.at(0, -1, 0)]) * udx2;
+  r0 = gxm(g, ic) - pc[0]; r1 = gym(g, jc) - pc[1]; r2 = gzc(g, kc) - pc[2];
+  double w_kmh = vc[2] + om[0] * r1 - om[1] * r0;
+  h33 = w_kmh * (t[s.at(0, 0, 1)] - t[s.at(0, 0, -1)]) * udx3;
+  return ht - (h31 + h32 + h33) * (1.0 - vofp);
+}
+
+struct RhsCoef {
+  double ga, ro, al, dt, alre, dx1q, dx2q, dx3q, dx1, dx2, dx3, pec;
+};
+
+// invtr1.f90:40-60
+__device__ inline double rhs_u(const RhsCoef& c, const double* __restrict__ q, const double* __restrict__ pr, const Cell& s, double h, double ru) {
+  double v = q[s.at(0, 0, 0)];
+  double d11 = (q[s.at(1, 0, 0)] - 2.0 * v + q[s.at(-1, 0, 0)]) * c.dx1q;
+  double d22 = (q[s.at(0, 1, 0)] - 2.0 * v + q[s.at(0, -1, 0)]) * c.dx2q;
+  double d33 = (q[s.at(0, 0, 1)] - 2.0 * v + q[s.at(0, 0, -1)]) * c.dx3q;
+  double dc = d11 + d22 + d33;
+  double dp = (pr[s.at(0, 0, 0)] - pr[s.at(-1, 0, 0)]) * (c.dx1 * c.al);
+  return (c.ga * h + c.ro * ru + c.alre * dc - dp) * c.dt;
+}
+// invtr2.f90:37-57
+__device__ inline double rhs_v(const RhsCoef& c, const double* __restrict__ q, const double* __restrict__ pr, const Cell& s, double h, double ru) {
+  double v = q[s.at(0, 0, 0)];
+  double d11 = (q[s.at(1, 0, 0)] - 2.0 * v + q[s.at(-1, 0, 0)]) * c.dx1q;
+  double d22 = (q[s.at(0, 1, 0)] - 2.0 * v + q[s.at(0, -1, 0)]) * c.dx2q;
+  double d33 = (q[s.at(0, 0, 1)] - 2.0 * v + q[s.at(0, 0, -1)]) * c.dx3q;
+  double dc = d11 + d22 + d33;
+  double dp = (pr[s.at(0, 0, 0)] - pr[s.at(0, -1, 0)]) * (c.dx2 * c.al);
+  return (c.ga * h + c.ro * ru + c.alre * dc - dp) * c.dt;
+}
+// invtr3.f90:35-56
+__device__ inline double rhs_w(const RhsCoef& c, const double* __restrict__ q, const double* __restrict__ pr, const Cell& s, double h, double ru) {
+  double v = q[s.at(0, 0, 0)];
+  double d1 = (q[s.at(-1, 0, 0)] - 2.0 * v + q[s.at(1, 0, 0)]) * c.dx1q;
+  double d2 = (q[s.at(0, -1, 0)] - 2.0 * v + q[s.at(0, 1, 0)]) * c.dx2q;
+  double d3 = (q[s.at(0, 0, 1)] - 2.0 * v + q[s.at(0, 0, -1)]) * c.dx3q;
+  double dc = d2 + d3 + d1;
+  double dp = (pr[s.at(0, 0, 0)] - pr[s.at(0, 0, -1)]) * (c.al * c.dx3);
+  return (c.ga * h + c.ro * ru + c.alre * dc - dp) * c.dt;
+}
+// invtrte.f90:32-53
+__device__ inline double rhs_t(const RhsCoef& c, const double* __restrict__ q, const Cell& s, double h, double ru) {
+  double v = q[s.at(0, 0, 0)];
+  double d1 = (q[s.at(-1, 0, 0)] - 2.0 * v + q[s.at(1, 0, 0)]) * c.dx1q;
+  double d2 = (q[s.at(0, -1, 0)] - 2.0 * v + q[s.at(0, 1, 0)]) * c.dx2q;
+  double d3 = (q[s.at(0, 0, 1)] - 2.0 * v + q[s.at(0, 0, -1)]) * c.dx3q;
+  double dc = d2 + d3 + d1;
+  return c.dt * (c.ga * h + c.ro * ru) + (c.al / c.pec) * dc * c.dt;
+}
+
+EulerArgs make_args(melt_ctx* ctx) {
+  EulerArgs A;
+  A.vx = ctx->f[MELT_VX]; A.vy = ctx->f[MELT_VY]; A.vz = ctx->f[MELT_VZ];
+  A.temp = ctx->f[MELT_TEMP]; A.pr = ctx->f[MELT_PR]; A.vofp = ctx->f[MELT_VOFP];
+  A.forcx = ctx->f[MELT_FORCX]; A.forcy = ctx->f[MELT_FORCY]; A.forcz = ctx->f[MELT_FORCZ];
+  A.body = ctx->d_body;
+  A.g = make_grid(ctx);
+  A.betagz = ctx->cfg.betagz; A.tliq = ctx->cfg.tliq;
+  return A;
+}
+
+RhsCoef make_coef(melt_ctx* ctx) {
+  RhsCoef c;
+  c.ga = ctx->ga; c.ro = ctx->ro; c.al = ctx->al; c.dt = ctx->dt;
+  c.alre = ctx->al / ctx->cfg.ren;
+  c.dx1q = ctx->dx1q; c.dx2q = ctx->dx2q; c.dx3q = ctx->dx3q;
+  c.dx1 = ctx->dx1; c.dx2 = ctx->dx2; c.dx3 = ctx->dx3;
+  c.pec = ctx->cfg.ren * ctx->cfg.prandtl;
+  return c;
+}
+
+// ------------------------------------------------------------- kernels
+__global__ __launch_bounds__(256) void k_hdnl(EulerArgs A, double* __restrict__ out, int which, size_t nc) {
+  unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  size_t c = (size_t)b * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  Cell s = make_cell(A.g, c);
+  const double* vp = A.vofp;
+  double p = vp[s.at(0, 0, 0)];
+  double r;
+  if (which == 0) r = h1_val(A, s, 0.5 * (vp[s.at(-1, 0, 0)] + p));
+  else if (which == 1) r = h2_val(A, s, 0.5 * (vp[s.at(0, -1, 0)] + p));
+  else if (which == 2) r = h3_val(A, s, 0.5 * (vp[s.at(0, 0, -1)] + p));
+  else r = ht_val(A, s, p);
+  out[c] = r;
+}
+
+__global__ __launch_bounds__(256) void k_invtr_rhs(Grid g, RhsCoef cf, const double* __restrict__ q, const double* __restrict__ pr,
+                                                   const double* __restrict__ h, double* __restrict__ ru, double* __restrict__ rhs, int which,
+                                                   size_t nc) {
+  unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  size_t c = (size_t)b * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  Cell s = make_cell(g, c);
+  double hv = h[c], rv = ru[c], r;
+  if (which == 0) r = rhs_u(cf, q, pr, s, hv, rv);
+  else if (which == 1) r = rhs_v(cf, q, pr, s, hv, rv);
+  else if (which == 2) r = rhs_w(cf, q, pr, s, hv, rv);
+  else r = rhs_t(cf, q, s, hv, rv);
+  rhs[c] = r;
+  ru[c] = hv;
+}
+
+// Fused explicit terms + RHS for all four transported fields: reads the state
+// once and writes rhs x4 and ru x4 (tsch.f90:49-57 up to the line solves).
+__global__ __launch_bounds__(256) void k_explicit_fused(EulerArgs A, RhsCoef cf, double* __restrict__ ru1, double* __restrict__ ru2,
+                                                        double* __restrict__ ru3, double* __restrict__ rut, double* __restrict__ r_u,
+                                                        double* __restrict__ r_v, double* __restrict__ r_w, double* __restrict__ r_t, size_t nc) {
+  unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  size_t c = (size_t)b * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  Cell s = make_cell(A.g, c);
+  const double* vp = A.vofp;
+  double p = vp[c];
+  double hu = h1_val(A, s, 0.5 * (vp[s.at(-1, 0, 0)] + p));
+  double hv = h2_val(A, s, 0.5 * (vp[s.at(0, -1, 0)] + p));
+  double hw = h3_val(A, s, 0.5 * (vp[s.at(0, 0, -1)] + p));
+  double ht = ht_val(A, s, p);
+  double o1 = ru1[c], o2 = ru2[c], o3 = ru3[c], ot = rut[c];
+  r_u[c] = rhs_u(cf, A.vx, A.pr, s, hu, o1);
+  r_v[c] = rhs_v(cf, A.vy, A.pr, s, hv, o2);
+  r_w[c] = rhs_w(cf, A.vz, A.pr, s, hw, o3);
+  r_t[c] = rhs_t(cf, A.temp, s, ht, ot);
+  ru1[c] = hu; ru2[c] = hv; ru3[c] = hw; rut[c] = ht;
+}
+
+// -------------------------------------------------- circulant PCR solver
+// The systems of solxi/solxj/solxk are periodic tridiagonal with constant
+// coefficients (diag 1, off-diagonals -G/(1+2G), rhs scaled by 1/(1+2G);
+// solxi.f90:33-37): circulant.  Parallel cyclic reduction keeps a circulant
+// circulant, so one scalar coefficient triple per step serves every row; the
+// ratio a/b squares each step and the reduction stops once it is below 1e-17
+// (after which x = d/b to round-off), or when the coupling offset reaches M/2
+// (M = 2^p: closed 2x2 solve) or M/3 (M = 3*2^p: closed 3x3 circulant solve).
+struct SolveField {
+  double* rhs;    // in/out (in only when qadd != null and store_rhs == 0)
+  double* qadd;   // solxk: q += solution
+  double betadx;
+  int store_rhs;  // with qadd: also leave the solution in rhs (solxk.f90:40)
+};
+struct SolveBatch {
+  SolveField f[4];
+};
+
+constexpr int PCR_EMAX = 16;
+
+template <int DIR>
+__global__ __launch_bounds__(256) void k_pcr(SolveBatch batch, int M, int W, int n1, int n2, int n3) {
+  extern __shared__ double lds[];
+  const SolveField F = batch.f[blockIdx.y];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int E = (M * W) / nt;
+  // tile base and strides
+  size_t base, es, cs;
+  unsigned t = xcd_block(blockIdx.x, gridDim.x);
+  if (DIR == 0) {
+    int ntj = n2 / W;
+    int jt = t % ntj, k = t / ntj;
+    base = (size_t)n1 * ((size_t)jt * W + (size_t)n2 * k);
+    es = 1; cs = n1;
+  } else if (DIR == 1) {
+    int nti = n1 / W;
+    int it = t % nti, k = t / nti;
+    base = (size_t)it * W + (size_t)n1 * n2 * k;
+    es = n1; cs = 1;
+  } else {
+    int nti = n1 / W;
+    int it = t % nti, j = t / nti;
+    base = (size_t)it * W + (size_t)n1 * j;
+    es = (size_t)n1 * n2; cs = 1;
+  }
+  const int ldx = M + 1;  // x layout: [line][x] padded
+  auto own_r = [&](int idx) { return DIR == 0 ? idx % M : idx / W; };
+  auto own_c = [&](int idx) { return DIR == 0 ? idx / M : idx % W; };
+  auto la = [&](int r, int c) { return DIR == 0 ? c * ldx + r : r * W + c; };
+
+  const double ackl_b = 1.0 / (1.0 + 2.0 * F.betadx);
+  double a = -F.betadx * ackl_b, bdiag = 1.0;
+  double d[PCR_EMAX];
+#pragma unroll
+  for (int q = 0; q < PCR_EMAX; ++q) {
+    if (q < E) {
+      int idx = tid + q * nt;
+      int r = own_r(idx), c = own_c(idx);
+      d[q] = F.rhs[base + (size_t)r * es + (size_t)c * cs] * ackl_b;
+      lds[la(r, c)] = d[q];
+    }
+  }
+  __syncthreads();
+  int p2 = M & (-M);          // largest power of two dividing M
+  int m3 = M / p2;            // 1 or 3
+  int o = 1;
+  int last = (m3 == 1) ? M / 4 : p2 / 2;  // offsets 1..last are reduction steps
+  bool done = false;
+  while (o <= last) {
+    if (fabs(a) < 1e-17 * fabs(bdiag)) { done = true; break; }
+    double k = a / bdiag;
+    double nd[PCR_EMAX];
+#pragma unroll
+    for (int q = 0; q < PCR_EMAX; ++q) {
+      if (q < E) {
+        int idx = tid + q * nt;
+        int r = own_r(idx), c = own_c(idx);
+        int rm = r - o; if (rm < 0) rm += M;
+        int rp = r + o; if (rp >= M) rp -= M;
+        nd[q] = d[q] - k * (lds[la(rm, c)] + lds[la(rp, c)]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PCR_EMAX; ++q) {
+      if (q < E) {
+        int idx = tid + q * nt;
+        d[q] = nd[q];
+        lds[la(own_r(idx), own_c(idx))] = d[q];
+      }
+    }
+    __syncthreads();
+    double bn = bdiag - 2.0 * a * k;
+    a = -a * k;
+    bdiag = bn;
+    o *= 2;
+  }
+  if (!done && fabs(a) < 1e-17 * fabs(bdiag)) done = true;
+  double x[PCR_EMAX];
+#pragma unroll
+  for (int q = 0; q < PCR_EMAX; ++q) {
+    if (q < E) {
+      int idx = tid + q * nt;
+      int r = own_r(idx), c = own_c(idx);
+      if (done) {
+        x[q] = d[q] / bdiag;
+      } else if (m3 == 1) {
+        int h = M / 2;
+        int rh = r + h; if (rh >= M) rh -= M;
+        double dh = lds[la(rh, c)];
+        x[q] = (bdiag * d[q] - 2.0 * a * dh) / (bdiag * bdiag - 4.0 * a * a);
+      } else {
+        int h = M / 3;
+        int r1 = r + h; if (r1 >= M) r1 -= M;
+        int r2 = r1 + h; if (r2 >= M) r2 -= M;
+        double D = d[q] + lds[la(r1, c)] + lds[la(r2, c)];
+        x[q] = (d[q] - a * (D / (bdiag + 2.0 * a))) / (bdiag - a);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < PCR_EMAX; ++q) {
+    if (q < E) {
+      int idx = tid + q * nt;
+      size_t gaddr = base + (size_t)own_r(idx) * es + (size_t)own_c(idx) * cs;
+      if (F.qadd) {
+        F.qadd[gaddr] = F.qadd[gaddr] + x[q];
+        if (F.store_rhs) F.rhs[gaddr] = x[q];
+      } else {
+        F.rhs[gaddr] = x[q];
+      }
+    }
+  }
+}
+
+int tile_width(int M, int ndim_other) {
+  int W = 16;
+  while (W > 1 && M * W > PCR_EMAX * 256) W /= 2;
+  while (W > 1 && ndim_other % W) W /= 2;
+  return W;
+}
+
+// ------------------------------------------------------------ divg etc
+__global__ __launch_bounds__(256) void k_divg(Grid g, const double* __restrict__ vx, const double* __restrict__ vy,
+                                              const double* __restrict__ vz, double* __restrict__ dph, double usdtal, size_t nc) {
+  unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  size_t c = (size_t)b * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  Cell s = make_cell(g, c);
+  double dq = (vx[s.at(1, 0, 0)] - vx[c]) * g.dx1 + (vy[s.at(0, 1, 0)] - vy[c]) * g.dx2 + (vz[s.at(0, 0, 1)] - vz[c]) * g.dx3;
+  dph[c] = dq * usdtal;
+}
+
+// updvp.f90:16-31 and prcalc.f90:17-33 in one pass; with mean != null the
+// mean MLS force deferred from velforce is subtracted first ((u+f)-fbar).
+__global__ __launch_bounds__(256) void k_project(Grid g, double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
+                                                 double* __restrict__ pr, const double* __restrict__ dph, const double* __restrict__ mean,
+                                                 double udx1, double udx2, double udx3, double be, double dx1q, double dx2q, double dx3q,
+                                                 int do_u, int do_p, size_t nc) {
+  unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  size_t c = (size_t)b * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  Cell s = make_cell(g, c);
+  double l = dph[c];
+  double pim = dph[s.at(-1, 0, 0)], pjm = dph[s.at(0, -1, 0)], pkm = dph[s.at(0, 0, -1)];
+  if (do_u) {
+    double u = vx[c], v = vy[c], w = vz[c];
+    if (mean) { u = u - mean[0]; v = v - mean[1]; w = w - mean[2]; }
+    vx[c] = u - (l - pim) * udx1;
+    vy[c] = v - (l - pjm) * udx2;
+    vz[c] = w - (l - pkm) * udx3;
+  }
+  if (do_p) {
+    double pip = dph[s.at(1, 0, 0)], pjp = dph[s.at(0, 1, 0)], pkp = dph[s.at(0, 0, 1)];
+    pr[c] = pr[c] + l - be * ((pip - 2.0 * l + pim) * dx1q + (pjp - 2.0 * l + pjm) * dx2q + (pkp - 2.0 * l + pkm) * dx3q);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_interp_vof(Grid g, const double* __restrict__ p, double* __restrict__ vx, double* __restrict__ vy,
+                                                    double* __restrict__ vz, size_t nc) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  Cell s = make_cell(g, c);
+  double v = p[c];
+  vx[c] = 0.5 * (p[s.at(-1, 0, 0)] + v);
+  vy[c] = 0.5 * (p[s.at(0, -1, 0)] + v);
+  vz[c] = 0.5 * (p[s.at(0, 0, -1)] + v);
+}
+
+// Max reductions of cfl.f90:9-25 (what = 0) and divgck divg.f90:36-50 (what = 1).
+__global__ __launch_bounds__(256) void k_reduce_max(Grid g, const double* __restrict__ vx, const double* __restrict__ vy,
+                                                    const double* __restrict__ vz, int what, size_t nc, unsigned long long* out) {
+  __shared__ double sm[256];
+  double m = what == 0 ? 0.00000001 : 0.0;
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += (size_t)gridDim.x * blockDim.x) {
+    Cell s = make_cell(g, c);
+    double q;
+    if (what == 0)
+      q = fabs((vx[c] + vx[s.at(1, 0, 0)]) * 0.5 * g.dx1) + fabs((vy[c] + vy[s.at(0, 1, 0)]) * 0.5 * g.dx2) +
+          fabs((vz[c] + vz[s.at(0, 0, 1)]) * 0.5 * g.dx3);
+    else
+      q = fabs((vx[s.at(1, 0, 0)] - vx[c]) * g.dx1 + (vy[s.at(0, 1, 0)] - vy[c]) * g.dx2 + (vz[s.at(0, 0, 1)] - vz[c]) * g.dx3);
+    m = fmax(m, q);
+  }
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + w]);
+    __syncthreads();
+  }
+  // non-negative doubles order like their bit patterns
+  if (threadIdx.x == 0) atomicMax(out, (unsigned long long)__double_as_longlong(sm[0]));
+}
+
+__global__ __launch_bounds__(256) void k_inqpr(Grid g, double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
+                                               double* __restrict__ t, const melt_body* __restrict__ body, double tliq, double tsol,
+                                               double rad_p, size_t nc) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  int i = (int)(c % g.n1) + 1, j = (int)((c / g.n1) % g.n2) + 1, k = (int)(c / ((size_t)g.n1 * g.n2)) + 1;
+  double v[3] = {gxm(g, i) - body->pos_cm[0], gym(g, j) - body->pos_cm[1], gzm(g, k) - body->pos_cm[2]};
+  double rr = fnorm2(v, 3);
+  vx[c] = 0.0; vy[c] = 0.0; vz[c] = 0.0;
+  t[c] = tliq - (tliq - tsol) * 0.5 * (1.0 - tanh((rr - rad_p) * g.dx1 / 2.0));
+}
+
+inline unsigned nblocks(size_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+// ------------------------------------------------------------ launchers
+int euler_launch_hdnl(melt_ctx* ctx, int which) {
+  static const int out_id[4] = {MELT_DQ, MELT_DPH, MELT_QCAP, MELT_HTEMP};
+  EulerArgs A = make_args(ctx);
+  hipLaunchKernelGGL(k_hdnl, dim3(nblocks(ctx->nc, 256)), dim3(256), 0, ctx->stream, A, ctx->f[out_id[which]], which, ctx->nc);
+  MELT_HIP(hipGetLastError());
+  return 0;
+}
+
+int euler_launch_invtr_rhs(melt_ctx* ctx, int which) {
+  static const int q_id[4] = {MELT_VX, MELT_VY, MELT_VZ, MELT_TEMP};
+  static const int h_id[4] = {MELT_DQ, MELT_DPH, MELT_QCAP, MELT_HTEMP};
+  static const int ru_id[4] = {MELT_RU1, MELT_RU2, MELT_RU3, MELT_RUT};
+  RhsCoef cf = make_coef(ctx);
+  hipLaunchKernelGGL(k_invtr_rhs, dim3(nblocks(ctx->nc, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), cf, ctx->f[q_id[which]],
+                     ctx->f[MELT_PR], ctx->f[h_id[which]], ctx->f[ru_id[which]], ctx->f[MELT_RHS], which, ctx->nc);
+  MELT_HIP(hipGetLastError());
+  return 0;
+}
+
+int euler_launch_explicit_fused(melt_ctx* ctx) {
+  EulerArgs A = make_args(ctx);
+  RhsCoef cf = make_coef(ctx);
+  hipLaunchKernelGGL(k_explicit_fused, dim3(nblocks(ctx->nc, 256)), dim3(256), 0, ctx->stream, A, cf, ctx->f[MELT_RU1], ctx->f[MELT_RU2],
+                     ctx->f[MELT_RU3], ctx->f[MELT_RUT], ctx->rhs4[RHS_U], ctx->rhs4[RHS_V], ctx->rhs4[RHS_W], ctx->rhs4[RHS_T], ctx->nc);
+  MELT_HIP(hipGetLastError());
+  return 0;
+}
+
+// dir 0/1/2 = x/y/z; fields[nfield] are solved in place, or (qadd != null)
+// added into qadd[] (solxk.f90:44-50).
+int euler_launch_solve(melt_ctx* ctx, int dir, double* const* fields, int nfield, const double* betadx, double* const* qadd, int store_rhs) {
+  int n[3] = {ctx->n1, ctx->n2, ctx->n3};
+  int M = n[dir];
+  int other = dir == 0 ? ctx->n2 : ctx->n1;
+  int W = tile_width(M, other);
+  // threads per tile: a divisor of M*W, at most 256, so every element is owned
+  int E = (M * W + 255) / 256;
+  while ((M * W) % E) ++E;
+  if (E > PCR_EMAX) return melt_fail(ctx, MELT_ERR_ARG, "line solve", "unsupported line length");
+  int nt = (M * W) / E;
+  SolveBatch B{};
+  for (int q = 0; q < nfield; ++q) {
+    B.f[q].rhs = fields[q];
+    B.f[q].qadd = qadd ? qadd[q] : nullptr;
+    B.f[q].betadx = betadx[q];
+    B.f[q].store_rhs = store_rhs;
+  }
+  size_t tiles = dir == 0 ? (size_t)(ctx->n2 / W) * ctx->n3 : dir == 1 ? (size_t)(ctx->n1 / W) * ctx->n3 : (size_t)(ctx->n1 / W) * ctx->n2;
+  size_t shm = dir == 0 ? sizeof(double) * (size_t)W * (M + 1) : sizeof(double) * (size_t)W * M;
+  dim3 grid((unsigned)tiles, nfield);
+  if (dir == 0) hipLaunchKernelGGL(k_pcr<0>, grid, dim3(nt), shm, ctx->stream, B, M, W, ctx->n1, ctx->n2, ctx->n3);
+  else if (dir == 1) hipLaunchKernelGGL(k_pcr<1>, grid, dim3(nt), shm, ctx->stream, B, M, W, ctx->n1, ctx->n2, ctx->n3);
+  else hipLaunchKernelGGL(k_pcr<2>, grid, dim3(nt), shm, ctx->stream, B, M, W, ctx->n1, ctx->n2, ctx->n3);
+  MELT_HIP(hipGetLastError());
+  return 0;
+}
+
+int euler_launch_divg(melt_ctx* ctx) {
+  double usdtal = 1.0 / (ctx->dt * ctx->al);
+  hipLaunchKernelGGL(k_divg, dim3(nblocks(ctx->nc, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), ctx->f[MELT_VX], ctx->f[MELT_VY],
+                     ctx->f[MELT_VZ], ctx->f[MELT_DPH], usdtal, ctx->nc);
+  MELT_HIP(hipGetLastError());
+  return 0;
+}
+
+int euler_launch_project(melt_ctx* ctx, int do_u, int do_p, int use_mean) {
+  double udx1 = ctx->al * ctx->dt * ctx->dx1, udx2 = ctx->al * ctx->dt * ctx->dx2, udx3 = ctx->al * ctx->dt * ctx->dx3;
+  double be = ctx->al * ctx->beta;
+  hipLaunchKernelGGL(k_project, dim3(nblocks(ctx->nc, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), ctx->f[MELT_VX], ctx->f[MELT_VY],
+                     ctx->f[MELT_VZ], ctx->f[MELT_PR], ctx->f[MELT_DPH], use_mean ? ctx->d_fmean : nullptr, udx1, udx2, udx3, be,
+                     ctx->dx1q, ctx->dx2q, ctx->dx3q, do_u, do_p, ctx->nc);
+  MELT_HIP(hipGetLastError());
+  return 0;
+}
+
+int euler_launch_interp_vof(melt_ctx* ctx) {
+  hipLaunchKernelGGL(k_interp_vof, dim3(nblocks(ctx->nc, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), ctx->f[MELT_VOFP],
+                     ctx->f[MELT_VOFX], ctx->f[MELT_VOFY], ctx->f[MELT_VOFZ], ctx->nc);
+  MELT_HIP(hipGetLastError());
+  return 0;
+}
+
+int euler_reduce_max(melt_ctx* ctx, int what, double* out) {
+  unsigned long long* d = reinterpret_cast<unsigned long long*>(ctx->d_red);
+  MELT_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), ctx->stream));
+  unsigned nb = (unsigned)std::min<size_t>(1024, nblocks(ctx->nc, 256));
+  hipLaunchKernelGGL(k_reduce_max, dim3(nb), dim3(256), 0, ctx->stream, make_grid(ctx), ctx->f[MELT_VX], ctx->f[MELT_VY], ctx->f[MELT_VZ],
+                     what, ctx->nc, d);
+  MELT_HIP(hipGetLastError());
+  unsigned long long h = 0;
+  MELT_HIP(hipMemcpyAsync(&h, d, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+  MELT_HIP(hipStreamSynchronize(ctx->stream));
+  double v;
+  memcpy(&v, &h, sizeof v);
+  *out = v;
+  return 0;
+}
+
+int euler_launch_inqpr(melt_ctx* ctx) {
+  hipLaunchKernelGGL(k_inqpr, dim3(nblocks(ctx->nc, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), ctx->f[MELT_VX], ctx->f[MELT_VY],
+                     ctx->f[MELT_VZ], ctx->f[MELT_TEMP], ctx->d_body, ctx->cfg.tliq, ctx->cfg.tsol, ctx->rad_p, ctx->nc);
+  MELT_HIP(hipGetLastError());
+  return 0;
+}

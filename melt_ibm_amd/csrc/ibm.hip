@@ -1,0 +1,987 @@
+// Immersed-boundary kernels of the RK3 substep on gfx950:
+//  - narrow-band signed-distance tagging (genSDF.f90) with a deterministic
+//    arg-min (smallest distance, then lowest face index: the reference's
+//    "first face wins"), so tags equal the serial CPU result bit for bit;
+//  - init-time ray tagging (pencil_rayTag.f90, rayAux.f90);
+//  - MLS shape functions (mlsWeight.f90), interpolation and force spreading
+//    (mlsForce.f90, forc1-3.f90, forctemp.f90).  Spreading accumulates each
+//    contribution as a 128-bit fixed-point integer, so the sum over faces is
+//    exact and independent of scheduling order;
+//  - MLS gradient probes for the heat flux (mls_heatFlux.f90) and the surface
+//    loads (mls_structLoads.f90).
+// All of this scales with the number of triangles, not with N^3.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+
+#include "common.h"
+
+namespace {
+
+constexpr double WCON = 0.3, WSCL = 1.5;  // initmls.f90:12-13
+
+__device__ inline double gco(const Grid& g, int dim, int centre, int i) {
+  if (dim == 0) return centre ? gxm(g, i) : gxc(g, i);
+  if (dim == 1) return centre ? gym(g, i) : gyc(g, i);
+  return centre ? gzm(g, i) : gzc(g, i);
+}
+
+__device__ inline double mls_gaussian(double r) {  // geom.f90:24-36
+  if (r > 1.0) return 0.0;
+  double t = r / WCON;
+  return det_exp(-(t * t));
+}
+__device__ inline double mls_gauss_deriv(double r) {  // geom.f90:38-53
+  if (r > 1.0) return 0.0;
+  double t = r / WCON;
+  return -2.0 * r / (WCON * WCON) * det_exp(-(t * t));
+}
+
+// A(i,j) += (w p_j) p_i : DGEMM('N','T',4,4,1,w,p,4,p,4,1,A,4)
+__device__ inline void rank1(double A[16], double w, const double p[4]) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    double temp = w * p[j];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) A[i + 4 * j] = A[i + 4 * j] + temp * p[i];
+  }
+}
+
+// MLS shape function over the 3x3x3 support i0..i0+2, j0.., k0.. (wght1/2/3/temp,
+// mlsWeight.f90:48-412); c[d] selects face (0) or centre (1) coordinates.
+__device__ int mls_phi(const Grid& g, const double pos[3], int i0, int j0, int k0, const int c[3], double phi[27]) {
+  double A[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) A[q] = 0.0;
+  for (int k = k0; k <= k0 + 2; ++k) {
+    double zk = gco(g, 2, c[2], k);
+    double w3 = mls_gaussian(fabs(zk - pos[2]) / (WSCL / g.dx3));
+    for (int j = j0; j <= j0 + 2; ++j) {
+      double yj = gco(g, 1, c[1], j);
+      double w23 = mls_gaussian(fabs(yj - pos[1]) / (WSCL / g.dx2)) * w3;
+      for (int i = i0; i <= i0 + 2; ++i) {
+        double xi = gco(g, 0, c[0], i);
+        double wtx = mls_gaussian(fabs(xi - pos[0]) / (WSCL / g.dx1)) * w23;
+        double p[4] = {1.0, xi, yj, zk};
+        rank1(A, wtx, p);
+      }
+    }
+  }
+  int info = lu_inverse<4>(A);
+  if (info) return info;
+  double ptx[4] = {1.0, pos[0], pos[1], pos[2]}, ptxA[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    double s = 0.0;
+#pragma unroll
+    for (int l = 0; l < 4; ++l) s = s + (1.0 * A[l + 4 * j]) * ptx[l];
+    ptxA[j] = s;
+  }
+  int inw = 0;
+  for (int k = k0; k <= k0 + 2; ++k) {
+    double zk = gco(g, 2, c[2], k);
+    double w3 = mls_gaussian(fabs(zk - pos[2]) / (WSCL / g.dx3));
+    for (int j = j0; j <= j0 + 2; ++j) {
+      double yj = gco(g, 1, c[1], j);
+      double w23 = mls_gaussian(fabs(yj - pos[1]) / (WSCL / g.dx2)) * w3;
+      for (int i = i0; i <= i0 + 2; ++i) {
+        double xi = gco(g, 0, c[0], i);
+        double wtx = mls_gaussian(fabs(xi - pos[0]) / (WSCL / g.dx1)) * w23;
+        double b[4] = {wtx * 1.0, wtx * xi, wtx * yj, wtx * zk};
+        double s = 0.0;
+#pragma unroll
+        for (int l = 0; l < 4; ++l) s = s + (1.0 * b[l]) * ptxA[l];
+        phi[inw++] = s;
+      }
+    }
+  }
+  return 0;
+}
+
+__device__ inline double dweight(double pc, double gcrd, double nrm, double scale) {
+  if (nrm < 2.220446049250313e-16) return 0.0;  // EPSILON(1.0d0)
+  double d = mls_gauss_deriv(nrm);
+  return d * ((pc - gcrd) / fabs(pc - gcrd) / scale);
+}
+
+// MLS gradient at a probe (wght_gradT mls_heatFlux.f90:85-300 and
+// wght_gradP/U/V/W mls_structLoads.f90:380-1297).  kz: grid indices for the z
+// coordinate (the reference uses k1-1, k1, k1); ke: field planes (1-based,
+// ghost-plane semantics) k1-1, k1, k1+1.
+__device__ int mls_grad(const Grid& g, const double* __restrict__ fld, const double pos[3], int ic, int jc, const int kz[3], const int ke[3],
+                        const int c[3], double grad[3]) {
+  double A[16], dA[3][16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) { A[q] = 0.0; dA[0][q] = 0.0; dA[1][q] = 0.0; dA[2][q] = 0.0; }
+  const double sx = WSCL / g.dx1, sy = WSCL / g.dx2, sz = WSCL / g.dx3;
+  for (int nk = 0; nk < 3; ++nk) {
+    double zk = gco(g, 2, c[2], kz[nk]);
+    double n3 = fabs(pos[2] - zk) / sz;
+    double w3 = mls_gaussian(n3), dw3 = dweight(pos[2], zk, n3, sz);
+    for (int j = jc - 1; j <= jc + 1; ++j) {
+      double yj = gco(g, 1, c[1], j);
+      double n2 = fabs(pos[1] - yj) / sy;
+      double w2 = mls_gaussian(n2), dw2 = dweight(pos[1], yj, n2, sy);
+      double w23 = w2 * w3;
+      for (int i = ic - 1; i <= ic + 1; ++i) {
+        double xi = gco(g, 0, c[0], i);
+        double n1 = fabs(pos[0] - xi) / sx;
+        double w1 = mls_gaussian(n1), dw1 = dweight(pos[0], xi, n1, sx);
+        double wtx = w1 * w23;
+        double p[4] = {1.0, xi, yj, zk};
+        rank1(A, wtx, p);
+        rank1(dA[0], dw1 * w2 * w3, p);
+        rank1(dA[1], w1 * dw2 * w3, p);
+        rank1(dA[2], w1 * w2 * dw3, p);
+      }
+    }
+  }
+  int info = lu_inverse<4>(A);
+  if (info) return info;
+  double ptx[4] = {1.0, pos[0], pos[1], pos[2]}, gam[4], dg[3][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) gam[i] = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    double t = 1.0 * ptx[j];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) gam[i] = gam[i] + t * A[i + 4 * j];
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double pv[4] = {0.0, 0.0, 0.0, 0.0};
+    pv[d + 1] = 1.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      double t = -1.0 * gam[j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) pv[i] = pv[i] + t * dA[d][i + 4 * j];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) dg[d][i] = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      double t = 1.0 * pv[j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) dg[d][i] = dg[d][i] + t * A[i + 4 * j];
+    }
+  }
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int nk = 0; nk < 3; ++nk) {
+    double zk = gco(g, 2, c[2], kz[nk]);
+    double n3 = fabs(pos[2] - zk) / sz;
+    double w3 = mls_gaussian(n3), dw3 = dweight(pos[2], zk, n3, sz);
+    for (int j = jc - 1; j <= jc + 1; ++j) {
+      double yj = gco(g, 1, c[1], j);
+      double n2 = fabs(pos[1] - yj) / sy;
+      double w2 = mls_gaussian(n2), dw2 = dweight(pos[1], yj, n2, sy);
+      double w23 = w2 * w3;
+      for (int i = ic - 1; i <= ic + 1; ++i) {
+        double xi = gco(g, 0, c[0], i);
+        double n1 = fabs(pos[0] - xi) / sx;
+        double w1 = mls_gaussian(n1), dw1 = dweight(pos[0], xi, n1, sx);
+        double wtx = w1 * w23;
+        double p[4] = {1.0, xi, yj, zk};
+        double dw[3] = {dw1 * w2 * w3, w1 * dw2 * w3, w1 * w2 * dw3};
+        double val = fld[widx(g, i, j, ke[nk])];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          double acc = 0.0;
+#pragma unroll
+          for (int l = 0; l < 4; ++l) acc = acc + (1.0 * (wtx * p[l])) * dg[d][l];
+#pragma unroll
+          for (int l = 0; l < 4; ++l) acc = acc + (1.0 * (dw[d] * p[l])) * gam[l];
+          s[d] = s[d] + acc * val;
+        }
+      }
+    }
+  }
+  grad[0] = s[0]; grad[1] = s[1]; grad[2] = s[2];
+  return 0;
+}
+
+// ---------------------------------------------------- 128-bit accumulation
+// A contribution v is held as hi + lo * 2^-64 with hi = floor(v) (two's
+// complement) and lo the fraction scaled by 2^64 (truncated below 2^-64).
+// Integer addition with carry is associative, so the accumulated sum does not
+// depend on the order in which faces arrive.
+__device__ inline void acc_add(unsigned long long* w /* [lo, hi] */, double v) {
+  if (v == 0.0) return;
+  double fl = floor(v);
+  long long hi = (long long)fl;
+  double frac = v - fl;
+  unsigned long long lo = (unsigned long long)(frac * 18446744073709551616.0);
+  unsigned long long old = atomicAdd(&w[0], lo);
+  unsigned long long carry = (old + lo < old) ? 1ull : 0ull;
+  atomicAdd(&w[1], (unsigned long long)hi + carry);
+}
+__device__ inline double acc_value(unsigned long long lo, unsigned long long hi) {
+  return (double)(long long)hi + (double)lo * 5.421010862427522170037264e-20;  // 2^-64
+}
+
+__device__ inline long long win_slot(const int org[3], const int E[3], int iu, int ju, int ku) {
+  int a = iu - org[0], b = ju - org[1], c = ku - org[2];
+  if (a < 0 || b < 0 || c < 0 || a >= E[0] || b >= E[1] || c >= E[2]) return -1;
+  return (long long)a + (long long)E[0] * ((long long)b + (long long)E[1] * c);
+}
+
+struct Win3 { int e[3]; };
+
+// ------------------------------------------------------------ windows
+// Face base indices floor(x*dx)+1 (unwrapped) bound every cell the MLS
+// supports (base-1..base+2) and the tagging band (base-2..base+2) touch.
+__global__ __launch_bounds__(256) void k_window(Grid g, const double* __restrict__ bar, const unsigned char* __restrict__ gf, int nf,
+                                                melt_window* win, Win3 ef, Win3 et, int which, int* err) {
+  __shared__ int mn[3], mx[3];
+  if (threadIdx.x < 3) { mn[threadIdx.x] = INT_MAX; mx[threadIdx.x] = INT_MIN; }
+  __syncthreads();
+  int lmn[3] = {INT_MAX, INT_MAX, INT_MAX}, lmx[3] = {INT_MIN, INT_MIN, INT_MIN};
+  double dx[3] = {g.dx1, g.dx2, g.dx3};
+  for (int f = threadIdx.x; f < nf; f += blockDim.x) {
+    if (gf[f]) continue;
+    for (int d = 0; d < 3; ++d) {
+      int b = (int)floor(bar[3 * f + d] * dx[d]) + 1;
+      lmn[d] = min(lmn[d], b); lmx[d] = max(lmx[d], b);
+    }
+  }
+  for (int d = 0; d < 3; ++d) { atomicMin(&mn[d], lmn[d]); atomicMax(&mx[d], lmx[d]); }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (which & 1) {  // forcing window
+      for (int d = 0; d < 3; ++d) {
+        win[0].org[d] = mn[d] - 2;
+        if (mx[d] + 3 - win[0].org[d] >= ef.e[d]) raise_error(err, MELT_ERR_WINDOW);
+      }
+    }
+    if (which & 2) {  // tagging window
+      if (win[1].prev_valid >= 0) {
+        for (int d = 0; d < 3; ++d) win[1].prev_org[d] = win[1].org[d];
+        win[1].prev_valid = win[1].pad;
+      }
+      for (int d = 0; d < 3; ++d) {
+        win[1].org[d] = mn[d] - 2;
+        if (mx[d] + 2 - win[1].org[d] >= et.e[d]) raise_error(err, MELT_ERR_WINDOW);
+      }
+      win[1].pad = 1;  // current box is valid from now on
+    }
+  }
+}
+
+// ------------------------------------------------------------ narrow band
+// genSDF.f90:85-234, Eberly's point-triangle distance.
+__device__ double dist_point_tri(const double P[3], const double* V0, const double* V1, const double* V2) {
+  double E0[3], E1[3], DE[3];
+  for (int d = 0; d < 3; ++d) { E0[d] = V1[d] - V0[d]; E1[d] = V2[d] - V0[d]; DE[d] = V0[d] - P[d]; }
+  double a = E0[0] * E0[0] + E0[1] * E0[1] + E0[2] * E0[2];
+  double b = E0[0] * E1[0] + E0[1] * E1[1] + E0[2] * E1[2];
+  double c = E1[0] * E1[0] + E1[1] * E1[1] + E1[2] * E1[2];
+  double d = E0[0] * DE[0] + E0[1] * DE[1] + E0[2] * DE[2];
+  double e = E1[0] * DE[0] + E1[1] * DE[1] + E1[2] * DE[2];
+  double det = a * c - b * b, s = b * e - c * d, t = b * d - a * e;
+  if (s + t <= det) {
+    if (s < 0.0) {
+      if (t < 0.0) {
+        if (d < 0.0) { t = 0.0; s = (-d >= a) ? 1.0 : -d / a; }
+        else { s = 0.0; t = (e >= 0.0) ? 0.0 : ((-e >= c) ? 1.0 : -e / c); }
+      } else {
+        s = 0.0; t = (e >= 0.0) ? 0.0 : ((-e >= c) ? 1.0 : -e / c);
+      }
+    } else if (t < 0.0) {
+      t = 0.0; s = (d >= 0.0) ? 0.0 : ((-d >= a) ? 1.0 : -d / a);
+    } else {
+      double inv = 1.0 / det;
+      s = s * inv; t = t * inv;
+    }
+  } else {
+    if (s < 0.0) {
+      double t0 = b + d, t1 = c + e;
+      if (t1 > t0) {
+        double num = t1 - t0, den = a - 2.0 * b + c;
+        if (num >= den) { s = 1.0; t = 0.0; } else { s = num / den; t = 1.0 - s; }
+      } else {
+        s = 0.0; t = (t1 <= 0.0) ? 1.0 : ((e >= 0.0) ? 0.0 : -e / c);
+      }
+    } else if (t < 0.0) {
+      double t0 = b + e, t1 = a + d;
+      if (t1 > t0) {
+        double num = t1 - t0, den = a - 2.0 * b + c;
+        if (num >= den) { t = 1.0; s = 0.0; } else { t = num / den; s = 1.0 - t; }
+      } else {
+        t = 0.0; s = (t1 <= 0.0) ? 1.0 : ((d >= 0.0) ? 0.0 : -d / a);
+      }
+    } else {
+      double num = c + e - b - d, den = a - 2.0 * b + c;
+      s = (num <= 0.0) ? 0.0 : ((num >= den) ? 1.0 : num / den);
+      t = 1.0 - s;
+    }
+  }
+  double q[3];
+  for (int k = 0; k < 3; ++k) q[k] = (V0[k] + s * E0[k] + t * E1[k]) - P[k];
+  return sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+}
+
+__global__ __launch_bounds__(256) void k_tag_reset(Grid g, double* __restrict__ sdf, const melt_window* __restrict__ win, Win3 et) {
+  if (win[1].prev_valid <= 0) return;
+  size_t n = (size_t)et.e[0] * et.e[1] * et.e[2];
+  for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += (size_t)gridDim.x * blockDim.x) {
+    int a = (int)(s % et.e[0]), b = (int)((s / et.e[0]) % et.e[1]), c = (int)(s / ((size_t)et.e[0] * et.e[1]));
+    sdf[widx(g, win[1].prev_org[0] + a, win[1].prev_org[1] + b, win[1].prev_org[2] + c)] = 999.0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_tag_init(unsigned long long* __restrict__ key, int* __restrict__ face, size_t n) {
+  for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += (size_t)gridDim.x * blockDim.x) {
+    key[s] = ~0ull;
+    face[s] = INT_MAX;
+  }
+}
+
+struct MeshArgs {
+  const double* __restrict__ xyzv;
+  const double* __restrict__ bar;
+  const double* __restrict__ nor;
+  const int* __restrict__ vof;
+  const unsigned char* __restrict__ gf;
+  int nf;
+};
+
+// mode 0: min distance; 1: lowest face among minima; 2: write sdf and VOFp
+__global__ __launch_bounds__(256) void k_tag_pass(Grid g, MeshArgs M, const melt_window* __restrict__ win, Win3 et,
+                                                  unsigned long long* __restrict__ key, int* __restrict__ face, double* __restrict__ sdf,
+                                                  double* __restrict__ vofp, int mode, double pi, int* err) {
+  size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (size_t)M.nf * 125) return;
+  int f = (int)(id / 125), q = (int)(id % 125);
+  if (M.gf[f]) return;
+  const double* pos = &M.bar[3 * f];
+  int i = (int)floor(pos[0] * g.dx1) + 1 + (q % 5) - 2;
+  int j = (int)floor(pos[1] * g.dx2) + 1 + (q / 5) % 5 - 2;
+  int k = (int)floor(pos[2] * g.dx3) + 1 + q / 25 - 2;
+  const double* V0 = &M.xyzv[3 * M.vof[3 * f + 0]];
+  const double* V1 = &M.xyzv[3 * M.vof[3 * f + 1]];
+  const double* V2 = &M.xyzv[3 * M.vof[3 * f + 2]];
+  double P[3] = {gxm(g, i), gym(g, j), gzm(g, k)};
+  double dist = dist_point_tri(P, V0, V1, V2);
+  long long s = win_slot(win[1].org, et.e, i, j, k);
+  if (s < 0) { raise_error(err, MELT_ERR_WINDOW); return; }
+  unsigned long long bits = (unsigned long long)__double_as_longlong(dist);
+  if (mode == 0) {
+    atomicMin(&key[s], bits);
+  } else if (mode == 1) {
+    if (key[s] == bits) atomicMin(&face[s], f);
+  } else if (face[s] == f) {
+    const double* n = &M.nor[3 * f];
+    double dot = n[0] * (pos[0] - P[0]) + n[1] * (pos[1] - P[1]) + n[2] * (pos[2] - P[2]);
+    double sgn = signbit(dot) ? -1.0 : 1.0;
+    double sg = dist * (-sgn);
+    size_t c = widx(g, i, j, k);
+    sdf[c] = sg;
+    double eps = 1.0 / g.dx1, v;  // sdf_to_VOF genSDF.f90:237-255
+    if (sg <= -eps) v = 0.0;
+    else if (sg >= eps) v = 1.0;
+    else v = 0.5 * (1.0 + sg * g.dx1 + (1.0 / pi) * det_sin(pi * sg * g.dx1));
+    vofp[c] = v;
+  }
+}
+
+// ------------------------------------------------------------ ray tagging
+__device__ bool edge_sq(double y1, double z1, double y2, double z2, double ymin, double ymax, double zmin, double zmax) {
+  double t, y, z;  // rayAux.f90:139-190
+  if (y1 != y2) {
+    t = (ymin - y1) / (y2 - y1);
+    if (t >= 0.0 && t <= 1.0) { z = z1 + t * (z2 - z1); if (z >= zmin && z <= zmax) return true; }
+    t = (ymax - y1) / (y2 - y1);
+    if (t >= 0.0 && t <= 1.0) { z = z1 + t * (z2 - z1); if (z >= zmin && z <= zmax) return true; }
+  }
+  if (z1 != z2) {
+    t = (zmin - z1) / (z2 - z1);
+    if (t >= 0.0 && t <= 1.0) { y = y1 + t * (y2 - y1); if (y >= ymin && y <= ymax) return true; }
+    t = (zmax - z1) / (z2 - z1);
+    if (t >= 0.0 && t <= 1.0) { y = y1 + t * (y2 - y1); if (y >= ymin && y <= ymax) return true; }
+  }
+  return false;
+}
+
+__device__ bool yz_range(const double* V0, const double* V1, const double* V2, double ymin, double ymax, double zmin, double zmax) {
+  double yv[3] = {V0[1], V1[1], V2[1]}, zv[3] = {V0[2], V1[2], V2[2]};  // rayAux.f90:86-113
+  for (int i = 0; i < 3; ++i)
+    if (yv[i] >= ymin && yv[i] <= ymax && zv[i] >= zmin && zv[i] <= zmax) return true;
+  double ymx = fmax(fmax(yv[0], yv[1]), yv[2]), ymn = fmin(fmin(yv[0], yv[1]), yv[2]);
+  double zmx = fmax(fmax(zv[0], zv[1]), zv[2]), zmn = fmin(fmin(zv[0], zv[1]), zv[2]);
+  if (ymx < ymin || ymn > ymax || zmx < zmin || zmn > zmax) return false;
+  for (int i = 0; i < 3; ++i) {
+    int j = (i + 1) % 3;
+    if (edge_sq(yv[i], zv[i], yv[j], zv[j], ymin, ymax, zmin, zmax)) return true;
+  }
+  return false;
+}
+
+struct Box { int lo[3], hi[3]; };
+
+__global__ __launch_bounds__(256) void k_pencil_build(Grid g, MeshArgs M, Box bx, int* __restrict__ count, int* __restrict__ list, int buff,
+                                                      int* err) {
+  int js = bx.hi[1] - bx.lo[1] + 1, ks = bx.hi[2] - bx.lo[2] + 1;
+  size_t np = (size_t)js * ks;
+  size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= np * M.nf) return;
+  int f = (int)(id % M.nf);
+  size_t p = id / M.nf;
+  if (M.gf[f]) return;
+  int j = bx.lo[1] + (int)(p % js), k = bx.lo[2] + (int)(p / js);
+  double ymin = gym(g, j) - 0.5 / g.dx2, ymax = gym(g, j) + 0.5 / g.dx2;
+  double zmin = gzm(g, k) - 0.5 / g.dx3, zmax = gzm(g, k) + 0.5 / g.dx3;
+  const double* V0 = &M.xyzv[3 * M.vof[3 * f + 0]];
+  const double* V1 = &M.xyzv[3 * M.vof[3 * f + 1]];
+  const double* V2 = &M.xyzv[3 * M.vof[3 * f + 2]];
+  if (yz_range(V0, V1, V2, ymin, ymax, zmin, zmax)) {
+    int n = atomicAdd(&count[p], 1);
+    if (n < buff) list[p * buff + n] = f;
+    else raise_error(err, MELT_ERR_PENCIL_OVERFLOW);
+  }
+}
+
+// rayTriangle_intersect (rayAux.f90:9-83).  A near-parallel ray re-enters with
+// an origin shifted by (0, 0.1/dx2, -0.1/dx3); the caller then re-applies its own
+// barycentric tests on top of the callee's verdict.  Evaluated here level by
+// level from the innermost call outwards.
+__device__ bool ray_tri(const Grid& g, const double C0[3], const double Q[3], const double* V0, const double* V1, const double* V2,
+                        double ip[3], int* err) {
+  const double EPS = 1.e-18;
+  const int MAXD = 8;
+  double Cs[MAXD + 1][3];
+  int L = 0;
+  Cs[0][0] = C0[0]; Cs[0][1] = C0[1]; Cs[0][2] = C0[2];
+  double e1[3], e2[3];
+  for (int d = 0; d < 3; ++d) { e1[d] = V1[d] - V0[d]; e2[d] = V2[d] - V0[d]; }
+  // find the first level whose ray is not parallel
+  for (;;) {
+    double r[3] = {Q[0] - Cs[L][0], Q[1] - Cs[L][1], Q[2] - Cs[L][2]}, p[3];
+    dcross(p, r, e2);
+    double a = e1[0] * p[0] + e1[1] * p[1] + e1[2] * p[2];
+    if (!(a > -EPS && a <= EPS)) break;
+    if (L == MAXD) { raise_error(err, MELT_ERR_RAY_RECURSION); return false; }
+    Cs[L + 1][0] = Cs[L][0]; Cs[L + 1][1] = Cs[L][1] + 0.1 / g.dx2; Cs[L + 1][2] = Cs[L][2] - 0.1 / g.dx3;
+    ++L;
+  }
+  bool hit = true;
+  for (int lev = L; lev >= 0; --lev) {
+    const double* C = Cs[lev];
+    double r[3], s[3], p[3], q1[3];
+    for (int d = 0; d < 3; ++d) { r[d] = Q[d] - C[d]; s[d] = C[d] - V0[d]; }
+    dcross(p, r, e2);
+    double a = e1[0] * p[0] + e1[1] * p[1] + e1[2] * p[2];
+    if (lev == L) hit = true;  // intersect = .true. at entry of the innermost call
+    double f = 1.0 / a;
+    double u = f * (s[0] * p[0] + s[1] * p[1] + s[2] * p[2]);
+    if (u < EPS || u > 1.0) hit = false;
+    dcross(q1, s, e1);
+    double v = f * (r[0] * q1[0] + r[1] * q1[1] + r[2] * q1[2]);
+    if (v < EPS || (u + v) > 1.0) hit = false;
+    double t = f * (e2[0] * q1[0] + e2[1] * q1[1] + e2[2] * q1[2]);
+    if (t < 0.0 || t > 1.0) hit = false;
+    if (hit)
+      for (int d = 0; d < 3; ++d) ip[d] = C[d] + t * r[d];
+  }
+  return hit;
+}
+
+__device__ double ray_tag_cell(const Grid& g, const MeshArgs& M, double xlen, const double Q[3], const int* list, int n, int* err) {
+  double C[3] = {-xlen * 10.0, Q[1], Q[2]};  // pencil_rayTag.f90:402
+  int cnt = 0;
+  double xmin = Q[0] - 0.5 / g.dx1, xmax = Q[0] + 0.5 / g.dx1;
+  for (int t = 0; t < n; ++t) {
+    int f = list[t];
+    if (M.gf[f]) continue;
+    const double* V0 = &M.xyzv[3 * M.vof[3 * f + 0]];
+    const double* V1 = &M.xyzv[3 * M.vof[3 * f + 1]];
+    const double* V2 = &M.xyzv[3 * M.vof[3 * f + 2]];
+    double mn = fmin(fmin(V0[0], V1[0]), V2[0]), mx = fmax(fmax(V0[0], V1[0]), V2[0]);  // triangleXIntersect
+    if (!(mx < xmin || mn > xmax)) {
+      if ((V0[0] >= xmin && V0[0] <= xmax) || (V1[0] >= xmin && V1[0] <= xmax) || (V2[0] >= xmin && V2[0] <= xmax)) return 0.5;
+    }
+    double ip[3] = {0.0, 0.0, 0.0};
+    if (ray_tri(g, C, Q, V0, V1, V2, ip, err)) {
+      bool inside = ip[0] >= xmin && ip[0] <= xmax && ip[1] >= Q[1] - 0.5 / g.dx2 && ip[1] <= Q[1] + 0.5 / g.dx2 && ip[2] >= Q[2] - 0.5 / g.dx3 &&
+                    ip[2] <= Q[2] + 0.5 / g.dx3;
+      if (inside) return 0.5;
+      cnt += 1;
+    }
+  }
+  return (cnt % 2 == 0) ? 1.0 : 0.0;
+}
+
+// pencil_tagCells (pencil_rayTag.f90:189-291): every wrapped cell of the box
+// visits its unwrapped preimages in the reference's order (i, then j, then k)
+// and keeps the first value below 1.
+__global__ __launch_bounds__(256) void k_pencil_tag(Grid g, MeshArgs M, Box bx, const int* __restrict__ count, const int* __restrict__ list,
+                                                    int buff, double* __restrict__ vofp, double xlen, int* err) {
+  int ext[3], cov[3];
+  int nn[3] = {g.n1, g.n2, g.n3};
+  for (int d = 0; d < 3; ++d) { ext[d] = bx.hi[d] - bx.lo[d] + 1; cov[d] = min(ext[d], nn[d]); }
+  size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (size_t)cov[0] * cov[1] * cov[2]) return;
+  int ta = (int)(id % cov[0]), tb = (int)((id / cov[0]) % cov[1]), tc = (int)(id / ((size_t)cov[0] * cov[1]));
+  int js = ext[1];
+  double cur = 1.0;
+  for (int i = bx.lo[0] + ta; i <= bx.hi[0]; i += g.n1)
+    for (int j = bx.lo[1] + tb; j <= bx.hi[1]; j += g.n2)
+      for (int kk = bx.lo[2] + tc; kk <= bx.hi[2]; kk += g.n3) {
+        double Q[3] = {gxm(g, i), gym(g, j), gzm(g, kk)};
+        size_t p = (size_t)(j - bx.lo[1]) + (size_t)js * (kk - bx.lo[2]);
+        int n = min(count[p], buff);
+        double v = ray_tag_cell(g, M, xlen, Q, &list[p * buff], n, err);
+        if (!(cur < 1.0)) cur = v;
+      }
+  vofp[widx(g, bx.lo[0] + ta, bx.lo[1] + tb, bx.lo[2] + tc)] = cur;
+}
+
+__global__ __launch_bounds__(256) void k_vbox(const double* __restrict__ xyzv, const unsigned char* __restrict__ gv, int nv, double* out) {
+  __shared__ double mn[3][256], mx[3][256];
+  double a[3] = {INFINITY, INFINITY, INFINITY}, b[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int v = threadIdx.x; v < nv; v += blockDim.x) {
+    if (gv[v]) continue;
+    for (int d = 0; d < 3; ++d) { a[d] = fmin(a[d], xyzv[3 * v + d]); b[d] = fmax(b[d], xyzv[3 * v + d]); }
+  }
+  for (int d = 0; d < 3; ++d) { mn[d][threadIdx.x] = a[d]; mx[d][threadIdx.x] = b[d]; }
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int d = 0; d < 3; ++d) {
+        mn[d][threadIdx.x] = fmin(mn[d][threadIdx.x], mn[d][threadIdx.x + w]);
+        mx[d][threadIdx.x] = fmax(mx[d][threadIdx.x], mx[d][threadIdx.x + w]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int d = 0; d < 3; ++d) { out[d] = mn[d][0]; out[3 + d] = mx[d][0]; }
+}
+
+__global__ __launch_bounds__(256) void k_fill(double* __restrict__ p, double v, size_t n) {
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) p[c] = v;
+}
+
+// ------------------------------------------------------------ MLS forcing
+__global__ __launch_bounds__(256) void k_centroid_idx(Grid g, const double* __restrict__ bar, const unsigned char* __restrict__ gf, int nf,
+                                                      int* __restrict__ pind) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;  // findCentroidIndices.f90:16-48
+  if (f >= nf || gf[f]) return;
+  const double* p = &bar[3 * f];
+  int* o = &pind[6 * f];
+  o[0] = (int)floor(p[0] * g.dx1) + 1; o[3] = fnint(p[0] * g.dx1) + 1;
+  o[1] = (int)floor(p[1] * g.dx2) + 1; o[4] = fnint(p[1] * g.dx2) + 1;
+  int k1 = (int)floor(p[2] * g.dx3) + 1, kst = fnint(p[2] * g.dx3) + 1;
+  o[2] = pmod(k1 - 1, g.n3) + 1; o[5] = pmod(kst - 1, g.n3) + 1;
+}
+
+struct PhiOut { double* q[4]; };  // ptxAB_q1, ptxAB_q2, ptxAB_temp, ptxAB_q3
+
+// component c: 0 = wght1 (u), 1 = wght2 (v), 2 = wghttemp (T), 3 = wght3 (w)
+__global__ __launch_bounds__(64) void k_mls_weights(Grid g, const double* __restrict__ bar, const unsigned char* __restrict__ gf, int nf,
+                                                    const int* __restrict__ pind, PhiOut out, int* err) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nf * 4) return;
+  int f = id >> 2, comp = id & 3;
+  if (gf[f]) return;
+  const double* pos = &bar[3 * f];
+  const int* p = &pind[6 * f];
+  int k1 = (int)floor(pos[2] * g.dx3) + 1, kst = fnint(pos[2] * g.dx3) + 1;
+  int i0, j0, k0, c[3];
+  if (comp == 0) { i0 = p[3] - 1; j0 = p[1] - 1; k0 = k1 - 1; c[0] = 0; c[1] = 1; c[2] = 1; }
+  else if (comp == 1) { i0 = p[0] - 1; j0 = p[4] - 1; k0 = k1 - 1; c[0] = 1; c[1] = 0; c[2] = 1; }
+  else if (comp == 2) { i0 = p[0] - 1; j0 = p[1] - 1; k0 = k1 - 1; c[0] = 1; c[1] = 1; c[2] = 1; }
+  else { i0 = p[0] - 1; j0 = p[1] - 1; k0 = kst - 1; c[0] = 1; c[1] = 1; c[2] = 0; }
+  double phi[27];
+  if (mls_phi(g, pos, i0, j0, k0, c, phi)) { raise_error(err, MELT_ERR_SINGULAR_MLS); return; }
+  double* o = &out.q[comp][27 * (size_t)f];
+  for (int q = 0; q < 27; ++q) o[q] = phi[q];
+}
+
+struct ForceArgs {
+  const double* fld[4];   // vx, vy, temp, vz
+  const double* phi[4];
+  const double* vel_tri;
+  const double* cfac;
+  double tmelt;
+};
+
+// forc1/forc2/forctemp/forc3: Um = sum q phi; spread cfac (V - Um) phi
+__global__ __launch_bounds__(64) void k_mls_force(Grid g, ForceArgs F, const double* __restrict__ bar, const unsigned char* __restrict__ gf,
+                                                  int nf, const int* __restrict__ pind, const melt_window* __restrict__ win, Win3 ef,
+                                                  unsigned long long* __restrict__ acc, size_t cells, int* err) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nf * 4) return;
+  int f = id >> 2, comp = id & 3;
+  if (gf[f]) return;
+  const int* p = &pind[6 * f];
+  const double* pos = &bar[3 * f];
+  int i0, j0, kw, ku;
+  int k1u = (int)floor(pos[2] * g.dx3) + 1, kstu = fnint(pos[2] * g.dx3) + 1;
+  if (comp == 0) { i0 = p[3]; j0 = p[1]; kw = p[2]; ku = k1u; }
+  else if (comp == 1) { i0 = p[0]; j0 = p[4]; kw = p[2]; ku = k1u; }
+  else if (comp == 2) { i0 = p[0]; j0 = p[1]; kw = p[2]; ku = k1u; }
+  else { i0 = p[0]; j0 = p[1]; kw = p[5]; ku = kstu; }
+  const double* phi = &F.phi[comp][27 * (size_t)f];
+  const double* q = F.fld[comp];
+  double Um = 0.0;
+  int inw = 0;
+  for (int k = kw - 1; k <= kw + 1; ++k)
+    for (int j = j0 - 1; j <= j0 + 1; ++j)
+      for (int i = i0 - 1; i <= i0 + 1; ++i) Um = Um + q[widx(g, i, j, k)] * phi[inw++];
+  double V = comp == 2 ? F.tmelt : F.vel_tri[3 * f + (comp == 0 ? 0 : comp == 1 ? 1 : 2)];
+  double cf = F.cfac[f];
+  unsigned long long* a = acc + (size_t)comp * cells * 2;
+  inw = 0;
+  for (int dk = -1; dk <= 1; ++dk)
+    for (int j = j0 - 1; j <= j0 + 1; ++j)
+      for (int i = i0 - 1; i <= i0 + 1; ++i) {
+        double v = cf * (V - Um) * phi[inw++];
+        long long s = win_slot(win[0].org, ef.e, i, j, ku + dk);
+        if (s < 0) { raise_error(err, MELT_ERR_WINDOW); return; }
+        acc_add(&a[2 * s], v);
+      }
+}
+
+// Drain the accumulators: mode 0 adds the forces to the fields (velocity
+// without its mean, which the projection subtracts later) and sums the
+// velocity forces exactly per block; mode 1 stores them into the for_* fields.
+__global__ __launch_bounds__(256) void k_force_apply(Grid g, const melt_window* __restrict__ win, Win3 ef, unsigned long long* __restrict__ acc,
+                                                     size_t cells, double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ t,
+                                                     double* __restrict__ vz, int mode, unsigned long long* __restrict__ part) {
+  __shared__ unsigned long long slo[3][256], shi[3][256];
+  unsigned long long mlo[3] = {0, 0, 0}, mhi[3] = {0, 0, 0};
+  double* dst[4] = {vx, vy, t, vz};
+  for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < cells; s += (size_t)gridDim.x * blockDim.x) {
+    int a = (int)(s % ef.e[0]), b = (int)((s / ef.e[0]) % ef.e[1]), c = (int)(s / ((size_t)ef.e[0] * ef.e[1]));
+    size_t cell = widx(g, win[0].org[0] + a, win[0].org[1] + b, win[0].org[2] + c);
+    for (int comp = 0; comp < 4; ++comp) {
+      unsigned long long* w = acc + ((size_t)comp * cells + s) * 2;
+      unsigned long long lo = w[0], hi = w[1];
+      if ((lo | hi) == 0) continue;
+      w[0] = 0; w[1] = 0;
+      double v = acc_value(lo, hi);
+      if (mode == 0) dst[comp][cell] = dst[comp][cell] + v;
+      else dst[comp][cell] = v;
+      if (comp != 2) {
+        int m = comp == 3 ? 2 : comp;
+        unsigned long long nlo = mlo[m] + lo;
+        mhi[m] += hi + (nlo < mlo[m] ? 1ull : 0ull);
+        mlo[m] = nlo;
+      }
+    }
+  }
+  if (!part) return;
+  for (int m = 0; m < 3; ++m) { slo[m][threadIdx.x] = mlo[m]; shi[m][threadIdx.x] = mhi[m]; }
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int m = 0; m < 3; ++m) {
+        unsigned long long l1 = slo[m][threadIdx.x], l2 = slo[m][threadIdx.x + w];
+        unsigned long long nl = l1 + l2;
+        shi[m][threadIdx.x] = shi[m][threadIdx.x] + shi[m][threadIdx.x + w] + (nl < l1 ? 1ull : 0ull);
+        slo[m][threadIdx.x] = nl;
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int m = 0; m < 3; ++m) { part[(blockIdx.x * 3 + m) * 2] = slo[m][0]; part[(blockIdx.x * 3 + m) * 2 + 1] = shi[m][0]; }
+}
+
+__global__ void k_fmean_final(const unsigned long long* __restrict__ part, int nb, double ncell, double* __restrict__ mean) {
+  int m = threadIdx.x;
+  if (m >= 3) return;
+  unsigned long long lo = 0, hi = 0;
+  for (int b = 0; b < nb; ++b) {
+    unsigned long long l2 = part[(b * 3 + m) * 2], h2 = part[(b * 3 + m) * 2 + 1];
+    unsigned long long nl = lo + l2;
+    hi = hi + h2 + (nl < lo ? 1ull : 0ull);
+    lo = nl;
+  }
+  mean[m] = acc_value(lo, hi) / ncell;
+}
+
+// velforce.f90:21-51 on full fields (per-routine entry point)
+__global__ __launch_bounds__(256) void k_sum3(const double* __restrict__ a, const double* __restrict__ b, const double* __restrict__ c,
+                                              size_t n, double* __restrict__ part) {
+  __shared__ double s[3][256];
+  double x = 0.0, y = 0.0, z = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    x = x + a[i]; y = y + b[i]; z = z + c[i];
+  }
+  s[0][threadIdx.x] = x; s[1][threadIdx.x] = y; s[2][threadIdx.x] = z;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int m = 0; m < 3; ++m) s[m][threadIdx.x] = s[m][threadIdx.x] + s[m][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int m = 0; m < 3; ++m) part[blockIdx.x * 3 + m] = s[m][0];
+}
+__global__ void k_sum3_final(const double* __restrict__ part, int nb, double ncell, double* __restrict__ mean) {
+  int m = threadIdx.x;
+  if (m >= 3) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s = s + part[b * 3 + m];
+  mean[m] = s / ncell;
+}
+__global__ __launch_bounds__(256) void k_add_force(double* __restrict__ q, const double* __restrict__ f, const double* __restrict__ mean, int m,
+                                                   size_t n) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  q[c] = mean ? q[c] + f[c] - mean[m] : q[c] + f[c];
+}
+
+__global__ __launch_bounds__(256) void k_sub_mean(double* __restrict__ q, const double* __restrict__ mean, int m, size_t n) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  q[c] = q[c] - mean[m];
+}
+
+// ------------------------------------------------------------ probes
+// kind 0/1: +/- temperature probe (findProbeIndices.f90 + wght_gradT);
+// kind 2: pressure (wght_press); 3..6: grad p, grad u, grad v, grad w.
+struct ProbeArgs {
+  const double* temp;
+  const double* pr;
+  const double* vx;
+  const double* vy;
+  const double* vz;
+  const double* bar;
+  const double* nor;
+  const unsigned char* gf;
+  int nf;
+  double h, pec;
+  int kind_lo, kind_hi;
+};
+
+__global__ __launch_bounds__(64) void k_probes(Grid g, ProbeArgs P, double* __restrict__ out, int* __restrict__ pind, int* err) {
+  int nk = P.kind_hi - P.kind_lo;
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= P.nf * nk) return;
+  int f = id / nk, kind = P.kind_lo + id % nk;
+  if (P.gf[f]) return;
+  const double* nor = &P.nor[3 * f];
+  const double* bar = &P.bar[3 * f];
+  double s = fnorm2(nor, 3), pos[3];
+  double sg = kind == 1 ? -1.0 : 1.0;
+  for (int d = 0; d < 3; ++d) pos[d] = sg > 0 ? bar[d] + P.h * nor[d] / s : bar[d] - P.h * nor[d] / s;
+  int icent = (int)floor(pos[0] * g.dx1) + 1, jcent = (int)floor(pos[1] * g.dx2) + 1;
+  int istag = fnint(pos[0] * g.dx1) + 1, jstag = fnint(pos[1] * g.dx2) + 1;
+  int k1 = (int)floor(pos[2] * g.dx3) + 1, kst = fnint(pos[2] * g.dx3) + 1;
+  int kw = pmod(k1 - 1, g.n3) + 1, kws = pmod(kst - 1, g.n3) + 1;
+  int kz[3] = {k1 - 1, k1, k1}, ke[3] = {kw - 1, kw, kw + 1};
+  double* o = &out[(size_t)PRB_N * f];
+  double gr[3];
+  int info = 0;
+  if (kind <= 1) {
+    const int cc[3] = {1, 1, 1};
+    info = mls_grad(g, P.temp, pos, icent, jcent, kz, ke, cc, gr);
+    double q = nor[0] * gr[0] + nor[1] * gr[1] + nor[2] * gr[2];
+    o[kind == 0 ? PRB_QO : PRB_QI] = q / P.pec;
+    int* pi = &pind[6 * f + 3 * kind];
+    pi[0] = icent; pi[1] = jcent; pi[2] = kw;
+  } else if (kind == 2) {
+    const int cc[3] = {1, 1, 1};
+    double phi[27];
+    info = mls_phi(g, pos, icent - 1, jcent - 1, k1 - 1, cc, phi);
+    double pp = 0.0;
+    int inw = 0;
+    for (int kk = 0; kk < 3; ++kk)
+      for (int j = jcent - 1; j <= jcent + 1; ++j)
+        for (int i = icent - 1; i <= icent + 1; ++i) pp = pp + P.pr[widx(g, i, j, kw - 1 + kk)] * phi[inw++];
+    o[PRB_P] = pp;
+  } else if (kind == 3) {
+    const int cc[3] = {1, 1, 1};
+    info = mls_grad(g, P.pr, pos, icent, jcent, kz, ke, cc, gr);
+    for (int d = 0; d < 3; ++d) o[PRB_GP + d] = gr[d];
+  } else if (kind == 4) {
+    const int cc[3] = {0, 1, 1};
+    info = mls_grad(g, P.vx, pos, istag, jcent, kz, ke, cc, gr);
+    for (int d = 0; d < 3; ++d) o[PRB_GU + d] = gr[d];
+  } else if (kind == 5) {
+    const int cc[3] = {1, 0, 1};
+    info = mls_grad(g, P.vy, pos, icent, jstag, kz, ke, cc, gr);
+    for (int d = 0; d < 3; ++d) o[PRB_GV + d] = gr[d];
+  } else {
+    const int cc[3] = {1, 1, 0};
+    int kzs[3] = {kst - 1, kst, kst}, kes[3] = {kws - 1, kws, kws + 1};
+    info = mls_grad(g, P.vz, pos, icent, jcent, kzs, kes, cc, gr);
+    for (int d = 0; d < 3; ++d) o[PRB_GW + d] = gr[d];
+  }
+  if (info) raise_error(err, MELT_ERR_SINGULAR_MLS);
+}
+
+inline unsigned nb(size_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+MeshArgs mesh_args(melt_ctx* ctx) {
+  MeshArgs M;
+  M.xyzv = ctx->d_lag[MELT_L_XYZV]; M.bar = ctx->d_lag[MELT_L_TRI_BAR]; M.nor = ctx->d_lag[MELT_L_TRI_NOR];
+  M.vof = ctx->d_vof; M.gf = ctx->d_gf; M.nf = ctx->nf;
+  return M;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------- host
+static int ray_tag_init(melt_ctx* ctx) {
+  Grid g = make_grid(ctx);
+  hipLaunchKernelGGL(k_vbox, dim3(1), dim3(256), 0, ctx->stream, ctx->d_lag[MELT_L_XYZV], ctx->d_gv, ctx->nv, ctx->d_red);
+  MELT_HIP(hipGetLastError());
+  double lim[6];
+  MELT_HIP(hipMemcpyAsync(lim, ctx->d_red, sizeof lim, hipMemcpyDeviceToHost, ctx->stream));
+  MELT_HIP(hipStreamSynchronize(ctx->stream));
+  Box bx;  // get_bbox_inds tagCells.f90:31-62 (the reference scales all three axes by dx1)
+  for (int d = 0; d < 3; ++d) {
+    bx.lo[d] = (int)floor(lim[d] * ctx->dx1) + 1 - 2;
+    bx.hi[d] = (int)floor(lim[3 + d] * ctx->dx1) + 1 + 2;
+  }
+  for (int id = MELT_VOFP; id <= MELT_VOFZ; ++id)
+    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, ctx->stream, ctx->f[id], 1.0, ctx->nc);
+  MELT_HIP(hipGetLastError());
+  int js = bx.hi[1] - bx.lo[1] + 1, ks = bx.hi[2] - bx.lo[2] + 1;
+  int buff = ctx->n1;  // BUFF_SIZE = n1m (pencil_rayTag.f90:84)
+  size_t np = (size_t)js * ks;
+  if (np * buff > ctx->plist_cap) {
+    if (ctx->d_plist) hipFree(ctx->d_plist);
+    if (ctx->d_pcount) hipFree(ctx->d_pcount);
+    MELT_HIP(hipMalloc(&ctx->d_plist, sizeof(int) * np * buff));
+    MELT_HIP(hipMalloc(&ctx->d_pcount, sizeof(int) * np));
+    ctx->plist_cap = np * buff;
+  }
+  MELT_HIP(hipMemsetAsync(ctx->d_pcount, 0, sizeof(int) * np, ctx->stream));
+  MeshArgs M = mesh_args(ctx);
+  hipLaunchKernelGGL(k_pencil_build, dim3(nb(np * ctx->nf, 256)), dim3(256), 0, ctx->stream, g, M, bx, ctx->d_pcount, ctx->d_plist, buff,
+                     ctx->d_err);
+  MELT_HIP(hipGetLastError());
+  size_t cov = 1;
+  int nn[3] = {ctx->n1, ctx->n2, ctx->n3};
+  for (int d = 0; d < 3; ++d) cov *= (size_t)std::min(bx.hi[d] - bx.lo[d] + 1, nn[d]);
+  hipLaunchKernelGGL(k_pencil_tag, dim3(nb(cov, 256)), dim3(256), 0, ctx->stream, g, M, bx, ctx->d_pcount, ctx->d_plist, buff,
+                     ctx->f[MELT_VOFP], ctx->cfg.xlen, ctx->d_err);
+  MELT_HIP(hipGetLastError());
+  return 0;
+}
+
+int ibm_tag(melt_ctx* ctx) {  // tagCells.f90:4-27 (tagType 1) + interp_vof when not fused
+  if (!ctx->has_geom) return melt_fail(ctx, MELT_ERR_NO_GEOMETRY, "tagCells", "no geometry");
+  Grid g = make_grid(ctx);
+  if (ctx->h_body.initial_tag) {
+    int rc = ray_tag_init(ctx);
+    if (rc) return rc;
+    ctx->h_body.initial_tag = 0;
+    MELT_HIP(hipMemcpyAsync(&ctx->d_body->initial_tag, &ctx->h_body.initial_tag, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  }
+  Win3 ef{{ctx->fwin[0], ctx->fwin[1], ctx->fwin[2]}}, et{{ctx->twin[0], ctx->twin[1], ctx->twin[2]}};
+  hipLaunchKernelGGL(k_window, dim3(1), dim3(256), 0, ctx->stream, g, ctx->d_lag[MELT_L_TRI_BAR], ctx->d_gf, ctx->nf, ctx->d_win, ef, et, 2,
+                     ctx->d_err);
+  hipLaunchKernelGGL(k_tag_reset, dim3(256), dim3(256), 0, ctx->stream, g, ctx->f[MELT_SDF], ctx->d_win, et);
+  hipLaunchKernelGGL(k_tag_init, dim3(256), dim3(256), 0, ctx->stream, ctx->d_tkey, ctx->d_tface, ctx->twin_cells);
+  MeshArgs M = mesh_args(ctx);
+  unsigned blocks = nb((size_t)ctx->nf * 125, 256);
+  for (int mode = 0; mode < 3; ++mode)
+    hipLaunchKernelGGL(k_tag_pass, dim3(blocks), dim3(256), 0, ctx->stream, g, M, ctx->d_win, et, ctx->d_tkey, ctx->d_tface, ctx->f[MELT_SDF],
+                       ctx->f[MELT_VOFP], mode, ctx->pi, ctx->d_err);
+  MELT_HIP(hipGetLastError());
+  return 0;
+}
+
+int ibm_find_centroid_indices(melt_ctx* ctx) {
+  hipLaunchKernelGGL(k_centroid_idx, dim3(nb(ctx->nf, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), ctx->d_lag[MELT_L_TRI_BAR], ctx->d_gf,
+                     ctx->nf, ctx->d_pind);
+  MELT_HIP(hipGetLastError());
+  return 0;
+}
+
+int ibm_weights(melt_ctx* ctx) {
+  PhiOut o{{ctx->d_lag[MELT_L_PTXAB_Q1], ctx->d_lag[MELT_L_PTXAB_Q2], ctx->d_lag[MELT_L_PTXAB_TEMP], ctx->d_lag[MELT_L_PTXAB_Q3]}};
+  hipLaunchKernelGGL(k_mls_weights, dim3(nb((size_t)ctx->nf * 4, 64)), dim3(64), 0, ctx->stream, make_grid(ctx), ctx->d_lag[MELT_L_TRI_BAR],
+                     ctx->d_gf, ctx->nf, ctx->d_pind, o, ctx->d_err);
+  MELT_HIP(hipGetLastError());
+  return 0;
+}
+
+// Interpolation + spreading into the exact accumulators (mlsForce.f90, forc*).
+int ibm_force_scatter(melt_ctx* ctx) {
+  Grid g = make_grid(ctx);
+  Win3 ef{{ctx->fwin[0], ctx->fwin[1], ctx->fwin[2]}}, et{{ctx->twin[0], ctx->twin[1], ctx->twin[2]}};
+  hipLaunchKernelGGL(k_window, dim3(1), dim3(256), 0, ctx->stream, g, ctx->d_lag[MELT_L_TRI_BAR], ctx->d_gf, ctx->nf, ctx->d_win, ef, et, 1,
+                     ctx->d_err);
+  ForceArgs F;
+  F.fld[0] = ctx->f[MELT_VX]; F.fld[1] = ctx->f[MELT_VY]; F.fld[2] = ctx->f[MELT_TEMP]; F.fld[3] = ctx->f[MELT_VZ];
+  F.phi[0] = ctx->d_lag[MELT_L_PTXAB_Q1]; F.phi[1] = ctx->d_lag[MELT_L_PTXAB_Q2];
+  F.phi[2] = ctx->d_lag[MELT_L_PTXAB_TEMP]; F.phi[3] = ctx->d_lag[MELT_L_PTXAB_Q3];
+  F.vel_tri = ctx->d_lag[MELT_L_VEL_TRI]; F.cfac = ctx->d_lag[MELT_L_CFAC]; F.tmelt = ctx->cfg.tmelt;
+  hipLaunchKernelGGL(k_mls_force, dim3(nb((size_t)ctx->nf * 4, 64)), dim3(64), 0, ctx->stream, g, F, ctx->d_lag[MELT_L_TRI_BAR], ctx->d_gf,
+                     ctx->nf, ctx->d_pind, ctx->d_win, ef, ctx->d_acc, ctx->fwin_cells, ctx->d_err);
+  MELT_HIP(hipGetLastError());
+  return 0;
+}
+
+// Drain the accumulators.  apply_now = 1 (fused substep): forces go straight
+// into u, v, w, T and the exact mean of the velocity forces lands in d_fmean
+// for the projection (velforce.f90:21-51 with the uniform shift deferred);
+// 0: the folded for_* fields are written (per-routine API, mlsForce output).
+int ibm_force_drain(melt_ctx* ctx, int apply_now) {
+  Grid g = make_grid(ctx);
+  Win3 ef{{ctx->fwin[0], ctx->fwin[1], ctx->fwin[2]}};
+  unsigned blocks = (unsigned)std::min<size_t>(512, nb(ctx->fwin_cells, 256));
+  unsigned long long* part = reinterpret_cast<unsigned long long*>(ctx->d_red + 64);
+  if (apply_now) {
+    hipLaunchKernelGGL(k_force_apply, dim3(blocks), dim3(256), 0, ctx->stream, g, ctx->d_win, ef, ctx->d_acc, ctx->fwin_cells, ctx->f[MELT_VX],
+                       ctx->f[MELT_VY], ctx->f[MELT_TEMP], ctx->f[MELT_VZ], 0, part);
+    hipLaunchKernelGGL(k_fmean_final, dim3(1), dim3(64), 0, ctx->stream, part, (int)blocks, (double)((long long)ctx->n1 * ctx->n2 * ctx->n3),
+                       ctx->d_fmean);
+  } else {
+    for (int id = MELT_FOR_XC; id <= MELT_FOR_TEMP; ++id) MELT_HIP(hipMemsetAsync(ctx->f[id], 0, sizeof(double) * ctx->nc, ctx->stream));
+    hipLaunchKernelGGL(k_force_apply, dim3(blocks), dim3(256), 0, ctx->stream, g, ctx->d_win, ef, ctx->d_acc, ctx->fwin_cells,
+                       ctx->f[MELT_FOR_XC], ctx->f[MELT_FOR_YC], ctx->f[MELT_FOR_TEMP], ctx->f[MELT_FOR_ZC], 1, nullptr);
+  }
+  MELT_HIP(hipGetLastError());
+  return 0;
+}
+
+// u -= fbar over the whole field (the shift the fused substep defers to the
+// projection), for callers that stop after particle.
+int ibm_sub_mean(melt_ctx* ctx) {
+  const int q[3] = {MELT_VX, MELT_VY, MELT_VZ};
+  for (int m = 0; m < 3; ++m)
+    hipLaunchKernelGGL(k_sub_mean, dim3(nb(ctx->nc, 256)), dim3(256), 0, ctx->stream, ctx->f[q[m]], ctx->d_fmean, m, ctx->nc);
+  MELT_HIP(hipGetLastError());
+  return 0;
+}
+
+int ibm_velforce(melt_ctx* ctx, int /*defer*/) {  // velforce.f90 on full for_* fields
+  unsigned blocks = (unsigned)std::min<size_t>(1024, nb(ctx->nc, 256));
+  double* part = ctx->d_red + 64;
+  hipLaunchKernelGGL(k_sum3, dim3(blocks), dim3(256), 0, ctx->stream, ctx->f[MELT_FOR_XC], ctx->f[MELT_FOR_YC], ctx->f[MELT_FOR_ZC], ctx->nc,
+                     part);
+  hipLaunchKernelGGL(k_sum3_final, dim3(1), dim3(64), 0, ctx->stream, part, (int)blocks, (double)((long long)ctx->n1 * ctx->n2 * ctx->n3),
+                     ctx->d_fmean);
+  const int q[3] = {MELT_VX, MELT_VY, MELT_VZ}, fo[3] = {MELT_FOR_XC, MELT_FOR_YC, MELT_FOR_ZC};
+  for (int m = 0; m < 3; ++m)
+    hipLaunchKernelGGL(k_add_force, dim3(nb(ctx->nc, 256)), dim3(256), 0, ctx->stream, ctx->f[q[m]], ctx->f[fo[m]], ctx->d_fmean, m, ctx->nc);
+  MELT_HIP(hipGetLastError());
+  return 0;
+}
+
+int ibm_tempforce(melt_ctx* ctx) {  // tempforce.f90
+  hipLaunchKernelGGL(k_add_force, dim3(nb(ctx->nc, 256)), dim3(256), 0, ctx->stream, ctx->f[MELT_TEMP], ctx->f[MELT_FOR_TEMP], nullptr, 0,
+                     ctx->nc);
+  MELT_HIP(hipGetLastError());
+  return 0;
+}
+
+int ibm_probes(melt_ctx* ctx) {
+  ProbeArgs P;
+  P.temp = ctx->f[MELT_TEMP]; P.pr = ctx->f[MELT_PR]; P.vx = ctx->f[MELT_VX]; P.vy = ctx->f[MELT_VY]; P.vz = ctx->f[MELT_VZ];
+  P.bar = ctx->d_lag[MELT_L_TRI_BAR]; P.nor = ctx->d_lag[MELT_L_TRI_NOR]; P.gf = ctx->d_gf; P.nf = ctx->nf;
+  P.h = ctx->h_body.h_eulerian; P.pec = ctx->cfg.ren * ctx->cfg.prandtl;
+  P.kind_lo = ctx->cfg.imelt == 1 ? 0 : 2;
+  P.kind_hi = ctx->cfg.imlsstr == 1 ? 7 : 2;
+  if (P.kind_hi <= P.kind_lo) return 0;
+  size_t n = (size_t)ctx->nf * (P.kind_hi - P.kind_lo);
+  hipLaunchKernelGGL(k_probes, dim3(nb(n, 64)), dim3(64), 0, ctx->stream, make_grid(ctx), P, ctx->d_probe, ctx->d_pind, ctx->d_err);
+  MELT_HIP(hipGetLastError());
+  return 0;
+}
