@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""RK3 substep throughput of libmeltgpu.so on one MI355X.
+
+Workload (default): 256^3 periodic cube of side 2*pi with low-wavenumber
+(|k| <= 2.5) HIT forcing fields, a melting sphere of radius 0.5 meshed as a
+level-5 icosphere (10242 vertices, 20480 faces), Stefan number 2 and
+Newton-Euler coupling on, FP64.  A "step" is one RK3 time step (three
+substeps of tsch.f90) of the whole field and body; inputs are resident in HBM
+before the timed region.  Inputs are synthetic and seeded (no datasets exist
+for this path).
+
+Prints one JSON line: value = Mcell-steps/s (whole job), plus
+ - roofline: algorithmic HBM bytes of the dominant kernel over its average
+   launch time (HIP events on the library's stream, over the timed region),
+   against the 8 TB/s HBM3E peak, and the same for the whole substep;
+ - cpu_baseline: the FP64 C restatement of the reference path on this host's
+   cores for one substep of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E bytes/s
+
+# Algorithmic HBM bytes per cell of each fused-path kernel (FP64, 8 B per value;
+# a half-spectrum complex field counts 8 B per cell).  Minimal traffic of the
+# pass: each field it must read or write, once.
+KERNEL_BYTES = {
+    "explicit_rhs": (13 + 8) * 8,      # u,v,w,T,p, ru x4, VOFp, forc x3 -> rhs x4, ru x4
+    "solve_x": (4 + 4) * 8,
+    "solve_y": (4 + 4) * 8,
+    "solve_z_update": (4 + 4 + 4) * 8,  # rhs x4, q x4 -> q x4
+    "divg_fft_x": (3 + 1) * 8,
+    "fft_y": 2 * 8,
+    "fft_z_solve": 2 * 8,
+    "ifft_y": 2 * 8,
+    "ifft_x": 2 * 8,
+    "project": (5 + 4) * 8,            # dph, u,v,w,p -> u,v,w,p
+}
+B_ALG_CELL = 480  # algorithmic bytes per cell per substep of the whole path (FP64)
+
+
+def workload(n: int):
+    from melt_ibm_amd.api import MeltConfig
+    st = 2.0
+    cfg = MeltConfig(n1m=n, n2m=n, n3m=n, ren=1000.0, prandtl=1.0, betagz=0.0,
+                     tmelt=0.0, tliq=1.0, tsol=0.0, cpliquid=1.0, latheat=1.0 / st,
+                     dens_ratio=1.0, imlsfor=1, imlsstr=1, imelt=1, iremesh=0, forcing=1)
+    return cfg, 5, 0.5
+
+
+def run_gpu(args):
+    from melt_ibm_amd.api import MeltContext
+    cfg, level, rad = workload(args.n)
+    ctx = MeltContext(cfg)
+    from melt_ibm_amd import geometry, inputs
+    # temperature from inqpr, then the seeded solenoidal velocity and forcing
+    m = geometry.sphere_level(level)
+    ctx.set_geometry(m.xyz, m.vert_of_edge, m.edge_of_face, rad)
+    ctx.inqpr()
+    u, v, w = inputs.solenoidal_velocity(cfg, seed=0, kmax=4.0, rms=0.1)
+    fx, fy, fz = inputs.low_mode_forcing(cfg, seed=1, kf=2.5, amplitude=0.5)
+    for name, a in (("vx", u), ("vy", v), ("vz", w), ("forcx", fx), ("forcy", fy), ("forcz", fz)):
+        ctx.upload(name, a)
+    dt = inputs.cfl_dt(cfg, u, v, w, cfl=0.5)
+    del u, v, w, fx, fy, fz
+    ctx.set_dt(dt)
+    ctx.tagCells()
+    ctx.sync()
+    print(f"# setup done: N={args.n} dt={dt:.4e} faces={m.counts[2]}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        ctx.tschem()
+    ctx.sync()
+    ctx.timing(True)
+    ctx.timing_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.tschem()
+    ctx.sync()
+    t1 = time.perf_counter()
+    tm = ctx.timing_read()
+    ctx.timing(False)
+    qmax = ctx.divgck()
+    body = ctx.body()
+    ctx.close()
+    return cfg, (t1 - t0), tm, qmax, body
+
+
+def cpu_baseline(cfg, level, rad):
+    """One substep of the same workload through the oracle (OpenMP threads)."""
+    import oracle
+    from melt_ibm_amd import geometry, inputs
+    lib = oracle.load()
+    o = oracle.context(cfg)
+    m = geometry.sphere_level(level)
+    o.set_geometry(m.xyz, m.vert_of_edge, m.edge_of_face, rad)
+    o.inqpr()
+    u, v, w = inputs.solenoidal_velocity(cfg, seed=0, kmax=4.0, rms=0.1)
+    fx, fy, fz = inputs.low_mode_forcing(cfg, seed=1, kf=2.5, amplitude=0.5)
+    for name, a in (("vx", u), ("vy", v), ("vz", w), ("forcx", fx), ("forcy", fy), ("forcz", fz)):
+        o.upload(name, a)
+    o.set_dt(inputs.cfl_dt(cfg, u, v, w, cfl=0.5))
+    del u, v, w, fx, fy, fz
+    o.tagCells()
+    t0 = time.perf_counter()
+    o.substep(1)
+    t1 = time.perf_counter()
+    cores = int(lib.orc_threads())
+    o.close()
+    cells = cfg.n1m * cfg.n2m * cfg.n3m
+    sub_s = t1 - t0
+    return {"value": cells / (3.0 * sub_s) / 1e6, "unit": "Mcell-steps/s", "cores": cores, "kind": "port",
+            "sample": f"1 RK3 substep of the {cfg.n1m}^3 workload (incl. IBM) on {cores} OpenMP threads: {sub_s:.2f} s; "
+                      f"value = cells / (3 x substep time)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256, help="cells per side")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this path runs one domain per GPU on a single-GPU box; "
+                                   "--gpus > 1 is not supported (multi-GPU sharding is a design note only)"}))
+        sys.exit(2)
+    cfg, elapsed, tm, qmax, body = run_gpu(args)
+    cells = cfg.n1m * cfg.n2m * cfg.n3m
+    ms_step = elapsed / args.steps * 1e3
+    value = cells * args.steps / elapsed / 1e6
+    substeps = 3 * args.steps
+    # dominant kernel by total time among the fused-path kernels
+    dom = max((k for k in KERNEL_BYTES if tm.get(k, (0, 0))[1] > 0), key=lambda k: tm[k][0])
+    avg_s = tm[dom][0] / tm[dom][1] * 1e-3
+    bytes_launch = KERNEL_BYTES[dom] * cells
+    achieved = bytes_launch / avg_s
+    sub_time = elapsed / substeps
+    out = {
+        "metric": "RK3 Mcell-steps/s @256^3 FP64 vs mixed (HBM roofline %); ensemble p99 step ms",
+        "value": round(value, 3),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded solenoidal velocity |k|<=4 rms 0.1, HIT-like forcing |k|<=2.5, inqpr temperature, level-5 icosphere)",
+        "config": {"workload": f"{cfg.n1m}^3 periodic 2pi cube, HIT forcing, melting sphere R=0.5 (level-5 icosphere, 20480 faces), "
+                               f"St=2, Newton-Euler on, Re=1000, FP64",
+                   "grid": [cfg.n1m, cfg.n2m, cfg.n3m], "faces": 20480, "precision": "fp64"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "bytes_per_launch": bytes_launch, "avg_launch_us": round(avg_s * 1e6, 2),
+                     "substep_frac": round(B_ALG_CELL * cells / sub_time / HBM_PEAK, 4),
+                     "substep_ms": round(sub_time * 1e3, 4)},
+        "kernels_ms_per_substep": {k: round(v[0] / max(v[1], 1) , 4) for k, v in tm.items() if v[1] > 0},
+        "checks": {"divgck": qmax, "volume": body.volume},
+    }
+    if not args.no_cpu:
+        cfg2, level, rad = workload(args.n)
+        try:
+            out["cpu_baseline"] = cpu_baseline(cfg2, level, rad)
+        except Exception as e:  # the baseline is reported, never required
+            out["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -651,3 +651,11 @@ int orc_tschem(orc_ctx* o) {  /* tsch.f90:1-99 */
   }
   return 0;
 }
+
+/* threads the OpenMP loops of the restatement use (for the CPU baseline record) */
+#ifdef _OPENMP
+#include <omp.h>
+int orc_threads(void) { return omp_get_max_threads(); }
+#else
+int orc_threads(void) { return 1; }
+#endif
