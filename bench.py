@@ -36,8 +36,7 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E bytes/s
 # a half-spectrum complex field counts 8 B per cell).  Minimal traffic of the
 # pass: each field it must read or write, once.
 KERNEL_BYTES = {
-    "explicit_rhs": (13 + 8) * 8,      # u,v,w,T,p, ru x4, VOFp, forc x3 -> rhs x4, ru x4
-    "solve_x": (4 + 4) * 8,
+    "explicit_xsolve": (13 + 8) * 8,   # u,v,w,T,p, ru x4, VOFp, forc x3 -> x-solved rhs x4, ru x4
     "solve_y": (4 + 4) * 8,
     "solve_z_update": (4 + 4 + 4) * 8,  # rhs x4, q x4 -> q x4
     "divg_fft_x": (3 + 1) * 8,

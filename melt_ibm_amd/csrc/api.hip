@@ -15,7 +15,7 @@ int melt_fail(melt_ctx* ctx, int code, const char* what, const char* detail) {
 namespace {
 
 const char* slot_names[SLOT_COUNT] = {
-    "tag_narrowband", "explicit_rhs", "solve_x", "solve_y", "solve_z_update", "mls_force", "lagrangian",
+    "tag_narrowband", "explicit_xsolve", "solve_x", "solve_y", "solve_z_update", "mls_force", "lagrangian",
     "divg_fft_x", "fft_y", "fft_z_solve", "ifft_y", "ifft_x", "project",
     "phase_tagging", "phase_eulerian", "phase_mls", "phase_pressure", "substep"};
 
@@ -464,15 +464,15 @@ extern "C" int melt_substep(melt_ctx* ctx, int32_t ns) {
 
   timing_begin(ctx, SLOT_PH_EULER, &e_ph);
   timing_begin(ctx, SLOT_EXPLICIT, &e);
-  if ((rc = euler_launch_explicit_fused(ctx))) return rc;
+  if ((rc = euler_launch_explicit_xsolve(ctx, betadx_q(ctx, 0, 0), betadx_q(ctx, 3, 0)))) return rc;
   timing_end(ctx, SLOT_EXPLICIT, e);
   double* r[4] = {ctx->rhs4[RHS_U], ctx->rhs4[RHS_V], ctx->rhs4[RHS_W], ctx->rhs4[RHS_T]};
   double* q[4] = {ctx->f[MELT_VX], ctx->f[MELT_VY], ctx->f[MELT_VZ], ctx->f[MELT_TEMP]};
   static const int comp[4] = {0, 1, 2, 3};
-  for (int dir = 0; dir < 3; ++dir) {
+  for (int dir = 1; dir < 3; ++dir) {
     double b[4];
     for (int m = 0; m < 4; ++m) b[m] = betadx_q(ctx, comp[m], dir);
-    int slot = dir == 0 ? SLOT_SOLVE_X : dir == 1 ? SLOT_SOLVE_Y : SLOT_SOLVE_Z;
+    int slot = dir == 1 ? SLOT_SOLVE_Y : SLOT_SOLVE_Z;
     timing_begin(ctx, slot, &e);
     if ((rc = euler_launch_solve(ctx, dir, r, 4, b, dir == 2 ? q : nullptr, 0))) return rc;
     timing_end(ctx, slot, e);

@@ -349,7 +349,7 @@ __device__ inline unsigned xcd_block(unsigned b, unsigned nb) {
 int euler_launch_hdnl(melt_ctx* ctx, int which);
 int euler_launch_invtr_rhs(melt_ctx* ctx, int which);
 int euler_launch_solve(melt_ctx* ctx, int dir, double* const* fields, int nfield, const double* betadx, double* const* qadd, int store_rhs);
-int euler_launch_explicit_fused(melt_ctx* ctx);
+int euler_launch_explicit_xsolve(melt_ctx* ctx, double bxu, double bxt);
 int euler_launch_divg(melt_ctx* ctx);
 int euler_launch_project(melt_ctx* ctx, int do_u, int do_p, int use_mean);
 int euler_launch_interp_vof(melt_ctx* ctx);

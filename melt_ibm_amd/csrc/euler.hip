@@ -219,29 +219,6 @@ __global__ __launch_bounds__(256) void k_invtr_rhs(Grid g, RhsCoef cf, const dou
   ru[c] = hv;
 }
 
-// Fused explicit terms + RHS for all four transported fields: reads the state
-// once and writes rhs x4 and ru x4 (tsch.f90:49-57 up to the line solves).
-__global__ __launch_bounds__(256) void k_explicit_fused(EulerArgs A, RhsCoef cf, double* __restrict__ ru1, double* __restrict__ ru2,
-                                                        double* __restrict__ ru3, double* __restrict__ rut, double* __restrict__ r_u,
-                                                        double* __restrict__ r_v, double* __restrict__ r_w, double* __restrict__ r_t, size_t nc) {
-  unsigned b = xcd_block(blockIdx.x, gridDim.x);
-  size_t c = (size_t)b * blockDim.x + threadIdx.x;
-  if (c >= nc) return;
-  Cell s = make_cell(A.g, c);
-  const double* vp = A.vofp;
-  double p = vp[c];
-  double hu = h1_val(A, s, 0.5 * (vp[s.at(-1, 0, 0)] + p));
-  double hv = h2_val(A, s, 0.5 * (vp[s.at(0, -1, 0)] + p));
-  double hw = h3_val(A, s, 0.5 * (vp[s.at(0, 0, -1)] + p));
-  double ht = ht_val(A, s, p);
-  double o1 = ru1[c], o2 = ru2[c], o3 = ru3[c], ot = rut[c];
-  r_u[c] = rhs_u(cf, A.vx, A.pr, s, hu, o1);
-  r_v[c] = rhs_v(cf, A.vy, A.pr, s, hv, o2);
-  r_w[c] = rhs_w(cf, A.vz, A.pr, s, hw, o3);
-  r_t[c] = rhs_t(cf, A.temp, s, ht, ot);
-  ru1[c] = hu; ru2[c] = hv; ru3[c] = hw; rut[c] = ht;
-}
-
 // -------------------------------------------------- circulant PCR solver
 // The systems of solxi/solxj/solxk are periodic tridiagonal with constant
 // coefficients (diag 1, off-diagonals -G/(1+2G), rhs scaled by 1/(1+2G);
@@ -262,49 +239,20 @@ struct SolveBatch {
 
 constexpr int PCR_EMAX = 16;
 
-template <int DIR>
-__global__ __launch_bounds__(256) void k_pcr(SolveBatch batch, int M, int W, int n1, int n2, int n3) {
-  extern __shared__ double lds[];
-  const SolveField F = batch.f[blockIdx.y];
+// Cyclic reduction of the W circulant systems of one LDS tile.  Element
+// (row r, line c) sits at lds[XL ? c*ldx + r : r*W + c]; thread tid owns the
+// elements idx = tid + q*nt, q < E (XL: r = idx % M, c = idx / M; else
+// r = idx / W, c = idx % W).  d[] holds the owned, already scaled right-hand
+// sides (also present in lds) and receives the solution.  The tile must be
+// complete in lds and visible (barrier) on entry.
+template <bool XL>
+__device__ void pcr_core(double* lds, int M, int W, int ldx, double betadx, int E, double* d) {
   const int tid = threadIdx.x, nt = blockDim.x;
-  const int E = (M * W) / nt;
-  // tile base and strides
-  size_t base, es, cs;
-  unsigned t = xcd_block(blockIdx.x, gridDim.x);
-  if (DIR == 0) {
-    int ntj = n2 / W;
-    int jt = t % ntj, k = t / ntj;
-    base = (size_t)n1 * ((size_t)jt * W + (size_t)n2 * k);
-    es = 1; cs = n1;
-  } else if (DIR == 1) {
-    int nti = n1 / W;
-    int it = t % nti, k = t / nti;
-    base = (size_t)it * W + (size_t)n1 * n2 * k;
-    es = n1; cs = 1;
-  } else {
-    int nti = n1 / W;
-    int it = t % nti, j = t / nti;
-    base = (size_t)it * W + (size_t)n1 * j;
-    es = (size_t)n1 * n2; cs = 1;
-  }
-  const int ldx = M + 1;  // x layout: [line][x] padded
-  auto own_r = [&](int idx) { return DIR == 0 ? idx % M : idx / W; };
-  auto own_c = [&](int idx) { return DIR == 0 ? idx / M : idx % W; };
-  auto la = [&](int r, int c) { return DIR == 0 ? c * ldx + r : r * W + c; };
-
-  const double ackl_b = 1.0 / (1.0 + 2.0 * F.betadx);
-  double a = -F.betadx * ackl_b, bdiag = 1.0;
-  double d[PCR_EMAX];
-#pragma unroll
-  for (int q = 0; q < PCR_EMAX; ++q) {
-    if (q < E) {
-      int idx = tid + q * nt;
-      int r = own_r(idx), c = own_c(idx);
-      d[q] = F.rhs[base + (size_t)r * es + (size_t)c * cs] * ackl_b;
-      lds[la(r, c)] = d[q];
-    }
-  }
-  __syncthreads();
+  auto own_r = [&](int idx) { return XL ? idx % M : idx / W; };
+  auto own_c = [&](int idx) { return XL ? idx / M : idx % W; };
+  auto la = [&](int r, int c) { return XL ? c * ldx + r : r * W + c; };
+  const double ackl_b = 1.0 / (1.0 + 2.0 * betadx);
+  double a = -betadx * ackl_b, bdiag = 1.0;
   int p2 = M & (-M);          // largest power of two dividing M
   int m3 = M / p2;            // 1 or 3
   int o = 1;
@@ -363,15 +311,124 @@ __global__ __launch_bounds__(256) void k_pcr(SolveBatch batch, int M, int W, int
     }
   }
 #pragma unroll
+  for (int q = 0; q < PCR_EMAX; ++q)
+    if (q < E) d[q] = x[q];
+}
+
+template <int DIR>
+__global__ __launch_bounds__(256) void k_pcr(SolveBatch batch, int M, int W, int n1, int n2, int n3) {
+  extern __shared__ double lds[];
+  const SolveField F = batch.f[blockIdx.y];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int E = (M * W) / nt;
+  size_t base, es, cs;
+  unsigned t = xcd_block(blockIdx.x, gridDim.x);
+  if (DIR == 0) {
+    int ntj = n2 / W;
+    int jt = t % ntj, k = t / ntj;
+    base = (size_t)n1 * ((size_t)jt * W + (size_t)n2 * k);
+    es = 1; cs = n1;
+  } else if (DIR == 1) {
+    int nti = n1 / W;
+    int it = t % nti, k = t / nti;
+    base = (size_t)it * W + (size_t)n1 * n2 * k;
+    es = n1; cs = 1;
+  } else {
+    int nti = n1 / W;
+    int it = t % nti, j = t / nti;
+    base = (size_t)it * W + (size_t)n1 * j;
+    es = (size_t)n1 * n2; cs = 1;
+  }
+  const int ldx = M + 1;
+  constexpr bool XL = DIR == 0;
+  auto own_r = [&](int idx) { return XL ? idx % M : idx / W; };
+  auto own_c = [&](int idx) { return XL ? idx / M : idx % W; };
+  auto la = [&](int r, int c) { return XL ? c * ldx + r : r * W + c; };
+  const double ackl_b = 1.0 / (1.0 + 2.0 * F.betadx);
+  double d[PCR_EMAX];
+#pragma unroll
+  for (int q = 0; q < PCR_EMAX; ++q) {
+    if (q < E) {
+      int idx = tid + q * nt;
+      int r = own_r(idx), c = own_c(idx);
+      d[q] = F.rhs[base + (size_t)r * es + (size_t)c * cs] * ackl_b;
+      lds[la(r, c)] = d[q];
+    }
+  }
+  __syncthreads();
+  pcr_core<XL>(lds, M, W, ldx, F.betadx, E, d);
+#pragma unroll
   for (int q = 0; q < PCR_EMAX; ++q) {
     if (q < E) {
       int idx = tid + q * nt;
       size_t gaddr = base + (size_t)own_r(idx) * es + (size_t)own_c(idx) * cs;
       if (F.qadd) {
-        F.qadd[gaddr] = F.qadd[gaddr] + x[q];
-        if (F.store_rhs) F.rhs[gaddr] = x[q];
+        F.qadd[gaddr] = F.qadd[gaddr] + d[q];
+        if (F.store_rhs) F.rhs[gaddr] = d[q];
       } else {
-        F.rhs[gaddr] = x[q];
+        F.rhs[gaddr] = d[q];
+      }
+    }
+  }
+}
+
+// Explicit terms + RHS of all four transported fields fused with the x-line
+// solve (pass A of the substep): a block owns W complete x-lines at one z
+// plane, computes rhs x4 and ru x4 per cell (reading the state through L1/L2),
+// keeps the scaled rhs in LDS, solves the four x-systems there and writes the
+// x-solved rhs.  Saves one read and one write of rhs x4 per substep.
+constexpr int XF_CELLS = 1024;   // cells per block (W * N1)
+
+__global__ __launch_bounds__(256) void k_explicit_xsolve(EulerArgs A, RhsCoef cf, double* __restrict__ ru1, double* __restrict__ ru2,
+                                                         double* __restrict__ ru3, double* __restrict__ rut, double* __restrict__ r_u,
+                                                         double* __restrict__ r_v, double* __restrict__ r_w, double* __restrict__ r_t,
+                                                         double bxu, double bxt, int W) {
+  extern __shared__ double lds[];
+  const Grid& g = A.g;
+  const int M = g.n1, ldx = M + 1, tid = threadIdx.x, nt = blockDim.x;
+  const int ncell = W * M;
+  const int E = ncell / nt;
+  unsigned t = xcd_block(blockIdx.x, gridDim.x);
+  int ntj = g.n2 / W;
+  int j0 = (t % ntj) * W, k = t / ntj;
+  size_t base = (size_t)M * ((size_t)j0 + (size_t)g.n2 * k);
+  double* L[4] = {lds, lds + (size_t)W * ldx, lds + 2 * (size_t)W * ldx, lds + 3 * (size_t)W * ldx};
+  double* R[4] = {r_u, r_v, r_w, r_t};
+  const double sc_u = 1.0 / (1.0 + 2.0 * bxu), sc_t = 1.0 / (1.0 + 2.0 * bxt);
+  for (int q = 0; q < E; ++q) {
+    int idx = tid + q * nt;
+    int r = idx % M, c = idx / M;
+    size_t gc = base + (size_t)M * c + r;
+    Cell s = make_cell(g, gc);
+    const double* vp = A.vofp;
+    double p = vp[gc];
+    double hu = h1_val(A, s, 0.5 * (vp[s.at(-1, 0, 0)] + p));
+    double hv = h2_val(A, s, 0.5 * (vp[s.at(0, -1, 0)] + p));
+    double hw = h3_val(A, s, 0.5 * (vp[s.at(0, 0, -1)] + p));
+    double ht = ht_val(A, s, p);
+    double o1 = ru1[gc], o2 = ru2[gc], o3 = ru3[gc], ot = rut[gc];
+    L[0][c * ldx + r] = rhs_u(cf, A.vx, A.pr, s, hu, o1) * sc_u;
+    L[1][c * ldx + r] = rhs_v(cf, A.vy, A.pr, s, hv, o2) * sc_u;
+    L[2][c * ldx + r] = rhs_w(cf, A.vz, A.pr, s, hw, o3) * sc_u;
+    L[3][c * ldx + r] = rhs_t(cf, A.temp, s, ht, ot) * sc_t;
+    ru1[gc] = hu; ru2[gc] = hv; ru3[gc] = hw; rut[gc] = ht;
+  }
+  __syncthreads();
+  for (int f = 0; f < 4; ++f) {
+    double d[PCR_EMAX];
+#pragma unroll
+    for (int q = 0; q < PCR_EMAX; ++q) {
+      if (q < E) {
+        int idx = tid + q * nt;
+        d[q] = L[f][(idx / M) * ldx + idx % M];
+      }
+    }
+    pcr_core<true>(L[f], M, W, ldx, f == 3 ? bxt : bxu, E, d);
+#pragma unroll
+    for (int q = 0; q < PCR_EMAX; ++q) {
+      if (q < E) {
+        int idx = tid + q * nt;
+        R[f][base + (size_t)M * (idx / M) + idx % M] = d[q];
       }
     }
   }
@@ -492,11 +549,21 @@ int euler_launch_invtr_rhs(melt_ctx* ctx, int which) {
   return 0;
 }
 
-int euler_launch_explicit_fused(melt_ctx* ctx) {
+// Pass A: explicit terms + RHS + x-solve of u, v, w, T into rhs4[].
+int euler_launch_explicit_xsolve(melt_ctx* ctx, double bxu, double bxt) {
   EulerArgs A = make_args(ctx);
   RhsCoef cf = make_coef(ctx);
-  hipLaunchKernelGGL(k_explicit_fused, dim3(nblocks(ctx->nc, 256)), dim3(256), 0, ctx->stream, A, cf, ctx->f[MELT_RU1], ctx->f[MELT_RU2],
-                     ctx->f[MELT_RU3], ctx->f[MELT_RUT], ctx->rhs4[RHS_U], ctx->rhs4[RHS_V], ctx->rhs4[RHS_W], ctx->rhs4[RHS_T], ctx->nc);
+  int W = 1;
+  for (int w = 1; w <= ctx->n2; ++w)
+    if (ctx->n2 % w == 0 && w * ctx->n1 <= XF_CELLS) W = w;
+  int ncell = W * ctx->n1;
+  int E = (ncell + 255) / 256;  // cells per thread; threads = ncell / E <= 256
+  while (ncell % E) ++E;
+  if (E > PCR_EMAX) return melt_fail(ctx, MELT_ERR_ARG, "explicit_xsolve", "unsupported line length");
+  size_t shm = sizeof(double) * 4 * (size_t)W * (ctx->n1 + 1);
+  unsigned blocks = (unsigned)((size_t)(ctx->n2 / W) * ctx->n3);
+  hipLaunchKernelGGL(k_explicit_xsolve, dim3(blocks), dim3(ncell / E), shm, ctx->stream, A, cf, ctx->f[MELT_RU1], ctx->f[MELT_RU2],
+                     ctx->f[MELT_RU3], ctx->f[MELT_RUT], ctx->rhs4[RHS_U], ctx->rhs4[RHS_V], ctx->rhs4[RHS_W], ctx->rhs4[RHS_T], bxu, bxt, W);
   MELT_HIP(hipGetLastError());
   return 0;
 }

@@ -59,7 +59,7 @@ def test_timing_slot_names_without_device():
     lib = load_library()
     lib.melt_timing_name.restype = C.c_char_p
     names = [lib.melt_timing_name(i).decode() for i in range(lib.melt_timing_count())]
-    assert "explicit_rhs" in names and "fft_z_solve" in names and "substep" in names
+    assert "explicit_xsolve" in names and "fft_z_solve" in names and "substep" in names
 
 
 def test_bench_refuses_multi_gpu():
