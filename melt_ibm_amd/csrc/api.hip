@@ -162,6 +162,12 @@ extern "C" int melt_ctx_create(const melt_config* cfg, melt_ctx** out) {
     b.e_thresh = cfg->perc_ethresh * b.h_eulerian;
     b.v_thresh = cfg->v_on_ve_perc * b.celvol;
     if (hipMemcpy(ctx->d_body, &b, sizeof b, hipMemcpyHostToDevice) != hipSuccess) rc = MELT_ERR_HIP;
+    // single-phase defaults of init.f90:30-33 (after the zero fills above,
+    // which are not ordered with this non-blocking stream)
+    if (hipDeviceSynchronize() != hipSuccess) { rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "initial fill failed"); break; }
+    for (int id : {MELT_VOFX, MELT_VOFY, MELT_VOFZ, MELT_VOFP})
+      hipLaunchKernelGGL(k_fill_d, dim3(1024), dim3(256), 0, ctx->stream, ctx->f[id], 1.0, ctx->nc);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "initial fill failed");
   } while (0);
   if (rc) {
     melt_ctx_destroy(ctx);

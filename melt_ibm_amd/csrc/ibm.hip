@@ -16,6 +16,7 @@
 #include <cstring>
 
 #include "common.h"
+#include "fixed128.h"
 
 namespace {
 
@@ -202,22 +203,15 @@ __device__ int mls_grad(const Grid& g, const double* __restrict__ fld, const dou
 }
 
 // ---------------------------------------------------- 128-bit accumulation
-// A contribution v is held as hi + lo * 2^-64 with hi = floor(v) (two's
-// complement) and lo the fraction scaled by 2^64 (truncated below 2^-64).
-// Integer addition with carry is associative, so the accumulated sum does not
-// depend on the order in which faces arrive.
+// acc_split / acc_value (fixed128.h) hold each contribution exactly as a
+// 64.64 fixed-point integer; the atomics add the two words with carry.
 __device__ inline void acc_add(unsigned long long* w /* [lo, hi] */, double v) {
-  if (v == 0.0) return;
-  double fl = floor(v);
-  long long hi = (long long)fl;
-  double frac = v - fl;
-  unsigned long long lo = (unsigned long long)(frac * 18446744073709551616.0);
+  unsigned long long lo, hi;
+  acc_split(v, lo, hi);
+  if ((lo | hi) == 0) return;
   unsigned long long old = atomicAdd(&w[0], lo);
   unsigned long long carry = (old + lo < old) ? 1ull : 0ull;
-  atomicAdd(&w[1], (unsigned long long)hi + carry);
-}
-__device__ inline double acc_value(unsigned long long lo, unsigned long long hi) {
-  return (double)(long long)hi + (double)lo * 5.421010862427522170037264e-20;  // 2^-64
+  atomicAdd(&w[1], hi + carry);
 }
 
 __device__ inline long long win_slot(const int org[3], const int E[3], int iu, int ju, int ku) {
@@ -664,9 +658,7 @@ __global__ __launch_bounds__(256) void k_force_apply(Grid g, const melt_window* 
       else dst[comp][cell] = v;
       if (comp != 2) {
         int m = comp == 3 ? 2 : comp;
-        unsigned long long nlo = mlo[m] + lo;
-        mhi[m] += hi + (nlo < mlo[m] ? 1ull : 0ull);
-        mlo[m] = nlo;
+        acc_add_words(mlo[m], mhi[m], lo, hi);
       }
     }
   }
@@ -693,9 +685,7 @@ __global__ void k_fmean_final(const unsigned long long* __restrict__ part, int n
   unsigned long long lo = 0, hi = 0;
   for (int b = 0; b < nb; ++b) {
     unsigned long long l2 = part[(b * 3 + m) * 2], h2 = part[(b * 3 + m) * 2 + 1];
-    unsigned long long nl = lo + l2;
-    hi = hi + h2 + (nl < lo ? 1ull : 0ull);
-    lo = nl;
+    acc_add_words(lo, hi, l2, h2);
   }
   mean[m] = acc_value(lo, hi) / ncell;
 }

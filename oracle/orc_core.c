@@ -92,6 +92,8 @@ int orc_ctx_create(const melt_config* cfg, orc_ctx** out) {
     size_t cnt = ghost ? (size_t)o->n1 * o->n2 * (o->n3 + 2) : o->nc;
     o->f[id] = (double*)xcalloc(cnt, sizeof(double));
   }
+  /* single-phase defaults of init.f90:30-33 */
+  for (int id = MELT_VOFP; id <= MELT_VOFZ; ++id) orc_fill_field(o, id, 1.0);
   o->dt = 0.0;
   *out = o;
   return MELT_OK;

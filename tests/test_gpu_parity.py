@@ -204,7 +204,11 @@ def test_tschem_with_melting_body(gpu_lib, n, n3, level, rad, steps):
     for q in ("vx", "vy", "vz", "temp", "pr"):
         e = rel_err(g.download(q), o.download(q))
         assert e < 1e-10, f"{q}: rel err {e}"
-    assert np.array_equal(g.download("VOFp"), o.download("VOFp")) or rel_err(g.download("VOFp"), o.download("VOFp")) < 1e-12
+    # the moved geometry differs at round-off after time stepping, so the tags
+    # follow it within the field bar (bit-exact tags on identical geometry are
+    # checked by test_tagging_bit_exact)
+    e = rel_err(g.download("VOFp"), o.download("VOFp"))
+    assert e < 1e-10, f"VOFp: rel err {e}"
     bg, bo = g.body().as_dict(), o.body().as_dict()
     for k in ("volume", "pos_cm", "omega_c", "vel_cm"):
         a, b = np.atleast_1d(bg[k]), np.atleast_1d(bo[k])
