@@ -3,7 +3,9 @@
 // and the fused RK3 substep.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 
 #include "common.h"
 
@@ -134,6 +136,21 @@ extern "C" int melt_ctx_create(const melt_config* cfg, melt_ctx** out) {
   int rc = 0;
   do {
     if (hipGetDevice(&ctx->device) != hipSuccess) { rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "no HIP device"); break; }
+    if (const char* tv = getenv("MELT_TUNE")) {
+      static const char* keys[TUNE_N] = {"xf_w", "xf_nt", "solve_w", "solve_nt"};
+      std::string spec(tv);
+      size_t pos = 0;
+      while (pos < spec.size()) {
+        size_t end = spec.find(',', pos);
+        if (end == std::string::npos) end = spec.size();
+        std::string item = spec.substr(pos, end - pos);
+        size_t eq = item.find('=');
+        if (eq != std::string::npos)
+          for (int t = 0; t < TUNE_N; ++t)
+            if (item.substr(0, eq) == keys[t]) ctx->tune[t] = atoi(item.c_str() + eq + 1);
+        pos = end + 1;
+      }
+    }
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { rc = MELT_ERR_HIP; break; }
     for (int i = 0; i < MELT_NFIELDS && !rc; ++i) {
       if (hipMalloc(&ctx->f[i], sizeof(double) * ctx->nc) != hipSuccess || hipMemset(ctx->f[i], 0, sizeof(double) * ctx->nc) != hipSuccess)
@@ -167,6 +184,7 @@ extern "C" int melt_ctx_create(const melt_config* cfg, melt_ctx** out) {
     if (hipDeviceSynchronize() != hipSuccess) { rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "initial fill failed"); break; }
     for (int id : {MELT_VOFX, MELT_VOFY, MELT_VOFZ, MELT_VOFP})
       hipLaunchKernelGGL(k_fill_d, dim3(1024), dim3(256), 0, ctx->stream, ctx->f[id], 1.0, ctx->nc);
+    if ((rc = ibm_window_reset(ctx))) break;
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "initial fill failed");
   } while (0);
   if (rc) {

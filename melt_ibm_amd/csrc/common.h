@@ -40,13 +40,23 @@ struct melt_window {
   int prev_org[3]; // box of the previous tagging call (its sdf must be reset)
   int prev_valid;
   int pad;
+  int bmin[3], bmax[3];  // running min / max of the face base indices (reset after use)
 };
 
 // Per-face probe results of the MLS gradient / interpolation evaluations.
-enum { PRB_QO = 0, PRB_QI, PRB_P, PRB_GP, PRB_GU = PRB_GP + 3, PRB_GV = PRB_GU + 3, PRB_GW = PRB_GV + 3, PRB_N = PRB_GW + 3 };
+enum {
+  PRB_QO = 0, PRB_QI, PRB_P, PRB_GP, PRB_GU = PRB_GP + 3, PRB_GV = PRB_GU + 3, PRB_GW = PRB_GV + 3,
+  PRB_GT0 = PRB_GW + 3, PRB_GT1 = PRB_GT0 + 3,  // temperature gradients behind QO / QI
+  PRB_N = PRB_GT1 + 3
+};
+
+// Launch-shape overrides (0 = built-in choice), set from MELT_TUNE at
+// context creation ("xf_w=4,solve_w=16,...") for tile sweeps.
+enum melt_tune { TUNE_XF_W = 0, TUNE_XF_NT, TUNE_SOLVE_W, TUNE_SOLVE_NT, TUNE_N };
 
 struct melt_ctx {
   melt_config cfg;
+  int tune[TUNE_N] = {};
   int n1, n2, n3;
   size_t nc;
   int n1h, n1hp;
@@ -366,6 +376,7 @@ int ibm_sub_mean(melt_ctx* ctx);
 int ibm_velforce(melt_ctx* ctx, int defer);
 int ibm_tempforce(melt_ctx* ctx);
 int ibm_probes(melt_ctx* ctx);
+int ibm_window_reset(melt_ctx* ctx);
 int lagr_after_force(melt_ctx* ctx);
 int lagr_alloc(melt_ctx* ctx, const double* xyz0, const int* voe, const int* eof, const int* vof, double rad_p);
 void timing_begin(melt_ctx* ctx, int slot, hipEvent_t* ev);

@@ -12,26 +12,33 @@
 
 namespace {
 
-// Wrapped neighbour coordinates of one cell (0-based).
+// One cell (0-based coordinates) with the element offsets to its wrapped
+// neighbours.  Fields hold at most 1024^3 elements, so 32-bit offsets suffice.
 struct Cell {
-  int i[3], j[3], k[3];  // [minus, centre, plus]
-  int n1, n2;
-  __device__ size_t at(int a, int b, int c) const {
-    return (size_t)i[a + 1] + (size_t)n1 * ((size_t)j[b + 1] + (size_t)n2 * (size_t)k[c + 1]);
+  int c, i, j, k;
+  int im, ip, jm, jp, km, kp;
+  __device__ __forceinline__ int at(int a, int b, int d) const {
+    return c + (a < 0 ? im : a > 0 ? ip : 0) + (b < 0 ? jm : b > 0 ? jp : 0) + (d < 0 ? km : d > 0 ? kp : 0);
   }
 };
 
-__device__ inline Cell make_cell(const Grid& g, size_t c) {
+__device__ __forceinline__ Cell cell_at(const Grid& g, int i, int j, int k) {
   Cell s;
-  s.n1 = g.n1; s.n2 = g.n2;
-  int i = (int)(c % g.n1);
-  size_t r = c / g.n1;
-  int j = (int)(r % g.n2);
-  int k = (int)(r / g.n2);
-  s.i[1] = i; s.i[0] = i == 0 ? g.n1 - 1 : i - 1; s.i[2] = i == g.n1 - 1 ? 0 : i + 1;
-  s.j[1] = j; s.j[0] = j == 0 ? g.n2 - 1 : j - 1; s.j[2] = j == g.n2 - 1 ? 0 : j + 1;
-  s.k[1] = k; s.k[0] = k == 0 ? g.n3 - 1 : k - 1; s.k[2] = k == g.n3 - 1 ? 0 : k + 1;
+  const int n12 = g.n1 * g.n2;
+  s.i = i; s.j = j; s.k = k;
+  s.c = i + g.n1 * j + n12 * k;
+  s.im = i == 0 ? g.n1 - 1 : -1;
+  s.ip = i == g.n1 - 1 ? 1 - g.n1 : 1;
+  s.jm = j == 0 ? (g.n2 - 1) * g.n1 : -g.n1;
+  s.jp = j == g.n2 - 1 ? (1 - g.n2) * g.n1 : g.n1;
+  s.km = k == 0 ? (g.n3 - 1) * n12 : -n12;
+  s.kp = k == g.n3 - 1 ? (1 - g.n3) * n12 : n12;
   return s;
+}
+
+__device__ __forceinline__ Cell make_cell(const Grid& g, int c) {
+  int i = c % g.n1, r = c / g.n1;
+  return cell_at(g, i, r % g.n2, r / g.n2);
 }
 
 struct EulerArgs {
@@ -103,7 +110,7 @@ __device__ inline double ht_val(const EulerArgs& A, const Cell& s, double vofp) 
   const double* pc = A.body->pos_cm;
   const double* vc = A.body->vel_cm;
   const double* om = A.body->omega_c;
-  int ic = s.i[1] + 1, jc = s.j[1] + 1, kc = s.k[1] + 1;
+  int ic = s.i + 1, jc = s.j + 1, kc = s.k + 1;
   double r0 = gxc(g, ic) - pc[0], r1 = gym(g, jc) - pc[1], r2 = gzm(g, kc) - pc[2];
   r0 = r0 - g.xlen * (double)fnint(r0 / g.xlen);
   r1 = r1 - g.ylen * (double)fnint(r1 / g.ylen);
@@ -237,16 +244,14 @@ struct SolveBatch {
   SolveField f[4];
 };
 
-constexpr int PCR_EMAX = 16;
-
 // Cyclic reduction of the W circulant systems of one LDS tile.  Element
 // (row r, line c) sits at lds[XL ? c*ldx + r : r*W + c]; thread tid owns the
 // elements idx = tid + q*nt, q < E (XL: r = idx % M, c = idx / M; else
 // r = idx / W, c = idx % W).  d[] holds the owned, already scaled right-hand
 // sides (also present in lds) and receives the solution.  The tile must be
 // complete in lds and visible (barrier) on entry.
-template <bool XL>
-__device__ void pcr_core(double* lds, int M, int W, int ldx, double betadx, int E, double* d) {
+template <bool XL, int E>
+__device__ __forceinline__ void pcr_core(double* lds, int M, int W, int ldx, double betadx, double* d) {
   const int tid = threadIdx.x, nt = blockDim.x;
   auto own_r = [&](int idx) { return XL ? idx % M : idx / W; };
   auto own_c = [&](int idx) { return XL ? idx / M : idx % W; };
@@ -261,25 +266,21 @@ __device__ void pcr_core(double* lds, int M, int W, int ldx, double betadx, int 
   while (o <= last) {
     if (fabs(a) < 1e-17 * fabs(bdiag)) { done = true; break; }
     double k = a / bdiag;
-    double nd[PCR_EMAX];
+    double nd[E];
 #pragma unroll
-    for (int q = 0; q < PCR_EMAX; ++q) {
-      if (q < E) {
-        int idx = tid + q * nt;
-        int r = own_r(idx), c = own_c(idx);
-        int rm = r - o; if (rm < 0) rm += M;
-        int rp = r + o; if (rp >= M) rp -= M;
-        nd[q] = d[q] - k * (lds[la(rm, c)] + lds[la(rp, c)]);
-      }
+    for (int q = 0; q < E; ++q) {
+      int idx = tid + q * nt;
+      int r = own_r(idx), c = own_c(idx);
+      int rm = r - o; if (rm < 0) rm += M;
+      int rp = r + o; if (rp >= M) rp -= M;
+      nd[q] = d[q] - k * (lds[la(rm, c)] + lds[la(rp, c)]);
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < PCR_EMAX; ++q) {
-      if (q < E) {
-        int idx = tid + q * nt;
-        d[q] = nd[q];
-        lds[la(own_r(idx), own_c(idx))] = d[q];
-      }
+    for (int q = 0; q < E; ++q) {
+      int idx = tid + q * nt;
+      d[q] = nd[q];
+      lds[la(own_r(idx), own_c(idx))] = d[q];
     }
     __syncthreads();
     double bn = bdiag - 2.0 * a * k;
@@ -288,56 +289,57 @@ __device__ void pcr_core(double* lds, int M, int W, int ldx, double betadx, int 
     o *= 2;
   }
   if (!done && fabs(a) < 1e-17 * fabs(bdiag)) done = true;
-  double x[PCR_EMAX];
+  if (done) {
 #pragma unroll
-  for (int q = 0; q < PCR_EMAX; ++q) {
-    if (q < E) {
-      int idx = tid + q * nt;
-      int r = own_r(idx), c = own_c(idx);
-      if (done) {
-        x[q] = d[q] / bdiag;
-      } else if (m3 == 1) {
-        int h = M / 2;
-        int rh = r + h; if (rh >= M) rh -= M;
-        double dh = lds[la(rh, c)];
-        x[q] = (bdiag * d[q] - 2.0 * a * dh) / (bdiag * bdiag - 4.0 * a * a);
-      } else {
-        int h = M / 3;
-        int r1 = r + h; if (r1 >= M) r1 -= M;
-        int r2 = r1 + h; if (r2 >= M) r2 -= M;
-        double D = d[q] + lds[la(r1, c)] + lds[la(r2, c)];
-        x[q] = (d[q] - a * (D / (bdiag + 2.0 * a))) / (bdiag - a);
-      }
+    for (int q = 0; q < E; ++q) d[q] = d[q] / bdiag;
+    return;
+  }
+  double x[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    int idx = tid + q * nt;
+    int r = own_r(idx), c = own_c(idx);
+    if (m3 == 1) {
+      int h = M / 2;
+      int rh = r + h; if (rh >= M) rh -= M;
+      double dh = lds[la(rh, c)];
+      x[q] = (bdiag * d[q] - 2.0 * a * dh) / (bdiag * bdiag - 4.0 * a * a);
+    } else {
+      int h = M / 3;
+      int r1 = r + h; if (r1 >= M) r1 -= M;
+      int r2 = r1 + h; if (r2 >= M) r2 -= M;
+      double D = d[q] + lds[la(r1, c)] + lds[la(r2, c)];
+      x[q] = (d[q] - a * (D / (bdiag + 2.0 * a))) / (bdiag - a);
     }
   }
 #pragma unroll
-  for (int q = 0; q < PCR_EMAX; ++q)
-    if (q < E) d[q] = x[q];
+  for (int q = 0; q < E; ++q) d[q] = x[q];
 }
 
-template <int DIR>
-__global__ __launch_bounds__(256) void k_pcr(SolveBatch batch, int M, int W, int n1, int n2, int n3) {
+// Line solves along y (DIR 1) or z (DIR 2) (and x, DIR 0, for the per-routine
+// entry points): a block owns W lines of M rows, E elements per thread.
+template <int DIR, int E>
+__global__ __launch_bounds__(1024) void k_pcr(SolveBatch batch, int M, int W, int n1, int n2, int n3) {
   extern __shared__ double lds[];
   const SolveField F = batch.f[blockIdx.y];
   const int tid = threadIdx.x, nt = blockDim.x;
-  const int E = (M * W) / nt;
-  size_t base, es, cs;
+  int base, es, cs;
   unsigned t = xcd_block(blockIdx.x, gridDim.x);
   if (DIR == 0) {
     int ntj = n2 / W;
     int jt = t % ntj, k = t / ntj;
-    base = (size_t)n1 * ((size_t)jt * W + (size_t)n2 * k);
+    base = n1 * (jt * W + n2 * k);
     es = 1; cs = n1;
   } else if (DIR == 1) {
     int nti = n1 / W;
     int it = t % nti, k = t / nti;
-    base = (size_t)it * W + (size_t)n1 * n2 * k;
+    base = it * W + n1 * n2 * k;
     es = n1; cs = 1;
   } else {
     int nti = n1 / W;
     int it = t % nti, j = t / nti;
-    base = (size_t)it * W + (size_t)n1 * j;
-    es = (size_t)n1 * n2; cs = 1;
+    base = it * W + n1 * j;
+    es = n1 * n2; cs = 1;
   }
   const int ldx = M + 1;
   constexpr bool XL = DIR == 0;
@@ -345,40 +347,36 @@ __global__ __launch_bounds__(256) void k_pcr(SolveBatch batch, int M, int W, int
   auto own_c = [&](int idx) { return XL ? idx / M : idx % W; };
   auto la = [&](int r, int c) { return XL ? c * ldx + r : r * W + c; };
   const double ackl_b = 1.0 / (1.0 + 2.0 * F.betadx);
-  double d[PCR_EMAX];
+  double d[E];
 #pragma unroll
-  for (int q = 0; q < PCR_EMAX; ++q) {
-    if (q < E) {
-      int idx = tid + q * nt;
-      int r = own_r(idx), c = own_c(idx);
-      d[q] = F.rhs[base + (size_t)r * es + (size_t)c * cs] * ackl_b;
-      lds[la(r, c)] = d[q];
-    }
+  for (int q = 0; q < E; ++q) {
+    int idx = tid + q * nt;
+    int r = own_r(idx), c = own_c(idx);
+    d[q] = F.rhs[base + r * es + c * cs] * ackl_b;
+    lds[la(r, c)] = d[q];
   }
   __syncthreads();
-  pcr_core<XL>(lds, M, W, ldx, F.betadx, E, d);
+  pcr_core<XL, E>(lds, M, W, ldx, F.betadx, d);
 #pragma unroll
-  for (int q = 0; q < PCR_EMAX; ++q) {
-    if (q < E) {
-      int idx = tid + q * nt;
-      size_t gaddr = base + (size_t)own_r(idx) * es + (size_t)own_c(idx) * cs;
-      if (F.qadd) {
-        F.qadd[gaddr] = F.qadd[gaddr] + d[q];
-        if (F.store_rhs) F.rhs[gaddr] = d[q];
-      } else {
-        F.rhs[gaddr] = d[q];
-      }
+  for (int q = 0; q < E; ++q) {
+    int idx = tid + q * nt;
+    int gaddr = base + own_r(idx) * es + own_c(idx) * cs;
+    if (F.qadd) {
+      F.qadd[gaddr] = F.qadd[gaddr] + d[q];
+      if (F.store_rhs) F.rhs[gaddr] = d[q];
+    } else {
+      F.rhs[gaddr] = d[q];
     }
   }
 }
 
 // Explicit terms + RHS of all four transported fields fused with the x-line
 // solve (pass A of the substep): a block owns W complete x-lines at one z
-// plane, computes rhs x4 and ru x4 per cell (reading the state through L1/L2),
-// keeps the scaled rhs in LDS, solves the four x-systems there and writes the
-// x-solved rhs.  Saves one read and one write of rhs x4 per substep.
-constexpr int XF_CELLS = 1024;   // cells per block (W * N1)
-
+// plane, computes rhs x4 and ru x4 per cell (E cells per thread, reading the
+// state through L1/L2), keeps the scaled rhs in LDS, solves the four
+// x-systems there and writes the x-solved rhs.  Saves one read and one write
+// of rhs x4 per substep against separate passes.
+template <int E>
 __global__ __launch_bounds__(256) void k_explicit_xsolve(EulerArgs A, RhsCoef cf, double* __restrict__ ru1, double* __restrict__ ru2,
                                                          double* __restrict__ ru3, double* __restrict__ rut, double* __restrict__ r_u,
                                                          double* __restrict__ r_v, double* __restrict__ r_w, double* __restrict__ r_t,
@@ -386,20 +384,18 @@ __global__ __launch_bounds__(256) void k_explicit_xsolve(EulerArgs A, RhsCoef cf
   extern __shared__ double lds[];
   const Grid& g = A.g;
   const int M = g.n1, ldx = M + 1, tid = threadIdx.x, nt = blockDim.x;
-  const int ncell = W * M;
-  const int E = ncell / nt;
   unsigned t = xcd_block(blockIdx.x, gridDim.x);
   int ntj = g.n2 / W;
   int j0 = (t % ntj) * W, k = t / ntj;
-  size_t base = (size_t)M * ((size_t)j0 + (size_t)g.n2 * k);
-  double* L[4] = {lds, lds + (size_t)W * ldx, lds + 2 * (size_t)W * ldx, lds + 3 * (size_t)W * ldx};
-  double* R[4] = {r_u, r_v, r_w, r_t};
+  const int base = M * (j0 + g.n2 * k);
+  const int plane = W * ldx;
   const double sc_u = 1.0 / (1.0 + 2.0 * bxu), sc_t = 1.0 / (1.0 + 2.0 * bxt);
+#pragma unroll 1
   for (int q = 0; q < E; ++q) {
     int idx = tid + q * nt;
     int r = idx % M, c = idx / M;
-    size_t gc = base + (size_t)M * c + r;
-    Cell s = make_cell(g, gc);
+    Cell s = cell_at(g, r, j0 + c, k);
+    const int gc = s.c;
     const double* vp = A.vofp;
     double p = vp[gc];
     double hu = h1_val(A, s, 0.5 * (vp[s.at(-1, 0, 0)] + p));
@@ -407,38 +403,45 @@ __global__ __launch_bounds__(256) void k_explicit_xsolve(EulerArgs A, RhsCoef cf
     double hw = h3_val(A, s, 0.5 * (vp[s.at(0, 0, -1)] + p));
     double ht = ht_val(A, s, p);
     double o1 = ru1[gc], o2 = ru2[gc], o3 = ru3[gc], ot = rut[gc];
-    L[0][c * ldx + r] = rhs_u(cf, A.vx, A.pr, s, hu, o1) * sc_u;
-    L[1][c * ldx + r] = rhs_v(cf, A.vy, A.pr, s, hv, o2) * sc_u;
-    L[2][c * ldx + r] = rhs_w(cf, A.vz, A.pr, s, hw, o3) * sc_u;
-    L[3][c * ldx + r] = rhs_t(cf, A.temp, s, ht, ot) * sc_t;
+    const int l = c * ldx + r;
+    lds[l] = rhs_u(cf, A.vx, A.pr, s, hu, o1) * sc_u;
+    lds[plane + l] = rhs_v(cf, A.vy, A.pr, s, hv, o2) * sc_u;
+    lds[2 * plane + l] = rhs_w(cf, A.vz, A.pr, s, hw, o3) * sc_u;
+    lds[3 * plane + l] = rhs_t(cf, A.temp, s, ht, ot) * sc_t;
     ru1[gc] = hu; ru2[gc] = hv; ru3[gc] = hw; rut[gc] = ht;
   }
   __syncthreads();
+  double* R[4] = {r_u, r_v, r_w, r_t};
+#pragma unroll 1
   for (int f = 0; f < 4; ++f) {
-    double d[PCR_EMAX];
+    double* L = lds + f * plane;
+    double d[E];
 #pragma unroll
-    for (int q = 0; q < PCR_EMAX; ++q) {
-      if (q < E) {
-        int idx = tid + q * nt;
-        d[q] = L[f][(idx / M) * ldx + idx % M];
-      }
+    for (int q = 0; q < E; ++q) {
+      int idx = tid + q * nt;
+      d[q] = L[(idx / M) * ldx + idx % M];
     }
-    pcr_core<true>(L[f], M, W, ldx, f == 3 ? bxt : bxu, E, d);
+    pcr_core<true, E>(L, M, W, ldx, f == 3 ? bxt : bxu, d);
 #pragma unroll
-    for (int q = 0; q < PCR_EMAX; ++q) {
-      if (q < E) {
-        int idx = tid + q * nt;
-        R[f][base + (size_t)M * (idx / M) + idx % M] = d[q];
-      }
+    for (int q = 0; q < E; ++q) {
+      int idx = tid + q * nt;
+      R[f][base + idx] = d[q];
     }
   }
 }
 
-int tile_width(int M, int ndim_other) {
-  int W = 16;
-  while (W > 1 && M * W > PCR_EMAX * 256) W /= 2;
-  while (W > 1 && ndim_other % W) W /= 2;
-  return W;
+// Elements per thread the templates are instantiated for.
+constexpr int kEs[] = {1, 2, 3, 4, 6, 8};
+
+// Threads per tile of n elements: the largest divisor of n that is at most
+// cap and whose element count per thread is one of kEs.
+int pick_threads(int n, int cap, int want) {
+  if (want > 0 && want <= cap && n % want == 0)
+    for (int e : kEs)
+      if (n / want == e) return want;
+  for (int e : kEs)
+    if (n % e == 0 && n / e <= cap) return n / e;
+  return 0;
 }
 
 // ------------------------------------------------------------ divg etc
@@ -553,17 +556,31 @@ int euler_launch_invtr_rhs(melt_ctx* ctx, int which) {
 int euler_launch_explicit_xsolve(melt_ctx* ctx, double bxu, double bxt) {
   EulerArgs A = make_args(ctx);
   RhsCoef cf = make_coef(ctx);
-  int W = 1;
-  for (int w = 1; w <= ctx->n2; ++w)
-    if (ctx->n2 % w == 0 && w * ctx->n1 <= XF_CELLS) W = w;
+  // W x-lines per block: the fewest that give a 256-cell tile (one cell per
+  // thread); small tiles keep the LDS and register footprint low.
+  int W = 0;
+  if (ctx->tune[TUNE_XF_W] > 0 && ctx->n2 % ctx->tune[TUNE_XF_W] == 0) W = ctx->tune[TUNE_XF_W];
+  else
+    for (int w = 1; w <= ctx->n2 && !W; ++w)
+      if (ctx->n2 % w == 0 && (w * ctx->n1 >= 256 || w == ctx->n2)) W = w;
+  if (W * ctx->n1 > 2048) return melt_fail(ctx, MELT_ERR_ARG, "explicit_xsolve", "tile too large");
   int ncell = W * ctx->n1;
-  int E = (ncell + 255) / 256;  // cells per thread; threads = ncell / E <= 256
-  while (ncell % E) ++E;
-  if (E > PCR_EMAX) return melt_fail(ctx, MELT_ERR_ARG, "explicit_xsolve", "unsupported line length");
+  int nt = pick_threads(ncell, 256, ctx->tune[TUNE_XF_NT]);
+  if (!nt) return melt_fail(ctx, MELT_ERR_ARG, "explicit_xsolve", "unsupported line length");
+  int E = ncell / nt;
   size_t shm = sizeof(double) * 4 * (size_t)W * (ctx->n1 + 1);
   unsigned blocks = (unsigned)((size_t)(ctx->n2 / W) * ctx->n3);
-  hipLaunchKernelGGL(k_explicit_xsolve, dim3(blocks), dim3(ncell / E), shm, ctx->stream, A, cf, ctx->f[MELT_RU1], ctx->f[MELT_RU2],
-                     ctx->f[MELT_RU3], ctx->f[MELT_RUT], ctx->rhs4[RHS_U], ctx->rhs4[RHS_V], ctx->rhs4[RHS_W], ctx->rhs4[RHS_T], bxu, bxt, W);
+#define XF_CASE(EE)                                                                                                            \
+  case EE:                                                                                                                     \
+    hipLaunchKernelGGL(k_explicit_xsolve<EE>, dim3(blocks), dim3(nt), shm, ctx->stream, A, cf, ctx->f[MELT_RU1], ctx->f[MELT_RU2], \
+                       ctx->f[MELT_RU3], ctx->f[MELT_RUT], ctx->rhs4[RHS_U], ctx->rhs4[RHS_V], ctx->rhs4[RHS_W], ctx->rhs4[RHS_T], bxu,  \
+                       bxt, W);                                                                                                \
+    break;
+  switch (E) {
+    XF_CASE(1) XF_CASE(2) XF_CASE(3) XF_CASE(4) XF_CASE(6) XF_CASE(8)
+    default: return melt_fail(ctx, MELT_ERR_ARG, "explicit_xsolve", "unsupported tile");
+  }
+#undef XF_CASE
   MELT_HIP(hipGetLastError());
   return 0;
 }
@@ -574,12 +591,13 @@ int euler_launch_solve(melt_ctx* ctx, int dir, double* const* fields, int nfield
   int n[3] = {ctx->n1, ctx->n2, ctx->n3};
   int M = n[dir];
   int other = dir == 0 ? ctx->n2 : ctx->n1;
-  int W = tile_width(M, other);
-  // threads per tile: a divisor of M*W, at most 256, so every element is owned
-  int E = (M * W + 255) / 256;
-  while ((M * W) % E) ++E;
-  if (E > PCR_EMAX) return melt_fail(ctx, MELT_ERR_ARG, "line solve", "unsupported line length");
-  int nt = (M * W) / E;
+  // W lines per tile: 16 by default (128-byte rows for y/z), at most 8 elements per thread
+  int W = ctx->tune[TUNE_SOLVE_W] > 0 ? ctx->tune[TUNE_SOLVE_W] : 16;
+  while (W > 1 && (other % W || M * W > 8 * 1024)) W /= 2;
+  if (other % W) W = 1;
+  int nt = pick_threads(M * W, 1024, ctx->tune[TUNE_SOLVE_NT] > 0 ? ctx->tune[TUNE_SOLVE_NT] : 256);
+  if (!nt) return melt_fail(ctx, MELT_ERR_ARG, "line solve", "unsupported line length");
+  int E = (M * W) / nt;
   SolveBatch B{};
   for (int q = 0; q < nfield; ++q) {
     B.f[q].rhs = fields[q];
@@ -589,10 +607,20 @@ int euler_launch_solve(melt_ctx* ctx, int dir, double* const* fields, int nfield
   }
   size_t tiles = dir == 0 ? (size_t)(ctx->n2 / W) * ctx->n3 : dir == 1 ? (size_t)(ctx->n1 / W) * ctx->n3 : (size_t)(ctx->n1 / W) * ctx->n2;
   size_t shm = dir == 0 ? sizeof(double) * (size_t)W * (M + 1) : sizeof(double) * (size_t)W * M;
+  if (shm > 64 * 1024) return melt_fail(ctx, MELT_ERR_ARG, "line solve", "tile exceeds LDS");
   dim3 grid((unsigned)tiles, nfield);
-  if (dir == 0) hipLaunchKernelGGL(k_pcr<0>, grid, dim3(nt), shm, ctx->stream, B, M, W, ctx->n1, ctx->n2, ctx->n3);
-  else if (dir == 1) hipLaunchKernelGGL(k_pcr<1>, grid, dim3(nt), shm, ctx->stream, B, M, W, ctx->n1, ctx->n2, ctx->n3);
-  else hipLaunchKernelGGL(k_pcr<2>, grid, dim3(nt), shm, ctx->stream, B, M, W, ctx->n1, ctx->n2, ctx->n3);
+#define PCR_CASE(D, EE) \
+  case EE: hipLaunchKernelGGL((k_pcr<D, EE>), grid, dim3(nt), shm, ctx->stream, B, M, W, ctx->n1, ctx->n2, ctx->n3); break;
+#define PCR_DIR(D)                                                                                          \
+  switch (E) {                                                                                              \
+    PCR_CASE(D, 1) PCR_CASE(D, 2) PCR_CASE(D, 3) PCR_CASE(D, 4) PCR_CASE(D, 6) PCR_CASE(D, 8)                 \
+    default: return melt_fail(ctx, MELT_ERR_ARG, "line solve", "unsupported tile");                          \
+  }
+  if (dir == 0) { PCR_DIR(0) }
+  else if (dir == 1) { PCR_DIR(1) }
+  else { PCR_DIR(2) }
+#undef PCR_DIR
+#undef PCR_CASE
   MELT_HIP(hipGetLastError());
   return 0;
 }

@@ -51,24 +51,35 @@ __device__ inline void rank1(double A[16], double w, const double p[4]) {
 
 // MLS shape function over the 3x3x3 support i0..i0+2, j0.., k0.. (wght1/2/3/temp,
 // mlsWeight.f90:48-412); c[d] selects face (0) or centre (1) coordinates.
-__device__ int mls_phi(const Grid& g, const double pos[3], int i0, int j0, int k0, const int c[3], double phi[27]) {
+// sink(inw, phi) receives the 27 values in the reference's k, j, i order.
+template <class Sink>
+__device__ int mls_phi(const Grid& g, const double pos[3], int i0, int j0, int k0, const int c[3], Sink sink) {
   double A[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) A[q] = 0.0;
-  for (int k = k0; k <= k0 + 2; ++k) {
-    double zk = gco(g, 2, c[2], k);
-    double w3 = mls_gaussian(fabs(zk - pos[2]) / (WSCL / g.dx3));
-    for (int j = j0; j <= j0 + 2; ++j) {
-      double yj = gco(g, 1, c[1], j);
-      double w23 = mls_gaussian(fabs(yj - pos[1]) / (WSCL / g.dx2)) * w3;
-      for (int i = i0; i <= i0 + 2; ++i) {
-        double xi = gco(g, 0, c[0], i);
-        double wtx = mls_gaussian(fabs(xi - pos[0]) / (WSCL / g.dx1)) * w23;
-        double p[4] = {1.0, xi, yj, zk};
+  // separable Gaussian factors, evaluated once per axis
+  double xs[3], ys[3], zs[3], wx[3], wy[3], wz[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    xs[q] = gco(g, 0, c[0], i0 + q);
+    wx[q] = mls_gaussian(fabs(xs[q] - pos[0]) / (WSCL / g.dx1));
+    ys[q] = gco(g, 1, c[1], j0 + q);
+    wy[q] = mls_gaussian(fabs(ys[q] - pos[1]) / (WSCL / g.dx2));
+    zs[q] = gco(g, 2, c[2], k0 + q);
+    wz[q] = mls_gaussian(fabs(zs[q] - pos[2]) / (WSCL / g.dx3));
+  }
+#pragma unroll 1
+  for (int kq = 0; kq < 3; ++kq)
+#pragma unroll
+    for (int jq = 0; jq < 3; ++jq) {
+      double w23 = wy[jq] * wz[kq];
+#pragma unroll
+      for (int iq = 0; iq < 3; ++iq) {
+        double wtx = wx[iq] * w23;
+        double p[4] = {1.0, xs[iq], ys[jq], zs[kq]};
         rank1(A, wtx, p);
       }
     }
-  }
   int info = lu_inverse<4>(A);
   if (info) return info;
   double ptx[4] = {1.0, pos[0], pos[1], pos[2]}, ptxA[4];
@@ -79,24 +90,21 @@ __device__ int mls_phi(const Grid& g, const double pos[3], int i0, int j0, int k
     for (int l = 0; l < 4; ++l) s = s + (1.0 * A[l + 4 * j]) * ptx[l];
     ptxA[j] = s;
   }
-  int inw = 0;
-  for (int k = k0; k <= k0 + 2; ++k) {
-    double zk = gco(g, 2, c[2], k);
-    double w3 = mls_gaussian(fabs(zk - pos[2]) / (WSCL / g.dx3));
-    for (int j = j0; j <= j0 + 2; ++j) {
-      double yj = gco(g, 1, c[1], j);
-      double w23 = mls_gaussian(fabs(yj - pos[1]) / (WSCL / g.dx2)) * w3;
-      for (int i = i0; i <= i0 + 2; ++i) {
-        double xi = gco(g, 0, c[0], i);
-        double wtx = mls_gaussian(fabs(xi - pos[0]) / (WSCL / g.dx1)) * w23;
-        double b[4] = {wtx * 1.0, wtx * xi, wtx * yj, wtx * zk};
+#pragma unroll 1
+  for (int kq = 0; kq < 3; ++kq)
+#pragma unroll
+    for (int jq = 0; jq < 3; ++jq) {
+      double w23 = wy[jq] * wz[kq];
+#pragma unroll
+      for (int iq = 0; iq < 3; ++iq) {
+        double wtx = wx[iq] * w23;
+        double b[4] = {wtx * 1.0, wtx * xs[iq], wtx * ys[jq], wtx * zs[kq]};
         double s = 0.0;
 #pragma unroll
         for (int l = 0; l < 4; ++l) s = s + (1.0 * b[l]) * ptxA[l];
-        phi[inw++] = s;
+        sink(9 * kq + 3 * jq + iq, s);
       }
     }
-  }
   return 0;
 }
 
@@ -107,40 +115,63 @@ __device__ inline double dweight(double pc, double gcrd, double nrm, double scal
 }
 
 // MLS gradient at a probe (wght_gradT mls_heatFlux.f90:85-300 and
-// wght_gradP/U/V/W mls_structLoads.f90:380-1297).  kz: grid indices for the z
-// coordinate (the reference uses k1-1, k1, k1); ke: field planes (1-based,
-// ghost-plane semantics) k1-1, k1, k1+1.
-__device__ int mls_grad(const Grid& g, const double* __restrict__ fld, const double pos[3], int ic, int jc, const int kz[3], const int ke[3],
-                        const int c[3], double grad[3]) {
-  double A[16], dA[3][16];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) { A[q] = 0.0; dA[0][q] = 0.0; dA[1][q] = 0.0; dA[2][q] = 0.0; }
+// wght_gradP/U/V/W mls_structLoads.f90:380-1297), computed by the 4 lanes of
+// one probe: lane 0 accumulates the moment matrix A, lane 1+d the derivative
+// matrix dA/dx_d; after A is shared through shA (16 doubles, block barrier),
+// lanes 1..3 each invert A and form gradient component d in the reference's
+// point order, so every value rounds as in the single-thread formulation.
+// kz: grid indices used for the z coordinate (the reference uses k1-1, k1,
+// k1); ke: field planes (1-based, ghost-plane semantics) k1-1, k1, k1+1.
+// Must be reached by every thread of the block (it contains a barrier);
+// active = 0 lanes only take part in the barrier.  Returns the LU info of A
+// on lanes 1..3 (0 elsewhere); *grad_d receives component lane-1.
+__device__ int mls_grad_lanes(const Grid& g, const double* __restrict__ fld, const double pos[3], int ic, int jc, const int kz[3],
+                              const int ke[3], const int c[3], int lane, bool active, double* shA, double* grad_d) {
   const double sx = WSCL / g.dx1, sy = WSCL / g.dx2, sz = WSCL / g.dx3;
-  for (int nk = 0; nk < 3; ++nk) {
-    double zk = gco(g, 2, c[2], kz[nk]);
-    double n3 = fabs(pos[2] - zk) / sz;
-    double w3 = mls_gaussian(n3), dw3 = dweight(pos[2], zk, n3, sz);
-    for (int j = jc - 1; j <= jc + 1; ++j) {
-      double yj = gco(g, 1, c[1], j);
-      double n2 = fabs(pos[1] - yj) / sy;
-      double w2 = mls_gaussian(n2), dw2 = dweight(pos[1], yj, n2, sy);
-      double w23 = w2 * w3;
-      for (int i = ic - 1; i <= ic + 1; ++i) {
-        double xi = gco(g, 0, c[0], i);
-        double n1 = fabs(pos[0] - xi) / sx;
-        double w1 = mls_gaussian(n1), dw1 = dweight(pos[0], xi, n1, sx);
-        double wtx = w1 * w23;
-        double p[4] = {1.0, xi, yj, zk};
-        rank1(A, wtx, p);
-        rank1(dA[0], dw1 * w2 * w3, p);
-        rank1(dA[1], w1 * dw2 * w3, p);
-        rank1(dA[2], w1 * w2 * dw3, p);
-      }
+  double xs[3], ys[3], zs[3], w1[3], w2[3], w3[3], dw1[3], dw2[3], dw3[3];
+  double M[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) M[q] = 0.0;
+  if (active) {
+    // separable Gaussian factors and derivatives, evaluated once per axis
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      xs[q] = gco(g, 0, c[0], ic - 1 + q);
+      double n1 = fabs(pos[0] - xs[q]) / sx;
+      w1[q] = mls_gaussian(n1); dw1[q] = dweight(pos[0], xs[q], n1, sx);
+      ys[q] = gco(g, 1, c[1], jc - 1 + q);
+      double n2 = fabs(pos[1] - ys[q]) / sy;
+      w2[q] = mls_gaussian(n2); dw2[q] = dweight(pos[1], ys[q], n2, sy);
+      zs[q] = gco(g, 2, c[2], kz[q]);
+      double n3 = fabs(pos[2] - zs[q]) / sz;
+      w3[q] = mls_gaussian(n3); dw3[q] = dweight(pos[2], zs[q], n3, sz);
     }
+#pragma unroll 1
+    for (int nk = 0; nk < 3; ++nk)
+#pragma unroll
+      for (int jq = 0; jq < 3; ++jq)
+#pragma unroll
+        for (int iq = 0; iq < 3; ++iq) {
+          double p[4] = {1.0, xs[iq], ys[jq], zs[nk]};
+          double wgt;
+          if (lane == 0) wgt = w1[iq] * (w2[jq] * w3[nk]);
+          else if (lane == 1) wgt = dw1[iq] * w2[jq] * w3[nk];
+          else if (lane == 2) wgt = w1[iq] * dw2[jq] * w3[nk];
+          else wgt = w1[iq] * w2[jq] * dw3[nk];
+          rank1(M, wgt, p);
+        }
+    if (lane == 0)
+#pragma unroll
+      for (int q = 0; q < 16; ++q) shA[q] = M[q];
   }
+  __syncthreads();
+  if (!active || lane == 0) return 0;
+  double A[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) A[q] = shA[q];
   int info = lu_inverse<4>(A);
   if (info) return info;
-  double ptx[4] = {1.0, pos[0], pos[1], pos[2]}, gam[4], dg[3][4];
+  double ptx[4] = {1.0, pos[0], pos[1], pos[2]}, gam[4], dg[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) gam[i] = 0.0;
 #pragma unroll
@@ -149,56 +180,42 @@ __device__ int mls_grad(const Grid& g, const double* __restrict__ fld, const dou
 #pragma unroll
     for (int i = 0; i < 4; ++i) gam[i] = gam[i] + t * A[i + 4 * j];
   }
+  const int d = lane - 1;
+  double pv[4] = {0.0, 0.0, 0.0, 0.0};
+  pv[d + 1] = 1.0;
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    double pv[4] = {0.0, 0.0, 0.0, 0.0};
-    pv[d + 1] = 1.0;
+  for (int j = 0; j < 4; ++j) {
+    double t = -1.0 * gam[j];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      double t = -1.0 * gam[j];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) pv[i] = pv[i] + t * dA[d][i + 4 * j];
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) dg[d][i] = 0.0;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      double t = 1.0 * pv[j];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) dg[d][i] = dg[d][i] + t * A[i + 4 * j];
-    }
+    for (int i = 0; i < 4; ++i) pv[i] = pv[i] + t * M[i + 4 * j];
   }
-  double s[3] = {0.0, 0.0, 0.0};
-  for (int nk = 0; nk < 3; ++nk) {
-    double zk = gco(g, 2, c[2], kz[nk]);
-    double n3 = fabs(pos[2] - zk) / sz;
-    double w3 = mls_gaussian(n3), dw3 = dweight(pos[2], zk, n3, sz);
-    for (int j = jc - 1; j <= jc + 1; ++j) {
-      double yj = gco(g, 1, c[1], j);
-      double n2 = fabs(pos[1] - yj) / sy;
-      double w2 = mls_gaussian(n2), dw2 = dweight(pos[1], yj, n2, sy);
-      double w23 = w2 * w3;
-      for (int i = ic - 1; i <= ic + 1; ++i) {
-        double xi = gco(g, 0, c[0], i);
-        double n1 = fabs(pos[0] - xi) / sx;
-        double w1 = mls_gaussian(n1), dw1 = dweight(pos[0], xi, n1, sx);
-        double wtx = w1 * w23;
-        double p[4] = {1.0, xi, yj, zk};
-        double dw[3] = {dw1 * w2 * w3, w1 * dw2 * w3, w1 * w2 * dw3};
-        double val = fld[widx(g, i, j, ke[nk])];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          double acc = 0.0;
+  for (int i = 0; i < 4; ++i) dg[i] = 0.0;
 #pragma unroll
-          for (int l = 0; l < 4; ++l) acc = acc + (1.0 * (wtx * p[l])) * dg[d][l];
+  for (int j = 0; j < 4; ++j) {
+    double t = 1.0 * pv[j];
 #pragma unroll
-          for (int l = 0; l < 4; ++l) acc = acc + (1.0 * (dw[d] * p[l])) * gam[l];
-          s[d] = s[d] + acc * val;
-        }
+    for (int i = 0; i < 4; ++i) dg[i] = dg[i] + t * A[i + 4 * j];
+  }
+  double s = 0.0;
+#pragma unroll 1
+  for (int nk = 0; nk < 3; ++nk)
+#pragma unroll
+    for (int jq = 0; jq < 3; ++jq)
+#pragma unroll
+      for (int iq = 0; iq < 3; ++iq) {
+        double wtx = w1[iq] * (w2[jq] * w3[nk]);
+        double p[4] = {1.0, xs[iq], ys[jq], zs[nk]};
+        double dw = d == 0 ? dw1[iq] * w2[jq] * w3[nk] : d == 1 ? w1[iq] * dw2[jq] * w3[nk] : w1[iq] * w2[jq] * dw3[nk];
+        double val = fld[widx(g, ic - 1 + iq, jc - 1 + jq, ke[nk])];
+        double acc = 0.0;
+#pragma unroll
+        for (int l = 0; l < 4; ++l) acc = acc + (1.0 * (wtx * p[l])) * dg[l];
+#pragma unroll
+        for (int l = 0; l < 4; ++l) acc = acc + (1.0 * (dw * p[l])) * gam[l];
+        s = s + acc * val;
       }
-    }
-  }
-  grad[0] = s[0]; grad[1] = s[1]; grad[2] = s[2];
+  *grad_d = s;
   return 0;
 }
 
@@ -225,40 +242,55 @@ struct Win3 { int e[3]; };
 // ------------------------------------------------------------ windows
 // Face base indices floor(x*dx)+1 (unwrapped) bound every cell the MLS
 // supports (base-1..base+2) and the tagging band (base-2..base+2) touch.
-__global__ __launch_bounds__(256) void k_window(Grid g, const double* __restrict__ bar, const unsigned char* __restrict__ gf, int nf,
-                                                melt_window* win, Win3 ef, Win3 et, int which, int* err) {
+// Min / max of the face base indices over all live faces: one block per 256
+// faces, combined with integer atomics (order-independent).
+__global__ __launch_bounds__(256) void k_window_box(Grid g, const double* __restrict__ bar, const unsigned char* __restrict__ gf, int nf,
+                                                    melt_window* win) {
   __shared__ int mn[3], mx[3];
   if (threadIdx.x < 3) { mn[threadIdx.x] = INT_MAX; mx[threadIdx.x] = INT_MIN; }
   __syncthreads();
-  int lmn[3] = {INT_MAX, INT_MAX, INT_MAX}, lmx[3] = {INT_MIN, INT_MIN, INT_MIN};
-  double dx[3] = {g.dx1, g.dx2, g.dx3};
-  for (int f = threadIdx.x; f < nf; f += blockDim.x) {
-    if (gf[f]) continue;
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f < nf && !gf[f]) {
+    const double dx[3] = {g.dx1, g.dx2, g.dx3};
     for (int d = 0; d < 3; ++d) {
       int b = (int)floor(bar[3 * f + d] * dx[d]) + 1;
-      lmn[d] = min(lmn[d], b); lmx[d] = max(lmx[d], b);
+      atomicMin(&mn[d], b);
+      atomicMax(&mx[d], b);
     }
   }
-  for (int d = 0; d < 3; ++d) { atomicMin(&mn[d], lmn[d]); atomicMax(&mx[d], lmx[d]); }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    if (which & 1) {  // forcing window
-      for (int d = 0; d < 3; ++d) {
-        win[0].org[d] = mn[d] - 2;
-        if (mx[d] + 3 - win[0].org[d] >= ef.e[d]) raise_error(err, MELT_ERR_WINDOW);
-      }
+  if (threadIdx.x < 3) {
+    atomicMin(&win->bmin[threadIdx.x], mn[threadIdx.x]);
+    atomicMax(&win->bmax[threadIdx.x], mx[threadIdx.x]);
+  }
+}
+
+__global__ void k_window_reset(melt_window* win, int n) {
+  for (int w = 0; w < n; ++w)
+    for (int d = 0; d < 3; ++d) { win[w].bmin[d] = INT_MAX; win[w].bmax[d] = INT_MIN; }
+}
+
+// Place the forcing (which = 1: win[0]) or tagging (which = 2: win[1]) box
+// from the reduced face extent, then reset the extent for the next call.
+__global__ void k_window_final(melt_window* win, Win3 ef, Win3 et, int which, int* err) {
+  melt_window& w = win[which == 1 ? 0 : 1];
+  int mn[3], mx[3];
+  for (int d = 0; d < 3; ++d) { mn[d] = w.bmin[d]; mx[d] = w.bmax[d]; w.bmin[d] = INT_MAX; w.bmax[d] = INT_MIN; }
+  if (which == 1) {  // forcing window
+    for (int d = 0; d < 3; ++d) {
+      w.org[d] = mn[d] - 2;
+      if (mx[d] + 3 - w.org[d] >= ef.e[d]) raise_error(err, MELT_ERR_WINDOW);
     }
-    if (which & 2) {  // tagging window
-      if (win[1].prev_valid >= 0) {
-        for (int d = 0; d < 3; ++d) win[1].prev_org[d] = win[1].org[d];
-        win[1].prev_valid = win[1].pad;
-      }
-      for (int d = 0; d < 3; ++d) {
-        win[1].org[d] = mn[d] - 2;
-        if (mx[d] + 2 - win[1].org[d] >= et.e[d]) raise_error(err, MELT_ERR_WINDOW);
-      }
-      win[1].pad = 1;  // current box is valid from now on
+  } else {  // tagging window
+    if (w.prev_valid >= 0) {
+      for (int d = 0; d < 3; ++d) w.prev_org[d] = w.org[d];
+      w.prev_valid = w.pad;
     }
+    for (int d = 0; d < 3; ++d) {
+      w.org[d] = mn[d] - 2;
+      if (mx[d] + 2 - w.org[d] >= et.e[d]) raise_error(err, MELT_ERR_WINDOW);
+    }
+    w.pad = 1;  // current box is valid from now on
   }
 }
 
@@ -585,10 +617,8 @@ __global__ __launch_bounds__(64) void k_mls_weights(Grid g, const double* __rest
   else if (comp == 1) { i0 = p[0] - 1; j0 = p[4] - 1; k0 = k1 - 1; c[0] = 1; c[1] = 0; c[2] = 1; }
   else if (comp == 2) { i0 = p[0] - 1; j0 = p[1] - 1; k0 = k1 - 1; c[0] = 1; c[1] = 1; c[2] = 1; }
   else { i0 = p[0] - 1; j0 = p[1] - 1; k0 = kst - 1; c[0] = 1; c[1] = 1; c[2] = 0; }
-  double phi[27];
-  if (mls_phi(g, pos, i0, j0, k0, c, phi)) { raise_error(err, MELT_ERR_SINGULAR_MLS); return; }
   double* o = &out.q[comp][27 * (size_t)f];
-  for (int q = 0; q < 27; ++q) o[q] = phi[q];
+  if (mls_phi(g, pos, i0, j0, k0, c, [&](int q, double v) { o[q] = v; })) raise_error(err, MELT_ERR_SINGULAR_MLS);
 }
 
 struct ForceArgs {
@@ -679,15 +709,26 @@ __global__ __launch_bounds__(256) void k_force_apply(Grid g, const melt_window* 
     for (int m = 0; m < 3; ++m) { part[(blockIdx.x * 3 + m) * 2] = slo[m][0]; part[(blockIdx.x * 3 + m) * 2 + 1] = shi[m][0]; }
 }
 
-__global__ void k_fmean_final(const unsigned long long* __restrict__ part, int nb, double ncell, double* __restrict__ mean) {
-  int m = threadIdx.x;
-  if (m >= 3) return;
-  unsigned long long lo = 0, hi = 0;
-  for (int b = 0; b < nb; ++b) {
-    unsigned long long l2 = part[(b * 3 + m) * 2], h2 = part[(b * 3 + m) * 2 + 1];
-    acc_add_words(lo, hi, l2, h2);
+// Exact sum of the per-block partial force sums (integer adds: any order
+// gives the same words), divided by the cell count.
+__global__ __launch_bounds__(256) void k_fmean_final(const unsigned long long* __restrict__ part, int nb, double ncell,
+                                                     double* __restrict__ mean) {
+  __shared__ unsigned long long slo[3][256], shi[3][256];
+  unsigned long long lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+    for (int m = 0; m < 3; ++m) acc_add_words(lo[m], hi[m], part[(b * 3 + m) * 2], part[(b * 3 + m) * 2 + 1]);
+  for (int m = 0; m < 3; ++m) { slo[m][threadIdx.x] = lo[m]; shi[m][threadIdx.x] = hi[m]; }
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int m = 0; m < 3; ++m) {
+        unsigned long long l = slo[m][threadIdx.x], h = shi[m][threadIdx.x];
+        acc_add_words(l, h, slo[m][threadIdx.x + w], shi[m][threadIdx.x + w]);
+        slo[m][threadIdx.x] = l; shi[m][threadIdx.x] = h;
+      }
+    __syncthreads();
   }
-  mean[m] = acc_value(lo, hi) / ncell;
+  if (threadIdx.x < 3) mean[threadIdx.x] = acc_value(slo[threadIdx.x][0], shi[threadIdx.x][0]) / ncell;
 }
 
 // velforce.f90:21-51 on full fields (per-routine entry point)
@@ -742,64 +783,108 @@ struct ProbeArgs {
   const unsigned char* gf;
   int nf;
   double h, pec;
-  int kind_lo, kind_hi;
+  int heat, loads;  // imelt == 1 (kinds 0, 1) / imlsstr == 1 (kinds 2..6)
 };
 
-__global__ __launch_bounds__(64) void k_probes(Grid g, ProbeArgs P, double* __restrict__ out, int* __restrict__ pind, int* err) {
-  int nk = P.kind_hi - P.kind_lo;
-  int id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= P.nf * nk) return;
-  int f = id / nk, kind = P.kind_lo + id % nk;
-  if (P.gf[f]) return;
+// Probe point of face f on the fluid (kind != 1) or solid (kind 1) side:
+// bar +- h n/|n| (mls_heatFlux.f90:33-60, mls_structLoads.f90:40-80).
+__device__ inline void probe_pos(const ProbeArgs& P, int f, int kind, double pos[3]) {
   const double* nor = &P.nor[3 * f];
   const double* bar = &P.bar[3 * f];
-  double s = fnorm2(nor, 3), pos[3];
+  double s = fnorm2(nor, 3);
   double sg = kind == 1 ? -1.0 : 1.0;
   for (int d = 0; d < 3; ++d) pos[d] = sg > 0 ? bar[d] + P.h * nor[d] / s : bar[d] - P.h * nor[d] / s;
-  int icent = (int)floor(pos[0] * g.dx1) + 1, jcent = (int)floor(pos[1] * g.dx2) + 1;
-  int istag = fnint(pos[0] * g.dx1) + 1, jstag = fnint(pos[1] * g.dx2) + 1;
-  int k1 = (int)floor(pos[2] * g.dx3) + 1, kst = fnint(pos[2] * g.dx3) + 1;
-  int kw = pmod(k1 - 1, g.n3) + 1, kws = pmod(kst - 1, g.n3) + 1;
-  int kz[3] = {k1 - 1, k1, k1}, ke[3] = {kw - 1, kw, kw + 1};
-  double* o = &out[(size_t)PRB_N * f];
-  double gr[3];
-  int info = 0;
-  if (kind <= 1) {
-    const int cc[3] = {1, 1, 1};
-    info = mls_grad(g, P.temp, pos, icent, jcent, kz, ke, cc, gr);
-    double q = nor[0] * gr[0] + nor[1] * gr[1] + nor[2] * gr[2];
-    o[kind == 0 ? PRB_QO : PRB_QI] = q / P.pec;
-    int* pi = &pind[6 * f + 3 * kind];
-    pi[0] = icent; pi[1] = jcent; pi[2] = kw;
-  } else if (kind == 2) {
-    const int cc[3] = {1, 1, 1};
-    double phi[27];
-    info = mls_phi(g, pos, icent - 1, jcent - 1, k1 - 1, cc, phi);
-    double pp = 0.0;
-    int inw = 0;
-    for (int kk = 0; kk < 3; ++kk)
-      for (int j = jcent - 1; j <= jcent + 1; ++j)
-        for (int i = icent - 1; i <= icent + 1; ++i) pp = pp + P.pr[widx(g, i, j, kw - 1 + kk)] * phi[inw++];
-    o[PRB_P] = pp;
-  } else if (kind == 3) {
-    const int cc[3] = {1, 1, 1};
-    info = mls_grad(g, P.pr, pos, icent, jcent, kz, ke, cc, gr);
-    for (int d = 0; d < 3; ++d) o[PRB_GP + d] = gr[d];
-  } else if (kind == 4) {
-    const int cc[3] = {0, 1, 1};
-    info = mls_grad(g, P.vx, pos, istag, jcent, kz, ke, cc, gr);
-    for (int d = 0; d < 3; ++d) o[PRB_GU + d] = gr[d];
-  } else if (kind == 5) {
-    const int cc[3] = {1, 0, 1};
-    info = mls_grad(g, P.vy, pos, icent, jstag, kz, ke, cc, gr);
-    for (int d = 0; d < 3; ++d) o[PRB_GV + d] = gr[d];
-  } else {
-    const int cc[3] = {1, 1, 0};
-    int kzs[3] = {kst - 1, kst, kst}, kes[3] = {kws - 1, kws, kws + 1};
-    info = mls_grad(g, P.vz, pos, icent, jcent, kzs, kes, cc, gr);
-    for (int d = 0; d < 3; ++d) o[PRB_GW + d] = gr[d];
+}
+
+// Gradient probes: kinds 0/1 (temperature, fluid/solid side), 3 (pressure),
+// 4/5/6 (u, v, w); four lanes per (face, kind), kinds major so that a wave
+// runs one kind.
+__device__ __forceinline__ int grad_kind(int q) { return q < 2 ? q : q + 1; }  // 0, 1, 3, 4, 5, 6
+
+__global__ __launch_bounds__(256) void k_probes_grad(Grid g, ProbeArgs P, double* __restrict__ out, int* __restrict__ pind, int klo,
+                                                     int nkinds, int* err) {
+  __shared__ double shA[64][16];
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  const int item = id >> 2, lane = id & 3, slot = threadIdx.x >> 2;
+  const bool inrange = item < P.nf * nkinds;
+  const int f = inrange ? item % P.nf : 0;
+  const int kind = grad_kind(klo + (inrange ? item / P.nf : 0));
+  const bool active = inrange && !P.gf[f];
+  double pos[3] = {0.0, 0.0, 0.0};
+  int ic = 1, jc = 1, kz[3] = {1, 1, 1}, ke[3] = {1, 1, 1}, cc[3] = {1, 1, 1};
+  const double* fld = P.temp;
+  if (active) {
+    probe_pos(P, f, kind, pos);
+    int icent = (int)floor(pos[0] * g.dx1) + 1, jcent = (int)floor(pos[1] * g.dx2) + 1;
+    int istag = fnint(pos[0] * g.dx1) + 1, jstag = fnint(pos[1] * g.dx2) + 1;
+    int k1 = (int)floor(pos[2] * g.dx3) + 1, kst = fnint(pos[2] * g.dx3) + 1;
+    int kw = pmod(k1 - 1, g.n3) + 1, kws = pmod(kst - 1, g.n3) + 1;
+    ic = icent; jc = jcent;
+    kz[0] = k1 - 1; kz[1] = k1; kz[2] = k1;
+    ke[0] = kw - 1; ke[1] = kw; ke[2] = kw + 1;
+    if (kind <= 1) {
+      fld = P.temp;
+      if (lane == 0) { int* pi = &pind[6 * f + 3 * kind]; pi[0] = icent; pi[1] = jcent; pi[2] = kw; }
+    } else if (kind == 3) {
+      fld = P.pr;
+    } else if (kind == 4) {
+      fld = P.vx; ic = istag; cc[0] = 0;
+    } else if (kind == 5) {
+      fld = P.vy; jc = jstag; cc[1] = 0;
+    } else {
+      fld = P.vz; cc[2] = 0;
+      kz[0] = kst - 1; kz[1] = kst; kz[2] = kst;
+      ke[0] = kws - 1; ke[1] = kws; ke[2] = kws + 1;
+    }
   }
-  if (info) raise_error(err, MELT_ERR_SINGULAR_MLS);
+  double gd = 0.0;
+  int info = mls_grad_lanes(g, fld, pos, ic, jc, kz, ke, cc, lane, active, shA[slot], &gd);
+  if (!active || lane == 0) return;
+  if (info) { raise_error(err, MELT_ERR_SINGULAR_MLS); return; }
+  double* o = &out[(size_t)PRB_N * f];
+  const int d = lane - 1;
+  if (kind <= 1) {
+    o[(kind == 0 ? PRB_GT0 : PRB_GT1) + d] = gd;
+  } else if (kind == 3) {
+    o[PRB_GP + d] = gd;
+  } else if (kind == 4) {
+    o[PRB_GU + d] = gd;
+  } else if (kind == 5) {
+    o[PRB_GV + d] = gd;
+  } else {
+    o[PRB_GW + d] = gd;
+  }
+}
+
+// Pressure interpolation probe (kind 2) and the heat-flux dot products of
+// kinds 0/1 (q = n . grad T / Pe), one thread per face.
+__global__ __launch_bounds__(64) void k_probes_point(Grid g, ProbeArgs P, double* __restrict__ out, int* err) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= P.nf || P.gf[f]) return;
+  double* o = &out[(size_t)PRB_N * f];
+  if (P.heat) {
+    const double* nor = &P.nor[3 * f];
+    for (int kind = 0; kind < 2; ++kind) {
+      const double* gr = &o[kind == 0 ? PRB_GT0 : PRB_GT1];
+      double q = nor[0] * gr[0] + nor[1] * gr[1] + nor[2] * gr[2];
+      o[kind == 0 ? PRB_QO : PRB_QI] = q / P.pec;
+    }
+  }
+  if (!P.loads) return;
+  double pos[3];
+  probe_pos(P, f, 2, pos);
+  int icent = (int)floor(pos[0] * g.dx1) + 1, jcent = (int)floor(pos[1] * g.dx2) + 1;
+  int k1 = (int)floor(pos[2] * g.dx3) + 1;
+  int kw = pmod(k1 - 1, g.n3) + 1;
+  const int cc[3] = {1, 1, 1};
+  double pp = 0.0;
+  const double* pr = P.pr;
+  int info = mls_phi(g, pos, icent - 1, jcent - 1, k1 - 1, cc, [&](int q, double phi) {
+    int kk = q / 9, j = jcent - 1 + (q / 3) % 3, i = icent - 1 + q % 3;
+    pp = pp + pr[widx(g, i, j, kw - 1 + kk)] * phi;
+  });
+  if (info) { raise_error(err, MELT_ERR_SINGULAR_MLS); return; }
+  o[PRB_P] = pp;
 }
 
 inline unsigned nb(size_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -863,8 +948,9 @@ int ibm_tag(melt_ctx* ctx) {  // tagCells.f90:4-27 (tagType 1) + interp_vof when
     MELT_HIP(hipMemcpyAsync(&ctx->d_body->initial_tag, &ctx->h_body.initial_tag, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
   }
   Win3 ef{{ctx->fwin[0], ctx->fwin[1], ctx->fwin[2]}}, et{{ctx->twin[0], ctx->twin[1], ctx->twin[2]}};
-  hipLaunchKernelGGL(k_window, dim3(1), dim3(256), 0, ctx->stream, g, ctx->d_lag[MELT_L_TRI_BAR], ctx->d_gf, ctx->nf, ctx->d_win, ef, et, 2,
-                     ctx->d_err);
+  hipLaunchKernelGGL(k_window_box, dim3(nb(ctx->nf, 256)), dim3(256), 0, ctx->stream, g, ctx->d_lag[MELT_L_TRI_BAR], ctx->d_gf, ctx->nf,
+                     ctx->d_win + 1);
+  hipLaunchKernelGGL(k_window_final, dim3(1), dim3(1), 0, ctx->stream, ctx->d_win, ef, et, 2, ctx->d_err);
   hipLaunchKernelGGL(k_tag_reset, dim3(256), dim3(256), 0, ctx->stream, g, ctx->f[MELT_SDF], ctx->d_win, et);
   hipLaunchKernelGGL(k_tag_init, dim3(256), dim3(256), 0, ctx->stream, ctx->d_tkey, ctx->d_tface, ctx->twin_cells);
   MeshArgs M = mesh_args(ctx);
@@ -895,8 +981,9 @@ int ibm_weights(melt_ctx* ctx) {
 int ibm_force_scatter(melt_ctx* ctx) {
   Grid g = make_grid(ctx);
   Win3 ef{{ctx->fwin[0], ctx->fwin[1], ctx->fwin[2]}}, et{{ctx->twin[0], ctx->twin[1], ctx->twin[2]}};
-  hipLaunchKernelGGL(k_window, dim3(1), dim3(256), 0, ctx->stream, g, ctx->d_lag[MELT_L_TRI_BAR], ctx->d_gf, ctx->nf, ctx->d_win, ef, et, 1,
-                     ctx->d_err);
+  hipLaunchKernelGGL(k_window_box, dim3(nb(ctx->nf, 256)), dim3(256), 0, ctx->stream, g, ctx->d_lag[MELT_L_TRI_BAR], ctx->d_gf, ctx->nf,
+                     ctx->d_win);
+  hipLaunchKernelGGL(k_window_final, dim3(1), dim3(1), 0, ctx->stream, ctx->d_win, ef, et, 1, ctx->d_err);
   ForceArgs F;
   F.fld[0] = ctx->f[MELT_VX]; F.fld[1] = ctx->f[MELT_VY]; F.fld[2] = ctx->f[MELT_TEMP]; F.fld[3] = ctx->f[MELT_VZ];
   F.phi[0] = ctx->d_lag[MELT_L_PTXAB_Q1]; F.phi[1] = ctx->d_lag[MELT_L_PTXAB_Q2];
@@ -920,7 +1007,7 @@ int ibm_force_drain(melt_ctx* ctx, int apply_now) {
   if (apply_now) {
     hipLaunchKernelGGL(k_force_apply, dim3(blocks), dim3(256), 0, ctx->stream, g, ctx->d_win, ef, ctx->d_acc, ctx->fwin_cells, ctx->f[MELT_VX],
                        ctx->f[MELT_VY], ctx->f[MELT_TEMP], ctx->f[MELT_VZ], 0, part);
-    hipLaunchKernelGGL(k_fmean_final, dim3(1), dim3(64), 0, ctx->stream, part, (int)blocks, (double)((long long)ctx->n1 * ctx->n2 * ctx->n3),
+    hipLaunchKernelGGL(k_fmean_final, dim3(1), dim3(256), 0, ctx->stream, part, (int)blocks, (double)((long long)ctx->n1 * ctx->n2 * ctx->n3),
                        ctx->d_fmean);
   } else {
     for (int id = MELT_FOR_XC; id <= MELT_FOR_TEMP; ++id) MELT_HIP(hipMemsetAsync(ctx->f[id], 0, sizeof(double) * ctx->nc, ctx->stream));
@@ -967,11 +1054,22 @@ int ibm_probes(melt_ctx* ctx) {
   P.temp = ctx->f[MELT_TEMP]; P.pr = ctx->f[MELT_PR]; P.vx = ctx->f[MELT_VX]; P.vy = ctx->f[MELT_VY]; P.vz = ctx->f[MELT_VZ];
   P.bar = ctx->d_lag[MELT_L_TRI_BAR]; P.nor = ctx->d_lag[MELT_L_TRI_NOR]; P.gf = ctx->d_gf; P.nf = ctx->nf;
   P.h = ctx->h_body.h_eulerian; P.pec = ctx->cfg.ren * ctx->cfg.prandtl;
-  P.kind_lo = ctx->cfg.imelt == 1 ? 0 : 2;
-  P.kind_hi = ctx->cfg.imlsstr == 1 ? 7 : 2;
-  if (P.kind_hi <= P.kind_lo) return 0;
-  size_t n = (size_t)ctx->nf * (P.kind_hi - P.kind_lo);
-  hipLaunchKernelGGL(k_probes, dim3(nb(n, 64)), dim3(64), 0, ctx->stream, make_grid(ctx), P, ctx->d_probe, ctx->d_pind, ctx->d_err);
+  P.heat = ctx->cfg.imelt == 1;
+  P.loads = ctx->cfg.imlsstr == 1;
+  int klo = P.heat ? 0 : 2, khi = P.loads ? 6 : 2;
+  if (khi > klo) {
+    size_t threads = (size_t)ctx->nf * (khi - klo) * 4;
+    hipLaunchKernelGGL(k_probes_grad, dim3(nb(threads, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), P, ctx->d_probe, ctx->d_pind, klo,
+                       khi - klo, ctx->d_err);
+  }
+  if (P.heat || P.loads)
+    hipLaunchKernelGGL(k_probes_point, dim3(nb(ctx->nf, 64)), dim3(64), 0, ctx->stream, make_grid(ctx), P, ctx->d_probe, ctx->d_err);
+  MELT_HIP(hipGetLastError());
+  return 0;
+}
+
+int ibm_window_reset(melt_ctx* ctx) {
+  hipLaunchKernelGGL(k_window_reset, dim3(1), dim3(1), 0, ctx->stream, ctx->d_win, 2);
   MELT_HIP(hipGetLastError());
   return 0;
 }

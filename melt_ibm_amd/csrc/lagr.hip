@@ -57,17 +57,19 @@ __device__ inline void face_rigid(const double* xyz, const int* vof, int f, doub
   o[9] = v * (z1 * y1 + z2 * y2 + z3 * y3 + z4 * y4);
 }
 
-// One block; each thread walks faces tid, tid+RB, ... in order, then a fixed
-// tree.  Output layout per mode:
+// Stage 1 of a deterministic two-stage sum: block b's threads walk faces
+// b*RB + tid, + gridDim*RB, ... in order, then a fixed tree writes the block's
+// partial to part[q * gridDim + b]; k_sum_partials adds the partials in a
+// fixed tree.  Output layout per mode:
 //  SUM_VOLUME: [vol]; SUM_RIGID: [10 rigid terms]; SUM_LOADS: [prn 3, rxp 3, tau 3, rxt 3];
 //  SUM_AREA_VOL_RIGID: [area, vol, 10 rigid terms].
 __global__ __launch_bounds__(RB) void k_face_sums(LagArgs A, const double* __restrict__ xyz, const double* __restrict__ rxp,
-                                                  const double* __restrict__ rxt, int mode, double* __restrict__ out) {
+                                                  const double* __restrict__ rxt, int mode, double* __restrict__ part) {
   __shared__ double sm[12][RB];
   const int ns = mode == SUM_VOLUME ? 1 : mode == SUM_RIGID ? 10 : 12;
   double acc[12];
   for (int q = 0; q < 12; ++q) acc[q] = 0.0;
-  for (int f = threadIdx.x; f < A.nf; f += RB) {
+  for (int f = blockIdx.x * RB + threadIdx.x; f < A.nf; f += gridDim.x * RB) {
     if (A.gf[f]) continue;
     if (mode == SUM_VOLUME) {
       acc[0] = acc[0] + face_volume(xyz, A.vof, f);
@@ -97,7 +99,31 @@ __global__ __launch_bounds__(RB) void k_face_sums(LagArgs A, const double* __res
       for (int q = 0; q < ns; ++q) sm[q][threadIdx.x] = sm[q][threadIdx.x] + sm[q][threadIdx.x + w];
     __syncthreads();
   }
-  if ((int)threadIdx.x < ns) out[threadIdx.x] = sm[threadIdx.x][0];
+  if ((int)threadIdx.x < ns) part[threadIdx.x * gridDim.x + blockIdx.x] = sm[threadIdx.x][0];
+}
+
+// Stage 2: out[q] = fixed-tree sum of the nb partials of quantity q (nb <= RB).
+__global__ __launch_bounds__(RB) void k_sum_partials(const double* __restrict__ part, int nb, int ns, double* __restrict__ out) {
+  __shared__ double sm[RB];
+  for (int q = 0; q < ns; ++q) {
+    sm[threadIdx.x] = (int)threadIdx.x < nb ? part[q * nb + threadIdx.x] : 0.0;
+    __syncthreads();
+    for (int w = RB / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) sm[threadIdx.x] = sm[threadIdx.x] + sm[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = sm[0];
+    __syncthreads();
+  }
+}
+
+// Two-stage face sum of `mode` into out[] (d_red scratch holds the partials).
+void face_sums(melt_ctx* ctx, const LagArgs& A, const double* xyz, int mode, double* out) {
+  int G = std::max(1, std::min(64, (A.nf + RB - 1) / RB));
+  int ns = mode == SUM_VOLUME ? 1 : mode == SUM_RIGID ? 10 : 12;
+  double* part = ctx->d_red + 8192;
+  hipLaunchKernelGGL(k_face_sums, dim3(G), dim3(RB), 0, ctx->stream, A, xyz, ctx->d_rxp, ctx->d_rxt, mode, part);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(RB), 0, ctx->stream, part, G, ns, out);
 }
 
 // calc_rigidBody_params epilogue (RigidAuxRoutines.f90:716-740) from the 10 sums
@@ -446,7 +472,7 @@ int refresh_geometry(melt_ctx* ctx, LagArgs& A) {
   hipLaunchKernelGGL(k_face_centroid_normal, dim3(nb(ctx->nf, 256)), dim3(256), 0, s, A, x, 1, 1, 1);
   hipLaunchKernelGGL(k_skewness, dim3(nb(ctx->nf, 256)), dim3(256), 0, s, A);
   hipLaunchKernelGGL(k_vert_geom, dim3(nb(ctx->nv, 256)), dim3(256), 0, s, A);
-  hipLaunchKernelGGL(k_face_sums, dim3(1), dim3(RB), 0, s, A, x, ctx->d_rxp, ctx->d_rxt, (int)SUM_AREA_VOL_RIGID, ctx->d_red);
+  face_sums(ctx, A, x, (int)SUM_AREA_VOL_RIGID, ctx->d_red);
   hipLaunchKernelGGL(k_body_final, dim3(1), dim3(1), 0, s, ctx->d_body, ctx->d_red, ctx->cfg.dens_ratio, ctx->d_err);
   if (ctx->cfg.imelt == 1) hipLaunchKernelGGL(k_cfac, dim3(nb(ctx->nf, 256)), dim3(256), 0, s, A, ctx->d_body);
   MELT_HIP(hipGetLastError());
@@ -465,19 +491,19 @@ int lagr_after_force(melt_ctx* ctx) {
     hipLaunchKernelGGL(k_face_post, dim3(nb(ctx->nf, 256)), dim3(256), 0, s, A, ctx->d_probe, ctx->d_body, ctx->d_rxp, ctx->d_rxt, ctx->cfg.ren,
                        heat, loads);
   if (loads) {
-    hipLaunchKernelGGL(k_face_sums, dim3(1), dim3(RB), 0, s, A, ctx->d_lag[MELT_L_XYZV], ctx->d_rxp, ctx->d_rxt, (int)SUM_LOADS, ctx->d_red);
+    face_sums(ctx, A, ctx->d_lag[MELT_L_XYZV], (int)SUM_LOADS, ctx->d_red);
     hipLaunchKernelGGL(k_body_loads, dim3(1), dim3(1), 0, s, ctx->d_body, ctx->d_red);
   }
   if (heat) {
     hipLaunchKernelGGL(k_vert_heat_stefan, dim3(nb(ctx->nv, 256)), dim3(256), 0, s, A, ctx->d_body, ctx->cfg.cpliquid / ctx->cfg.latheat, ctx->dt,
                        ctx->ga, ctx->ro, 1);
-    hipLaunchKernelGGL(k_face_sums, dim3(1), dim3(RB), 0, s, A, ctx->d_lag[MELT_L_XYZV], ctx->d_rxp, ctx->d_rxt, (int)SUM_VOLUME, ctx->d_red);
+    face_sums(ctx, A, ctx->d_lag[MELT_L_XYZV], (int)SUM_VOLUME, ctx->d_red);
     hipLaunchKernelGGL(k_body_volume, dim3(1), dim3(1), 0, s, ctx->d_body, ctx->d_red);
   }
   if (loads) {
     const double* x = ctx->d_lag[MELT_L_XYZV];
     hipLaunchKernelGGL(k_face_centroid_normal, dim3(nb(ctx->nf, 256)), dim3(256), 0, s, A, x, 1, 1, 1);
-    hipLaunchKernelGGL(k_face_sums, dim3(1), dim3(RB), 0, s, A, x, ctx->d_rxp, ctx->d_rxt, (int)SUM_RIGID, ctx->d_red);
+    face_sums(ctx, A, x, (int)SUM_RIGID, ctx->d_red);
     hipLaunchKernelGGL(k_body_rigid, dim3(1), dim3(1), 0, s, ctx->d_body, ctx->d_red, ctx->cfg.dens_ratio);
     hipLaunchKernelGGL(k_rel, dim3(nb((size_t)ctx->nv + ctx->nf, 256)), dim3(256), 0, s, A, ctx->d_body);
     hipLaunchKernelGGL(k_newton, dim3(1), dim3(1), 0, s, ctx->d_body, ctx->ga, ctx->ro, ctx->al, ctx->dt, ctx->cfg.dens_ratio, ctx->cfg.xlen,
@@ -496,7 +522,7 @@ static int geometry_init_device(melt_ctx* ctx, const double* d_xyz0) {
   LagArgs A = lag_args(ctx);
   hipStream_t s = ctx->stream;
   const melt_body& h = ctx->h_body;
-  hipLaunchKernelGGL(k_face_sums, dim3(1), dim3(RB), 0, s, A, d_xyz0, ctx->d_rxp, ctx->d_rxt, (int)SUM_RIGID, ctx->d_red);
+  face_sums(ctx, A, d_xyz0, (int)SUM_RIGID, ctx->d_red);
   hipLaunchKernelGGL(k_body_rigid, dim3(1), dim3(1), 0, s, ctx->d_body, ctx->d_red, ctx->cfg.dens_ratio);
   hipLaunchKernelGGL(k_geom_init_body, dim3(1), dim3(1), 0, s, ctx->d_body, ctx->cfg.xlen, ctx->cfg.ylen, ctx->cfg.zlen, h.h_eulerian, h.celvol,
                      h.e_thresh, h.v_thresh);
@@ -506,7 +532,7 @@ static int geometry_init_device(melt_ctx* ctx, const double* d_xyz0) {
   // E_thresh is still unset when tri_geo measures the edges
   hipLaunchKernelGGL(k_edge_len, dim3(nb(ctx->ne, 256)), dim3(256), 0, s, A, d_xyz0, ctx->d_body, 0.0);
   hipLaunchKernelGGL(k_face_area, dim3(nb(ctx->nf, 256)), dim3(256), 0, s, A, d_xyz0);
-  hipLaunchKernelGGL(k_face_sums, dim3(1), dim3(RB), 0, s, A, d_xyz0, ctx->d_rxp, ctx->d_rxt, (int)SUM_AREA_VOL_RIGID, ctx->d_red + 16);
+  face_sums(ctx, A, d_xyz0, (int)SUM_AREA_VOL_RIGID, ctx->d_red + 16);
   hipLaunchKernelGGL(k_surface, dim3(1), dim3(1), 0, s, ctx->d_body, ctx->d_red + 16);
   hipLaunchKernelGGL(k_skewness, dim3(nb(ctx->nf, 256)), dim3(256), 0, s, A);
   hipLaunchKernelGGL(k_face_centroid_normal, dim3(nb(ctx->nf, 256)), dim3(256), 0, s, A, d_xyz0, 0, 1, 0);

@@ -9,6 +9,11 @@
 // reads and writes the spectrum once.  The z pass does forward FFT, division
 // and inverse FFT in one visit.
 //
+// The real transforms along x pack two real lines into one complex line
+// (z = a + i b) and separate the two half spectra afterwards (inverse: the
+// two Hermitian half spectra are packed the same way), which halves their LDS
+// work.  Radix stages are templates, so every butterfly lives in registers.
+//
 // Spectrum layout: complex (n1hp x N2 x N3), kx fastest, n1hp = N1/2+1 rounded
 // up to a multiple of 8 so every row starts on a 128-byte boundary.
 #include <cmath>
@@ -18,204 +23,252 @@
 
 namespace {
 
-constexpr int FFT_TILE = 2048;  // complex elements per LDS tile (32 KiB)
-constexpr int FFT_MAXB = 4;     // butterflies per thread per stage (256 threads, radix >= 2)
+template <typename T>
+struct alignas(2 * sizeof(T)) cx {
+  T x, y;
+};
+template <typename T>
+__device__ __forceinline__ cx<T> mkc(T a, T b) {
+  cx<T> r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ cx<T> cadd(cx<T> a, cx<T> b) { return mkc<T>(a.x + b.x, a.y + b.y); }
+template <typename T>
+__device__ __forceinline__ cx<T> csub(cx<T> a, cx<T> b) { return mkc<T>(a.x - b.x, a.y - b.y); }
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) { return mkc<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+
+constexpr int FFT_NT = 256;   // threads per FFT block
+constexpr int FFT_EPT = 8;    // LDS elements per thread: a tile holds <= 2048 complex values
 
 struct FftPlan {
   int N, nst;
   int radix[24];
-  const double2* tw;  // exp(-2 pi i t / N), t < N
 };
 
-__device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
-__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+// One Stockham stage of radix R over the tile: element (line l, position p)
+// at s[l*ls + p*ps].  LFAST: consecutive butterflies run over lines (tile
+// layouts with the line index contiguous) instead of positions.
+template <typename T, int R, bool LFAST>
+__device__ __forceinline__ void fft_stage(cx<T>* s, int nl, int ls, int ps, int N, int Ns, const cx<T>* __restrict__ tw, bool inv) {
+  constexpr int Q = (FFT_EPT + R - 1) / R;  // butterflies per thread
+  const int NR = N / R, nbf = nl * NR, tid = threadIdx.x, nt = blockDim.x;
+  const int step = N / (Ns * R);
+  cx<T> v[Q][R];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    int b = tid + q * nt;
+    if (b < nbf) {
+      int l = LFAST ? b % nl : b / NR;
+      int j = LFAST ? b / nl : b % NR;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = s[l * ls + (j + r * NR) * ps];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    int b = tid + q * nt;
+    if (b < nbf) {
+      int l = LFAST ? b % nl : b / NR;
+      int j = LFAST ? b / nl : b % NR;
+      int k = j % Ns;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        cx<T> w = tw[k * r * step];
+        if (inv) w.y = -w.y;
+        v[q][r] = cmul(v[q][r], w);
+      }
+      cx<T> y[R];
+      if constexpr (R == 4) {
+        cx<T> a0 = cadd(v[q][0], v[q][2]), a1 = csub(v[q][0], v[q][2]);
+        cx<T> a2 = cadd(v[q][1], v[q][3]), d = csub(v[q][1], v[q][3]);
+        cx<T> a3 = inv ? mkc<T>(-d.y, d.x) : mkc<T>(d.y, -d.x);  // +i d (backward) or -i d (forward)
+        y[0] = cadd(a0, a2); y[1] = cadd(a1, a3); y[2] = csub(a0, a2); y[3] = csub(a1, a3);
+      } else if constexpr (R == 2) {
+        y[0] = cadd(v[q][0], v[q][1]); y[1] = csub(v[q][0], v[q][1]);
+      } else {
+        const T c = T(-0.5), sn = (inv ? T(1) : T(-1)) * T(0.86602540378443864676);
+        cx<T> sm = cadd(v[q][1], v[q][2]), df = csub(v[q][1], v[q][2]);
+        y[0] = cadd(v[q][0], sm);
+        cx<T> t1 = mkc<T>(v[q][0].x + c * sm.x, v[q][0].y + c * sm.y);
+        cx<T> t2 = mkc<T>(-sn * df.y, sn * df.x);  // i sn df
+        y[1] = cadd(t1, t2); y[2] = csub(t1, t2);
+      }
+      int o = (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) s[l * ls + (o + r * Ns) * ps] = y[r];
+    }
+  }
+  __syncthreads();
+}
 
-// In-LDS Stockham FFT of nl lines: element (line l, position p) at s[l*ls + p*ps].
-// lfast: consecutive butterflies run over lines (tile layouts with the line
-// index contiguous) instead of positions.  sign -1 forward, +1 backward
-// (unnormalised, as FFTW's FFTW_FORWARD / FFTW_BACKWARD).
-__device__ void lds_fft(double2* s, int nl, int ls, int ps, bool lfast, const FftPlan& P, int sign) {
-  const int N = P.N, tid = threadIdx.x, nt = blockDim.x;
+// Unnormalised in-LDS FFT of nl lines (forward: exp(-i...), as FFTW_FORWARD).
+// The tile must hold at most FFT_EPT * blockDim.x elements, be complete in LDS
+// and visible (barrier) on entry; it is visible again on return.
+template <typename T, bool LFAST>
+__device__ void lds_fft(cx<T>* s, int nl, int ls, int ps, const FftPlan& P, const cx<T>* __restrict__ tw, bool inv) {
   int Ns = 1;
   for (int st = 0; st < P.nst; ++st) {
     const int R = P.radix[st];
-    const int NR = N / R;
-    const int nbf = nl * NR;
-    double2 v[FFT_MAXB][4];
-#pragma unroll
-    for (int q = 0; q < FFT_MAXB; ++q) {
-      int b = tid + q * nt;
-      if (b < nbf) {
-        int l = lfast ? b % nl : b / NR;
-        int j = lfast ? b / nl : b % NR;
-        for (int r = 0; r < R; ++r) v[q][r] = s[l * ls + (j + r * NR) * ps];
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < FFT_MAXB; ++q) {
-      int b = tid + q * nt;
-      if (b < nbf) {
-        int l = lfast ? b % nl : b / NR;
-        int j = lfast ? b / nl : b % NR;
-        int k = j % Ns;
-        int step = N / (Ns * R);
-        for (int r = 1; r < R; ++r) {
-          double2 w = P.tw[k * r * step];
-          if (sign > 0) w.y = -w.y;
-          v[q][r] = cmul(v[q][r], w);
-        }
-        double2 y0, y1, y2, y3;
-        if (R == 4) {
-          double2 a0 = cadd(v[q][0], v[q][2]), a1 = csub(v[q][0], v[q][2]);
-          double2 a2 = cadd(v[q][1], v[q][3]), d = csub(v[q][1], v[q][3]);
-          // times -i (forward) or +i (backward)
-          double2 a3 = sign < 0 ? make_double2(d.y, -d.x) : make_double2(-d.y, d.x);
-          y0 = cadd(a0, a2); y1 = cadd(a1, a3); y2 = csub(a0, a2); y3 = csub(a1, a3);
-        } else if (R == 2) {
-          y0 = cadd(v[q][0], v[q][1]); y1 = csub(v[q][0], v[q][1]);
-          y2 = y3 = make_double2(0, 0);
-        } else {  // R == 3
-          const double c = -0.5, sn = (sign < 0 ? -1.0 : 1.0) * 0.86602540378443864676;
-          double2 sm = cadd(v[q][1], v[q][2]), df = csub(v[q][1], v[q][2]);
-          y0 = cadd(v[q][0], sm);
-          double2 t1 = make_double2(v[q][0].x + c * sm.x, v[q][0].y + c * sm.y);
-          double2 t2 = make_double2(-sn * df.y, sn * df.x);  // i*sn*df
-          y1 = cadd(t1, t2); y2 = csub(t1, t2);
-          y3 = make_double2(0, 0);
-        }
-        int o = (j / Ns) * Ns * R + k;
-        s[l * ls + o * ps] = y0;
-        s[l * ls + (o + Ns) * ps] = y1;
-        if (R >= 3) s[l * ls + (o + 2 * Ns) * ps] = y2;
-        if (R == 4) s[l * ls + (o + 3 * Ns) * ps] = y3;
-      }
-    }
-    __syncthreads();
+    if (R == 4) fft_stage<T, 4, LFAST>(s, nl, ls, ps, P.N, Ns, tw, inv);
+    else if (R == 2) fft_stage<T, 2, LFAST>(s, nl, ls, ps, P.N, Ns, tw, inv);
+    else fft_stage<T, 3, LFAST>(s, nl, ls, ps, P.N, Ns, tw, inv);
     Ns *= R;
   }
 }
 
+template <typename T>
 struct SpecArgs {
   Grid g;
   int n1h, n1hp;
-  double2* spec;
+  cx<T>* spec;
 };
 
-// x pass, forward: real line (divergence or dph) -> bins 0..N1/2.
-__global__ __launch_bounds__(256) void k_fft_x_fwd(SpecArgs S, FftPlan P, int nl, const double* __restrict__ vx,
-                                                   const double* __restrict__ vy, const double* __restrict__ vz,
-                                                   const double* __restrict__ dph, double usdtal) {
-  extern __shared__ double2 sh[];
+// x pass, forward: nl real lines (divergence divg.f90:17-20, or a stored field)
+// -> bins 0..N1/2 of each, two lines per complex transform.
+template <typename Rf, typename T>
+__global__ __launch_bounds__(FFT_NT) void k_fft_x_fwd(SpecArgs<T> S, FftPlan P, const cx<T>* __restrict__ tw, int nl,
+                                                      const Rf* __restrict__ vx, const Rf* __restrict__ vy, const Rf* __restrict__ vz,
+                                                      const Rf* __restrict__ src, T usdtal) {
+  extern __shared__ unsigned char smem[];
+  cx<T>* sh = reinterpret_cast<cx<T>*>(smem);
+  T* shr = reinterpret_cast<T*>(smem);
   const Grid& g = S.g;
   const int N = g.n1, ls = N + 1;
   unsigned t = xcd_block(blockIdx.x, gridDim.x);
   int ntj = g.n2 / nl;
   int j0 = (t % ntj) * nl, k = t / ntj;
+  const T dx1 = (T)g.dx1, dx2 = (T)g.dx2, dx3 = (T)g.dx3;
+  const int kp = k == g.n3 - 1 ? 0 : k + 1;
   for (int c = threadIdx.x; c < nl * N; c += blockDim.x) {
     int l = c / N, i = c % N, j = j0 + l;
     size_t idx = (size_t)i + (size_t)N * ((size_t)j + (size_t)g.n2 * k);
-    double v;
-    if (dph) {
-      v = dph[idx];
-    } else {  // divg.f90:17-20
-      int ip = i == N - 1 ? 0 : i + 1, jp = j == g.n2 - 1 ? 0 : j + 1, kp = k == g.n3 - 1 ? 0 : k + 1;
-      double dq = (vx[idx - i + ip] - vx[idx]) * g.dx1 + (vy[(size_t)i + (size_t)N * ((size_t)jp + (size_t)g.n2 * k)] - vy[idx]) * g.dx2 +
-                  (vz[(size_t)i + (size_t)N * ((size_t)j + (size_t)g.n2 * kp)] - vz[idx]) * g.dx3;
+    T v;
+    if (src) {
+      v = (T)src[idx];
+    } else {
+      int ip = i == N - 1 ? 0 : i + 1, jp = j == g.n2 - 1 ? 0 : j + 1;
+      T dq = ((T)vx[idx - i + ip] - (T)vx[idx]) * dx1 + ((T)vy[(size_t)i + (size_t)N * ((size_t)jp + (size_t)g.n2 * k)] - (T)vy[idx]) * dx2 +
+             ((T)vz[(size_t)i + (size_t)N * ((size_t)j + (size_t)g.n2 * kp)] - (T)vz[idx]) * dx3;
       v = dq * usdtal;
     }
-    sh[l * ls + i] = make_double2(v, 0.0);
+    shr[2 * ((l >> 1) * ls + i) + (l & 1)] = v;  // line 2p -> real part, 2p+1 -> imaginary part
   }
   __syncthreads();
-  lds_fft(sh, nl, ls, 1, false, P, -1);
-  for (int c = threadIdx.x; c < nl * S.n1hp; c += blockDim.x) {
-    int l = c / S.n1hp, m = c % S.n1hp;
-    double2 v = m < S.n1h ? sh[l * ls + m] : make_double2(0.0, 0.0);
-    S.spec[(size_t)m + (size_t)S.n1hp * ((size_t)(j0 + l) + (size_t)g.n2 * k)] = v;
+  lds_fft<T, false>(sh, nl / 2, ls, 1, P, tw, false);
+  const int np = nl / 2;
+  for (int c = threadIdx.x; c < np * S.n1hp; c += blockDim.x) {
+    int pl = c / S.n1hp, m = c % S.n1hp;
+    cx<T> a = mkc<T>(0, 0), b = mkc<T>(0, 0);
+    if (m < S.n1h) {
+      cx<T> z = sh[pl * ls + m], zr = sh[pl * ls + (m == 0 ? 0 : N - m)];
+      // A = (Z[m] + conj Z[N-m]) / 2,  B = (Z[m] - conj Z[N-m]) / (2i)
+      a = mkc<T>((z.x + zr.x) * T(0.5), (z.y - zr.y) * T(0.5));
+      b = mkc<T>((z.y + zr.y) * T(0.5), (zr.x - z.x) * T(0.5));
+    }
+    size_t row = (size_t)S.n1hp * ((size_t)(j0 + 2 * pl) + (size_t)g.n2 * k);
+    S.spec[row + m] = a;
+    S.spec[row + S.n1hp + m] = b;
   }
 }
 
-// y pass: tile of nl kx-columns for one z plane.
-__global__ __launch_bounds__(256) void k_fft_y(SpecArgs S, FftPlan P, int nl, int sign) {
-  extern __shared__ double2 sh[];
+// y pass: tile of nl kx-columns (the last tile may be partial) for one z plane.
+template <typename T>
+__global__ __launch_bounds__(FFT_NT) void k_fft_y(SpecArgs<T> S, FftPlan P, const cx<T>* __restrict__ tw, int nl, int inv) {
+  extern __shared__ unsigned char smem[];
+  cx<T>* sh = reinterpret_cast<cx<T>*>(smem);
   const Grid& g = S.g;
   const int N = g.n2;
   unsigned t = xcd_block(blockIdx.x, gridDim.x);
-  int ntm = S.n1hp / nl;
+  int ntm = (S.n1hp + nl - 1) / nl;
   int m0 = (t % ntm) * nl, k = t / ntm;
   size_t base = (size_t)m0 + (size_t)S.n1hp * (size_t)g.n2 * k;
   for (int c = threadIdx.x; c < nl * N; c += blockDim.x) {
     int l = c % nl, p = c / nl;
-    sh[p * nl + l] = S.spec[base + l + (size_t)S.n1hp * p];
+    sh[p * nl + l] = m0 + l < S.n1hp ? S.spec[base + l + (size_t)S.n1hp * p] : mkc<T>(0, 0);
   }
   __syncthreads();
-  lds_fft(sh, nl, 1, nl, true, P, sign);
+  lds_fft<T, true>(sh, nl, 1, nl, P, tw, inv != 0);
   for (int c = threadIdx.x; c < nl * N; c += blockDim.x) {
     int l = c % nl, p = c / nl;
-    S.spec[base + l + (size_t)S.n1hp * p] = sh[p * nl + l];
+    if (m0 + l < S.n1hp) S.spec[base + l + (size_t)S.n1hp * p] = sh[p * nl + l];
   }
 }
 
 // z pass: forward FFT, 1/(N1 N2 N3), division by -(ak3+ak1+ak2) with the zero
 // mode set to 0 (phcalc.f90:40-73), inverse FFT.
-__global__ __launch_bounds__(256) void k_fft_z_solve(SpecArgs S, FftPlan P, int nl, const double* __restrict__ ak1,
-                                                     const double* __restrict__ ak2, const double* __restrict__ ak3, double nrm) {
-  extern __shared__ double2 sh[];
+template <typename T>
+__global__ __launch_bounds__(FFT_NT) void k_fft_z_solve(SpecArgs<T> S, FftPlan P, const cx<T>* __restrict__ tw, int nl,
+                                                        const double* __restrict__ ak1, const double* __restrict__ ak2,
+                                                        const double* __restrict__ ak3, double nrm) {
+  extern __shared__ unsigned char smem[];
+  cx<T>* sh = reinterpret_cast<cx<T>*>(smem);
   const Grid& g = S.g;
   const int N = g.n3;
   unsigned t = xcd_block(blockIdx.x, gridDim.x);
-  int ntm = S.n1hp / nl;
+  int ntm = (S.n1hp + nl - 1) / nl;
   int m0 = (t % ntm) * nl, j = t / ntm;
   size_t plane = (size_t)S.n1hp * g.n2;
   size_t base = (size_t)m0 + (size_t)S.n1hp * j;
   for (int c = threadIdx.x; c < nl * N; c += blockDim.x) {
     int l = c % nl, p = c / nl;
-    sh[p * nl + l] = S.spec[base + l + plane * p];
+    sh[p * nl + l] = m0 + l < S.n1hp ? S.spec[base + l + plane * p] : mkc<T>(0, 0);
   }
   __syncthreads();
-  lds_fft(sh, nl, 1, nl, true, P, -1);
+  lds_fft<T, true>(sh, nl, 1, nl, P, tw, false);
   for (int c = threadIdx.x; c < nl * N; c += blockDim.x) {
     int l = c % nl, p = c / nl, m = m0 + l;
-    double2 v = sh[p * nl + l];
+    cx<T> v = sh[p * nl + l];
     if (m < S.n1h) {
-      v.x = v.x / nrm; v.y = v.y / nrm;
       double ev = -ak3[p] - ak1[m] - ak2[j];
-      if (m == 0 && j == 0 && p == 0) { v.x = 0.0; v.y = 0.0; }
-      else { v.x = v.x / ev; v.y = v.y / ev; }
+      double vx = (double)v.x / nrm, vy = (double)v.y / nrm;
+      if (m == 0 && j == 0 && p == 0) { vx = 0.0; vy = 0.0; }
+      else { vx = vx / ev; vy = vy / ev; }
+      v = mkc<T>((T)vx, (T)vy);
     } else {
-      v = make_double2(0.0, 0.0);
+      v = mkc<T>(0, 0);
     }
     sh[p * nl + l] = v;
   }
   __syncthreads();
-  lds_fft(sh, nl, 1, nl, true, P, +1);
+  lds_fft<T, true>(sh, nl, 1, nl, P, tw, true);
   for (int c = threadIdx.x; c < nl * N; c += blockDim.x) {
     int l = c % nl, p = c / nl;
-    S.spec[base + l + plane * p] = sh[p * nl + l];
+    if (m0 + l < S.n1hp) S.spec[base + l + plane * p] = sh[p * nl + l];
   }
 }
 
-// x pass, backward (complex-to-real, FFTW c2r semantics): rebuild the full
-// Hermitian line, inverse FFT, keep the real part.
-__global__ __launch_bounds__(256) void k_fft_x_bwd(SpecArgs S, FftPlan P, int nl, double* __restrict__ dph) {
-  extern __shared__ double2 sh[];
+// x pass, backward (complex-to-real with FFTW c2r semantics: the imaginary
+// parts of bins 0 and N1/2 are ignored): the half spectra A, B of lines 2p,
+// 2p+1 are packed as Z = A + iB over the full Hermitian range, one inverse
+// FFT gives line 2p in the real and line 2p+1 in the imaginary part.
+template <typename Rf, typename T>
+__global__ __launch_bounds__(FFT_NT) void k_fft_x_bwd(SpecArgs<T> S, FftPlan P, const cx<T>* __restrict__ tw, int nl, Rf* __restrict__ dst) {
+  extern __shared__ unsigned char smem[];
+  cx<T>* sh = reinterpret_cast<cx<T>*>(smem);
   const Grid& g = S.g;
-  const int N = g.n1, ls = N + 1;
+  const int N = g.n1, ls = N + 1, np = nl / 2;
   unsigned t = xcd_block(blockIdx.x, gridDim.x);
   int ntj = g.n2 / nl;
   int j0 = (t % ntj) * nl, k = t / ntj;
-  for (int c = threadIdx.x; c < nl * S.n1h; c += blockDim.x) {
-    int l = c / S.n1h, m = c % S.n1h;
-    double2 v = S.spec[(size_t)m + (size_t)S.n1hp * ((size_t)(j0 + l) + (size_t)g.n2 * k)];
-    sh[l * ls + m] = v;
-    if (m > 0 && m < N - m) sh[l * ls + (N - m)] = make_double2(v.x, -v.y);
+  for (int c = threadIdx.x; c < np * S.n1h; c += blockDim.x) {
+    int pl = c / S.n1h, m = c % S.n1h;
+    size_t row = (size_t)S.n1hp * ((size_t)(j0 + 2 * pl) + (size_t)g.n2 * k);
+    cx<T> a = S.spec[row + m], b = S.spec[row + S.n1hp + m];
+    if (m == 0 || 2 * m == N) { a.y = T(0); b.y = T(0); }
+    sh[pl * ls + m] = mkc<T>(a.x - b.y, a.y + b.x);                         // A + iB
+    if (m > 0 && 2 * m < N) sh[pl * ls + (N - m)] = mkc<T>(a.x + b.y, b.x - a.y);  // conj(A) + i conj(B)
   }
   __syncthreads();
-  lds_fft(sh, nl, ls, 1, false, P, +1);
+  lds_fft<T, false>(sh, np, ls, 1, P, tw, true);
   for (int c = threadIdx.x; c < nl * N; c += blockDim.x) {
     int l = c / N, i = c % N;
-    dph[(size_t)i + (size_t)N * ((size_t)(j0 + l) + (size_t)g.n2 * k)] = sh[l * ls + i].x;
+    cx<T> z = sh[(l >> 1) * ls + i];
+    dst[(size_t)i + (size_t)N * ((size_t)(j0 + l) + (size_t)g.n2 * k)] = (Rf)((l & 1) ? z.y : z.x);
   }
 }
 
@@ -229,10 +282,13 @@ bool make_plan(int N, FftPlan& P) {
   return m == 1 && P.nst <= 24;
 }
 
-int largest_divisor_le(int n, int cap) {
-  for (int d = cap; d >= 1; --d)
-    if (n % d == 0) return d;
-  return 1;
+// Largest even divisor of n2 with (nl/2) * N1 complex values per tile.
+int x_lines(int n1, int n2) {
+  int cap = std::max(2, 2 * (FFT_EPT * FFT_NT) / n1);
+  int best = 2;
+  for (int d = 2; d <= std::min(cap, n2); d += 2)
+    if (n2 % d == 0) best = d;
+  return best;
 }
 
 }  // namespace
@@ -242,6 +298,7 @@ int poisson_init(melt_ctx* ctx) {
   double2** tw[3] = {&ctx->tw1, &ctx->tw2, &ctx->tw3};
   double** ak[3] = {&ctx->ak1, &ctx->ak2, &ctx->ak3};
   double len[3] = {ctx->cfg.xlen, ctx->cfg.ylen, ctx->cfg.zlen};
+  if (ctx->n2 % 2) return melt_fail(ctx, MELT_ERR_ARG, "poisson_init", "n2m must be even");
   for (int d = 0; d < 3; ++d) {
     FftPlan P;
     if (!make_plan(n[d], P)) return melt_fail(ctx, MELT_ERR_ARG, "poisson_init", "grid size must be 2^a 3^b");
@@ -272,46 +329,51 @@ int poisson_init(melt_ctx* ctx) {
 // fused_divg = 1: dph := divergence / (al dt) is formed on the fly from the
 // velocities (divg.f90) and never stored; 0: transforms MELT_DPH as phcalc.
 int poisson_solve(melt_ctx* ctx, int fused_divg) {
-  SpecArgs S;
+  using T = double;
+  SpecArgs<T> S;
   S.g = make_grid(ctx);
-  S.n1h = ctx->n1h; S.n1hp = ctx->n1hp; S.spec = ctx->spec;
+  S.n1h = ctx->n1h; S.n1hp = ctx->n1hp; S.spec = reinterpret_cast<cx<T>*>(ctx->spec);
   FftPlan Px, Py, Pz;
   make_plan(ctx->n1, Px); make_plan(ctx->n2, Py); make_plan(ctx->n3, Pz);
-  Px.tw = ctx->tw1; Py.tw = ctx->tw2; Pz.tw = ctx->tw3;
+  const cx<T>* twx = reinterpret_cast<const cx<T>*>(ctx->tw1);
+  const cx<T>* twy = reinterpret_cast<const cx<T>*>(ctx->tw2);
+  const cx<T>* twz = reinterpret_cast<const cx<T>*>(ctx->tw3);
   hipEvent_t ev = nullptr;
 
-  int nlx = largest_divisor_le(ctx->n2, std::max(1, FFT_TILE / ctx->n1));
-  size_t shx = sizeof(double2) * (size_t)nlx * (ctx->n1 + 1);
-  double usdtal = 1.0 / (ctx->dt * ctx->al);
+  int nlx = x_lines(ctx->n1, ctx->n2);
+  size_t shx = sizeof(cx<T>) * (size_t)(nlx / 2) * (ctx->n1 + 1);
+  T usdtal = 1.0 / (ctx->dt * ctx->al);
+  unsigned gx = (unsigned)((ctx->n2 / nlx) * ctx->n3);
   timing_begin(ctx, SLOT_DIVG_FFTX, &ev);
-  hipLaunchKernelGGL(k_fft_x_fwd, dim3((unsigned)((ctx->n2 / nlx) * ctx->n3)), dim3(256), shx, ctx->stream, S, Px, nlx, ctx->f[MELT_VX],
-                     ctx->f[MELT_VY], ctx->f[MELT_VZ], fused_divg ? nullptr : ctx->f[MELT_DPH], usdtal);
+  hipLaunchKernelGGL((k_fft_x_fwd<double, T>), dim3(gx), dim3(FFT_NT), shx, ctx->stream, S, Px, twx, nlx, ctx->f[MELT_VX], ctx->f[MELT_VY],
+                     ctx->f[MELT_VZ], fused_divg ? nullptr : ctx->f[MELT_DPH], usdtal);
   MELT_HIP(hipGetLastError());
   timing_end(ctx, SLOT_DIVG_FFTX, ev);
 
-  int nly = largest_divisor_le(ctx->n1hp, std::max(1, FFT_TILE / ctx->n2));
-  size_t shy = sizeof(double2) * (size_t)nly * ctx->n2;
+  int nly = std::max(1, std::min(ctx->n1hp, (FFT_EPT * FFT_NT) / ctx->n2));
+  size_t shy = sizeof(cx<T>) * (size_t)nly * ctx->n2;
+  unsigned gy = (unsigned)(((ctx->n1hp + nly - 1) / nly) * ctx->n3);
   timing_begin(ctx, SLOT_FFTY, &ev);
-  hipLaunchKernelGGL(k_fft_y, dim3((unsigned)((ctx->n1hp / nly) * ctx->n3)), dim3(256), shy, ctx->stream, S, Py, nly, -1);
+  hipLaunchKernelGGL((k_fft_y<T>), dim3(gy), dim3(FFT_NT), shy, ctx->stream, S, Py, twy, nly, 0);
   MELT_HIP(hipGetLastError());
   timing_end(ctx, SLOT_FFTY, ev);
 
-  int nlz = largest_divisor_le(ctx->n1hp, std::max(1, FFT_TILE / ctx->n3));
-  size_t shz = sizeof(double2) * (size_t)nlz * ctx->n3;
+  int nlz = std::max(1, std::min(ctx->n1hp, (FFT_EPT * FFT_NT) / ctx->n3));
+  size_t shz = sizeof(cx<T>) * (size_t)nlz * ctx->n3;
+  unsigned gz = (unsigned)(((ctx->n1hp + nlz - 1) / nlz) * ctx->n2);
   double nrm = (double)((long long)ctx->n1 * ctx->n2 * ctx->n3);
   timing_begin(ctx, SLOT_FFTZ_SOLVE, &ev);
-  hipLaunchKernelGGL(k_fft_z_solve, dim3((unsigned)((ctx->n1hp / nlz) * ctx->n2)), dim3(256), shz, ctx->stream, S, Pz, nlz, ctx->ak1,
-                     ctx->ak2, ctx->ak3, nrm);
+  hipLaunchKernelGGL((k_fft_z_solve<T>), dim3(gz), dim3(FFT_NT), shz, ctx->stream, S, Pz, twz, nlz, ctx->ak1, ctx->ak2, ctx->ak3, nrm);
   MELT_HIP(hipGetLastError());
   timing_end(ctx, SLOT_FFTZ_SOLVE, ev);
 
   timing_begin(ctx, SLOT_IFFTY, &ev);
-  hipLaunchKernelGGL(k_fft_y, dim3((unsigned)((ctx->n1hp / nly) * ctx->n3)), dim3(256), shy, ctx->stream, S, Py, nly, +1);
+  hipLaunchKernelGGL((k_fft_y<T>), dim3(gy), dim3(FFT_NT), shy, ctx->stream, S, Py, twy, nly, 1);
   MELT_HIP(hipGetLastError());
   timing_end(ctx, SLOT_IFFTY, ev);
 
   timing_begin(ctx, SLOT_IFFTX, &ev);
-  hipLaunchKernelGGL(k_fft_x_bwd, dim3((unsigned)((ctx->n2 / nlx) * ctx->n3)), dim3(256), shx, ctx->stream, S, Px, nlx, ctx->f[MELT_DPH]);
+  hipLaunchKernelGGL((k_fft_x_bwd<double, T>), dim3(gx), dim3(FFT_NT), shx, ctx->stream, S, Px, twx, nlx, ctx->f[MELT_DPH]);
   MELT_HIP(hipGetLastError());
   timing_end(ctx, SLOT_IFFTX, ev);
   return 0;
