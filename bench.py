@@ -47,6 +47,9 @@ KERNEL_BYTES = {
     "project": (5 + 4) * 8,            # dph, u,v,w,p -> u,v,w,p
 }
 B_ALG_CELL = 480  # algorithmic bytes per cell per substep of the whole path (FP64)
+# precision form: the same passes at 4 B per value (240 B) plus its two FP64
+# residual reductions over u, v, w (provisional and projected field, 12 B each)
+B_ALG_CELL_MIXED = 240 + 24
 
 
 def workload(n: int):
@@ -58,9 +61,10 @@ def workload(n: int):
     return cfg, 5, 0.5
 
 
-def run_gpu(args):
+def run_gpu(args, precision=0, keep_fields=False):
     from melt_ibm_amd.api import MeltContext
     cfg, level, rad = workload(args.n)
+    cfg.precision = precision
     ctx = MeltContext(cfg)
     from melt_ibm_amd import geometry, inputs
     # temperature from inqpr, then the seeded solenoidal velocity and forcing
@@ -76,7 +80,8 @@ def run_gpu(args):
     ctx.set_dt(dt)
     ctx.tagCells()
     ctx.sync()
-    print(f"# setup done: N={args.n} dt={dt:.4e} faces={m.counts[2]}", file=sys.stderr, flush=True)
+    tag = "fp64" if precision == 0 else "mixed"
+    print(f"# {tag} setup done: N={args.n} dt={dt:.4e} faces={m.counts[2]}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         ctx.tschem()
     ctx.sync()
@@ -89,10 +94,13 @@ def run_gpu(args):
     t1 = time.perf_counter()
     tm = ctx.timing_read()
     ctx.timing(False)
+    print(f"# {tag} timed {args.steps} steps: {(t1 - t0) / args.steps * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
     qmax = ctx.divgck()
     body = ctx.body()
+    fields = {q: ctx.download(q) for q in ("vx", "vy", "vz", "temp", "pr")} if keep_fields else None
+    stats = ctx.refinement_stats() if precision else None
     ctx.close()
-    return cfg, (t1 - t0), tm, qmax, body
+    return cfg, (t1 - t0), tm, qmax, body, fields, stats
 
 
 def cpu_baseline(cfg, level, rad):
@@ -130,12 +138,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256, help="cells per side")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the precision-form leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this path runs one domain per GPU on a single-GPU box; "
                                    "--gpus > 1 is not supported (multi-GPU sharding is a design note only)"}))
         sys.exit(2)
-    cfg, elapsed, tm, qmax, body = run_gpu(args)
+    cfg, elapsed, tm, qmax, body, f64, _ = run_gpu(args, 0, keep_fields=not args.no_mixed)
     cells = cfg.n1m * cfg.n2m * cfg.n3m
     ms_step = elapsed / args.steps * 1e3
     value = cells * args.steps / elapsed / 1e6
@@ -170,6 +179,23 @@ def main():
         "kernels_ms_per_substep": {k: round(v[0] / max(v[1], 1) , 4) for k, v in tm.items() if v[1] > 0},
         "checks": {"divgck": qmax, "volume": body.volume},
     }
+    if not args.no_mixed:
+        _, el32, tm32, q32, body32, f32, stats = run_gpu(args, 1, keep_fields=True)
+        sub32 = el32 / substeps
+        errs = {q: float(np.max(np.abs(f32[q] - f64[q])) / max(np.max(np.abs(f64[q])), 1e-300)) for q in f64}
+        del f32, f64
+        out["mixed"] = {
+            "value": round(cells * args.steps / el32 / 1e6, 3), "unit": "Mcell-steps/s", "dtype": "f32 storage, f64 refinement",
+            "ms_per_step": round(el32 / args.steps * 1e3, 4), "speedup_vs_fp64": round(elapsed / el32, 3),
+            "substep_ms": round(sub32 * 1e3, 4), "substep_frac": round(B_ALG_CELL_MIXED * cells / sub32 / HBM_PEAK, 4),
+            "bytes_per_cell_substep": B_ALG_CELL_MIXED,
+            "field_rel_err_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()},
+            "volume_rel_err_vs_fp64": float(f"{abs(body32.volume - body.volume) / body.volume:.3e}"),
+            "divgck": q32,
+            "refinement": {"checks": stats[0], "refined": stats[1], "last_ratio": stats[2]},
+            "kernels_ms_per_substep": {k: round(v[0] / max(v[1], 1), 4) for k, v in tm32.items() if v[1] > 0},
+            "error_sample": f"both forms from the same state after {args.warmup + args.steps} RK3 steps",
+        }
     if not args.no_cpu:
         cfg2, level, rad = workload(args.n)
         try:

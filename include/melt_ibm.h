@@ -202,6 +202,18 @@ int melt_timing_count(void);
 const char* melt_timing_name(int32_t slot);
 int melt_timing_read(melt_ctx* ctx, double* ms, int64_t* samples);
 
+/* Precision form (melt_config.precision = 1; no counterpart in the reference,
+ * which is FP64 only).  Eulerian fields are stored as FP32 and the stencils,
+ * FFTs and MLS spreading run in FP32; the periodic line reductions, the MLS
+ * moment inversions, the Lagrangian / Newton-Euler / Stefan updates and the
+ * Poisson residual run in FP64.  After each projection the FP64 divergence of
+ * the projected velocity is compared with that of the provisional one; while
+ * it exceeds rtol times it (at most max_iter times) a correction Poisson solve
+ * and projection are applied.  Defaults: max_iter 1, rtol 1e-4.  Host arrays
+ * of upload/download stay double in both forms. */
+int melt_set_refinement(melt_ctx* ctx, int32_t max_iter, double rtol);
+int melt_get_refinement_stats(melt_ctx* ctx, int64_t* checks, int64_t* refined, double* last_ratio);
+
 #ifdef __cplusplus
 }
 #endif

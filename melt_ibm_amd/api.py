@@ -182,8 +182,11 @@ _SIGS = {
     "timing_enable": (C.c_int, [_P, C.c_int32]), "timing_reset": (C.c_int, [_P]),
     "timing_count": (C.c_int, []), "timing_name": (C.c_char_p, [C.c_int32]),
     "timing_read": (C.c_int, [_P, _P, _P]),
+    "set_refinement": (C.c_int, [_P, C.c_int32, C.c_double]),
+    "get_refinement_stats": (C.c_int, [_P, _P, _P, _P]),
 }
-OPTIONAL = {"timing_enable", "timing_reset", "timing_count", "timing_name", "timing_read", "device_ok"}
+OPTIONAL = {"timing_enable", "timing_reset", "timing_count", "timing_name", "timing_read", "device_ok", "set_refinement",
+            "get_refinement_stats"}
 
 
 def bind(lib: C.CDLL, prefix: str) -> dict:
@@ -366,6 +369,17 @@ class MeltContext:
 
     def timing_reset(self) -> None:
         self._chk(self._f["timing_reset"](self._ctx))
+
+    # ------------------------------------------------------ precision form
+    def set_refinement(self, max_iter: int = 1, rtol: float = 1e-4) -> None:
+        """Gated FP64-residual refinement of the precision form's pressure."""
+        self._chk(self._f["set_refinement"](self._ctx, int(max_iter), float(rtol)))
+
+    def refinement_stats(self) -> tuple[int, int, float]:
+        """(checks, refinements applied, last residual ratio after/before)."""
+        a, b, r = C.c_int64(), C.c_int64(), C.c_double()
+        self._chk(self._f["get_refinement_stats"](self._ctx, C.byref(a), C.byref(b), C.byref(r)))
+        return a.value, b.value, r.value
 
     def timing_read(self) -> dict:
         n = self._f["timing_count"]()

@@ -19,7 +19,7 @@ namespace {
 const char* slot_names[SLOT_COUNT] = {
     "tag_narrowband", "explicit_xsolve", "solve_x", "solve_y", "solve_z_update", "mls_force", "lagrangian",
     "divg_fft_x", "fft_y", "fft_z_solve", "ifft_y", "ifft_x", "project",
-    "phase_tagging", "phase_eulerian", "phase_mls", "phase_pressure", "substep"};
+    "phase_tagging", "phase_eulerian", "phase_mls", "phase_pressure", "substep", "refine"};
 
 const char* err_text(int code) {
   switch (code) {
@@ -44,8 +44,15 @@ __global__ void k_reset_loads(melt_body* b) {  // tsch.f90:19-23
   for (int d = 0; d < 3; ++d) { b->fp[d] = 0.0; b->ftau[d] = 0.0; b->torq_p[d] = 0.0; b->torq_tau[d] = 0.0; }
 }
 
-__global__ void k_fill_d(double* __restrict__ p, double v, size_t n) {
-  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) p[c] = v;
+template <typename R>
+__global__ void k_fill_d(R* __restrict__ p, double v, size_t n) {
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) p[c] = (R)v;
+}
+
+// Fill a field with a value (fill kernel, not a memset: safe under capture).
+void fill_field(melt_ctx* ctx, int id, double v) {
+  if (ctx->fp32) hipLaunchKernelGGL(k_fill_d<float>, dim3(1024), dim3(256), 0, ctx->stream, fld<float>(ctx, id), v, ctx->nc);
+  else hipLaunchKernelGGL(k_fill_d<double>, dim3(1024), dim3(256), 0, ctx->stream, fld<double>(ctx, id), v, ctx->nc);
 }
 
 int check_ctx(melt_ctx* ctx) { return ctx ? 0 : MELT_ERR_ARG; }
@@ -106,7 +113,7 @@ extern "C" void melt_ctx_destroy(melt_ctx* ctx) {
   for (int i = 1; i < 4; ++i) if (ctx->rhs4[i]) hipFree(ctx->rhs4[i]);
   void* ps[] = {ctx->spec, ctx->ak1, ctx->ak2, ctx->ak3, ctx->tw1, ctx->tw2, ctx->tw3, ctx->d_body, ctx->d_vof, ctx->d_eof, ctx->d_voe,
                 ctx->d_v2f_ptr, ctx->d_v2f, ctx->d_gv, ctx->d_ge, ctx->d_gf, ctx->d_pind, ctx->d_probe, ctx->d_rxp, ctx->d_rxt, ctx->d_red,
-                ctx->d_vbox, ctx->d_acc, ctx->d_fmean, ctx->d_tkey, ctx->d_tface, ctx->d_win, ctx->d_pcount, ctx->d_plist, ctx->d_err};
+                ctx->d_vbox, ctx->d_acc, ctx->d_fmean, ctx->d_tkey, ctx->d_tface, ctx->d_win, ctx->d_pcount, ctx->d_plist, ctx->d_err, ctx->d_gate, ctx->d_refine_stats};
   for (void* p : ps) if (p) hipFree(p);
   for (int i = 0; i < MELT_L_NLAG; ++i) if (ctx->d_lag[i]) hipFree(ctx->d_lag[i]);
   if (ctx->h_err) hipHostFree(ctx->h_err);
@@ -120,9 +127,11 @@ extern "C" int melt_ctx_create(const melt_config* cfg, melt_ctx** out) {
   *out = nullptr;
   if (!cfg || cfg->n1m != cfg->n2m || !size_ok(cfg->n1m) || !size_ok(cfg->n2m) || !size_ok(cfg->n3m) || cfg->n1m > 1024 || cfg->n3m > 1024)
     return MELT_ERR_ARG;
-  if (cfg->precision != 0) return MELT_ERR_ARG;
+  if (cfg->precision != 0 && cfg->precision != 1) return MELT_ERR_ARG;
   melt_ctx* ctx = new melt_ctx();
   ctx->cfg = *cfg;
+  ctx->fp32 = cfg->precision == 1;
+  ctx->esz = ctx->fp32 ? sizeof(float) : sizeof(double);
   ctx->n1 = cfg->n1m; ctx->n2 = cfg->n2m; ctx->n3 = cfg->n3m;
   ctx->nc = (size_t)ctx->n1 * ctx->n2 * ctx->n3;
   ctx->n1h = ctx->n1 / 2 + 1;
@@ -153,19 +162,22 @@ extern "C" int melt_ctx_create(const melt_config* cfg, melt_ctx** out) {
     }
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { rc = MELT_ERR_HIP; break; }
     for (int i = 0; i < MELT_NFIELDS && !rc; ++i) {
-      if (hipMalloc(&ctx->f[i], sizeof(double) * ctx->nc) != hipSuccess || hipMemset(ctx->f[i], 0, sizeof(double) * ctx->nc) != hipSuccess)
+      if (hipMalloc(&ctx->f[i], ctx->esz * ctx->nc) != hipSuccess || hipMemset(ctx->f[i], 0, ctx->esz * ctx->nc) != hipSuccess)
         rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "field allocation failed");
     }
     if (rc) break;
     ctx->rhs4[0] = ctx->f[MELT_RHS];
     for (int i = 1; i < 4 && !rc; ++i)
-      if (hipMalloc(&ctx->rhs4[i], sizeof(double) * ctx->nc) != hipSuccess) rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "rhs allocation");
+      if (hipMalloc(&ctx->rhs4[i], ctx->esz * ctx->nc) != hipSuccess) rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "rhs allocation");
     if (rc) break;
     if (hipMalloc(&ctx->d_body, sizeof(melt_body)) != hipSuccess || hipMemset(ctx->d_body, 0, sizeof(melt_body)) != hipSuccess ||
         hipMalloc(&ctx->d_red, sizeof(double) * 16384) != hipSuccess || hipMalloc(&ctx->d_fmean, sizeof(double) * 4) != hipSuccess ||
         hipMemset(ctx->d_fmean, 0, sizeof(double) * 4) != hipSuccess || hipMalloc(&ctx->d_err, sizeof(int)) != hipSuccess ||
         hipMemset(ctx->d_err, 0, sizeof(int)) != hipSuccess || hipHostMalloc(&ctx->h_err, sizeof(int)) != hipSuccess ||
-        hipMalloc(&ctx->d_win, sizeof(melt_window) * 2) != hipSuccess || hipMemset(ctx->d_win, 0, sizeof(melt_window) * 2) != hipSuccess) {
+        hipMalloc(&ctx->d_win, sizeof(melt_window) * 2) != hipSuccess || hipMemset(ctx->d_win, 0, sizeof(melt_window) * 2) != hipSuccess ||
+        hipMalloc(&ctx->d_gate, sizeof(int)) != hipSuccess || hipMemset(ctx->d_gate, 0, sizeof(int)) != hipSuccess ||
+        hipMalloc(&ctx->d_refine_stats, sizeof(long long) * 8) != hipSuccess ||
+        hipMemset(ctx->d_refine_stats, 0, sizeof(long long) * 8) != hipSuccess) {
       rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "scratch allocation failed");
       break;
     }
@@ -182,8 +194,7 @@ extern "C" int melt_ctx_create(const melt_config* cfg, melt_ctx** out) {
     // single-phase defaults of init.f90:30-33 (after the zero fills above,
     // which are not ordered with this non-blocking stream)
     if (hipDeviceSynchronize() != hipSuccess) { rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "initial fill failed"); break; }
-    for (int id : {MELT_VOFX, MELT_VOFY, MELT_VOFZ, MELT_VOFP})
-      hipLaunchKernelGGL(k_fill_d, dim3(1024), dim3(256), 0, ctx->stream, ctx->f[id], 1.0, ctx->nc);
+    for (int id : {MELT_VOFX, MELT_VOFY, MELT_VOFZ, MELT_VOFP}) fill_field(ctx, id, 1.0);
     if ((rc = ibm_window_reset(ctx))) break;
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "initial fill failed");
   } while (0);
@@ -211,22 +222,42 @@ extern "C" int melt_sync(melt_ctx* ctx) {
 }
 
 // ---------------------------------------------------------- data movement
+// Host arrays are double; the precision form stores fields as float (the
+// conversion happens here, on the host).
+static int copy_to_device(melt_ctx* ctx, int field, const double* src) {
+  if (!ctx->fp32) {
+    MELT_HIP(hipMemcpy(ctx->f[field], src, sizeof(double) * ctx->nc, hipMemcpyHostToDevice));
+    return 0;
+  }
+  std::vector<float> t(ctx->nc);
+  for (size_t c = 0; c < ctx->nc; ++c) t[c] = (float)src[c];
+  MELT_HIP(hipMemcpy(ctx->f[field], t.data(), sizeof(float) * ctx->nc, hipMemcpyHostToDevice));
+  return 0;
+}
+
+static int copy_from_device(melt_ctx* ctx, int field, double* dst) {
+  if (!ctx->fp32) {
+    MELT_HIP(hipMemcpy(dst, ctx->f[field], sizeof(double) * ctx->nc, hipMemcpyDeviceToHost));
+    return 0;
+  }
+  std::vector<float> t(ctx->nc);
+  MELT_HIP(hipMemcpy(t.data(), ctx->f[field], sizeof(float) * ctx->nc, hipMemcpyDeviceToHost));
+  for (size_t c = 0; c < ctx->nc; ++c) dst[c] = (double)t[c];
+  return 0;
+}
+
 extern "C" int melt_upload_field(melt_ctx* ctx, int field, const double* host, const melt_layout* lay) {
   if (check_ctx(ctx) || field < 0 || field >= MELT_NFIELDS || !host) return MELT_ERR_ARG;
   melt_layout L = lay ? *lay : melt_layout{ctx->n1, ctx->n2, 0, 0};
   if (L.ld1 < ctx->n1 || L.ld2 < ctx->n2) return melt_fail(ctx, MELT_ERR_ARG, "upload", "leading dimension too small");
   MELT_HIP(hipStreamSynchronize(ctx->stream));
-  if (L.ld1 == ctx->n1 && L.ld2 == ctx->n2 && L.kghost == 0) {
-    MELT_HIP(hipMemcpy(ctx->f[field], host, sizeof(double) * ctx->nc, hipMemcpyHostToDevice));
-    return 0;
-  }
+  if (L.ld1 == ctx->n1 && L.ld2 == ctx->n2 && L.kghost == 0) return copy_to_device(ctx, field, host);
   std::vector<double> tmp(ctx->nc);
   for (int k = 0; k < ctx->n3; ++k)
     for (int j = 0; j < ctx->n2; ++j)
       memcpy(&tmp[(size_t)ctx->n1 * (j + (size_t)ctx->n2 * k)], &host[(size_t)L.ld1 * (j + (size_t)L.ld2 * (k + L.kghost))],
              sizeof(double) * ctx->n1);
-  MELT_HIP(hipMemcpy(ctx->f[field], tmp.data(), sizeof(double) * ctx->nc, hipMemcpyHostToDevice));
-  return 0;
+  return copy_to_device(ctx, field, tmp.data());
 }
 
 extern "C" int melt_download_field(melt_ctx* ctx, int field, double* host, const melt_layout* lay) {
@@ -234,12 +265,10 @@ extern "C" int melt_download_field(melt_ctx* ctx, int field, double* host, const
   melt_layout L = lay ? *lay : melt_layout{ctx->n1, ctx->n2, 0, 0};
   if (L.ld1 < ctx->n1 || L.ld2 < ctx->n2) return melt_fail(ctx, MELT_ERR_ARG, "download", "leading dimension too small");
   MELT_HIP(hipStreamSynchronize(ctx->stream));
-  if (L.ld1 == ctx->n1 && L.ld2 == ctx->n2 && L.kghost == 0) {
-    MELT_HIP(hipMemcpy(host, ctx->f[field], sizeof(double) * ctx->nc, hipMemcpyDeviceToHost));
-    return 0;
-  }
+  if (L.ld1 == ctx->n1 && L.ld2 == ctx->n2 && L.kghost == 0) return copy_from_device(ctx, field, host);
   std::vector<double> tmp(ctx->nc);
-  MELT_HIP(hipMemcpy(tmp.data(), ctx->f[field], sizeof(double) * ctx->nc, hipMemcpyDeviceToHost));
+  int rc = copy_from_device(ctx, field, tmp.data());
+  if (rc) return rc;
   int k0 = L.kghost ? -1 : 0, k1 = L.kghost ? ctx->n3 + 1 : ctx->n3;
   for (int k = k0; k < k1; ++k) {
     int ks = pmod(k, ctx->n3);
@@ -252,7 +281,7 @@ extern "C" int melt_download_field(melt_ctx* ctx, int field, double* host, const
 
 extern "C" int melt_fill_field(melt_ctx* ctx, int field, double value) {
   if (check_ctx(ctx) || field < 0 || field >= MELT_NFIELDS) return MELT_ERR_ARG;
-  hipLaunchKernelGGL(k_fill_d, dim3(1024), dim3(256), 0, ctx->stream, ctx->f[field], value, ctx->nc);
+  fill_field(ctx, field, value);
   MELT_HIP(hipGetLastError());
   return 0;
 }
@@ -267,7 +296,7 @@ extern "C" int melt_set_geometry(melt_ctx* ctx, int32_t nv, int32_t ne, int32_t 
   MELT_HIP(hipStreamSynchronize(ctx->stream));
   int rc = lagr_alloc(ctx, xyz0, vert_of_edge, edge_of_face, nullptr, rad_p);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_fill_d, dim3(1024), dim3(256), 0, ctx->stream, ctx->f[MELT_SDF], 999.0, ctx->nc);
+  fill_field(ctx, MELT_SDF, 999.0);
   MELT_HIP(hipGetLastError());
   ctx->has_geom = 1;
   MELT_HIP(hipMemcpyAsync(&ctx->h_body, ctx->d_body, sizeof(melt_body), hipMemcpyDeviceToHost, ctx->stream));
@@ -378,11 +407,11 @@ static int invtr(melt_ctx* ctx, int which) {
   static const int q_id[4] = {MELT_VX, MELT_VY, MELT_VZ, MELT_TEMP};
   rc = euler_launch_invtr_rhs(ctx, which);
   if (rc) return rc;
-  double* r[1] = {ctx->f[MELT_RHS]};
+  void* r[1] = {ctx->f[MELT_RHS]};
   double b0 = betadx_q(ctx, which, 0), b1 = betadx_q(ctx, which, 1), b2 = betadx_q(ctx, which, 2);
   if ((rc = euler_launch_solve(ctx, 0, r, 1, &b0, nullptr, 1))) return rc;
   if ((rc = euler_launch_solve(ctx, 1, r, 1, &b1, nullptr, 1))) return rc;
-  double* q[1] = {ctx->f[q_id[which]]};
+  void* q[1] = {ctx->f[q_id[which]]};
   return euler_launch_solve(ctx, 2, r, 1, &b2, q, 1);
 }
 extern "C" int melt_invtr1(melt_ctx* ctx) { return invtr(ctx, 0); }
@@ -392,18 +421,18 @@ extern "C" int melt_invtrte(melt_ctx* ctx) { return invtr(ctx, 3); }
 
 extern "C" int melt_solxi(melt_ctx* ctx, double betadx) {
   if (check_ctx(ctx)) return MELT_ERR_ARG;
-  double* r[1] = {ctx->f[MELT_RHS]};
+  void* r[1] = {ctx->f[MELT_RHS]};
   return euler_launch_solve(ctx, 0, r, 1, &betadx, nullptr, 1);
 }
 extern "C" int melt_solxj(melt_ctx* ctx, double betadx) {
   if (check_ctx(ctx)) return MELT_ERR_ARG;
-  double* r[1] = {ctx->f[MELT_RHS]};
+  void* r[1] = {ctx->f[MELT_RHS]};
   return euler_launch_solve(ctx, 1, r, 1, &betadx, nullptr, 1);
 }
 extern "C" int melt_solxk(melt_ctx* ctx, int field, double betadx) {
   if (check_ctx(ctx) || field < 0 || field >= MELT_NFIELDS || field == MELT_RHS) return MELT_ERR_ARG;
-  double* r[1] = {ctx->f[MELT_RHS]};
-  double* q[1] = {ctx->f[field]};
+  void* r[1] = {ctx->f[MELT_RHS]};
+  void* q[1] = {ctx->f[field]};
   return euler_launch_solve(ctx, 2, r, 1, &betadx, q, 1);
 }
 
@@ -490,8 +519,8 @@ extern "C" int melt_substep(melt_ctx* ctx, int32_t ns) {
   timing_begin(ctx, SLOT_EXPLICIT, &e);
   if ((rc = euler_launch_explicit_xsolve(ctx, betadx_q(ctx, 0, 0), betadx_q(ctx, 3, 0)))) return rc;
   timing_end(ctx, SLOT_EXPLICIT, e);
-  double* r[4] = {ctx->rhs4[RHS_U], ctx->rhs4[RHS_V], ctx->rhs4[RHS_W], ctx->rhs4[RHS_T]};
-  double* q[4] = {ctx->f[MELT_VX], ctx->f[MELT_VY], ctx->f[MELT_VZ], ctx->f[MELT_TEMP]};
+  void* r[4] = {ctx->rhs4[RHS_U], ctx->rhs4[RHS_V], ctx->rhs4[RHS_W], ctx->rhs4[RHS_T]};
+  void* q[4] = {ctx->f[MELT_VX], ctx->f[MELT_VY], ctx->f[MELT_VZ], ctx->f[MELT_TEMP]};
   static const int comp[4] = {0, 1, 2, 3};
   for (int dir = 1; dir < 3; ++dir) {
     double b[4];
@@ -517,10 +546,27 @@ extern "C" int melt_substep(melt_ctx* ctx, int32_t ns) {
   timing_end(ctx, SLOT_PH_MLS, e_ph);
 
   timing_begin(ctx, SLOT_PH_PRESSURE, &e_ph);
+  // the precision form measures the FP64 divergence of the provisional field
+  // (before) and of the projected one (after) for its refinement gate
+  unsigned long long* dmax = reinterpret_cast<unsigned long long*>(ctx->d_refine_stats + 4);
+  const bool refine = ctx->fp32 && ctx->refine_max > 0;
+  if (refine && (rc = euler_launch_reduce_max(ctx, 1, dmax))) return rc;
   if ((rc = poisson_solve(ctx, 1))) return rc;
   timing_begin(ctx, SLOT_PROJECT, &e);
   if ((rc = euler_launch_project(ctx, 1, 1, ibm ? 1 : 0))) return rc;
   timing_end(ctx, SLOT_PROJECT, e);
+  if (refine) {
+    // iterative refinement: while the FP64 residual div(u) exceeds rtol times
+    // the provisional one, solve for the correction of the FP32 solve and
+    // project again (updvp/prcalc are linear in phi); gated on the device
+    timing_begin(ctx, SLOT_REFINE, &e);
+    for (int it = 0; it < ctx->refine_max; ++it) {
+      if ((rc = euler_launch_reduce_max(ctx, 1, dmax + 1)) || (rc = euler_launch_refine_gate(ctx, dmax, ctx->d_gate)) ||
+          (rc = poisson_solve(ctx, 1, ctx->d_gate)) || (rc = euler_launch_project(ctx, 1, 1, 0, ctx->d_gate)))
+        return rc;
+    }
+    timing_end(ctx, SLOT_REFINE, e);
+  }
   timing_end(ctx, SLOT_PH_PRESSURE, e_ph);
   timing_end(ctx, SLOT_SUBSTEP, e_sub);
   if (ctx->timing && ctx->ev_pending.size() > 4096) timing_flush(ctx);
@@ -574,5 +620,24 @@ extern "C" int melt_timing_read(melt_ctx* ctx, double* ms, int64_t* samples) {
   if (check_ctx(ctx) || !ms || !samples) return MELT_ERR_ARG;
   timing_flush(ctx);
   for (int i = 0; i < SLOT_COUNT; ++i) { ms[i] = ctx->t_ms[i]; samples[i] = ctx->t_n[i]; }
+  return 0;
+}
+
+// ------------------------------------------------------ precision form
+extern "C" int melt_set_refinement(melt_ctx* ctx, int32_t max_iter, double rtol) {
+  if (check_ctx(ctx) || max_iter < 0 || max_iter > 8 || !(rtol >= 0.0)) return MELT_ERR_ARG;
+  ctx->refine_max = max_iter;
+  ctx->refine_rtol = rtol;
+  return 0;
+}
+
+extern "C" int melt_get_refinement_stats(melt_ctx* ctx, int64_t* checks, int64_t* refined, double* last_ratio) {
+  if (check_ctx(ctx) || !checks || !refined || !last_ratio) return MELT_ERR_ARG;
+  long long h[3];
+  MELT_HIP(hipMemcpyAsync(h, ctx->d_refine_stats, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+  MELT_HIP(hipStreamSynchronize(ctx->stream));
+  *checks = h[0];
+  *refined = h[1];
+  memcpy(last_ratio, &h[2], sizeof(double));
   return 0;
 }

@@ -29,7 +29,7 @@ enum { RHS_U = 0, RHS_V, RHS_W, RHS_T };
 enum melt_slot {
   SLOT_TAG = 0, SLOT_EXPLICIT, SLOT_SOLVE_X, SLOT_SOLVE_Y, SLOT_SOLVE_Z, SLOT_IBM, SLOT_LAGR,
   SLOT_DIVG_FFTX, SLOT_FFTY, SLOT_FFTZ_SOLVE, SLOT_IFFTY, SLOT_IFFTX, SLOT_PROJECT,
-  SLOT_PH_TAG, SLOT_PH_EULER, SLOT_PH_MLS, SLOT_PH_PRESSURE, SLOT_SUBSTEP,
+  SLOT_PH_TAG, SLOT_PH_EULER, SLOT_PH_MLS, SLOT_PH_PRESSURE, SLOT_SUBSTEP, SLOT_REFINE,
   SLOT_COUNT
 };
 
@@ -67,11 +67,20 @@ struct melt_ctx {
   hipStream_t stream = nullptr;
   int device = 0;
 
-  double* f[MELT_NFIELDS] = {};
-  double* rhs4[4] = {};
-  double2* spec = nullptr;
+  // Eulerian fields: double, or float in the precision form (fp32 = 1); fld<R>()
+  int fp32 = 0;
+  size_t esz = sizeof(double);
+  void* f[MELT_NFIELDS] = {};
+  void* rhs4[4] = {};
+  void* spec = nullptr;                                    // complex, element type as the fields
   double *ak1 = nullptr, *ak2 = nullptr, *ak3 = nullptr;   // 0-based wavenumber index
-  double2 *tw1 = nullptr, *tw2 = nullptr, *tw3 = nullptr;  // exp(-2 pi i t / N)
+  void *tw1 = nullptr, *tw2 = nullptr, *tw3 = nullptr;     // exp(-2 pi i t / N), complex of the field type
+  // precision-form pressure refinement: up to refine_max correction solves while
+  // the FP64 divergence after projection exceeds refine_rtol times the one before
+  int refine_max = 1;
+  double refine_rtol = 1e-4;
+  int* d_gate = nullptr;                 // device flag read by the gated refinement kernels
+  long long* d_refine_stats = nullptr;   // [checks, refinements, bits of the last ratio]
 
   // body / mesh
   int has_geom = 0;
@@ -122,6 +131,10 @@ struct melt_ctx {
 };
 
 int melt_fail(melt_ctx* ctx, int code, const char* what, const char* detail);
+
+// Typed view of an Eulerian field (R = double, or float in the precision form).
+template <typename R>
+inline R* fld(melt_ctx* c, int id) { return static_cast<R*>(c->f[id]); }
 
 // ---------------------------------------------------------- device helpers
 struct Grid {
@@ -358,15 +371,17 @@ __device__ inline unsigned xcd_block(unsigned b, unsigned nb) {
 // host-side helpers implemented in the kernel files
 int euler_launch_hdnl(melt_ctx* ctx, int which);
 int euler_launch_invtr_rhs(melt_ctx* ctx, int which);
-int euler_launch_solve(melt_ctx* ctx, int dir, double* const* fields, int nfield, const double* betadx, double* const* qadd, int store_rhs);
+int euler_launch_solve(melt_ctx* ctx, int dir, void* const* fields, int nfield, const double* betadx, void* const* qadd, int store_rhs);
 int euler_launch_explicit_xsolve(melt_ctx* ctx, double bxu, double bxt);
 int euler_launch_divg(melt_ctx* ctx);
-int euler_launch_project(melt_ctx* ctx, int do_u, int do_p, int use_mean);
+int euler_launch_project(melt_ctx* ctx, int do_u, int do_p, int use_mean, const int* gate = nullptr);
+int euler_launch_reduce_max(melt_ctx* ctx, int what, unsigned long long* d);
+int euler_launch_refine_gate(melt_ctx* ctx, const unsigned long long* dmax, int* gate);
 int euler_launch_interp_vof(melt_ctx* ctx);
 int euler_reduce_max(melt_ctx* ctx, int what, double* out);
 int euler_launch_inqpr(melt_ctx* ctx);
 int poisson_init(melt_ctx* ctx);
-int poisson_solve(melt_ctx* ctx, int fused_divg);
+int poisson_solve(melt_ctx* ctx, int fused_divg, const int* gate = nullptr);
 int ibm_tag(melt_ctx* ctx);
 int ibm_find_centroid_indices(melt_ctx* ctx);
 int ibm_weights(melt_ctx* ctx);

@@ -41,72 +41,80 @@ __device__ __forceinline__ Cell make_cell(const Grid& g, int c) {
   return cell_at(g, i, r % g.n2, r / g.n2);
 }
 
+// Field storage type R (double; float in the precision form).  The stencil
+// arithmetic runs in R; geometry (rigid-body velocity at the cell faces) in
+// double.  For R = double every expression is the reference's, in its order.
+template <typename R>
 struct EulerArgs {
-  const double* __restrict__ vx;
-  const double* __restrict__ vy;
-  const double* __restrict__ vz;
-  const double* __restrict__ temp;
-  const double* __restrict__ pr;
-  const double* __restrict__ vofp;
-  const double* __restrict__ forcx;
-  const double* __restrict__ forcy;
-  const double* __restrict__ forcz;
+  const R* __restrict__ vx;
+  const R* __restrict__ vy;
+  const R* __restrict__ vz;
+  const R* __restrict__ temp;
+  const R* __restrict__ pr;
+  const R* __restrict__ vofp;
+  const R* __restrict__ forcx;
+  const R* __restrict__ forcy;
+  const R* __restrict__ forcz;
   const melt_body* __restrict__ body;
   Grid g;
   double betagz, tliq;
 };
 
 // hdnl1.f90:32-74
-__device__ inline double h1_val(const EulerArgs& A, const Cell& s, double vofx) {
-  const double* vx = A.vx; const double* vy = A.vy; const double* vz = A.vz;
-  double udx1 = A.g.dx1 * 0.25, udx2 = A.g.dx2 * 0.25, udx3 = A.g.dx3 * 0.25;
-  double h11 = ((vx[s.at(1, 0, 0)] + vx[s.at(0, 0, 0)]) * (vx[s.at(1, 0, 0)] + vx[s.at(0, 0, 0)]) -
-                (vx[s.at(-1, 0, 0)] + vx[s.at(0, 0, 0)]) * (vx[s.at(-1, 0, 0)] + vx[s.at(0, 0, 0)])) * udx1;
-  double h12 = ((vy[s.at(0, 1, 0)] + vy[s.at(-1, 1, 0)]) * (vx[s.at(0, 1, 0)] + vx[s.at(0, 0, 0)]) -
-                (vy[s.at(0, 0, 0)] + vy[s.at(-1, 0, 0)]) * (vx[s.at(0, 0, 0)] + vx[s.at(0, -1, 0)])) * udx2;
-  double h13 = ((vz[s.at(0, 0, 1)] + vz[s.at(-1, 0, 1)]) * (vx[s.at(0, 0, 1)] + vx[s.at(0, 0, 0)]) -
-                (vz[s.at(0, 0, 0)] + vz[s.at(-1, 0, 0)]) * (vx[s.at(0, 0, 0)] + vx[s.at(0, 0, -1)])) * udx3;
-  return -(h11 + h12 + h13) + vofx * A.forcx[s.at(0, 0, 0)] / A.g.xlen;
+template <typename R>
+__device__ inline R h1_val(const EulerArgs<R>& A, const Cell& s, R vofx) {
+  const R* vx = A.vx; const R* vy = A.vy; const R* vz = A.vz;
+  R udx1 = R(A.g.dx1) * R(0.25), udx2 = R(A.g.dx2) * R(0.25), udx3 = R(A.g.dx3) * R(0.25);
+  R h11 = ((vx[s.at(1, 0, 0)] + vx[s.at(0, 0, 0)]) * (vx[s.at(1, 0, 0)] + vx[s.at(0, 0, 0)]) -
+           (vx[s.at(-1, 0, 0)] + vx[s.at(0, 0, 0)]) * (vx[s.at(-1, 0, 0)] + vx[s.at(0, 0, 0)])) * udx1;
+  R h12 = ((vy[s.at(0, 1, 0)] + vy[s.at(-1, 1, 0)]) * (vx[s.at(0, 1, 0)] + vx[s.at(0, 0, 0)]) -
+           (vy[s.at(0, 0, 0)] + vy[s.at(-1, 0, 0)]) * (vx[s.at(0, 0, 0)] + vx[s.at(0, -1, 0)])) * udx2;
+  R h13 = ((vz[s.at(0, 0, 1)] + vz[s.at(-1, 0, 1)]) * (vx[s.at(0, 0, 1)] + vx[s.at(0, 0, 0)]) -
+           (vz[s.at(0, 0, 0)] + vz[s.at(-1, 0, 0)]) * (vx[s.at(0, 0, 0)] + vx[s.at(0, 0, -1)])) * udx3;
+  return -(h11 + h12 + h13) + vofx * A.forcx[s.at(0, 0, 0)] / R(A.g.xlen);
 }
 
 // hdnl2.f90:33-75
-__device__ inline double h2_val(const EulerArgs& A, const Cell& s, double vofy) {
-  const double* vx = A.vx; const double* vy = A.vy; const double* vz = A.vz;
-  double udx1 = A.g.dx1 * 0.25, udx2 = A.g.dx2 * 0.25, udx3 = A.g.dx3 * 0.25;
-  double h21 = ((vy[s.at(1, 0, 0)] + vy[s.at(0, 0, 0)]) * (vx[s.at(1, 0, 0)] + vx[s.at(1, -1, 0)]) -
-                (vy[s.at(0, 0, 0)] + vy[s.at(-1, 0, 0)]) * (vx[s.at(0, 0, 0)] + vx[s.at(0, -1, 0)])) * udx1;
-  double h22 = ((vy[s.at(0, 1, 0)] + vy[s.at(0, 0, 0)]) * (vy[s.at(0, 1, 0)] + vy[s.at(0, 0, 0)]) -
-                (vy[s.at(0, -1, 0)] + vy[s.at(0, 0, 0)]) * (vy[s.at(0, -1, 0)] + vy[s.at(0, 0, 0)])) * udx2;
-  double h23 = ((vz[s.at(0, 0, 1)] + vz[s.at(0, -1, 1)]) * (vy[s.at(0, 0, 1)] + vy[s.at(0, 0, 0)]) -
-                (vz[s.at(0, 0, 0)] + vz[s.at(0, -1, 0)]) * (vy[s.at(0, 0, 0)] + vy[s.at(0, 0, -1)])) * udx3;
-  return -(h21 + h22 + h23) + vofy * A.forcy[s.at(0, 0, 0)] / A.g.ylen;
+template <typename R>
+__device__ inline R h2_val(const EulerArgs<R>& A, const Cell& s, R vofy) {
+  const R* vx = A.vx; const R* vy = A.vy; const R* vz = A.vz;
+  R udx1 = R(A.g.dx1) * R(0.25), udx2 = R(A.g.dx2) * R(0.25), udx3 = R(A.g.dx3) * R(0.25);
+  R h21 = ((vy[s.at(1, 0, 0)] + vy[s.at(0, 0, 0)]) * (vx[s.at(1, 0, 0)] + vx[s.at(1, -1, 0)]) -
+           (vy[s.at(0, 0, 0)] + vy[s.at(-1, 0, 0)]) * (vx[s.at(0, 0, 0)] + vx[s.at(0, -1, 0)])) * udx1;
+  R h22 = ((vy[s.at(0, 1, 0)] + vy[s.at(0, 0, 0)]) * (vy[s.at(0, 1, 0)] + vy[s.at(0, 0, 0)]) -
+           (vy[s.at(0, -1, 0)] + vy[s.at(0, 0, 0)]) * (vy[s.at(0, -1, 0)] + vy[s.at(0, 0, 0)])) * udx2;
+  R h23 = ((vz[s.at(0, 0, 1)] + vz[s.at(0, -1, 1)]) * (vy[s.at(0, 0, 1)] + vy[s.at(0, 0, 0)]) -
+           (vz[s.at(0, 0, 0)] + vz[s.at(0, -1, 0)]) * (vy[s.at(0, 0, 0)] + vy[s.at(0, 0, -1)])) * udx3;
+  return -(h21 + h22 + h23) + vofy * A.forcy[s.at(0, 0, 0)] / R(A.g.ylen);
 }
 
 // hdnl3.f90:38-82
-__device__ inline double h3_val(const EulerArgs& A, const Cell& s, double vofz) {
-  const double* vx = A.vx; const double* vy = A.vy; const double* vz = A.vz;
-  double udx1 = A.g.dx1 * 0.25, udx2 = A.g.dx2 * 0.25, udx3 = A.g.dx3 * 0.25;
-  double h31 = (((vx[s.at(1, 0, 0)] + vx[s.at(1, 0, -1)]) * (vz[s.at(1, 0, 0)] + vz[s.at(0, 0, 0)])) -
-                ((vx[s.at(0, 0, 0)] + vx[s.at(0, 0, -1)]) * (vz[s.at(0, 0, 0)] + vz[s.at(-1, 0, 0)]))) * udx1;
-  double h32 = (((vy[s.at(0, 1, 0)] + vy[s.at(0, 1, -1)]) * (vz[s.at(0, 1, 0)] + vz[s.at(0, 0, 0)])) -
-                ((vy[s.at(0, 0, 0)] + vy[s.at(0, 0, -1)]) * (vz[s.at(0, 0, 0)] + vz[s.at(0, -1, 0)]))) * udx2;
-  double h33 = (((vz[s.at(0, 0, 1)] + vz[s.at(0, 0, 0)]) * (vz[s.at(0, 0, 1)] + vz[s.at(0, 0, 0)])) -
-                ((vz[s.at(0, 0, 0)] + vz[s.at(0, 0, -1)]) * (vz[s.at(0, 0, 0)] + vz[s.at(0, 0, -1)]))) * udx3;
-  double T_interp = (A.temp[s.at(0, 0, 0)] + A.temp[s.at(0, 0, -1)]) * 0.5;
-  double fbz = A.betagz * (T_interp - A.tliq);
-  return (-(h31 + h32 + h33) + fbz) + vofz * A.forcz[s.at(0, 0, 0)] / A.g.zlen;
+template <typename R>
+__device__ inline R h3_val(const EulerArgs<R>& A, const Cell& s, R vofz) {
+  const R* vx = A.vx; const R* vy = A.vy; const R* vz = A.vz;
+  R udx1 = R(A.g.dx1) * R(0.25), udx2 = R(A.g.dx2) * R(0.25), udx3 = R(A.g.dx3) * R(0.25);
+  R h31 = (((vx[s.at(1, 0, 0)] + vx[s.at(1, 0, -1)]) * (vz[s.at(1, 0, 0)] + vz[s.at(0, 0, 0)])) -
+           ((vx[s.at(0, 0, 0)] + vx[s.at(0, 0, -1)]) * (vz[s.at(0, 0, 0)] + vz[s.at(-1, 0, 0)]))) * udx1;
+  R h32 = (((vy[s.at(0, 1, 0)] + vy[s.at(0, 1, -1)]) * (vz[s.at(0, 1, 0)] + vz[s.at(0, 0, 0)])) -
+           ((vy[s.at(0, 0, 0)] + vy[s.at(0, 0, -1)]) * (vz[s.at(0, 0, 0)] + vz[s.at(0, -1, 0)]))) * udx2;
+  R h33 = (((vz[s.at(0, 0, 1)] + vz[s.at(0, 0, 0)]) * (vz[s.at(0, 0, 1)] + vz[s.at(0, 0, 0)])) -
+           ((vz[s.at(0, 0, 0)] + vz[s.at(0, 0, -1)]) * (vz[s.at(0, 0, 0)] + vz[s.at(0, 0, -1)]))) * udx3;
+  R T_interp = (A.temp[s.at(0, 0, 0)] + A.temp[s.at(0, 0, -1)]) * R(0.5);
+  R fbz = R(A.betagz) * (T_interp - R(A.tliq));
+  return (-(h31 + h32 + h33) + fbz) + vofz * A.forcz[s.at(0, 0, 0)] / R(A.g.zlen);
 }
 
 // hdnlte.f90:45-158 (Nparticle = 1)
-__device__ inline double ht_val(const EulerArgs& A, const Cell& s, double vofp) {
-  const double* vx = A.vx; const double* vy = A.vy; const double* vz = A.vz; const double* t = A.temp;
+template <typename R>
+__device__ inline R ht_val(const EulerArgs<R>& A, const Cell& s, R vofp) {
+  const R* vx = A.vx; const R* vy = A.vy; const R* vz = A.vz; const R* t = A.temp;
   const Grid& g = A.g;
-  double udx1 = g.dx1 * 0.5, udx2 = g.dx2 * 0.5, udx3 = g.dx3 * 0.5;
-  double T = t[s.at(0, 0, 0)];
-  double h31 = (vx[s.at(1, 0, 0)] * (t[s.at(1, 0, 0)] + T) - vx[s.at(0, 0, 0)] * (T + t[s.at(-1, 0, 0)])) * udx1;
-  double h32 = (vy[s.at(0, 1, 0)] * (t[s.at(0, 1, 0)] + T) - vy[s.at(0, 0, 0)] * (T + t[s.at(0, -1, 0)])) * udx2;
-  double h33 = (vz[s.at(0, 0, 1)] * (t[s.at(0, 0, 1)] + T) - vz[s.at(0, 0, 0)] * (T + t[s.at(0, 0, -1)])) * udx3;
-  double ht = -(h31 + h32 + h33) * vofp;
+  R udx1 = R(g.dx1) * R(0.5), udx2 = R(g.dx2) * R(0.5), udx3 = R(g.dx3) * R(0.5);
+  R T = t[s.at(0, 0, 0)];
+  R h31 = (vx[s.at(1, 0, 0)] * (t[s.at(1, 0, 0)] + T) - vx[s.at(0, 0, 0)] * (T + t[s.at(-1, 0, 0)])) * udx1;
+  R h32 = (vy[s.at(0, 1, 0)] * (t[s.at(0, 1, 0)] + T) - vy[s.at(0, 0, 0)] * (T + t[s.at(0, -1, 0)])) * udx2;
+  R h33 = (vz[s.at(0, 0, 1)] * (t[s.at(0, 0, 1)] + T) - vz[s.at(0, 0, 0)] * (T + t[s.at(0, 0, -1)])) * udx3;
+  R ht = -(h31 + h32 + h33) * vofp;
   const double* pc = A.body->pos_cm;
   const double* vc = A.body->vel_cm;
   const double* om = A.body->omega_c;
@@ -116,108 +124,116 @@ __device__ inline double ht_val(const EulerArgs& A, const Cell& s, double vofp) 
   r1 = r1 - g.ylen * (double)fnint(r1 / g.ylen);
   r2 = r2 - g.zlen * (double)fnint(r2 / g.zlen);
   double u_imh = vc[0] + om[1] * r2 - om[2] * r1;
-  h31 = u_imh * (t[s.at(1, 0, 0)] - t[s.at(-1, 0, 0)]) * udx1;
+  h31 = R(u_imh) * (t[s.at(1, 0, 0)] - t[s.at(-1, 0, 0)]) * udx1;
   r0 = gxm(g, ic) - pc[0]; r1 = gyc(g, jc) - pc[1]; r2 = gzm(g, kc) - pc[2];
   r0 = r0 - g.xlen * (double)fnint(r0 / g.xlen);
   r1 = r1 - g.ylen * (double)fnint(r1 / g.ylen);
   r2 = r2 - g.zlen * (double)fnint(r2 / g.zlen);
   double v_jmh = vc[1] + om[2] * r0 - om[0] * r2;
-  h32 = v_jmh * (t[s.at(0, 1, 0)] - t[s.at(0, -1, 0)]) * udx2;
+  h32 = R(v_jmh) * (t[s.at(0, 1, 0)] - t[s.at(0, -1, 0)]) * udx2;
   r0 = gxm(g, ic) - pc[0]; r1 = gym(g, jc) - pc[1]; r2 = gzc(g, kc) - pc[2];
   double w_kmh = vc[2] + om[0] * r1 - om[1] * r0;
-  h33 = w_kmh * (t[s.at(0, 0, 1)] - t[s.at(0, 0, -1)]) * udx3;
-  return ht - (h31 + h32 + h33) * (1.0 - vofp);
+  h33 = R(w_kmh) * (t[s.at(0, 0, 1)] - t[s.at(0, 0, -1)]) * udx3;
+  return ht - (h31 + h32 + h33) * (R(1) - vofp);
 }
 
+template <typename R>
 struct RhsCoef {
-  double ga, ro, al, dt, alre, dx1q, dx2q, dx3q, dx1, dx2, dx3, pec;
+  R ga, ro, al, dt, alre, dx1q, dx2q, dx3q, dx1, dx2, dx3, pec;
 };
 
 // invtr1.f90:40-60
-__device__ inline double rhs_u(const RhsCoef& c, const double* __restrict__ q, const double* __restrict__ pr, const Cell& s, double h, double ru) {
-  double v = q[s.at(0, 0, 0)];
-  double d11 = (q[s.at(1, 0, 0)] - 2.0 * v + q[s.at(-1, 0, 0)]) * c.dx1q;
-  double d22 = (q[s.at(0, 1, 0)] - 2.0 * v + q[s.at(0, -1, 0)]) * c.dx2q;
-  double d33 = (q[s.at(0, 0, 1)] - 2.0 * v + q[s.at(0, 0, -1)]) * c.dx3q;
-  double dc = d11 + d22 + d33;
-  double dp = (pr[s.at(0, 0, 0)] - pr[s.at(-1, 0, 0)]) * (c.dx1 * c.al);
+template <typename R>
+__device__ inline R rhs_u(const RhsCoef<R>& c, const R* __restrict__ q, const R* __restrict__ pr, const Cell& s, R h, R ru) {
+  R v = q[s.at(0, 0, 0)];
+  R d11 = (q[s.at(1, 0, 0)] - R(2) * v + q[s.at(-1, 0, 0)]) * c.dx1q;
+  R d22 = (q[s.at(0, 1, 0)] - R(2) * v + q[s.at(0, -1, 0)]) * c.dx2q;
+  R d33 = (q[s.at(0, 0, 1)] - R(2) * v + q[s.at(0, 0, -1)]) * c.dx3q;
+  R dc = d11 + d22 + d33;
+  R dp = (pr[s.at(0, 0, 0)] - pr[s.at(-1, 0, 0)]) * (c.dx1 * c.al);
   return (c.ga * h + c.ro * ru + c.alre * dc - dp) * c.dt;
 }
 // invtr2.f90:37-57
-__device__ inline double rhs_v(const RhsCoef& c, const double* __restrict__ q, const double* __restrict__ pr, const Cell& s, double h, double ru) {
-  double v = q[s.at(0, 0, 0)];
-  double d11 = (q[s.at(1, 0, 0)] - 2.0 * v + q[s.at(-1, 0, 0)]) * c.dx1q;
-  double d22 = (q[s.at(0, 1, 0)] - 2.0 * v + q[s.at(0, -1, 0)]) * c.dx2q;
-  double d33 = (q[s.at(0, 0, 1)] - 2.0 * v + q[s.at(0, 0, -1)]) * c.dx3q;
-  double dc = d11 + d22 + d33;
-  double dp = (pr[s.at(0, 0, 0)] - pr[s.at(0, -1, 0)]) * (c.dx2 * c.al);
+template <typename R>
+__device__ inline R rhs_v(const RhsCoef<R>& c, const R* __restrict__ q, const R* __restrict__ pr, const Cell& s, R h, R ru) {
+  R v = q[s.at(0, 0, 0)];
+  R d11 = (q[s.at(1, 0, 0)] - R(2) * v + q[s.at(-1, 0, 0)]) * c.dx1q;
+  R d22 = (q[s.at(0, 1, 0)] - R(2) * v + q[s.at(0, -1, 0)]) * c.dx2q;
+  R d33 = (q[s.at(0, 0, 1)] - R(2) * v + q[s.at(0, 0, -1)]) * c.dx3q;
+  R dc = d11 + d22 + d33;
+  R dp = (pr[s.at(0, 0, 0)] - pr[s.at(0, -1, 0)]) * (c.dx2 * c.al);
   return (c.ga * h + c.ro * ru + c.alre * dc - dp) * c.dt;
 }
 // invtr3.f90:35-56
-__device__ inline double rhs_w(const RhsCoef& c, const double* __restrict__ q, const double* __restrict__ pr, const Cell& s, double h, double ru) {
-  double v = q[s.at(0, 0, 0)];
-  double d1 = (q[s.at(-1, 0, 0)] - 2.0 * v + q[s.at(1, 0, 0)]) * c.dx1q;
-  double d2 = (q[s.at(0, -1, 0)] - 2.0 * v + q[s.at(0, 1, 0)]) * c.dx2q;
-  double d3 = (q[s.at(0, 0, 1)] - 2.0 * v + q[s.at(0, 0, -1)]) * c.dx3q;
-  double dc = d2 + d3 + d1;
-  double dp = (pr[s.at(0, 0, 0)] - pr[s.at(0, 0, -1)]) * (c.al * c.dx3);
+template <typename R>
+__device__ inline R rhs_w(const RhsCoef<R>& c, const R* __restrict__ q, const R* __restrict__ pr, const Cell& s, R h, R ru) {
+  R v = q[s.at(0, 0, 0)];
+  R d1 = (q[s.at(-1, 0, 0)] - R(2) * v + q[s.at(1, 0, 0)]) * c.dx1q;
+  R d2 = (q[s.at(0, -1, 0)] - R(2) * v + q[s.at(0, 1, 0)]) * c.dx2q;
+  R d3 = (q[s.at(0, 0, 1)] - R(2) * v + q[s.at(0, 0, -1)]) * c.dx3q;
+  R dc = d2 + d3 + d1;
+  R dp = (pr[s.at(0, 0, 0)] - pr[s.at(0, 0, -1)]) * (c.al * c.dx3);
   return (c.ga * h + c.ro * ru + c.alre * dc - dp) * c.dt;
 }
 // invtrte.f90:32-53
-__device__ inline double rhs_t(const RhsCoef& c, const double* __restrict__ q, const Cell& s, double h, double ru) {
-  double v = q[s.at(0, 0, 0)];
-  double d1 = (q[s.at(-1, 0, 0)] - 2.0 * v + q[s.at(1, 0, 0)]) * c.dx1q;
-  double d2 = (q[s.at(0, -1, 0)] - 2.0 * v + q[s.at(0, 1, 0)]) * c.dx2q;
-  double d3 = (q[s.at(0, 0, 1)] - 2.0 * v + q[s.at(0, 0, -1)]) * c.dx3q;
-  double dc = d2 + d3 + d1;
+template <typename R>
+__device__ inline R rhs_t(const RhsCoef<R>& c, const R* __restrict__ q, const Cell& s, R h, R ru) {
+  R v = q[s.at(0, 0, 0)];
+  R d1 = (q[s.at(-1, 0, 0)] - R(2) * v + q[s.at(1, 0, 0)]) * c.dx1q;
+  R d2 = (q[s.at(0, -1, 0)] - R(2) * v + q[s.at(0, 1, 0)]) * c.dx2q;
+  R d3 = (q[s.at(0, 0, 1)] - R(2) * v + q[s.at(0, 0, -1)]) * c.dx3q;
+  R dc = d2 + d3 + d1;
   return c.dt * (c.ga * h + c.ro * ru) + (c.al / c.pec) * dc * c.dt;
 }
 
-EulerArgs make_args(melt_ctx* ctx) {
-  EulerArgs A;
-  A.vx = ctx->f[MELT_VX]; A.vy = ctx->f[MELT_VY]; A.vz = ctx->f[MELT_VZ];
-  A.temp = ctx->f[MELT_TEMP]; A.pr = ctx->f[MELT_PR]; A.vofp = ctx->f[MELT_VOFP];
-  A.forcx = ctx->f[MELT_FORCX]; A.forcy = ctx->f[MELT_FORCY]; A.forcz = ctx->f[MELT_FORCZ];
+template <typename R>
+EulerArgs<R> make_args(melt_ctx* ctx) {
+  EulerArgs<R> A;
+  A.vx = fld<R>(ctx, MELT_VX); A.vy = fld<R>(ctx, MELT_VY); A.vz = fld<R>(ctx, MELT_VZ);
+  A.temp = fld<R>(ctx, MELT_TEMP); A.pr = fld<R>(ctx, MELT_PR); A.vofp = fld<R>(ctx, MELT_VOFP);
+  A.forcx = fld<R>(ctx, MELT_FORCX); A.forcy = fld<R>(ctx, MELT_FORCY); A.forcz = fld<R>(ctx, MELT_FORCZ);
   A.body = ctx->d_body;
   A.g = make_grid(ctx);
   A.betagz = ctx->cfg.betagz; A.tliq = ctx->cfg.tliq;
   return A;
 }
 
-RhsCoef make_coef(melt_ctx* ctx) {
-  RhsCoef c;
-  c.ga = ctx->ga; c.ro = ctx->ro; c.al = ctx->al; c.dt = ctx->dt;
-  c.alre = ctx->al / ctx->cfg.ren;
-  c.dx1q = ctx->dx1q; c.dx2q = ctx->dx2q; c.dx3q = ctx->dx3q;
-  c.dx1 = ctx->dx1; c.dx2 = ctx->dx2; c.dx3 = ctx->dx3;
-  c.pec = ctx->cfg.ren * ctx->cfg.prandtl;
+template <typename R>
+RhsCoef<R> make_coef(melt_ctx* ctx) {
+  RhsCoef<R> c;
+  c.ga = R(ctx->ga); c.ro = R(ctx->ro); c.al = R(ctx->al); c.dt = R(ctx->dt);
+  c.alre = R(ctx->al / ctx->cfg.ren);
+  c.dx1q = R(ctx->dx1q); c.dx2q = R(ctx->dx2q); c.dx3q = R(ctx->dx3q);
+  c.dx1 = R(ctx->dx1); c.dx2 = R(ctx->dx2); c.dx3 = R(ctx->dx3);
+  c.pec = R(ctx->cfg.ren * ctx->cfg.prandtl);
   return c;
 }
 
 // ------------------------------------------------------------- kernels
-__global__ __launch_bounds__(256) void k_hdnl(EulerArgs A, double* __restrict__ out, int which, size_t nc) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_hdnl(EulerArgs<R> A, R* __restrict__ out, int which, int nc) {
   unsigned b = xcd_block(blockIdx.x, gridDim.x);
-  size_t c = (size_t)b * blockDim.x + threadIdx.x;
+  int c = (int)(b * blockDim.x + threadIdx.x);
   if (c >= nc) return;
   Cell s = make_cell(A.g, c);
-  const double* vp = A.vofp;
-  double p = vp[s.at(0, 0, 0)];
-  double r;
-  if (which == 0) r = h1_val(A, s, 0.5 * (vp[s.at(-1, 0, 0)] + p));
-  else if (which == 1) r = h2_val(A, s, 0.5 * (vp[s.at(0, -1, 0)] + p));
-  else if (which == 2) r = h3_val(A, s, 0.5 * (vp[s.at(0, 0, -1)] + p));
+  const R* vp = A.vofp;
+  R p = vp[s.at(0, 0, 0)];
+  R r;
+  if (which == 0) r = h1_val(A, s, R(0.5) * (vp[s.at(-1, 0, 0)] + p));
+  else if (which == 1) r = h2_val(A, s, R(0.5) * (vp[s.at(0, -1, 0)] + p));
+  else if (which == 2) r = h3_val(A, s, R(0.5) * (vp[s.at(0, 0, -1)] + p));
   else r = ht_val(A, s, p);
   out[c] = r;
 }
 
-__global__ __launch_bounds__(256) void k_invtr_rhs(Grid g, RhsCoef cf, const double* __restrict__ q, const double* __restrict__ pr,
-                                                   const double* __restrict__ h, double* __restrict__ ru, double* __restrict__ rhs, int which,
-                                                   size_t nc) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_invtr_rhs(Grid g, RhsCoef<R> cf, const R* __restrict__ q, const R* __restrict__ pr,
+                                                   const R* __restrict__ h, R* __restrict__ ru, R* __restrict__ rhs, int which, int nc) {
   unsigned b = xcd_block(blockIdx.x, gridDim.x);
-  size_t c = (size_t)b * blockDim.x + threadIdx.x;
+  int c = (int)(b * blockDim.x + threadIdx.x);
   if (c >= nc) return;
   Cell s = make_cell(g, c);
-  double hv = h[c], rv = ru[c], r;
+  R hv = h[c], rv = ru[c], r;
   if (which == 0) r = rhs_u(cf, q, pr, s, hv, rv);
   else if (which == 1) r = rhs_v(cf, q, pr, s, hv, rv);
   else if (which == 2) r = rhs_w(cf, q, pr, s, hv, rv);
@@ -234,14 +250,16 @@ __global__ __launch_bounds__(256) void k_invtr_rhs(Grid g, RhsCoef cf, const dou
 // ratio a/b squares each step and the reduction stops once it is below 1e-17
 // (after which x = d/b to round-off), or when the coupling offset reaches M/2
 // (M = 2^p: closed 2x2 solve) or M/3 (M = 3*2^p: closed 3x3 circulant solve).
+template <typename R>
 struct SolveField {
-  double* rhs;    // in/out (in only when qadd != null and store_rhs == 0)
-  double* qadd;   // solxk: q += solution
+  R* rhs;         // in/out (in only when qadd != null and store_rhs == 0)
+  R* qadd;        // solxk: q += solution
   double betadx;
   int store_rhs;  // with qadd: also leave the solution in rhs (solxk.f90:40)
 };
+template <typename R>
 struct SolveBatch {
-  SolveField f[4];
+  SolveField<R> f[4];
 };
 
 // Cyclic reduction of the W circulant systems of one LDS tile.  Element
@@ -318,10 +336,10 @@ __device__ __forceinline__ void pcr_core(double* lds, int M, int W, int ldx, dou
 
 // Line solves along y (DIR 1) or z (DIR 2) (and x, DIR 0, for the per-routine
 // entry points): a block owns W lines of M rows, E elements per thread.
-template <int DIR, int E>
-__global__ __launch_bounds__(1024) void k_pcr(SolveBatch batch, int M, int W, int n1, int n2, int n3) {
+template <int DIR, int E, typename R>
+__global__ __launch_bounds__(1024) void k_pcr(SolveBatch<R> batch, int M, int W, int n1, int n2, int n3) {
   extern __shared__ double lds[];
-  const SolveField F = batch.f[blockIdx.y];
+  const SolveField<R> F = batch.f[blockIdx.y];
   const int tid = threadIdx.x, nt = blockDim.x;
   int base, es, cs;
   unsigned t = xcd_block(blockIdx.x, gridDim.x);
@@ -352,7 +370,7 @@ __global__ __launch_bounds__(1024) void k_pcr(SolveBatch batch, int M, int W, in
   for (int q = 0; q < E; ++q) {
     int idx = tid + q * nt;
     int r = own_r(idx), c = own_c(idx);
-    d[q] = F.rhs[base + r * es + c * cs] * ackl_b;
+    d[q] = (double)F.rhs[base + r * es + c * cs] * ackl_b;
     lds[la(r, c)] = d[q];
   }
   __syncthreads();
@@ -362,10 +380,10 @@ __global__ __launch_bounds__(1024) void k_pcr(SolveBatch batch, int M, int W, in
     int idx = tid + q * nt;
     int gaddr = base + own_r(idx) * es + own_c(idx) * cs;
     if (F.qadd) {
-      F.qadd[gaddr] = F.qadd[gaddr] + d[q];
-      if (F.store_rhs) F.rhs[gaddr] = d[q];
+      F.qadd[gaddr] = F.qadd[gaddr] + (R)d[q];
+      if (F.store_rhs) F.rhs[gaddr] = (R)d[q];
     } else {
-      F.rhs[gaddr] = d[q];
+      F.rhs[gaddr] = (R)d[q];
     }
   }
 }
@@ -376,11 +394,10 @@ __global__ __launch_bounds__(1024) void k_pcr(SolveBatch batch, int M, int W, in
 // state through L1/L2), keeps the scaled rhs in LDS, solves the four
 // x-systems there and writes the x-solved rhs.  Saves one read and one write
 // of rhs x4 per substep against separate passes.
-template <int E>
-__global__ __launch_bounds__(256) void k_explicit_xsolve(EulerArgs A, RhsCoef cf, double* __restrict__ ru1, double* __restrict__ ru2,
-                                                         double* __restrict__ ru3, double* __restrict__ rut, double* __restrict__ r_u,
-                                                         double* __restrict__ r_v, double* __restrict__ r_w, double* __restrict__ r_t,
-                                                         double bxu, double bxt, int W) {
+template <int E, typename R>
+__global__ __launch_bounds__(256) void k_explicit_xsolve(EulerArgs<R> A, RhsCoef<R> cf, R* __restrict__ ru1, R* __restrict__ ru2,
+                                                         R* __restrict__ ru3, R* __restrict__ rut, R* __restrict__ r_u, R* __restrict__ r_v,
+                                                         R* __restrict__ r_w, R* __restrict__ r_t, double bxu, double bxt, int W) {
   extern __shared__ double lds[];
   const Grid& g = A.g;
   const int M = g.n1, ldx = M + 1, tid = threadIdx.x, nt = blockDim.x;
@@ -396,22 +413,22 @@ __global__ __launch_bounds__(256) void k_explicit_xsolve(EulerArgs A, RhsCoef cf
     int r = idx % M, c = idx / M;
     Cell s = cell_at(g, r, j0 + c, k);
     const int gc = s.c;
-    const double* vp = A.vofp;
-    double p = vp[gc];
-    double hu = h1_val(A, s, 0.5 * (vp[s.at(-1, 0, 0)] + p));
-    double hv = h2_val(A, s, 0.5 * (vp[s.at(0, -1, 0)] + p));
-    double hw = h3_val(A, s, 0.5 * (vp[s.at(0, 0, -1)] + p));
-    double ht = ht_val(A, s, p);
-    double o1 = ru1[gc], o2 = ru2[gc], o3 = ru3[gc], ot = rut[gc];
+    const R* vp = A.vofp;
+    R p = vp[gc];
+    R hu = h1_val(A, s, R(0.5) * (vp[s.at(-1, 0, 0)] + p));
+    R hv = h2_val(A, s, R(0.5) * (vp[s.at(0, -1, 0)] + p));
+    R hw = h3_val(A, s, R(0.5) * (vp[s.at(0, 0, -1)] + p));
+    R ht = ht_val(A, s, p);
+    R o1 = ru1[gc], o2 = ru2[gc], o3 = ru3[gc], ot = rut[gc];
     const int l = c * ldx + r;
-    lds[l] = rhs_u(cf, A.vx, A.pr, s, hu, o1) * sc_u;
-    lds[plane + l] = rhs_v(cf, A.vy, A.pr, s, hv, o2) * sc_u;
-    lds[2 * plane + l] = rhs_w(cf, A.vz, A.pr, s, hw, o3) * sc_u;
-    lds[3 * plane + l] = rhs_t(cf, A.temp, s, ht, ot) * sc_t;
+    lds[l] = (double)rhs_u(cf, A.vx, A.pr, s, hu, o1) * sc_u;
+    lds[plane + l] = (double)rhs_v(cf, A.vy, A.pr, s, hv, o2) * sc_u;
+    lds[2 * plane + l] = (double)rhs_w(cf, A.vz, A.pr, s, hw, o3) * sc_u;
+    lds[3 * plane + l] = (double)rhs_t(cf, A.temp, s, ht, ot) * sc_t;
     ru1[gc] = hu; ru2[gc] = hv; ru3[gc] = hw; rut[gc] = ht;
   }
   __syncthreads();
-  double* R[4] = {r_u, r_v, r_w, r_t};
+  R* outs[4] = {r_u, r_v, r_w, r_t};
 #pragma unroll 1
   for (int f = 0; f < 4; ++f) {
     double* L = lds + f * plane;
@@ -425,7 +442,7 @@ __global__ __launch_bounds__(256) void k_explicit_xsolve(EulerArgs A, RhsCoef cf
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       int idx = tid + q * nt;
-      R[f][base + idx] = d[q];
+      outs[f][base + idx] = (R)d[q];
     }
   }
 }
@@ -445,65 +462,73 @@ int pick_threads(int n, int cap, int want) {
 }
 
 // ------------------------------------------------------------ divg etc
-__global__ __launch_bounds__(256) void k_divg(Grid g, const double* __restrict__ vx, const double* __restrict__ vy,
-                                              const double* __restrict__ vz, double* __restrict__ dph, double usdtal, size_t nc) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_divg(Grid g, const R* __restrict__ vx, const R* __restrict__ vy, const R* __restrict__ vz,
+                                              R* __restrict__ dph, R usdtal, int nc) {
   unsigned b = xcd_block(blockIdx.x, gridDim.x);
-  size_t c = (size_t)b * blockDim.x + threadIdx.x;
+  int c = (int)(b * blockDim.x + threadIdx.x);
   if (c >= nc) return;
   Cell s = make_cell(g, c);
-  double dq = (vx[s.at(1, 0, 0)] - vx[c]) * g.dx1 + (vy[s.at(0, 1, 0)] - vy[c]) * g.dx2 + (vz[s.at(0, 0, 1)] - vz[c]) * g.dx3;
+  R dq = (vx[s.at(1, 0, 0)] - vx[c]) * R(g.dx1) + (vy[s.at(0, 1, 0)] - vy[c]) * R(g.dx2) + (vz[s.at(0, 0, 1)] - vz[c]) * R(g.dx3);
   dph[c] = dq * usdtal;
 }
 
 // updvp.f90:16-31 and prcalc.f90:17-33 in one pass; with mean != null the
 // mean MLS force deferred from velforce is subtracted first ((u+f)-fbar).
-__global__ __launch_bounds__(256) void k_project(Grid g, double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
-                                                 double* __restrict__ pr, const double* __restrict__ dph, const double* __restrict__ mean,
-                                                 double udx1, double udx2, double udx3, double be, double dx1q, double dx2q, double dx3q,
-                                                 int do_u, int do_p, size_t nc) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_project(Grid g, R* __restrict__ vx, R* __restrict__ vy, R* __restrict__ vz, R* __restrict__ pr,
+                                                 const R* __restrict__ dph, const double* __restrict__ mean, R udx1, R udx2, R udx3, R be,
+                                                 R dx1q, R dx2q, R dx3q, int do_u, int do_p, int nc, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
   unsigned b = xcd_block(blockIdx.x, gridDim.x);
-  size_t c = (size_t)b * blockDim.x + threadIdx.x;
+  int c = (int)(b * blockDim.x + threadIdx.x);
   if (c >= nc) return;
   Cell s = make_cell(g, c);
-  double l = dph[c];
-  double pim = dph[s.at(-1, 0, 0)], pjm = dph[s.at(0, -1, 0)], pkm = dph[s.at(0, 0, -1)];
+  R l = dph[c];
+  R pim = dph[s.at(-1, 0, 0)], pjm = dph[s.at(0, -1, 0)], pkm = dph[s.at(0, 0, -1)];
   if (do_u) {
-    double u = vx[c], v = vy[c], w = vz[c];
-    if (mean) { u = u - mean[0]; v = v - mean[1]; w = w - mean[2]; }
+    R u = vx[c], v = vy[c], w = vz[c];
+    if (mean) { u = u - R(mean[0]); v = v - R(mean[1]); w = w - R(mean[2]); }
     vx[c] = u - (l - pim) * udx1;
     vy[c] = v - (l - pjm) * udx2;
     vz[c] = w - (l - pkm) * udx3;
   }
   if (do_p) {
-    double pip = dph[s.at(1, 0, 0)], pjp = dph[s.at(0, 1, 0)], pkp = dph[s.at(0, 0, 1)];
-    pr[c] = pr[c] + l - be * ((pip - 2.0 * l + pim) * dx1q + (pjp - 2.0 * l + pjm) * dx2q + (pkp - 2.0 * l + pkm) * dx3q);
+    R pip = dph[s.at(1, 0, 0)], pjp = dph[s.at(0, 1, 0)], pkp = dph[s.at(0, 0, 1)];
+    pr[c] = pr[c] + l - be * ((pip - R(2) * l + pim) * dx1q + (pjp - R(2) * l + pjm) * dx2q + (pkp - R(2) * l + pkm) * dx3q);
   }
 }
 
-__global__ __launch_bounds__(256) void k_interp_vof(Grid g, const double* __restrict__ p, double* __restrict__ vx, double* __restrict__ vy,
-                                                    double* __restrict__ vz, size_t nc) {
-  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+template <typename R>
+__global__ __launch_bounds__(256) void k_interp_vof(Grid g, const R* __restrict__ p, R* __restrict__ vx, R* __restrict__ vy, R* __restrict__ vz,
+                                                    int nc) {
+  int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (c >= nc) return;
   Cell s = make_cell(g, c);
-  double v = p[c];
-  vx[c] = 0.5 * (p[s.at(-1, 0, 0)] + v);
-  vy[c] = 0.5 * (p[s.at(0, -1, 0)] + v);
-  vz[c] = 0.5 * (p[s.at(0, 0, -1)] + v);
+  R v = p[c];
+  vx[c] = R(0.5) * (p[s.at(-1, 0, 0)] + v);
+  vy[c] = R(0.5) * (p[s.at(0, -1, 0)] + v);
+  vz[c] = R(0.5) * (p[s.at(0, 0, -1)] + v);
 }
 
-// Max reductions of cfl.f90:9-25 (what = 0) and divgck divg.f90:36-50 (what = 1).
-__global__ __launch_bounds__(256) void k_reduce_max(Grid g, const double* __restrict__ vx, const double* __restrict__ vy,
-                                                    const double* __restrict__ vz, int what, size_t nc, unsigned long long* out) {
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+
+// Max reductions of cfl.f90:9-25 (what = 0) and divgck divg.f90:36-50
+// (what = 1), in FP64 arithmetic whatever the storage type; the result
+// (non-negative) is combined by atomicMax on its bit pattern.
+template <typename R>
+__global__ __launch_bounds__(256) void k_reduce_max(Grid g, const R* __restrict__ vx, const R* __restrict__ vy, const R* __restrict__ vz,
+                                                    int what, int nc, unsigned long long* out) {
   __shared__ double sm[256];
   double m = what == 0 ? 0.00000001 : 0.0;
-  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += (size_t)gridDim.x * blockDim.x) {
+  for (int c = (int)(blockIdx.x * blockDim.x + threadIdx.x); c < nc; c += (int)(gridDim.x * blockDim.x)) {
     Cell s = make_cell(g, c);
     double q;
+    double x0 = vx[c], x1 = vx[s.at(1, 0, 0)], y0 = vy[c], y1 = vy[s.at(0, 1, 0)], z0 = vz[c], z1 = vz[s.at(0, 0, 1)];
     if (what == 0)
-      q = fabs((vx[c] + vx[s.at(1, 0, 0)]) * 0.5 * g.dx1) + fabs((vy[c] + vy[s.at(0, 1, 0)]) * 0.5 * g.dx2) +
-          fabs((vz[c] + vz[s.at(0, 0, 1)]) * 0.5 * g.dx3);
+      q = fabs((x0 + x1) * 0.5 * g.dx1) + fabs((y0 + y1) * 0.5 * g.dx2) + fabs((z0 + z1) * 0.5 * g.dx3);
     else
-      q = fabs((vx[s.at(1, 0, 0)] - vx[c]) * g.dx1 + (vy[s.at(0, 1, 0)] - vy[c]) * g.dx2 + (vz[s.at(0, 0, 1)] - vz[c]) * g.dx3);
+      q = fabs((x1 - x0) * g.dx1 + (y1 - y0) * g.dx2 + (z1 - z0) * g.dx3);
     m = fmax(m, q);
   }
   sm[threadIdx.x] = m;
@@ -516,46 +541,58 @@ __global__ __launch_bounds__(256) void k_reduce_max(Grid g, const double* __rest
   if (threadIdx.x == 0) atomicMax(out, (unsigned long long)__double_as_longlong(sm[0]));
 }
 
-__global__ __launch_bounds__(256) void k_inqpr(Grid g, double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
-                                               double* __restrict__ t, const melt_body* __restrict__ body, double tliq, double tsol,
-                                               double rad_p, size_t nc) {
-  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Refinement gate of the precision form: refine while the FP64 divergence of
+// the projected velocity exceeds rtol times that of the provisional one.
+__global__ void k_refine_gate(const unsigned long long* __restrict__ dmax, double rtol, int* gate, long long* stats) {
+  double before = __longlong_as_double((long long)dmax[0]), after = __longlong_as_double((long long)dmax[1]);
+  int on = after > rtol * before;
+  *gate = on;
+  stats[0] += 1;
+  stats[1] += on;
+  stats[2] = __double_as_longlong(before > 0.0 ? after / before : 0.0);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_inqpr(Grid g, R* __restrict__ vx, R* __restrict__ vy, R* __restrict__ vz, R* __restrict__ t,
+                                               const melt_body* __restrict__ body, double tliq, double tsol, double rad_p, int nc) {
+  int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (c >= nc) return;
-  int i = (int)(c % g.n1) + 1, j = (int)((c / g.n1) % g.n2) + 1, k = (int)(c / ((size_t)g.n1 * g.n2)) + 1;
+  int i = c % g.n1 + 1, j = (c / g.n1) % g.n2 + 1, k = c / (g.n1 * g.n2) + 1;
   double v[3] = {gxm(g, i) - body->pos_cm[0], gym(g, j) - body->pos_cm[1], gzm(g, k) - body->pos_cm[2]};
   double rr = fnorm2(v, 3);
-  vx[c] = 0.0; vy[c] = 0.0; vz[c] = 0.0;
-  t[c] = tliq - (tliq - tsol) * 0.5 * (1.0 - tanh((rr - rad_p) * g.dx1 / 2.0));
+  vx[c] = R(0); vy[c] = R(0); vz[c] = R(0);
+  t[c] = (R)(tliq - (tliq - tsol) * 0.5 * (1.0 - tanh((rr - rad_p) * g.dx1 / 2.0)));
 }
 
 inline unsigned nblocks(size_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-}  // namespace
-
 // ------------------------------------------------------------ launchers
-int euler_launch_hdnl(melt_ctx* ctx, int which) {
+template <typename R>
+int launch_hdnl(melt_ctx* ctx, int which) {
   static const int out_id[4] = {MELT_DQ, MELT_DPH, MELT_QCAP, MELT_HTEMP};
-  EulerArgs A = make_args(ctx);
-  hipLaunchKernelGGL(k_hdnl, dim3(nblocks(ctx->nc, 256)), dim3(256), 0, ctx->stream, A, ctx->f[out_id[which]], which, ctx->nc);
+  EulerArgs<R> A = make_args<R>(ctx);
+  hipLaunchKernelGGL(k_hdnl<R>, dim3(nblocks(ctx->nc, 256)), dim3(256), 0, ctx->stream, A, fld<R>(ctx, out_id[which]), which, (int)ctx->nc);
   MELT_HIP(hipGetLastError());
   return 0;
 }
 
-int euler_launch_invtr_rhs(melt_ctx* ctx, int which) {
+template <typename R>
+int launch_invtr_rhs(melt_ctx* ctx, int which) {
   static const int q_id[4] = {MELT_VX, MELT_VY, MELT_VZ, MELT_TEMP};
   static const int h_id[4] = {MELT_DQ, MELT_DPH, MELT_QCAP, MELT_HTEMP};
   static const int ru_id[4] = {MELT_RU1, MELT_RU2, MELT_RU3, MELT_RUT};
-  RhsCoef cf = make_coef(ctx);
-  hipLaunchKernelGGL(k_invtr_rhs, dim3(nblocks(ctx->nc, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), cf, ctx->f[q_id[which]],
-                     ctx->f[MELT_PR], ctx->f[h_id[which]], ctx->f[ru_id[which]], ctx->f[MELT_RHS], which, ctx->nc);
+  RhsCoef<R> cf = make_coef<R>(ctx);
+  hipLaunchKernelGGL(k_invtr_rhs<R>, dim3(nblocks(ctx->nc, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), cf, fld<R>(ctx, q_id[which]),
+                     fld<R>(ctx, MELT_PR), fld<R>(ctx, h_id[which]), fld<R>(ctx, ru_id[which]), fld<R>(ctx, MELT_RHS), which, (int)ctx->nc);
   MELT_HIP(hipGetLastError());
   return 0;
 }
 
 // Pass A: explicit terms + RHS + x-solve of u, v, w, T into rhs4[].
-int euler_launch_explicit_xsolve(melt_ctx* ctx, double bxu, double bxt) {
-  EulerArgs A = make_args(ctx);
-  RhsCoef cf = make_coef(ctx);
+template <typename R>
+int launch_explicit_xsolve(melt_ctx* ctx, double bxu, double bxt) {
+  EulerArgs<R> A = make_args<R>(ctx);
+  RhsCoef<R> cf = make_coef<R>(ctx);
   // W x-lines per block: the fewest that give a 256-cell tile (one cell per
   // thread); small tiles keep the LDS and register footprint low.
   int W = 0;
@@ -570,11 +607,12 @@ int euler_launch_explicit_xsolve(melt_ctx* ctx, double bxu, double bxt) {
   int E = ncell / nt;
   size_t shm = sizeof(double) * 4 * (size_t)W * (ctx->n1 + 1);
   unsigned blocks = (unsigned)((size_t)(ctx->n2 / W) * ctx->n3);
-#define XF_CASE(EE)                                                                                                            \
-  case EE:                                                                                                                     \
-    hipLaunchKernelGGL(k_explicit_xsolve<EE>, dim3(blocks), dim3(nt), shm, ctx->stream, A, cf, ctx->f[MELT_RU1], ctx->f[MELT_RU2], \
-                       ctx->f[MELT_RU3], ctx->f[MELT_RUT], ctx->rhs4[RHS_U], ctx->rhs4[RHS_V], ctx->rhs4[RHS_W], ctx->rhs4[RHS_T], bxu,  \
-                       bxt, W);                                                                                                \
+  R* r4[4] = {static_cast<R*>(ctx->rhs4[RHS_U]), static_cast<R*>(ctx->rhs4[RHS_V]), static_cast<R*>(ctx->rhs4[RHS_W]),
+              static_cast<R*>(ctx->rhs4[RHS_T])};
+#define XF_CASE(EE)                                                                                                                   \
+  case EE:                                                                                                                            \
+    hipLaunchKernelGGL((k_explicit_xsolve<EE, R>), dim3(blocks), dim3(nt), shm, ctx->stream, A, cf, fld<R>(ctx, MELT_RU1),             \
+                       fld<R>(ctx, MELT_RU2), fld<R>(ctx, MELT_RU3), fld<R>(ctx, MELT_RUT), r4[0], r4[1], r4[2], r4[3], bxu, bxt, W); \
     break;
   switch (E) {
     XF_CASE(1) XF_CASE(2) XF_CASE(3) XF_CASE(4) XF_CASE(6) XF_CASE(8)
@@ -587,7 +625,8 @@ int euler_launch_explicit_xsolve(melt_ctx* ctx, double bxu, double bxt) {
 
 // dir 0/1/2 = x/y/z; fields[nfield] are solved in place, or (qadd != null)
 // added into qadd[] (solxk.f90:44-50).
-int euler_launch_solve(melt_ctx* ctx, int dir, double* const* fields, int nfield, const double* betadx, double* const* qadd, int store_rhs) {
+template <typename R>
+int launch_solve(melt_ctx* ctx, int dir, void* const* fields, int nfield, const double* betadx, void* const* qadd, int store_rhs) {
   int n[3] = {ctx->n1, ctx->n2, ctx->n3};
   int M = n[dir];
   int other = dir == 0 ? ctx->n2 : ctx->n1;
@@ -598,10 +637,10 @@ int euler_launch_solve(melt_ctx* ctx, int dir, double* const* fields, int nfield
   int nt = pick_threads(M * W, 1024, ctx->tune[TUNE_SOLVE_NT] > 0 ? ctx->tune[TUNE_SOLVE_NT] : 256);
   if (!nt) return melt_fail(ctx, MELT_ERR_ARG, "line solve", "unsupported line length");
   int E = (M * W) / nt;
-  SolveBatch B{};
+  SolveBatch<R> B{};
   for (int q = 0; q < nfield; ++q) {
-    B.f[q].rhs = fields[q];
-    B.f[q].qadd = qadd ? qadd[q] : nullptr;
+    B.f[q].rhs = static_cast<R*>(fields[q]);
+    B.f[q].qadd = qadd ? static_cast<R*>(qadd[q]) : nullptr;
     B.f[q].betadx = betadx[q];
     B.f[q].store_rhs = store_rhs;
   }
@@ -610,7 +649,7 @@ int euler_launch_solve(melt_ctx* ctx, int dir, double* const* fields, int nfield
   if (shm > 64 * 1024) return melt_fail(ctx, MELT_ERR_ARG, "line solve", "tile exceeds LDS");
   dim3 grid((unsigned)tiles, nfield);
 #define PCR_CASE(D, EE) \
-  case EE: hipLaunchKernelGGL((k_pcr<D, EE>), grid, dim3(nt), shm, ctx->stream, B, M, W, ctx->n1, ctx->n2, ctx->n3); break;
+  case EE: hipLaunchKernelGGL((k_pcr<D, EE, R>), grid, dim3(nt), shm, ctx->stream, B, M, W, ctx->n1, ctx->n2, ctx->n3); break;
 #define PCR_DIR(D)                                                                                          \
   switch (E) {                                                                                              \
     PCR_CASE(D, 1) PCR_CASE(D, 2) PCR_CASE(D, 3) PCR_CASE(D, 4) PCR_CASE(D, 6) PCR_CASE(D, 8)                 \
@@ -625,38 +664,87 @@ int euler_launch_solve(melt_ctx* ctx, int dir, double* const* fields, int nfield
   return 0;
 }
 
-int euler_launch_divg(melt_ctx* ctx) {
-  double usdtal = 1.0 / (ctx->dt * ctx->al);
-  hipLaunchKernelGGL(k_divg, dim3(nblocks(ctx->nc, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), ctx->f[MELT_VX], ctx->f[MELT_VY],
-                     ctx->f[MELT_VZ], ctx->f[MELT_DPH], usdtal, ctx->nc);
+template <typename R>
+int launch_divg(melt_ctx* ctx) {
+  R usdtal = R(1.0 / (ctx->dt * ctx->al));
+  hipLaunchKernelGGL(k_divg<R>, dim3(nblocks(ctx->nc, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), fld<R>(ctx, MELT_VX),
+                     fld<R>(ctx, MELT_VY), fld<R>(ctx, MELT_VZ), fld<R>(ctx, MELT_DPH), usdtal, (int)ctx->nc);
   MELT_HIP(hipGetLastError());
   return 0;
 }
 
-int euler_launch_project(melt_ctx* ctx, int do_u, int do_p, int use_mean) {
-  double udx1 = ctx->al * ctx->dt * ctx->dx1, udx2 = ctx->al * ctx->dt * ctx->dx2, udx3 = ctx->al * ctx->dt * ctx->dx3;
-  double be = ctx->al * ctx->beta;
-  hipLaunchKernelGGL(k_project, dim3(nblocks(ctx->nc, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), ctx->f[MELT_VX], ctx->f[MELT_VY],
-                     ctx->f[MELT_VZ], ctx->f[MELT_PR], ctx->f[MELT_DPH], use_mean ? ctx->d_fmean : nullptr, udx1, udx2, udx3, be,
-                     ctx->dx1q, ctx->dx2q, ctx->dx3q, do_u, do_p, ctx->nc);
+template <typename R>
+int launch_project(melt_ctx* ctx, int do_u, int do_p, int use_mean, const int* gate) {
+  R udx1 = R(ctx->al * ctx->dt * ctx->dx1), udx2 = R(ctx->al * ctx->dt * ctx->dx2), udx3 = R(ctx->al * ctx->dt * ctx->dx3);
+  R be = R(ctx->al * ctx->beta);
+  hipLaunchKernelGGL(k_project<R>, dim3(nblocks(ctx->nc, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), fld<R>(ctx, MELT_VX),
+                     fld<R>(ctx, MELT_VY), fld<R>(ctx, MELT_VZ), fld<R>(ctx, MELT_PR), fld<R>(ctx, MELT_DPH),
+                     use_mean ? ctx->d_fmean : nullptr, udx1, udx2, udx3, be, R(ctx->dx1q), R(ctx->dx2q), R(ctx->dx3q), do_u, do_p,
+                     (int)ctx->nc, gate);
   MELT_HIP(hipGetLastError());
   return 0;
 }
 
-int euler_launch_interp_vof(melt_ctx* ctx) {
-  hipLaunchKernelGGL(k_interp_vof, dim3(nblocks(ctx->nc, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), ctx->f[MELT_VOFP],
-                     ctx->f[MELT_VOFX], ctx->f[MELT_VOFY], ctx->f[MELT_VOFZ], ctx->nc);
+template <typename R>
+int launch_interp_vof(melt_ctx* ctx) {
+  hipLaunchKernelGGL(k_interp_vof<R>, dim3(nblocks(ctx->nc, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), fld<R>(ctx, MELT_VOFP),
+                     fld<R>(ctx, MELT_VOFX), fld<R>(ctx, MELT_VOFY), fld<R>(ctx, MELT_VOFZ), (int)ctx->nc);
   MELT_HIP(hipGetLastError());
   return 0;
 }
 
+// Launch the max reduction into d (zeroed by a fill kernel first, so the
+// sequence may be captured into a graph).
+template <typename R>
+int launch_reduce_max(melt_ctx* ctx, int what, unsigned long long* d) {
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, ctx->stream, d);
+  unsigned nb = (unsigned)std::min<size_t>(1024, nblocks(ctx->nc, 256));
+  hipLaunchKernelGGL(k_reduce_max<R>, dim3(nb), dim3(256), 0, ctx->stream, make_grid(ctx), fld<R>(ctx, MELT_VX), fld<R>(ctx, MELT_VY),
+                     fld<R>(ctx, MELT_VZ), what, (int)ctx->nc, d);
+  MELT_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename R>
+int launch_inqpr(melt_ctx* ctx) {
+  hipLaunchKernelGGL(k_inqpr<R>, dim3(nblocks(ctx->nc, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), fld<R>(ctx, MELT_VX),
+                     fld<R>(ctx, MELT_VY), fld<R>(ctx, MELT_VZ), fld<R>(ctx, MELT_TEMP), ctx->d_body, ctx->cfg.tliq, ctx->cfg.tsol, ctx->rad_p,
+                     (int)ctx->nc);
+  MELT_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------- precision dispatch
+int euler_launch_hdnl(melt_ctx* ctx, int which) { return ctx->fp32 ? launch_hdnl<float>(ctx, which) : launch_hdnl<double>(ctx, which); }
+int euler_launch_invtr_rhs(melt_ctx* ctx, int which) {
+  return ctx->fp32 ? launch_invtr_rhs<float>(ctx, which) : launch_invtr_rhs<double>(ctx, which);
+}
+int euler_launch_explicit_xsolve(melt_ctx* ctx, double bxu, double bxt) {
+  return ctx->fp32 ? launch_explicit_xsolve<float>(ctx, bxu, bxt) : launch_explicit_xsolve<double>(ctx, bxu, bxt);
+}
+int euler_launch_solve(melt_ctx* ctx, int dir, void* const* fields, int nfield, const double* betadx, void* const* qadd, int store_rhs) {
+  return ctx->fp32 ? launch_solve<float>(ctx, dir, fields, nfield, betadx, qadd, store_rhs)
+                   : launch_solve<double>(ctx, dir, fields, nfield, betadx, qadd, store_rhs);
+}
+int euler_launch_divg(melt_ctx* ctx) { return ctx->fp32 ? launch_divg<float>(ctx) : launch_divg<double>(ctx); }
+int euler_launch_project(melt_ctx* ctx, int do_u, int do_p, int use_mean, const int* gate) {
+  return ctx->fp32 ? launch_project<float>(ctx, do_u, do_p, use_mean, gate) : launch_project<double>(ctx, do_u, do_p, use_mean, gate);
+}
+int euler_launch_interp_vof(melt_ctx* ctx) { return ctx->fp32 ? launch_interp_vof<float>(ctx) : launch_interp_vof<double>(ctx); }
+int euler_launch_reduce_max(melt_ctx* ctx, int what, unsigned long long* d) {
+  return ctx->fp32 ? launch_reduce_max<float>(ctx, what, d) : launch_reduce_max<double>(ctx, what, d);
+}
+int euler_launch_refine_gate(melt_ctx* ctx, const unsigned long long* dmax, int* gate) {
+  hipLaunchKernelGGL(k_refine_gate, dim3(1), dim3(1), 0, ctx->stream, dmax, ctx->refine_rtol, gate, ctx->d_refine_stats);
+  MELT_HIP(hipGetLastError());
+  return 0;
+}
 int euler_reduce_max(melt_ctx* ctx, int what, double* out) {
   unsigned long long* d = reinterpret_cast<unsigned long long*>(ctx->d_red);
-  MELT_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), ctx->stream));
-  unsigned nb = (unsigned)std::min<size_t>(1024, nblocks(ctx->nc, 256));
-  hipLaunchKernelGGL(k_reduce_max, dim3(nb), dim3(256), 0, ctx->stream, make_grid(ctx), ctx->f[MELT_VX], ctx->f[MELT_VY], ctx->f[MELT_VZ],
-                     what, ctx->nc, d);
-  MELT_HIP(hipGetLastError());
+  int rc = euler_launch_reduce_max(ctx, what, d);
+  if (rc) return rc;
   unsigned long long h = 0;
   MELT_HIP(hipMemcpyAsync(&h, d, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
   MELT_HIP(hipStreamSynchronize(ctx->stream));
@@ -665,10 +753,4 @@ int euler_reduce_max(melt_ctx* ctx, int what, double* out) {
   *out = v;
   return 0;
 }
-
-int euler_launch_inqpr(melt_ctx* ctx) {
-  hipLaunchKernelGGL(k_inqpr, dim3(nblocks(ctx->nc, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), ctx->f[MELT_VX], ctx->f[MELT_VY],
-                     ctx->f[MELT_VZ], ctx->f[MELT_TEMP], ctx->d_body, ctx->cfg.tliq, ctx->cfg.tsol, ctx->rad_p, ctx->nc);
-  MELT_HIP(hipGetLastError());
-  return 0;
-}
+int euler_launch_inqpr(melt_ctx* ctx) { return ctx->fp32 ? launch_inqpr<float>(ctx) : launch_inqpr<double>(ctx); }

@@ -125,7 +125,8 @@ __device__ inline double dweight(double pc, double gcrd, double nrm, double scal
 // Must be reached by every thread of the block (it contains a barrier);
 // active = 0 lanes only take part in the barrier.  Returns the LU info of A
 // on lanes 1..3 (0 elsewhere); *grad_d receives component lane-1.
-__device__ int mls_grad_lanes(const Grid& g, const double* __restrict__ fld, const double pos[3], int ic, int jc, const int kz[3],
+template <typename R>
+__device__ int mls_grad_lanes(const Grid& g, const R* __restrict__ fld, const double pos[3], int ic, int jc, const int kz[3],
                               const int ke[3], const int c[3], int lane, bool active, double* shA, double* grad_d) {
   const double sx = WSCL / g.dx1, sy = WSCL / g.dx2, sz = WSCL / g.dx3;
   double xs[3], ys[3], zs[3], w1[3], w2[3], w3[3], dw1[3], dw2[3], dw3[3];
@@ -347,12 +348,13 @@ __device__ double dist_point_tri(const double P[3], const double* V0, const doub
   return sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
 }
 
-__global__ __launch_bounds__(256) void k_tag_reset(Grid g, double* __restrict__ sdf, const melt_window* __restrict__ win, Win3 et) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_tag_reset(Grid g, R* __restrict__ sdf, const melt_window* __restrict__ win, Win3 et) {
   if (win[1].prev_valid <= 0) return;
   size_t n = (size_t)et.e[0] * et.e[1] * et.e[2];
   for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += (size_t)gridDim.x * blockDim.x) {
     int a = (int)(s % et.e[0]), b = (int)((s / et.e[0]) % et.e[1]), c = (int)(s / ((size_t)et.e[0] * et.e[1]));
-    sdf[widx(g, win[1].prev_org[0] + a, win[1].prev_org[1] + b, win[1].prev_org[2] + c)] = 999.0;
+    sdf[widx(g, win[1].prev_org[0] + a, win[1].prev_org[1] + b, win[1].prev_org[2] + c)] = R(999.0);
   }
 }
 
@@ -373,9 +375,10 @@ struct MeshArgs {
 };
 
 // mode 0: min distance; 1: lowest face among minima; 2: write sdf and VOFp
+template <typename R>
 __global__ __launch_bounds__(256) void k_tag_pass(Grid g, MeshArgs M, const melt_window* __restrict__ win, Win3 et,
-                                                  unsigned long long* __restrict__ key, int* __restrict__ face, double* __restrict__ sdf,
-                                                  double* __restrict__ vofp, int mode, double pi, int* err) {
+                                                  unsigned long long* __restrict__ key, int* __restrict__ face, R* __restrict__ sdf,
+                                                  R* __restrict__ vofp, int mode, double pi, int* err) {
   size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= (size_t)M.nf * 125) return;
   int f = (int)(id / 125), q = (int)(id % 125);
@@ -402,12 +405,12 @@ __global__ __launch_bounds__(256) void k_tag_pass(Grid g, MeshArgs M, const melt
     double sgn = signbit(dot) ? -1.0 : 1.0;
     double sg = dist * (-sgn);
     size_t c = widx(g, i, j, k);
-    sdf[c] = sg;
+    sdf[c] = (R)sg;
     double eps = 1.0 / g.dx1, v;  // sdf_to_VOF genSDF.f90:237-255
     if (sg <= -eps) v = 0.0;
     else if (sg >= eps) v = 1.0;
     else v = 0.5 * (1.0 + sg * g.dx1 + (1.0 / pi) * det_sin(pi * sg * g.dx1));
-    vofp[c] = v;
+    vofp[c] = (R)v;
   }
 }
 
@@ -540,8 +543,9 @@ __device__ double ray_tag_cell(const Grid& g, const MeshArgs& M, double xlen, co
 // pencil_tagCells (pencil_rayTag.f90:189-291): every wrapped cell of the box
 // visits its unwrapped preimages in the reference's order (i, then j, then k)
 // and keeps the first value below 1.
+template <typename R>
 __global__ __launch_bounds__(256) void k_pencil_tag(Grid g, MeshArgs M, Box bx, const int* __restrict__ count, const int* __restrict__ list,
-                                                    int buff, double* __restrict__ vofp, double xlen, int* err) {
+                                                    int buff, R* __restrict__ vofp, double xlen, int* err) {
   int ext[3], cov[3];
   int nn[3] = {g.n1, g.n2, g.n3};
   for (int d = 0; d < 3; ++d) { ext[d] = bx.hi[d] - bx.lo[d] + 1; cov[d] = min(ext[d], nn[d]); }
@@ -559,7 +563,7 @@ __global__ __launch_bounds__(256) void k_pencil_tag(Grid g, MeshArgs M, Box bx, 
         double v = ray_tag_cell(g, M, xlen, Q, &list[p * buff], n, err);
         if (!(cur < 1.0)) cur = v;
       }
-  vofp[widx(g, bx.lo[0] + ta, bx.lo[1] + tb, bx.lo[2] + tc)] = cur;
+  vofp[widx(g, bx.lo[0] + ta, bx.lo[1] + tb, bx.lo[2] + tc)] = (R)cur;
 }
 
 __global__ __launch_bounds__(256) void k_vbox(const double* __restrict__ xyzv, const unsigned char* __restrict__ gv, int nv, double* out) {
@@ -583,8 +587,9 @@ __global__ __launch_bounds__(256) void k_vbox(const double* __restrict__ xyzv, c
     for (int d = 0; d < 3; ++d) { out[d] = mn[d][0]; out[3 + d] = mx[d][0]; }
 }
 
-__global__ __launch_bounds__(256) void k_fill(double* __restrict__ p, double v, size_t n) {
-  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) p[c] = v;
+template <typename R>
+__global__ __launch_bounds__(256) void k_fill(R* __restrict__ p, double v, size_t n) {
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) p[c] = (R)v;
 }
 
 // ------------------------------------------------------------ MLS forcing
@@ -621,8 +626,9 @@ __global__ __launch_bounds__(64) void k_mls_weights(Grid g, const double* __rest
   if (mls_phi(g, pos, i0, j0, k0, c, [&](int q, double v) { o[q] = v; })) raise_error(err, MELT_ERR_SINGULAR_MLS);
 }
 
+template <typename R>
 struct ForceArgs {
-  const double* fld[4];   // vx, vy, temp, vz
+  const R* fld[4];        // vx, vy, temp, vz
   const double* phi[4];
   const double* vel_tri;
   const double* cfac;
@@ -630,7 +636,8 @@ struct ForceArgs {
 };
 
 // forc1/forc2/forctemp/forc3: Um = sum q phi; spread cfac (V - Um) phi
-__global__ __launch_bounds__(64) void k_mls_force(Grid g, ForceArgs F, const double* __restrict__ bar, const unsigned char* __restrict__ gf,
+template <typename R>
+__global__ __launch_bounds__(64) void k_mls_force(Grid g, ForceArgs<R> F, const double* __restrict__ bar, const unsigned char* __restrict__ gf,
                                                   int nf, const int* __restrict__ pind, const melt_window* __restrict__ win, Win3 ef,
                                                   unsigned long long* __restrict__ acc, size_t cells, int* err) {
   int id = blockIdx.x * blockDim.x + threadIdx.x;
@@ -646,7 +653,7 @@ __global__ __launch_bounds__(64) void k_mls_force(Grid g, ForceArgs F, const dou
   else if (comp == 2) { i0 = p[0]; j0 = p[1]; kw = p[2]; ku = k1u; }
   else { i0 = p[0]; j0 = p[1]; kw = p[5]; ku = kstu; }
   const double* phi = &F.phi[comp][27 * (size_t)f];
-  const double* q = F.fld[comp];
+  const R* q = F.fld[comp];
   double Um = 0.0;
   int inw = 0;
   for (int k = kw - 1; k <= kw + 1; ++k)
@@ -669,12 +676,13 @@ __global__ __launch_bounds__(64) void k_mls_force(Grid g, ForceArgs F, const dou
 // Drain the accumulators: mode 0 adds the forces to the fields (velocity
 // without its mean, which the projection subtracts later) and sums the
 // velocity forces exactly per block; mode 1 stores them into the for_* fields.
+template <typename R>
 __global__ __launch_bounds__(256) void k_force_apply(Grid g, const melt_window* __restrict__ win, Win3 ef, unsigned long long* __restrict__ acc,
-                                                     size_t cells, double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ t,
-                                                     double* __restrict__ vz, int mode, unsigned long long* __restrict__ part) {
+                                                     size_t cells, R* __restrict__ vx, R* __restrict__ vy, R* __restrict__ t,
+                                                     R* __restrict__ vz, int mode, unsigned long long* __restrict__ part) {
   __shared__ unsigned long long slo[3][256], shi[3][256];
   unsigned long long mlo[3] = {0, 0, 0}, mhi[3] = {0, 0, 0};
-  double* dst[4] = {vx, vy, t, vz};
+  R* dst[4] = {vx, vy, t, vz};
   for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < cells; s += (size_t)gridDim.x * blockDim.x) {
     int a = (int)(s % ef.e[0]), b = (int)((s / ef.e[0]) % ef.e[1]), c = (int)(s / ((size_t)ef.e[0] * ef.e[1]));
     size_t cell = widx(g, win[0].org[0] + a, win[0].org[1] + b, win[0].org[2] + c);
@@ -732,7 +740,8 @@ __global__ __launch_bounds__(256) void k_fmean_final(const unsigned long long* _
 }
 
 // velforce.f90:21-51 on full fields (per-routine entry point)
-__global__ __launch_bounds__(256) void k_sum3(const double* __restrict__ a, const double* __restrict__ b, const double* __restrict__ c,
+template <typename R>
+__global__ __launch_bounds__(256) void k_sum3(const R* __restrict__ a, const R* __restrict__ b, const R* __restrict__ c,
                                               size_t n, double* __restrict__ part) {
   __shared__ double s[3][256];
   double x = 0.0, y = 0.0, z = 0.0;
@@ -756,14 +765,16 @@ __global__ void k_sum3_final(const double* __restrict__ part, int nb, double nce
   for (int b = 0; b < nb; ++b) s = s + part[b * 3 + m];
   mean[m] = s / ncell;
 }
-__global__ __launch_bounds__(256) void k_add_force(double* __restrict__ q, const double* __restrict__ f, const double* __restrict__ mean, int m,
+template <typename R>
+__global__ __launch_bounds__(256) void k_add_force(R* __restrict__ q, const R* __restrict__ f, const double* __restrict__ mean, int m,
                                                    size_t n) {
   size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   q[c] = mean ? q[c] + f[c] - mean[m] : q[c] + f[c];
 }
 
-__global__ __launch_bounds__(256) void k_sub_mean(double* __restrict__ q, const double* __restrict__ mean, int m, size_t n) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_sub_mean(R* __restrict__ q, const double* __restrict__ mean, int m, size_t n) {
   size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   q[c] = q[c] - mean[m];
@@ -772,12 +783,13 @@ __global__ __launch_bounds__(256) void k_sub_mean(double* __restrict__ q, const 
 // ------------------------------------------------------------ probes
 // kind 0/1: +/- temperature probe (findProbeIndices.f90 + wght_gradT);
 // kind 2: pressure (wght_press); 3..6: grad p, grad u, grad v, grad w.
+template <typename R>
 struct ProbeArgs {
-  const double* temp;
-  const double* pr;
-  const double* vx;
-  const double* vy;
-  const double* vz;
+  const R* temp;
+  const R* pr;
+  const R* vx;
+  const R* vy;
+  const R* vz;
   const double* bar;
   const double* nor;
   const unsigned char* gf;
@@ -788,7 +800,8 @@ struct ProbeArgs {
 
 // Probe point of face f on the fluid (kind != 1) or solid (kind 1) side:
 // bar +- h n/|n| (mls_heatFlux.f90:33-60, mls_structLoads.f90:40-80).
-__device__ inline void probe_pos(const ProbeArgs& P, int f, int kind, double pos[3]) {
+template <typename R>
+__device__ inline void probe_pos(const ProbeArgs<R>& P, int f, int kind, double pos[3]) {
   const double* nor = &P.nor[3 * f];
   const double* bar = &P.bar[3 * f];
   double s = fnorm2(nor, 3);
@@ -801,7 +814,8 @@ __device__ inline void probe_pos(const ProbeArgs& P, int f, int kind, double pos
 // runs one kind.
 __device__ __forceinline__ int grad_kind(int q) { return q < 2 ? q : q + 1; }  // 0, 1, 3, 4, 5, 6
 
-__global__ __launch_bounds__(256) void k_probes_grad(Grid g, ProbeArgs P, double* __restrict__ out, int* __restrict__ pind, int klo,
+template <typename R>
+__global__ __launch_bounds__(256) void k_probes_grad(Grid g, ProbeArgs<R> P, double* __restrict__ out, int* __restrict__ pind, int klo,
                                                      int nkinds, int* err) {
   __shared__ double shA[64][16];
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
@@ -812,7 +826,7 @@ __global__ __launch_bounds__(256) void k_probes_grad(Grid g, ProbeArgs P, double
   const bool active = inrange && !P.gf[f];
   double pos[3] = {0.0, 0.0, 0.0};
   int ic = 1, jc = 1, kz[3] = {1, 1, 1}, ke[3] = {1, 1, 1}, cc[3] = {1, 1, 1};
-  const double* fld = P.temp;
+  const R* fld = P.temp;
   if (active) {
     probe_pos(P, f, kind, pos);
     int icent = (int)floor(pos[0] * g.dx1) + 1, jcent = (int)floor(pos[1] * g.dx2) + 1;
@@ -858,7 +872,8 @@ __global__ __launch_bounds__(256) void k_probes_grad(Grid g, ProbeArgs P, double
 
 // Pressure interpolation probe (kind 2) and the heat-flux dot products of
 // kinds 0/1 (q = n . grad T / Pe), one thread per face.
-__global__ __launch_bounds__(64) void k_probes_point(Grid g, ProbeArgs P, double* __restrict__ out, int* err) {
+template <typename R>
+__global__ __launch_bounds__(64) void k_probes_point(Grid g, ProbeArgs<R> P, double* __restrict__ out, int* err) {
   int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= P.nf || P.gf[f]) return;
   double* o = &out[(size_t)PRB_N * f];
@@ -878,7 +893,7 @@ __global__ __launch_bounds__(64) void k_probes_point(Grid g, ProbeArgs P, double
   int kw = pmod(k1 - 1, g.n3) + 1;
   const int cc[3] = {1, 1, 1};
   double pp = 0.0;
-  const double* pr = P.pr;
+  const R* pr = P.pr;
   int info = mls_phi(g, pos, icent - 1, jcent - 1, k1 - 1, cc, [&](int q, double phi) {
     int kk = q / 9, j = jcent - 1 + (q / 3) % 3, i = icent - 1 + q % 3;
     pp = pp + pr[widx(g, i, j, kw - 1 + kk)] * phi;
@@ -899,7 +914,8 @@ MeshArgs mesh_args(melt_ctx* ctx) {
 }  // namespace
 
 // ----------------------------------------------------------------- host
-static int ray_tag_init(melt_ctx* ctx) {
+template <typename R>
+static int ray_tag_init_t(melt_ctx* ctx) {
   Grid g = make_grid(ctx);
   hipLaunchKernelGGL(k_vbox, dim3(1), dim3(256), 0, ctx->stream, ctx->d_lag[MELT_L_XYZV], ctx->d_gv, ctx->nv, ctx->d_red);
   MELT_HIP(hipGetLastError());
@@ -912,7 +928,7 @@ static int ray_tag_init(melt_ctx* ctx) {
     bx.hi[d] = (int)floor(lim[3 + d] * ctx->dx1) + 1 + 2;
   }
   for (int id = MELT_VOFP; id <= MELT_VOFZ; ++id)
-    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, ctx->stream, ctx->f[id], 1.0, ctx->nc);
+    hipLaunchKernelGGL(k_fill<R>, dim3(1024), dim3(256), 0, ctx->stream, fld<R>(ctx, id), 1.0, ctx->nc);
   MELT_HIP(hipGetLastError());
   int js = bx.hi[1] - bx.lo[1] + 1, ks = bx.hi[2] - bx.lo[2] + 1;
   int buff = ctx->n1;  // BUFF_SIZE = n1m (pencil_rayTag.f90:84)
@@ -932,17 +948,18 @@ static int ray_tag_init(melt_ctx* ctx) {
   size_t cov = 1;
   int nn[3] = {ctx->n1, ctx->n2, ctx->n3};
   for (int d = 0; d < 3; ++d) cov *= (size_t)std::min(bx.hi[d] - bx.lo[d] + 1, nn[d]);
-  hipLaunchKernelGGL(k_pencil_tag, dim3(nb(cov, 256)), dim3(256), 0, ctx->stream, g, M, bx, ctx->d_pcount, ctx->d_plist, buff,
-                     ctx->f[MELT_VOFP], ctx->cfg.xlen, ctx->d_err);
+  hipLaunchKernelGGL(k_pencil_tag<R>, dim3(nb(cov, 256)), dim3(256), 0, ctx->stream, g, M, bx, ctx->d_pcount, ctx->d_plist, buff,
+                     fld<R>(ctx, MELT_VOFP), ctx->cfg.xlen, ctx->d_err);
   MELT_HIP(hipGetLastError());
   return 0;
 }
 
-int ibm_tag(melt_ctx* ctx) {  // tagCells.f90:4-27 (tagType 1) + interp_vof when not fused
+template <typename R>
+static int ibm_tag_t(melt_ctx* ctx) {  // tagCells.f90:4-27 (tagType 1) + interp_vof when not fused
   if (!ctx->has_geom) return melt_fail(ctx, MELT_ERR_NO_GEOMETRY, "tagCells", "no geometry");
   Grid g = make_grid(ctx);
   if (ctx->h_body.initial_tag) {
-    int rc = ray_tag_init(ctx);
+    int rc = ray_tag_init_t<R>(ctx);
     if (rc) return rc;
     ctx->h_body.initial_tag = 0;
     MELT_HIP(hipMemcpyAsync(&ctx->d_body->initial_tag, &ctx->h_body.initial_tag, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
@@ -951,13 +968,13 @@ int ibm_tag(melt_ctx* ctx) {  // tagCells.f90:4-27 (tagType 1) + interp_vof when
   hipLaunchKernelGGL(k_window_box, dim3(nb(ctx->nf, 256)), dim3(256), 0, ctx->stream, g, ctx->d_lag[MELT_L_TRI_BAR], ctx->d_gf, ctx->nf,
                      ctx->d_win + 1);
   hipLaunchKernelGGL(k_window_final, dim3(1), dim3(1), 0, ctx->stream, ctx->d_win, ef, et, 2, ctx->d_err);
-  hipLaunchKernelGGL(k_tag_reset, dim3(256), dim3(256), 0, ctx->stream, g, ctx->f[MELT_SDF], ctx->d_win, et);
+  hipLaunchKernelGGL(k_tag_reset<R>, dim3(256), dim3(256), 0, ctx->stream, g, fld<R>(ctx, MELT_SDF), ctx->d_win, et);
   hipLaunchKernelGGL(k_tag_init, dim3(256), dim3(256), 0, ctx->stream, ctx->d_tkey, ctx->d_tface, ctx->twin_cells);
   MeshArgs M = mesh_args(ctx);
   unsigned blocks = nb((size_t)ctx->nf * 125, 256);
   for (int mode = 0; mode < 3; ++mode)
-    hipLaunchKernelGGL(k_tag_pass, dim3(blocks), dim3(256), 0, ctx->stream, g, M, ctx->d_win, et, ctx->d_tkey, ctx->d_tface, ctx->f[MELT_SDF],
-                       ctx->f[MELT_VOFP], mode, ctx->pi, ctx->d_err);
+    hipLaunchKernelGGL(k_tag_pass<R>, dim3(blocks), dim3(256), 0, ctx->stream, g, M, ctx->d_win, et, ctx->d_tkey, ctx->d_tface, fld<R>(ctx, MELT_SDF),
+                       fld<R>(ctx, MELT_VOFP), mode, ctx->pi, ctx->d_err);
   MELT_HIP(hipGetLastError());
   return 0;
 }
@@ -978,18 +995,19 @@ int ibm_weights(melt_ctx* ctx) {
 }
 
 // Interpolation + spreading into the exact accumulators (mlsForce.f90, forc*).
-int ibm_force_scatter(melt_ctx* ctx) {
+template <typename R>
+static int ibm_force_scatter_t(melt_ctx* ctx) {
   Grid g = make_grid(ctx);
   Win3 ef{{ctx->fwin[0], ctx->fwin[1], ctx->fwin[2]}}, et{{ctx->twin[0], ctx->twin[1], ctx->twin[2]}};
   hipLaunchKernelGGL(k_window_box, dim3(nb(ctx->nf, 256)), dim3(256), 0, ctx->stream, g, ctx->d_lag[MELT_L_TRI_BAR], ctx->d_gf, ctx->nf,
                      ctx->d_win);
   hipLaunchKernelGGL(k_window_final, dim3(1), dim3(1), 0, ctx->stream, ctx->d_win, ef, et, 1, ctx->d_err);
-  ForceArgs F;
-  F.fld[0] = ctx->f[MELT_VX]; F.fld[1] = ctx->f[MELT_VY]; F.fld[2] = ctx->f[MELT_TEMP]; F.fld[3] = ctx->f[MELT_VZ];
+  ForceArgs<R> F;
+  F.fld[0] = fld<R>(ctx, MELT_VX); F.fld[1] = fld<R>(ctx, MELT_VY); F.fld[2] = fld<R>(ctx, MELT_TEMP); F.fld[3] = fld<R>(ctx, MELT_VZ);
   F.phi[0] = ctx->d_lag[MELT_L_PTXAB_Q1]; F.phi[1] = ctx->d_lag[MELT_L_PTXAB_Q2];
   F.phi[2] = ctx->d_lag[MELT_L_PTXAB_TEMP]; F.phi[3] = ctx->d_lag[MELT_L_PTXAB_Q3];
   F.vel_tri = ctx->d_lag[MELT_L_VEL_TRI]; F.cfac = ctx->d_lag[MELT_L_CFAC]; F.tmelt = ctx->cfg.tmelt;
-  hipLaunchKernelGGL(k_mls_force, dim3(nb((size_t)ctx->nf * 4, 64)), dim3(64), 0, ctx->stream, g, F, ctx->d_lag[MELT_L_TRI_BAR], ctx->d_gf,
+  hipLaunchKernelGGL(k_mls_force<R>, dim3(nb((size_t)ctx->nf * 4, 64)), dim3(64), 0, ctx->stream, g, F, ctx->d_lag[MELT_L_TRI_BAR], ctx->d_gf,
                      ctx->nf, ctx->d_pind, ctx->d_win, ef, ctx->d_acc, ctx->fwin_cells, ctx->d_err);
   MELT_HIP(hipGetLastError());
   return 0;
@@ -999,20 +1017,22 @@ int ibm_force_scatter(melt_ctx* ctx) {
 // into u, v, w, T and the exact mean of the velocity forces lands in d_fmean
 // for the projection (velforce.f90:21-51 with the uniform shift deferred);
 // 0: the folded for_* fields are written (per-routine API, mlsForce output).
-int ibm_force_drain(melt_ctx* ctx, int apply_now) {
+template <typename R>
+static int ibm_force_drain_t(melt_ctx* ctx, int apply_now) {
   Grid g = make_grid(ctx);
   Win3 ef{{ctx->fwin[0], ctx->fwin[1], ctx->fwin[2]}};
   unsigned blocks = (unsigned)std::min<size_t>(512, nb(ctx->fwin_cells, 256));
   unsigned long long* part = reinterpret_cast<unsigned long long*>(ctx->d_red + 64);
   if (apply_now) {
-    hipLaunchKernelGGL(k_force_apply, dim3(blocks), dim3(256), 0, ctx->stream, g, ctx->d_win, ef, ctx->d_acc, ctx->fwin_cells, ctx->f[MELT_VX],
-                       ctx->f[MELT_VY], ctx->f[MELT_TEMP], ctx->f[MELT_VZ], 0, part);
+    hipLaunchKernelGGL(k_force_apply<R>, dim3(blocks), dim3(256), 0, ctx->stream, g, ctx->d_win, ef, ctx->d_acc, ctx->fwin_cells, fld<R>(ctx, MELT_VX),
+                       fld<R>(ctx, MELT_VY), fld<R>(ctx, MELT_TEMP), fld<R>(ctx, MELT_VZ), 0, part);
     hipLaunchKernelGGL(k_fmean_final, dim3(1), dim3(256), 0, ctx->stream, part, (int)blocks, (double)((long long)ctx->n1 * ctx->n2 * ctx->n3),
                        ctx->d_fmean);
   } else {
-    for (int id = MELT_FOR_XC; id <= MELT_FOR_TEMP; ++id) MELT_HIP(hipMemsetAsync(ctx->f[id], 0, sizeof(double) * ctx->nc, ctx->stream));
-    hipLaunchKernelGGL(k_force_apply, dim3(blocks), dim3(256), 0, ctx->stream, g, ctx->d_win, ef, ctx->d_acc, ctx->fwin_cells,
-                       ctx->f[MELT_FOR_XC], ctx->f[MELT_FOR_YC], ctx->f[MELT_FOR_TEMP], ctx->f[MELT_FOR_ZC], 1, nullptr);
+    for (int id = MELT_FOR_XC; id <= MELT_FOR_TEMP; ++id)
+      hipLaunchKernelGGL(k_fill<R>, dim3(1024), dim3(256), 0, ctx->stream, fld<R>(ctx, id), 0.0, ctx->nc);
+    hipLaunchKernelGGL(k_force_apply<R>, dim3(blocks), dim3(256), 0, ctx->stream, g, ctx->d_win, ef, ctx->d_acc, ctx->fwin_cells,
+                       fld<R>(ctx, MELT_FOR_XC), fld<R>(ctx, MELT_FOR_YC), fld<R>(ctx, MELT_FOR_TEMP), fld<R>(ctx, MELT_FOR_ZC), 1, nullptr);
   }
   MELT_HIP(hipGetLastError());
   return 0;
@@ -1020,38 +1040,42 @@ int ibm_force_drain(melt_ctx* ctx, int apply_now) {
 
 // u -= fbar over the whole field (the shift the fused substep defers to the
 // projection), for callers that stop after particle.
-int ibm_sub_mean(melt_ctx* ctx) {
+template <typename R>
+static int ibm_sub_mean_t(melt_ctx* ctx) {
   const int q[3] = {MELT_VX, MELT_VY, MELT_VZ};
   for (int m = 0; m < 3; ++m)
-    hipLaunchKernelGGL(k_sub_mean, dim3(nb(ctx->nc, 256)), dim3(256), 0, ctx->stream, ctx->f[q[m]], ctx->d_fmean, m, ctx->nc);
+    hipLaunchKernelGGL(k_sub_mean<R>, dim3(nb(ctx->nc, 256)), dim3(256), 0, ctx->stream, fld<R>(ctx, q[m]), ctx->d_fmean, m, ctx->nc);
   MELT_HIP(hipGetLastError());
   return 0;
 }
 
-int ibm_velforce(melt_ctx* ctx, int /*defer*/) {  // velforce.f90 on full for_* fields
+template <typename R>
+static int ibm_velforce_t(melt_ctx* ctx, int /*defer*/) {  // velforce.f90 on full for_* fields
   unsigned blocks = (unsigned)std::min<size_t>(1024, nb(ctx->nc, 256));
   double* part = ctx->d_red + 64;
-  hipLaunchKernelGGL(k_sum3, dim3(blocks), dim3(256), 0, ctx->stream, ctx->f[MELT_FOR_XC], ctx->f[MELT_FOR_YC], ctx->f[MELT_FOR_ZC], ctx->nc,
+  hipLaunchKernelGGL(k_sum3<R>, dim3(blocks), dim3(256), 0, ctx->stream, fld<R>(ctx, MELT_FOR_XC), fld<R>(ctx, MELT_FOR_YC), fld<R>(ctx, MELT_FOR_ZC), ctx->nc,
                      part);
   hipLaunchKernelGGL(k_sum3_final, dim3(1), dim3(64), 0, ctx->stream, part, (int)blocks, (double)((long long)ctx->n1 * ctx->n2 * ctx->n3),
                      ctx->d_fmean);
   const int q[3] = {MELT_VX, MELT_VY, MELT_VZ}, fo[3] = {MELT_FOR_XC, MELT_FOR_YC, MELT_FOR_ZC};
   for (int m = 0; m < 3; ++m)
-    hipLaunchKernelGGL(k_add_force, dim3(nb(ctx->nc, 256)), dim3(256), 0, ctx->stream, ctx->f[q[m]], ctx->f[fo[m]], ctx->d_fmean, m, ctx->nc);
+    hipLaunchKernelGGL(k_add_force<R>, dim3(nb(ctx->nc, 256)), dim3(256), 0, ctx->stream, fld<R>(ctx, q[m]), fld<R>(ctx, fo[m]), ctx->d_fmean, m, ctx->nc);
   MELT_HIP(hipGetLastError());
   return 0;
 }
 
-int ibm_tempforce(melt_ctx* ctx) {  // tempforce.f90
-  hipLaunchKernelGGL(k_add_force, dim3(nb(ctx->nc, 256)), dim3(256), 0, ctx->stream, ctx->f[MELT_TEMP], ctx->f[MELT_FOR_TEMP], nullptr, 0,
+template <typename R>
+static int ibm_tempforce_t(melt_ctx* ctx) {  // tempforce.f90
+  hipLaunchKernelGGL(k_add_force<R>, dim3(nb(ctx->nc, 256)), dim3(256), 0, ctx->stream, fld<R>(ctx, MELT_TEMP), fld<R>(ctx, MELT_FOR_TEMP), nullptr, 0,
                      ctx->nc);
   MELT_HIP(hipGetLastError());
   return 0;
 }
 
-int ibm_probes(melt_ctx* ctx) {
-  ProbeArgs P;
-  P.temp = ctx->f[MELT_TEMP]; P.pr = ctx->f[MELT_PR]; P.vx = ctx->f[MELT_VX]; P.vy = ctx->f[MELT_VY]; P.vz = ctx->f[MELT_VZ];
+template <typename R>
+static int ibm_probes_t(melt_ctx* ctx) {
+  ProbeArgs<R> P;
+  P.temp = fld<R>(ctx, MELT_TEMP); P.pr = fld<R>(ctx, MELT_PR); P.vx = fld<R>(ctx, MELT_VX); P.vy = fld<R>(ctx, MELT_VY); P.vz = fld<R>(ctx, MELT_VZ);
   P.bar = ctx->d_lag[MELT_L_TRI_BAR]; P.nor = ctx->d_lag[MELT_L_TRI_NOR]; P.gf = ctx->d_gf; P.nf = ctx->nf;
   P.h = ctx->h_body.h_eulerian; P.pec = ctx->cfg.ren * ctx->cfg.prandtl;
   P.heat = ctx->cfg.imelt == 1;
@@ -1059,11 +1083,11 @@ int ibm_probes(melt_ctx* ctx) {
   int klo = P.heat ? 0 : 2, khi = P.loads ? 6 : 2;
   if (khi > klo) {
     size_t threads = (size_t)ctx->nf * (khi - klo) * 4;
-    hipLaunchKernelGGL(k_probes_grad, dim3(nb(threads, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), P, ctx->d_probe, ctx->d_pind, klo,
+    hipLaunchKernelGGL(k_probes_grad<R>, dim3(nb(threads, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), P, ctx->d_probe, ctx->d_pind, klo,
                        khi - klo, ctx->d_err);
   }
   if (P.heat || P.loads)
-    hipLaunchKernelGGL(k_probes_point, dim3(nb(ctx->nf, 64)), dim3(64), 0, ctx->stream, make_grid(ctx), P, ctx->d_probe, ctx->d_err);
+    hipLaunchKernelGGL(k_probes_point<R>, dim3(nb(ctx->nf, 64)), dim3(64), 0, ctx->stream, make_grid(ctx), P, ctx->d_probe, ctx->d_err);
   MELT_HIP(hipGetLastError());
   return 0;
 }
@@ -1073,3 +1097,12 @@ int ibm_window_reset(melt_ctx* ctx) {
   MELT_HIP(hipGetLastError());
   return 0;
 }
+
+// ------------------------------------------------- precision dispatch
+int ibm_tag(melt_ctx* ctx) { return ctx->fp32 ? ibm_tag_t<float>(ctx) : ibm_tag_t<double>(ctx); }
+int ibm_force_scatter(melt_ctx* ctx) { return ctx->fp32 ? ibm_force_scatter_t<float>(ctx) : ibm_force_scatter_t<double>(ctx); }
+int ibm_force_drain(melt_ctx* ctx, int apply_now) { return ctx->fp32 ? ibm_force_drain_t<float>(ctx, apply_now) : ibm_force_drain_t<double>(ctx, apply_now); }
+int ibm_sub_mean(melt_ctx* ctx) { return ctx->fp32 ? ibm_sub_mean_t<float>(ctx) : ibm_sub_mean_t<double>(ctx); }
+int ibm_velforce(melt_ctx* ctx, int defer) { return ctx->fp32 ? ibm_velforce_t<float>(ctx, defer) : ibm_velforce_t<double>(ctx, defer); }
+int ibm_tempforce(melt_ctx* ctx) { return ctx->fp32 ? ibm_tempforce_t<float>(ctx) : ibm_tempforce_t<double>(ctx); }
+int ibm_probes(melt_ctx* ctx) { return ctx->fp32 ? ibm_probes_t<float>(ctx) : ibm_probes_t<double>(ctx); }

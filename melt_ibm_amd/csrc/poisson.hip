@@ -133,7 +133,8 @@ struct SpecArgs {
 template <typename Rf, typename T>
 __global__ __launch_bounds__(FFT_NT) void k_fft_x_fwd(SpecArgs<T> S, FftPlan P, const cx<T>* __restrict__ tw, int nl,
                                                       const Rf* __restrict__ vx, const Rf* __restrict__ vy, const Rf* __restrict__ vz,
-                                                      const Rf* __restrict__ src, T usdtal) {
+                                                      const Rf* __restrict__ src, T usdtal, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
   extern __shared__ unsigned char smem[];
   cx<T>* sh = reinterpret_cast<cx<T>*>(smem);
   T* shr = reinterpret_cast<T*>(smem);
@@ -178,7 +179,9 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_x_fwd(SpecArgs<T> S, FftPlan P, 
 
 // y pass: tile of nl kx-columns (the last tile may be partial) for one z plane.
 template <typename T>
-__global__ __launch_bounds__(FFT_NT) void k_fft_y(SpecArgs<T> S, FftPlan P, const cx<T>* __restrict__ tw, int nl, int inv) {
+__global__ __launch_bounds__(FFT_NT) void k_fft_y(SpecArgs<T> S, FftPlan P, const cx<T>* __restrict__ tw, int nl, int inv,
+                                                  const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
   extern __shared__ unsigned char smem[];
   cx<T>* sh = reinterpret_cast<cx<T>*>(smem);
   const Grid& g = S.g;
@@ -204,7 +207,8 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_y(SpecArgs<T> S, FftPlan P, cons
 template <typename T>
 __global__ __launch_bounds__(FFT_NT) void k_fft_z_solve(SpecArgs<T> S, FftPlan P, const cx<T>* __restrict__ tw, int nl,
                                                         const double* __restrict__ ak1, const double* __restrict__ ak2,
-                                                        const double* __restrict__ ak3, double nrm) {
+                                                        const double* __restrict__ ak3, double nrm, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
   extern __shared__ unsigned char smem[];
   cx<T>* sh = reinterpret_cast<cx<T>*>(smem);
   const Grid& g = S.g;
@@ -247,7 +251,9 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_z_solve(SpecArgs<T> S, FftPlan P
 // 2p+1 are packed as Z = A + iB over the full Hermitian range, one inverse
 // FFT gives line 2p in the real and line 2p+1 in the imaginary part.
 template <typename Rf, typename T>
-__global__ __launch_bounds__(FFT_NT) void k_fft_x_bwd(SpecArgs<T> S, FftPlan P, const cx<T>* __restrict__ tw, int nl, Rf* __restrict__ dst) {
+__global__ __launch_bounds__(FFT_NT) void k_fft_x_bwd(SpecArgs<T> S, FftPlan P, const cx<T>* __restrict__ tw, int nl, Rf* __restrict__ dst,
+                                                      const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
   extern __shared__ unsigned char smem[];
   cx<T>* sh = reinterpret_cast<cx<T>*>(smem);
   const Grid& g = S.g;
@@ -293,22 +299,30 @@ int x_lines(int n1, int n2) {
 
 }  // namespace
 
+template <typename T>
+int upload_twiddles(melt_ctx* ctx, int n, void** dst) {
+  std::vector<cx<T>> h(n);
+  for (int t = 0; t < n; ++t) {
+    long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)t / (long double)n;
+    h[t].x = (T)cosl(ang);
+    h[t].y = (T)-sinl(ang);
+  }
+  MELT_HIP(hipMalloc(dst, sizeof(cx<T>) * n));
+  MELT_HIP(hipMemcpy(*dst, h.data(), sizeof(cx<T>) * n, hipMemcpyHostToDevice));
+  return 0;
+}
+
 int poisson_init(melt_ctx* ctx) {
   int n[3] = {ctx->n1, ctx->n2, ctx->n3};
-  double2** tw[3] = {&ctx->tw1, &ctx->tw2, &ctx->tw3};
+  void** tw[3] = {&ctx->tw1, &ctx->tw2, &ctx->tw3};
   double** ak[3] = {&ctx->ak1, &ctx->ak2, &ctx->ak3};
   double len[3] = {ctx->cfg.xlen, ctx->cfg.ylen, ctx->cfg.zlen};
   if (ctx->n2 % 2) return melt_fail(ctx, MELT_ERR_ARG, "poisson_init", "n2m must be even");
   for (int d = 0; d < 3; ++d) {
     FftPlan P;
     if (!make_plan(n[d], P)) return melt_fail(ctx, MELT_ERR_ARG, "poisson_init", "grid size must be 2^a 3^b");
-    std::vector<double2> h(n[d]);
-    for (int t = 0; t < n[d]; ++t) {
-      long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)t / (long double)n[d];
-      h[t] = make_double2((double)cosl(ang), (double)-sinl(ang));
-    }
-    MELT_HIP(hipMalloc(tw[d], sizeof(double2) * n[d]));
-    MELT_HIP(hipMemcpy(*tw[d], h.data(), sizeof(double2) * n[d], hipMemcpyHostToDevice));
+    int rc = ctx->fp32 ? upload_twiddles<float>(ctx, n[d], tw[d]) : upload_twiddles<double>(ctx, n[d], tw[d]);
+    if (rc) return rc;
     // phini.f90:35-55, 0-based index m -> reference index m+1
     std::vector<double> a(n[d]);
     int nm = n[d], nmh = nm / 2 + 1;
@@ -320,7 +334,7 @@ int poisson_init(melt_ctx* ctx) {
     MELT_HIP(hipMalloc(ak[d], sizeof(double) * n[d]));
     MELT_HIP(hipMemcpy(*ak[d], a.data(), sizeof(double) * n[d], hipMemcpyHostToDevice));
   }
-  size_t sz = sizeof(double2) * (size_t)ctx->n1hp * ctx->n2 * ctx->n3;
+  size_t sz = 2 * ctx->esz * (size_t)ctx->n1hp * ctx->n2 * ctx->n3;
   MELT_HIP(hipMalloc(&ctx->spec, sz));
   MELT_HIP(hipMemset(ctx->spec, 0, sz));
   return 0;
@@ -328,53 +342,59 @@ int poisson_init(melt_ctx* ctx) {
 
 // fused_divg = 1: dph := divergence / (al dt) is formed on the fly from the
 // velocities (divg.f90) and never stored; 0: transforms MELT_DPH as phcalc.
-int poisson_solve(melt_ctx* ctx, int fused_divg) {
-  using T = double;
+// gate != null: every kernel returns at once unless *gate != 0 (the
+// refinement solve of the precision form).
+template <typename T>
+int solve_t(melt_ctx* ctx, int fused_divg, const int* gate) {
   SpecArgs<T> S;
   S.g = make_grid(ctx);
-  S.n1h = ctx->n1h; S.n1hp = ctx->n1hp; S.spec = reinterpret_cast<cx<T>*>(ctx->spec);
+  S.n1h = ctx->n1h; S.n1hp = ctx->n1hp; S.spec = static_cast<cx<T>*>(ctx->spec);
   FftPlan Px, Py, Pz;
   make_plan(ctx->n1, Px); make_plan(ctx->n2, Py); make_plan(ctx->n3, Pz);
-  const cx<T>* twx = reinterpret_cast<const cx<T>*>(ctx->tw1);
-  const cx<T>* twy = reinterpret_cast<const cx<T>*>(ctx->tw2);
-  const cx<T>* twz = reinterpret_cast<const cx<T>*>(ctx->tw3);
+  const cx<T>* twx = static_cast<const cx<T>*>(ctx->tw1);
+  const cx<T>* twy = static_cast<const cx<T>*>(ctx->tw2);
+  const cx<T>* twz = static_cast<const cx<T>*>(ctx->tw3);
   hipEvent_t ev = nullptr;
 
   int nlx = x_lines(ctx->n1, ctx->n2);
   size_t shx = sizeof(cx<T>) * (size_t)(nlx / 2) * (ctx->n1 + 1);
-  T usdtal = 1.0 / (ctx->dt * ctx->al);
+  T usdtal = T(1.0 / (ctx->dt * ctx->al));
   unsigned gx = (unsigned)((ctx->n2 / nlx) * ctx->n3);
-  timing_begin(ctx, SLOT_DIVG_FFTX, &ev);
-  hipLaunchKernelGGL((k_fft_x_fwd<double, T>), dim3(gx), dim3(FFT_NT), shx, ctx->stream, S, Px, twx, nlx, ctx->f[MELT_VX], ctx->f[MELT_VY],
-                     ctx->f[MELT_VZ], fused_divg ? nullptr : ctx->f[MELT_DPH], usdtal);
+  if (!gate) timing_begin(ctx, SLOT_DIVG_FFTX, &ev);
+  hipLaunchKernelGGL((k_fft_x_fwd<T, T>), dim3(gx), dim3(FFT_NT), shx, ctx->stream, S, Px, twx, nlx, fld<T>(ctx, MELT_VX), fld<T>(ctx, MELT_VY),
+                     fld<T>(ctx, MELT_VZ), fused_divg ? nullptr : fld<T>(ctx, MELT_DPH), usdtal, gate);
   MELT_HIP(hipGetLastError());
-  timing_end(ctx, SLOT_DIVG_FFTX, ev);
+  if (!gate) timing_end(ctx, SLOT_DIVG_FFTX, ev);
 
   int nly = std::max(1, std::min(ctx->n1hp, (FFT_EPT * FFT_NT) / ctx->n2));
   size_t shy = sizeof(cx<T>) * (size_t)nly * ctx->n2;
   unsigned gy = (unsigned)(((ctx->n1hp + nly - 1) / nly) * ctx->n3);
-  timing_begin(ctx, SLOT_FFTY, &ev);
-  hipLaunchKernelGGL((k_fft_y<T>), dim3(gy), dim3(FFT_NT), shy, ctx->stream, S, Py, twy, nly, 0);
+  if (!gate) timing_begin(ctx, SLOT_FFTY, &ev);
+  hipLaunchKernelGGL((k_fft_y<T>), dim3(gy), dim3(FFT_NT), shy, ctx->stream, S, Py, twy, nly, 0, gate);
   MELT_HIP(hipGetLastError());
-  timing_end(ctx, SLOT_FFTY, ev);
+  if (!gate) timing_end(ctx, SLOT_FFTY, ev);
 
   int nlz = std::max(1, std::min(ctx->n1hp, (FFT_EPT * FFT_NT) / ctx->n3));
   size_t shz = sizeof(cx<T>) * (size_t)nlz * ctx->n3;
   unsigned gz = (unsigned)(((ctx->n1hp + nlz - 1) / nlz) * ctx->n2);
   double nrm = (double)((long long)ctx->n1 * ctx->n2 * ctx->n3);
-  timing_begin(ctx, SLOT_FFTZ_SOLVE, &ev);
-  hipLaunchKernelGGL((k_fft_z_solve<T>), dim3(gz), dim3(FFT_NT), shz, ctx->stream, S, Pz, twz, nlz, ctx->ak1, ctx->ak2, ctx->ak3, nrm);
+  if (!gate) timing_begin(ctx, SLOT_FFTZ_SOLVE, &ev);
+  hipLaunchKernelGGL((k_fft_z_solve<T>), dim3(gz), dim3(FFT_NT), shz, ctx->stream, S, Pz, twz, nlz, ctx->ak1, ctx->ak2, ctx->ak3, nrm, gate);
   MELT_HIP(hipGetLastError());
-  timing_end(ctx, SLOT_FFTZ_SOLVE, ev);
+  if (!gate) timing_end(ctx, SLOT_FFTZ_SOLVE, ev);
 
-  timing_begin(ctx, SLOT_IFFTY, &ev);
-  hipLaunchKernelGGL((k_fft_y<T>), dim3(gy), dim3(FFT_NT), shy, ctx->stream, S, Py, twy, nly, 1);
+  if (!gate) timing_begin(ctx, SLOT_IFFTY, &ev);
+  hipLaunchKernelGGL((k_fft_y<T>), dim3(gy), dim3(FFT_NT), shy, ctx->stream, S, Py, twy, nly, 1, gate);
   MELT_HIP(hipGetLastError());
-  timing_end(ctx, SLOT_IFFTY, ev);
+  if (!gate) timing_end(ctx, SLOT_IFFTY, ev);
 
-  timing_begin(ctx, SLOT_IFFTX, &ev);
-  hipLaunchKernelGGL((k_fft_x_bwd<double, T>), dim3(gx), dim3(FFT_NT), shx, ctx->stream, S, Px, twx, nlx, ctx->f[MELT_DPH]);
+  if (!gate) timing_begin(ctx, SLOT_IFFTX, &ev);
+  hipLaunchKernelGGL((k_fft_x_bwd<T, T>), dim3(gx), dim3(FFT_NT), shx, ctx->stream, S, Px, twx, nlx, fld<T>(ctx, MELT_DPH), gate);
   MELT_HIP(hipGetLastError());
-  timing_end(ctx, SLOT_IFFTX, ev);
+  if (!gate) timing_end(ctx, SLOT_IFFTX, ev);
   return 0;
+}
+
+int poisson_solve(melt_ctx* ctx, int fused_divg, const int* gate) {
+  return ctx->fp32 ? solve_t<float>(ctx, fused_divg, gate) : solve_t<double>(ctx, fused_divg, gate);
 }
