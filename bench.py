@@ -139,6 +139,11 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="cells per side")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the precision-form leg")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--ens-steps", type=int, default=10, help="timed ensemble steps")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-window", type=float, default=4.0, help="arrival window of the serving leg (s)")
+    ap.add_argument("--serve-steps-scale", type=float, default=0.1, help="scale of the 50-500 step counts in the serving leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this path runs one domain per GPU on a single-GPU box; "
@@ -196,6 +201,27 @@ def main():
             "kernels_ms_per_substep": {k: round(v[0] / max(v[1], 1), 4) for k, v in tm32.items() if v[1] > 0},
             "error_sample": f"both forms from the same state after {args.warmup + args.steps} RK3 steps",
         }
+    if not args.no_ensemble:
+        # batched form: 48 melting simulations of uneven grids, graph-replayed
+        from melt_ibm_amd.ensemble import Ensemble, ensemble_members
+        print("# ensemble: building 48 simulations", file=sys.stderr, flush=True)
+        ens = Ensemble(ensemble_members(48, seed=0), precision=1, tune=True)
+        st = ens.run(steps=args.ens_steps, warmup=2)
+        ens.close()
+        st["workload"] = ("48 independent melting spheres (R 0.3-0.7, level-3/4 icosahedra), grids uniform from {48,64,96,128}^3, "
+                          "St in [0.5,4], Re in [200,1000], seeded; FP32 bulk with FP64 refinement; one graph replay per simulation "
+                          "and step on its own stream")
+        out["ensemble"] = st
+        print(f"# ensemble: p50 {st['step_ms_p50']:.2f} ms, p99 {st['step_ms_p99']:.2f} ms", file=sys.stderr, flush=True)
+    if not args.no_serve:
+        from melt_ibm_amd.serve import Server, request_stream
+        reqs = request_stream(args.serve_window, rate=4.0, seed=7, steps_scale=args.serve_steps_scale)
+        print(f"# serving: {len(reqs)} requests over {args.serve_window} s", file=sys.stderr, flush=True)
+        rep = Server(200e9, precision=1, tune=False).run(reqs, log=sys.stderr)
+        rep["workload"] = (f"Poisson arrivals (4/s, seed 7) over {args.serve_window} s; grids from {{64,128,256,384}}^3; spheres or "
+                           f"spheroids (aspect 1-3, 2k-60k faces); St in [0.5,4]; 50-500 RK3 steps x {args.serve_steps_scale}; "
+                           f"200 GB budget; FP32 bulk with FP64 refinement")
+        out["serving"] = rep
     if not args.no_cpu:
         cfg2, level, rad = workload(args.n)
         try:

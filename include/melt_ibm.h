@@ -212,6 +212,20 @@ int melt_timing_read(melt_ctx* ctx, double* ms, int64_t* samples);
  * and projection are applied.  Defaults: max_iter 1, rtol 1e-4.  Host arrays
  * of upload/download stay double in both forms. */
 int melt_set_refinement(melt_ctx* ctx, int32_t max_iter, double rtol);
+
+/* Graph replay (no counterpart in the reference, which has no device): with
+ * on = 1, melt_tschem captures one RK3 step into a HIP graph on first use and
+ * replays it afterwards; melt_set_dt with a new dt and melt_set_refinement
+ * drop the graph (their values are launch arguments).  Not used while timing
+ * is enabled or before the first tagging.  melt_stream returns the context's
+ * hipStream_t, so that a caller can overlap several contexts. */
+int melt_graph_enable(melt_ctx* ctx, int32_t on);
+/* Launch-shape override used by the autotuner (melt_ibm_amd/tune.py): key 0 =
+ * x-lines per block of the fused explicit/x-solve pass, 1 = its threads per
+ * block, 2 = lines per block of the y/z line solves, 3 = their threads per
+ * block; value 0 = built-in choice.  Results do not depend on it. */
+int melt_set_tuning(melt_ctx* ctx, int32_t key, int32_t value);
+void* melt_stream(melt_ctx* ctx);
 int melt_get_refinement_stats(melt_ctx* ctx, int64_t* checks, int64_t* refined, double* last_ratio);
 
 #ifdef __cplusplus

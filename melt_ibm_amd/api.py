@@ -184,9 +184,13 @@ _SIGS = {
     "timing_read": (C.c_int, [_P, _P, _P]),
     "set_refinement": (C.c_int, [_P, C.c_int32, C.c_double]),
     "get_refinement_stats": (C.c_int, [_P, _P, _P, _P]),
+    "graph_enable": (C.c_int, [_P, C.c_int32]),
+    "set_tuning": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "stream": (C.c_void_p, [_P]),
 }
 OPTIONAL = {"timing_enable", "timing_reset", "timing_count", "timing_name", "timing_read", "device_ok", "set_refinement",
-            "get_refinement_stats"}
+            "get_refinement_stats", "graph_enable", "stream",
+            "set_tuning"}
 
 
 def bind(lib: C.CDLL, prefix: str) -> dict:
@@ -380,6 +384,17 @@ class MeltContext:
         a, b, r = C.c_int64(), C.c_int64(), C.c_double()
         self._chk(self._f["get_refinement_stats"](self._ctx, C.byref(a), C.byref(b), C.byref(r)))
         return a.value, b.value, r.value
+
+    def graph(self, on: bool = True) -> None:
+        """Replay tschem as one captured HIP graph."""
+        self._chk(self._f["graph_enable"](self._ctx, 1 if on else 0))
+
+    TUNE_KEYS = {"xf_w": 0, "xf_nt": 1, "solve_w": 2, "solve_nt": 3}
+
+    def set_tuning(self, **kw) -> None:
+        """Launch-shape overrides (xf_w, xf_nt, solve_w, solve_nt; 0 = default)."""
+        for k, v in kw.items():
+            self._chk(self._f["set_tuning"](self._ctx, self.TUNE_KEYS[k], int(v)))
 
     def timing_read(self) -> dict:
         n = self._f["timing_count"]()

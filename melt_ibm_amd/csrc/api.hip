@@ -57,6 +57,15 @@ void fill_field(melt_ctx* ctx, int id, double v) {
 
 int check_ctx(melt_ctx* ctx) { return ctx ? 0 : MELT_ERR_ARG; }
 
+// Drop a captured step graph (its kernels carry dt and the stage coefficients
+// as arguments).
+void graph_drop(melt_ctx* ctx) {
+  if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
+  if (ctx->graph) hipGraphDestroy(ctx->graph);
+  ctx->graph_exec = nullptr;
+  ctx->graph = nullptr;
+}
+
 }  // namespace
 
 // --------------------------------------------------------------- timing
@@ -109,6 +118,7 @@ extern "C" int melt_device_ok(void) {
 extern "C" void melt_ctx_destroy(melt_ctx* ctx) {
   if (!ctx) return;
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  graph_drop(ctx);
   for (int i = 0; i < MELT_NFIELDS; ++i) if (ctx->f[i]) hipFree(ctx->f[i]);
   for (int i = 1; i < 4; ++i) if (ctx->rhs4[i]) hipFree(ctx->rhs4[i]);
   void* ps[] = {ctx->spec, ctx->ak1, ctx->ak2, ctx->ak3, ctx->tw1, ctx->tw2, ctx->tw3, ctx->d_body, ctx->d_vof, ctx->d_eof, ctx->d_voe,
@@ -358,6 +368,7 @@ extern "C" int melt_get_vert_of_face(melt_ctx* ctx, int32_t* host) {
 // ------------------------------------------------------------ time control
 extern "C" int melt_set_dt(melt_ctx* ctx, double dt) {
   if (check_ctx(ctx) || !(dt > 0.0)) return MELT_ERR_ARG;
+  if (dt != ctx->dt) graph_drop(ctx);
   ctx->dt = dt;
   return 0;
 }
@@ -573,9 +584,7 @@ extern "C" int melt_substep(melt_ctx* ctx, int32_t ns) {
   return 0;
 }
 
-extern "C" int melt_tschem(melt_ctx* ctx) {
-  if (check_ctx(ctx)) return MELT_ERR_ARG;
-  if (!(ctx->dt > 0.0)) return melt_fail(ctx, MELT_ERR_ARG, "tschem", "dt not set");
+static int tschem_launch(melt_ctx* ctx) {
   if (ctx->has_geom) {
     hipLaunchKernelGGL(k_reset_loads, dim3(1), dim3(1), 0, ctx->stream, ctx->d_body);
     MELT_HIP(hipGetLastError());
@@ -586,6 +595,47 @@ extern "C" int melt_tschem(melt_ctx* ctx) {
   }
   return 0;
 }
+
+extern "C" int melt_tschem(melt_ctx* ctx) {
+  if (check_ctx(ctx)) return MELT_ERR_ARG;
+  if (!(ctx->dt > 0.0)) return melt_fail(ctx, MELT_ERR_ARG, "tschem", "dt not set");
+  // the first tagging (ray casting, host round trip) happens outside graphs
+  bool graph = ctx->use_graph && !ctx->timing && !(ctx->has_geom && ctx->h_body.initial_tag);
+  if (!graph) return tschem_launch(ctx);
+  if (!ctx->graph_exec) {
+    MELT_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = tschem_launch(ctx);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) { if (g) hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return melt_fail(ctx, MELT_ERR_HIP, "tschem graph capture", hipGetErrorString(e));
+    ctx->graph = g;
+    MELT_HIP(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
+  }
+  MELT_HIP(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+  return 0;
+}
+
+// Replay tschem as one captured HIP graph (per context, per dt).
+extern "C" int melt_graph_enable(melt_ctx* ctx, int32_t on) {
+  if (check_ctx(ctx)) return MELT_ERR_ARG;
+  if (!on) graph_drop(ctx);
+  ctx->use_graph = on ? 1 : 0;
+  return 0;
+}
+
+// Launch-shape override (tile sweeps / autotuning): key 0 xf_w, 1 xf_nt,
+// 2 solve_w, 3 solve_nt; value 0 restores the built-in choice.
+extern "C" int melt_set_tuning(melt_ctx* ctx, int32_t key, int32_t value) {
+  if (check_ctx(ctx) || key < 0 || key >= TUNE_N || value < 0) return MELT_ERR_ARG;
+  graph_drop(ctx);
+  ctx->tune[key] = value;
+  return 0;
+}
+
+// The context's HIP stream (hipStream_t), for callers that order their own work
+// against it or overlap several contexts.
+extern "C" void* melt_stream(melt_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
 // ------------------------------------------------------------ diagnostics
 extern "C" int melt_cfl(melt_ctx* ctx, double* cflm) {
@@ -626,6 +676,7 @@ extern "C" int melt_timing_read(melt_ctx* ctx, double* ms, int64_t* samples) {
 // ------------------------------------------------------ precision form
 extern "C" int melt_set_refinement(melt_ctx* ctx, int32_t max_iter, double rtol) {
   if (check_ctx(ctx) || max_iter < 0 || max_iter > 8 || !(rtol >= 0.0)) return MELT_ERR_ARG;
+  graph_drop(ctx);
   ctx->refine_max = max_iter;
   ctx->refine_rtol = rtol;
   return 0;

@@ -65,6 +65,11 @@ struct melt_ctx {
   double al = 0, ga = 0, ro = 0, dt = 0, beta = 0;
   int stage = 0;
   hipStream_t stream = nullptr;
+  // graph replay of one RK3 step (melt_graph_enable): captured on first use,
+  // dropped whenever a host-side launch parameter (dt, refinement) changes
+  int use_graph = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
   int device = 0;
 
   // Eulerian fields: double, or float in the precision form (fp32 = 1); fld<R>()
