@@ -208,6 +208,7 @@ def main():
         ens = Ensemble(ensemble_members(48, seed=0), precision=1, tune=True)
         st = ens.run(steps=args.ens_steps, warmup=2)
         ens.close()
+        del ens
         st["workload"] = ("48 independent melting spheres (R 0.3-0.7, level-3/4 icosahedra), grids uniform from {48,64,96,128}^3, "
                           "St in [0.5,4], Re in [200,1000], seeded; FP32 bulk with FP64 refinement; one graph replay per simulation "
                           "and step on its own stream")
@@ -215,10 +216,12 @@ def main():
         print(f"# ensemble: p50 {st['step_ms_p50']:.2f} ms, p99 {st['step_ms_p99']:.2f} ms", file=sys.stderr, flush=True)
     if not args.no_serve:
         from melt_ibm_amd.serve import Server, request_stream
-        reqs = request_stream(args.serve_window, rate=4.0, seed=7, steps_scale=args.serve_steps_scale)
+        # the bench leg draws grids up to 256^3 (a 384^3 request spends ~10 s in host-side
+        # initial-field synthesis); python -m melt_ibm_amd.serve runs the full stream
+        reqs = request_stream(args.serve_window, rate=4.0, seed=7, steps_scale=args.serve_steps_scale, grids=(64, 128, 256))
         print(f"# serving: {len(reqs)} requests over {args.serve_window} s", file=sys.stderr, flush=True)
         rep = Server(200e9, precision=1, tune=False).run(reqs, log=sys.stderr)
-        rep["workload"] = (f"Poisson arrivals (4/s, seed 7) over {args.serve_window} s; grids from {{64,128,256,384}}^3; spheres or "
+        rep["workload"] = (f"Poisson arrivals (4/s, seed 7) over {args.serve_window} s; grids from {{64,128,256}}^3; spheres or "
                            f"spheroids (aspect 1-3, 2k-60k faces); St in [0.5,4]; 50-500 RK3 steps x {args.serve_steps_scale}; "
                            f"200 GB budget; FP32 bulk with FP64 refinement")
         out["serving"] = rep

@@ -373,6 +373,20 @@ __device__ inline unsigned xcd_block(unsigned b, unsigned nb) {
   return base + q;
 }
 
+// Tiled order of (a, b) line indices (na x nb lines): consecutive logical
+// blocks walk a ta x tb tile before moving on, so the lines a block's
+// stencils share with its neighbours in both directions are loaded into the
+// same L2 at about the same time.  ta | na and tb | nb (else 1 is used).
+__device__ __forceinline__ void tiled_2d(unsigned t, int na, int nb, int ta, int tb, int& a, int& b) {
+  if (na % ta) ta = 1;
+  if (nb % tb) tb = 1;
+  const unsigned per = (unsigned)(ta * tb);
+  unsigned tile = t / per, r = t % per;
+  int nta = na / ta;
+  a = (int)(tile % nta) * ta + (int)(r % ta);
+  b = (int)(tile / nta) * tb + (int)(r / ta);
+}
+
 // host-side helpers implemented in the kernel files
 int euler_launch_hdnl(melt_ctx* ctx, int which);
 int euler_launch_invtr_rhs(melt_ctx* ctx, int which);

@@ -402,8 +402,9 @@ __global__ __launch_bounds__(256) void k_explicit_xsolve(EulerArgs<R> A, RhsCoef
   const Grid& g = A.g;
   const int M = g.n1, ldx = M + 1, tid = threadIdx.x, nt = blockDim.x;
   unsigned t = xcd_block(blockIdx.x, gridDim.x);
-  int ntj = g.n2 / W;
-  int j0 = (t % ntj) * W, k = t / ntj;
+  int jt, k;
+  tiled_2d(t, g.n2 / W, g.n3, 16, 8, jt, k);
+  const int j0 = jt * W;
   const int base = M * (j0 + g.n2 * k);
   const int plane = W * ldx;
   const double sc_u = 1.0 / (1.0 + 2.0 * bxu), sc_t = 1.0 / (1.0 + 2.0 * bxt);
@@ -481,9 +482,18 @@ __global__ __launch_bounds__(256) void k_project(Grid g, R* __restrict__ vx, R* 
                                                  R dx1q, R dx2q, R dx3q, int do_u, int do_p, int nc, const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
   unsigned b = xcd_block(blockIdx.x, gridDim.x);
-  int c = (int)(b * blockDim.x + threadIdx.x);
-  if (c >= nc) return;
-  Cell s = make_cell(g, c);
+  Cell s;
+  if (g.n1 % blockDim.x == 0) {  // blocks tile whole x-lines: walk (j, k) in 16 x 8 tiles
+    const int per = g.n1 / blockDim.x;
+    int jj, kk;
+    tiled_2d(b / per, g.n2, g.n3, 16, 8, jj, kk);
+    s = cell_at(g, (int)((b % per) * blockDim.x + threadIdx.x), jj, kk);
+  } else {
+    int c = (int)(b * blockDim.x + threadIdx.x);
+    if (c >= nc) return;
+    s = make_cell(g, c);
+  }
+  const int c = s.c;
   R l = dph[c];
   R pim = dph[s.at(-1, 0, 0)], pjm = dph[s.at(0, -1, 0)], pkm = dph[s.at(0, 0, -1)];
   if (do_u) {

@@ -266,9 +266,16 @@ __global__ __launch_bounds__(256) void k_window_box(Grid g, const double* __rest
   }
 }
 
+// Initial state of the windows: no box placed yet, empty extents.
 __global__ void k_window_reset(melt_window* win, int n) {
-  for (int w = 0; w < n; ++w)
-    for (int d = 0; d < 3; ++d) { win[w].bmin[d] = INT_MAX; win[w].bmax[d] = INT_MIN; }
+  for (int w = 0; w < n; ++w) {
+    for (int d = 0; d < 3; ++d) {
+      win[w].org[d] = 0; win[w].prev_org[d] = 0;
+      win[w].bmin[d] = INT_MAX; win[w].bmax[d] = INT_MIN;
+    }
+    win[w].prev_valid = 0;
+    win[w].pad = 0;
+  }
 }
 
 // Place the forcing (which = 1: win[0]) or tagging (which = 2: win[1]) box

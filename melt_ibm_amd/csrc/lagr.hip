@@ -630,7 +630,8 @@ int lagr_alloc(melt_ctx* ctx, const double* xyz0, const int* voe, const int* eof
   rc |= dalloc((void**)&ctx->d_tkey, sizeof(unsigned long long) * ctx->twin_cells);
   rc |= dalloc((void**)&ctx->d_tface, sizeof(int) * ctx->twin_cells);
   if (rc) return MELT_ERR_HIP;
-  MELT_HIP(hipMemset(ctx->d_win, 0, sizeof(melt_window) * 2));
+  MELT_HIP(hipDeviceSynchronize());  // the zero fills above are not ordered with the context's stream
+  if (int rc2 = ibm_window_reset(ctx)) return rc2;
   MELT_HIP(hipMemcpy(ctx->d_vof, vof.data(), sizeof(int) * 3 * nf, hipMemcpyHostToDevice));
   MELT_HIP(hipMemcpy(ctx->d_eof, e0.data(), sizeof(int) * 3 * nf, hipMemcpyHostToDevice));
   MELT_HIP(hipMemcpy(ctx->d_voe, v0.data(), sizeof(int) * 2 * ne, hipMemcpyHostToDevice));

@@ -141,8 +141,9 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_x_fwd(SpecArgs<T> S, FftPlan P, 
   const Grid& g = S.g;
   const int N = g.n1, ls = N + 1;
   unsigned t = xcd_block(blockIdx.x, gridDim.x);
-  int ntj = g.n2 / nl;
-  int j0 = (t % ntj) * nl, k = t / ntj;
+  int jt, k;
+  tiled_2d(t, g.n2 / nl, g.n3, 16, 8, jt, k);  // the divergence reads lines j+1 and planes k+1
+  const int j0 = jt * nl;
   const T dx1 = (T)g.dx1, dx2 = (T)g.dx2, dx3 = (T)g.dx3;
   const int kp = k == g.n3 - 1 ? 0 : k + 1;
   for (int c = threadIdx.x; c < nl * N; c += blockDim.x) {

@@ -16,13 +16,14 @@ forcing, dt from CFL 0.5, FP32 bulk with FP64 refinement.
 """
 from __future__ import annotations
 
+import sys
 import time
 from dataclasses import dataclass, field
 
 import numpy as np
 
 from . import geometry, inputs
-from .api import MeltConfig, MeltContext
+from .api import MeltConfig, MeltContext, MeltError
 
 GRIDS = (48, 64, 96, 128)
 
@@ -76,6 +77,27 @@ class Simulation:
 
     @classmethod
     def build(cls, m: Member, precision: int = 1, lib=None, tuning: dict | None = None, graph: bool = True) -> "Simulation":
+        """Set the member up.  A mesh finer than the reference's ray tagging
+        allows (more than n1m faces in one pencil, pencil_rayTag.f90:126-130)
+        is replaced by the next coarser icosahedron level (or a frequency 3/4
+        as fine)."""
+        while True:
+            try:
+                return cls._build(m, precision, lib, tuning, graph)
+            except MeltError as e:
+                f0 = m.freq or 2 ** m.level
+                if e.code != 5 or f0 <= 2:
+                    raise
+                if m.freq:
+                    m.freq = max(2, (3 * m.freq) // 4)
+                else:
+                    m.level -= 1
+                f1 = m.freq or 2 ** m.level
+                print(f"# member seed {m.seed} ({m.n}^3, R {m.rad:.2f}): pencil overflow, {20 * f0 * f0} -> {20 * f1 * f1} faces",
+                      file=sys.stderr, flush=True)
+
+    @classmethod
+    def _build(cls, m: Member, precision: int, lib, tuning: dict | None, graph: bool) -> "Simulation":
         cfg = m.config(precision)
         ctx = MeltContext(cfg, lib=lib) if lib is not None else MeltContext(cfg)
         try:
@@ -136,6 +158,7 @@ class Ensemble:
         return {"sims": len(self.sims), "cells": self.cells, "steps": steps, "warmup": warmup,
                 "step_ms_p50": float(np.percentile(a, 50)), "step_ms_p99": float(np.percentile(a, 99)),
                 "step_ms_mean": float(a.mean()), "mcell_steps_per_s": self.cells * steps / (a.sum() / 1e3) / 1e6,
+                "faces": sorted({20 * (s.member.freq or 2 ** s.member.level) ** 2 for s in self.sims}),
                 "tuning": {str(k): v for k, v in self.tuning.items()}}
 
     def close(self) -> None:

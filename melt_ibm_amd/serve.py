@@ -26,6 +26,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+from .api import MeltError
 from .ensemble import Member, Simulation
 
 SERVE_GRIDS = (64, 128, 256, 384)
@@ -58,6 +59,7 @@ class Request:
     done_steps: int = 0
     step_ms: list = field(default_factory=list)
     bytes: int = 0
+    status: str = "ok"
 
 
 def request_stream(duration_s: float, rate: float = 4.0, seed: int = 7, steps_scale: float = 1.0, grids=SERVE_GRIDS) -> list[Request]:
@@ -121,15 +123,20 @@ class Server:
             for r in self.active:
                 r.sim.ctx.tschem()
             for r in self.active:
-                r.sim.ctx.sync()
+                try:
+                    r.sim.ctx.sync()
+                except MeltError as e:
+                    # the reference's terminal conditions end the request, not the server:
+                    # code 4 is a body melted below V_thresh (particle.f90:218-224)
+                    r.status = "melted" if e.code == 4 else f"error {e.code}"
                 r.step_ms.append((time.perf_counter() - ts) * 1e3)
                 r.done_steps += 1
                 cell_steps += r.member.cells
             still = []
             for r in self.active:
-                if r.done_steps >= r.member.steps:
+                if r.done_steps >= r.member.steps or r.status != "ok":
                     r.finished = time.perf_counter() - t0
-                    if keep_results:
+                    if keep_results and r.status == "ok":
                         results[r.rid] = {q: r.sim.ctx.download(q) for q in ("vx", "vy", "vz", "temp", "pr")}
                         results[r.rid]["volume"] = r.sim.ctx.body().volume
                     r.sim.close()
@@ -137,17 +144,24 @@ class Server:
                     self.used -= r.bytes
                     self.done.append(r)
                     if log:
-                        print(f"# request {r.rid} ({r.member.n}^3, {r.member.steps} steps) done at {r.finished:.2f} s", file=log, flush=True)
+                        print(f"# request {r.rid} ({r.member.n}^3, {r.done_steps}/{r.member.steps} steps, {r.status}) done at "
+                              f"{r.finished:.2f} s", file=log, flush=True)
                 else:
                     still.append(r)
             self.active = still
         wall = time.perf_counter() - t0
         lat = np.concatenate([np.array(r.step_ms) for r in self.done]) if self.done else np.zeros(1)
         wait = np.array([r.admitted - r.arrival for r in self.done]) if self.done else np.zeros(1)
-        rep = {"requests": len(self.done), "wall_s": wall, "mcell_steps_per_s": cell_steps / wall / 1e6,
+        status = {}
+        for r in self.done:
+            status[r.status] = status.get(r.status, 0) + 1
+        rep = {"requests": len(self.done), "status": status, "wall_s": wall, "mcell_steps_per_s": cell_steps / wall / 1e6,
                "step_ms_p50": float(np.percentile(lat, 50)), "step_ms_p99": float(np.percentile(lat, 99)),
                "queue_wait_s_p50": float(np.percentile(wait, 50)), "queue_wait_s_p99": float(np.percentile(wait, 99)),
-               "peak_budget_gb": self.peak / 1e9, "budget_gb": self.budget / 1e9}
+               "peak_budget_gb": self.peak / 1e9, "budget_gb": self.budget / 1e9,
+               "grids": sorted({r.member.n for r in self.done}),
+               "faces_range": [min(20 * (r.member.freq or 2 ** r.member.level) ** 2 for r in self.done),
+                               max(20 * (r.member.freq or 2 ** r.member.level) ** 2 for r in self.done)] if self.done else []}
         if keep_results:
             rep["results"] = results
         return rep

