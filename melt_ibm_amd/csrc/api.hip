@@ -121,6 +121,7 @@ extern "C" void melt_ctx_destroy(melt_ctx* ctx) {
   graph_drop(ctx);
   for (int i = 0; i < MELT_NFIELDS; ++i) if (ctx->f[i]) hipFree(ctx->f[i]);
   for (int i = 1; i < 4; ++i) if (ctx->rhs4[i]) hipFree(ctx->rhs4[i]);
+  for (int i = 0; i < 4; ++i) if (ctx->tmp4[i]) hipFree(ctx->tmp4[i]);
   void* ps[] = {ctx->spec, ctx->ak1, ctx->ak2, ctx->ak3, ctx->tw1, ctx->tw2, ctx->tw3, ctx->d_body, ctx->d_vof, ctx->d_eof, ctx->d_voe,
                 ctx->d_v2f_ptr, ctx->d_v2f, ctx->d_gv, ctx->d_ge, ctx->d_gf, ctx->d_pind, ctx->d_probe, ctx->d_rxp, ctx->d_rxt, ctx->d_red,
                 ctx->d_vbox, ctx->d_acc, ctx->d_fmean, ctx->d_tkey, ctx->d_tface, ctx->d_win, ctx->d_pcount, ctx->d_plist, ctx->d_err, ctx->d_gate, ctx->d_refine_stats};
@@ -164,9 +165,11 @@ extern "C" int melt_ctx_create(const melt_config* cfg, melt_ctx** out) {
         if (end == std::string::npos) end = spec.size();
         std::string item = spec.substr(pos, end - pos);
         size_t eq = item.find('=');
-        if (eq != std::string::npos)
+        if (eq != std::string::npos) {
           for (int t = 0; t < TUNE_N; ++t)
             if (item.substr(0, eq) == keys[t]) ctx->tune[t] = atoi(item.c_str() + eq + 1);
+          if (item.substr(0, eq) == "green") ctx->no_green = atoi(item.c_str() + eq + 1) == 0;
+        }
         pos = end + 1;
       }
     }
@@ -179,6 +182,8 @@ extern "C" int melt_ctx_create(const melt_config* cfg, melt_ctx** out) {
     ctx->rhs4[0] = ctx->f[MELT_RHS];
     for (int i = 1; i < 4 && !rc; ++i)
       if (hipMalloc(&ctx->rhs4[i], ctx->esz * ctx->nc) != hipSuccess) rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "rhs allocation");
+    for (int i = 0; i < 4 && !rc; ++i)
+      if (hipMalloc(&ctx->tmp4[i], ctx->esz * ctx->nc) != hipSuccess) rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "rhs allocation");
     if (rc) break;
     if (hipMalloc(&ctx->d_body, sizeof(melt_body)) != hipSuccess || hipMemset(ctx->d_body, 0, sizeof(melt_body)) != hipSuccess ||
         hipMalloc(&ctx->d_red, sizeof(double) * 16384) != hipSuccess || hipMalloc(&ctx->d_fmean, sizeof(double) * 4) != hipSuccess ||
@@ -532,15 +537,22 @@ extern "C" int melt_substep(melt_ctx* ctx, int32_t ns) {
   timing_end(ctx, SLOT_EXPLICIT, e);
   void* r[4] = {ctx->rhs4[RHS_U], ctx->rhs4[RHS_V], ctx->rhs4[RHS_W], ctx->rhs4[RHS_T]};
   void* q[4] = {ctx->f[MELT_VX], ctx->f[MELT_VY], ctx->f[MELT_VZ], ctx->f[MELT_TEMP]};
-  static const int comp[4] = {0, 1, 2, 3};
-  for (int dir = 1; dir < 3; ++dir) {
-    double b[4];
-    for (int m = 0; m < 4; ++m) b[m] = betadx_q(ctx, comp[m], dir);
-    int slot = dir == 1 ? SLOT_SOLVE_Y : SLOT_SOLVE_Z;
-    timing_begin(ctx, slot, &e);
-    if ((rc = euler_launch_solve(ctx, dir, r, 4, b, dir == 2 ? q : nullptr, 0))) return rc;
-    timing_end(ctx, slot, e);
-  }
+  double b[4];
+  // y: Green filter rhs -> tmp (or cyclic reduction in place)
+  for (int m = 0; m < 4; ++m) b[m] = betadx_q(ctx, m, 1);
+  timing_begin(ctx, SLOT_SOLVE_Y, &e);
+  int used_y = 0;
+  if ((rc = euler_launch_green(ctx, 1, r, ctx->tmp4, nullptr, b, 4, &used_y))) return rc;
+  if (!used_y && (rc = euler_launch_solve(ctx, 1, r, 4, b, nullptr, 0))) return rc;
+  timing_end(ctx, SLOT_SOLVE_Y, e);
+  // z: q += solution (solxk.f90:44-50)
+  void* const* zsrc = used_y ? ctx->tmp4 : r;
+  for (int m = 0; m < 4; ++m) b[m] = betadx_q(ctx, m, 2);
+  timing_begin(ctx, SLOT_SOLVE_Z, &e);
+  int used_z = 0;
+  if ((rc = euler_launch_green(ctx, 2, zsrc, nullptr, q, b, 4, &used_z))) return rc;
+  if (!used_z && (rc = euler_launch_solve(ctx, 2, zsrc, 4, b, q, 0))) return rc;
+  timing_end(ctx, SLOT_SOLVE_Z, e);
   timing_end(ctx, SLOT_PH_EULER, e_ph);
 
   timing_begin(ctx, SLOT_PH_MLS, &e_ph);

@@ -77,6 +77,8 @@ struct melt_ctx {
   size_t esz = sizeof(double);
   void* f[MELT_NFIELDS] = {};
   void* rhs4[4] = {};
+  void* tmp4[4] = {};      // output of the y-direction Green filters (the z pass reads it)
+  int no_green = 0;        // MELT_TUNE green=0: cyclic reduction for every line solve
   void* spec = nullptr;                                    // complex, element type as the fields
   double *ak1 = nullptr, *ak2 = nullptr, *ak3 = nullptr;   // 0-based wavenumber index
   void *tw1 = nullptr, *tw2 = nullptr, *tw3 = nullptr;     // exp(-2 pi i t / N), complex of the field type
@@ -393,6 +395,8 @@ int euler_launch_invtr_rhs(melt_ctx* ctx, int which);
 int euler_launch_solve(melt_ctx* ctx, int dir, void* const* fields, int nfield, const double* betadx, void* const* qadd, int store_rhs);
 int euler_launch_explicit_xsolve(melt_ctx* ctx, double bxu, double bxt);
 int euler_launch_divg(melt_ctx* ctx);
+int euler_launch_green(melt_ctx* ctx, int dir, void* const* src, void* const* dst, void* const* qadd, const double* betadx, int nfield,
+                       int* used);
 int euler_launch_project(melt_ctx* ctx, int do_u, int do_p, int use_mean, const int* gate = nullptr);
 int euler_launch_reduce_max(melt_ctx* ctx, int what, unsigned long long* d);
 int euler_launch_refine_gate(melt_ctx* ctx, const unsigned long long* dmax, int* gate);

@@ -388,16 +388,46 @@ __global__ __launch_bounds__(1024) void k_pcr(SolveBatch<R> batch, int M, int W,
   }
 }
 
+// ------------------------------------------- truncated Green's function
+// The exact inverse of the normalised periodic system x_i + a (x_{i-1} +
+// x_{i+1}) = d_i (a = -G/(1+2G)) is a circulant with entries
+//   g_m = C (rho^m + rho^(N-m)) / (1 - rho^N),  rho = -2a / (1 + sqrt(1 - 4a^2)),
+//   C = 1 / (1 + 2 a rho)  (0 <= m < N, g_{-m} = g_m),
+// so the solve is the symmetric filter x_i = g_0 d_i + sum_{m=1..K} g_m
+// (d_{i-m} + d_{i+m}) once |g_{K+1}| is below the storage precision.  For the
+// diagonally dominant systems of this path (rho ~ 1e-2 at 256^3) K is 8 in
+// FP64, so the line solve becomes a short stencil: coalesced, no LDS
+// reductions, no barriers.  Systems that decay more slowly (K > FIR_KMAX) use
+// the cyclic reduction above.  The result equals the exact solution to
+// round-off (the reference's Thomas + Sherman-Morrison is another route to it).
+constexpr int FIR_KMAX = 12;
+struct Green {
+  int K;
+  double g[FIR_KMAX + 1];
+};
+
+// x of element r of an LDS line (elements L[r], r < M, periodic).
+__device__ __forceinline__ double green_lds(const Green& G, const double* L, int r, int M) {
+  double x = G.g[0] * L[r];
+  for (int m = 1; m <= G.K; ++m) {
+    int rm = r - m; if (rm < 0) rm += M;
+    int rp = r + m; if (rp >= M) rp -= M;
+    x += G.g[m] * (L[rm] + L[rp]);
+  }
+  return x;
+}
+
 // Explicit terms + RHS of all four transported fields fused with the x-line
 // solve (pass A of the substep): a block owns W complete x-lines at one z
 // plane, computes rhs x4 and ru x4 per cell (E cells per thread, reading the
 // state through L1/L2), keeps the scaled rhs in LDS, solves the four
 // x-systems there and writes the x-solved rhs.  Saves one read and one write
 // of rhs x4 per substep against separate passes.
-template <int E, typename R>
+template <int E, typename R, bool FIR>
 __global__ __launch_bounds__(256) void k_explicit_xsolve(EulerArgs<R> A, RhsCoef<R> cf, R* __restrict__ ru1, R* __restrict__ ru2,
                                                          R* __restrict__ ru3, R* __restrict__ rut, R* __restrict__ r_u, R* __restrict__ r_v,
-                                                         R* __restrict__ r_w, R* __restrict__ r_t, double bxu, double bxt, int W) {
+                                                         R* __restrict__ r_w, R* __restrict__ r_t, double bxu, double bxt, int W, Green gu,
+                                                         Green gt) {
   extern __shared__ double lds[];
   const Grid& g = A.g;
   const int M = g.n1, ldx = M + 1, tid = threadIdx.x, nt = blockDim.x;
@@ -430,6 +460,17 @@ __global__ __launch_bounds__(256) void k_explicit_xsolve(EulerArgs<R> A, RhsCoef
   }
   __syncthreads();
   R* outs[4] = {r_u, r_v, r_w, r_t};
+  if constexpr (FIR) {
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      int idx = tid + q * nt;
+      int r = idx % M, c = idx / M;
+      const double* L = lds + c * ldx;
+#pragma unroll
+      for (int f = 0; f < 4; ++f) outs[f][base + idx] = (R)green_lds(f == 3 ? gt : gu, L + f * plane, r, M);
+    }
+    return;
+  }
 #pragma unroll 1
   for (int f = 0; f < 4; ++f) {
     double* L = lds + f * plane;
@@ -460,6 +501,83 @@ int pick_threads(int n, int cap, int want) {
   for (int e : kEs)
     if (n % e == 0 && n / e <= cap) return n / e;
   return 0;
+}
+
+// y (DIR 1) or z (DIR 2) line solve as the truncated Green's function: one
+// thread per element, reading src and writing dst (dst != src), or adding into
+// qadd (solxk.f90:44-50).  G folds in the 1/(1+2G) scaling of the rhs.
+template <typename R>
+struct GreenBatch {
+  const R* src[4];
+  R* dst[4];
+  R* qadd[4];
+  int which[4];   // index into gr[] of each field's coefficients
+  Green gr[2];
+};
+
+// A block is 64 consecutive i (coalesced rows) x 4 segments of GQ points of
+// the line; each thread slides a register window of GQ + 2 KT values along
+// its segment, so every input is read about (GQ + 2 KT) / GQ times (the halo
+// rows of neighbouring segments mostly from L2).  Coefficients beyond the
+// system's K are zero.  Needs n1 % 64 == 0 and N % (4 GQ) == 0.
+constexpr int GQ = 16;
+
+template <int DIR, typename R, int KT>
+__global__ __launch_bounds__(256) void k_green(GreenBatch<R> B, int n1, int n2, int n3) {
+  const int f = blockIdx.y;
+  const int N = DIR == 1 ? n2 : n3;
+  const int s = DIR == 1 ? n1 : n1 * n2;
+  const int nsg = N / (4 * GQ), nit = n1 / 64;
+  unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  const int sg = (int)(b % nsg), rest = (int)(b / nsg);
+  const int it = rest % nit, other = rest / nit;   // other: k (DIR 1) or j (DIR 2)
+  const int i = it * 64 + (threadIdx.x & 63);
+  const int p0 = (sg * 4 + (threadIdx.x >> 6)) * GQ;
+  const int base = i + (DIR == 1 ? n1 * n2 * other : n1 * other);
+  const Green& G = B.gr[B.which[f]];
+  double g[KT + 1];
+#pragma unroll
+  for (int m = 0; m <= KT; ++m) g[m] = m <= G.K ? G.g[m] : 0.0;
+  const R* __restrict__ d = B.src[f];
+  double v[GQ + 2 * KT];
+#pragma unroll
+  for (int q = 0; q < GQ + 2 * KT; ++q) {
+    int p = p0 - KT + q;
+    p = p < 0 ? p + N : (p >= N ? p - N : p);
+    v[q] = (double)d[base + p * s];
+  }
+#pragma unroll
+  for (int q = 0; q < GQ; ++q) {
+    double x = g[0] * v[q + KT];
+#pragma unroll
+    for (int m = 1; m <= KT; ++m) x += g[m] * (v[q + KT - m] + v[q + KT + m]);
+    const int c = base + (p0 + q) * s;
+    if (B.qadd[f]) B.qadd[f][c] = B.qadd[f][c] + (R)x;
+    else B.dst[f][c] = (R)x;
+  }
+}
+
+// Coefficients of the truncated Green's function of the normalised system
+// with off-diagonal ratio G = betadx on N points, scaled by `scale`; false
+// when more than FIR_KMAX terms are needed for relative truncation `tol`.
+bool green_coeffs(double betadx, int N, double tol, double scale, Green& out) {
+  long double G = betadx;
+  long double a = -G / (1.0L + 2.0L * G);
+  if (a == 0.0L) {
+    out.K = 0;
+    out.g[0] = scale;
+    return true;
+  }
+  long double rho = -2.0L * a / (1.0L + sqrtl(1.0L - 4.0L * a * a));
+  long double C = 1.0L / (1.0L + 2.0L * a * rho);
+  long double rN = powl(rho, (long double)N);
+  int K = 0;
+  while (K < N / 2 - 1 && fabsl(powl(rho, (long double)(K + 1))) > tol) ++K;
+  if (K > FIR_KMAX || 2 * K + 1 > N) return false;
+  out.K = K;
+  for (int m = 0; m <= K; ++m)
+    out.g[m] = (double)(scale * C * (powl(rho, (long double)m) + powl(rho, (long double)(N - m))) / (1.0L - rN));
+  return true;
 }
 
 // ------------------------------------------------------------ divg etc
@@ -619,10 +737,20 @@ int launch_explicit_xsolve(melt_ctx* ctx, double bxu, double bxt) {
   unsigned blocks = (unsigned)((size_t)(ctx->n2 / W) * ctx->n3);
   R* r4[4] = {static_cast<R*>(ctx->rhs4[RHS_U]), static_cast<R*>(ctx->rhs4[RHS_V]), static_cast<R*>(ctx->rhs4[RHS_W]),
               static_cast<R*>(ctx->rhs4[RHS_T])};
+  // the LDS line holds the scaled rhs, so the filter is the unscaled one
+  Green gu{}, gt{};
+  const double tol = ctx->fp32 ? 1e-8 : 2e-17;
+  const bool fir = !ctx->no_green && green_coeffs(bxu, ctx->n1, tol, 1.0, gu) && green_coeffs(bxt, ctx->n1, tol, 1.0, gt);
 #define XF_CASE(EE)                                                                                                                   \
   case EE:                                                                                                                            \
-    hipLaunchKernelGGL((k_explicit_xsolve<EE, R>), dim3(blocks), dim3(nt), shm, ctx->stream, A, cf, fld<R>(ctx, MELT_RU1),             \
-                       fld<R>(ctx, MELT_RU2), fld<R>(ctx, MELT_RU3), fld<R>(ctx, MELT_RUT), r4[0], r4[1], r4[2], r4[3], bxu, bxt, W); \
+    if (fir)                                                                                                                          \
+      hipLaunchKernelGGL((k_explicit_xsolve<EE, R, true>), dim3(blocks), dim3(nt), shm, ctx->stream, A, cf, fld<R>(ctx, MELT_RU1),     \
+                         fld<R>(ctx, MELT_RU2), fld<R>(ctx, MELT_RU3), fld<R>(ctx, MELT_RUT), r4[0], r4[1], r4[2], r4[3], bxu, bxt, W, \
+                         gu, gt);                                                                                                     \
+    else                                                                                                                              \
+      hipLaunchKernelGGL((k_explicit_xsolve<EE, R, false>), dim3(blocks), dim3(nt), shm, ctx->stream, A, cf, fld<R>(ctx, MELT_RU1),    \
+                         fld<R>(ctx, MELT_RU2), fld<R>(ctx, MELT_RU3), fld<R>(ctx, MELT_RUT), r4[0], r4[1], r4[2], r4[3], bxu, bxt, W, \
+                         gu, gt);                                                                                                     \
     break;
   switch (E) {
     XF_CASE(1) XF_CASE(2) XF_CASE(3) XF_CASE(4) XF_CASE(6) XF_CASE(8)
@@ -671,6 +799,48 @@ int launch_solve(melt_ctx* ctx, int dir, void* const* fields, int nfield, const 
 #undef PCR_DIR
 #undef PCR_CASE
   MELT_HIP(hipGetLastError());
+  return 0;
+}
+
+// y/z solves of the fused substep as Green filters: returns 1 (and launches
+// nothing) when a system needs the cyclic reduction instead.
+template <typename R>
+int launch_green(melt_ctx* ctx, int dir, void* const* src, void* const* dst, void* const* qadd, const double* betadx, int nfield,
+                 int* used) {
+  *used = 0;
+  if (ctx->no_green) return 0;
+  const int N = dir == 1 ? ctx->n2 : ctx->n3;
+  const double tol = ctx->fp32 ? 1e-8 : 2e-17;
+  GreenBatch<R> B{};
+  int ng = 0;
+  double bseen[2] = {0.0, 0.0};
+  for (int q = 0; q < nfield; ++q) {
+    int w = -1;
+    for (int i = 0; i < ng; ++i)
+      if (bseen[i] == betadx[q]) w = i;
+    if (w < 0) {
+      if (ng == 2) return 0;
+      if (!green_coeffs(betadx[q], N, tol, 1.0 / (1.0 + 2.0 * betadx[q]), B.gr[ng])) return 0;
+      bseen[ng] = betadx[q];
+      w = ng++;
+    }
+    B.which[q] = w;
+    B.src[q] = static_cast<const R*>(src[q]);
+    B.dst[q] = dst ? static_cast<R*>(dst[q]) : nullptr;
+    B.qadd[q] = qadd ? static_cast<R*>(qadd[q]) : nullptr;
+  }
+  int kmax = std::max(B.gr[0].K, ng > 1 ? B.gr[1].K : 0);
+  int KT = kmax <= 4 ? 4 : kmax <= 8 ? 8 : 12;
+  // KT < N / 2 keeps every window inside one period
+  if (ctx->n1 % 64 || N % (4 * GQ) || 2 * KT >= N) return 0;
+  const int nsg = N / (4 * GQ), nit = ctx->n1 / 64, other = dir == 1 ? ctx->n3 : ctx->n2;
+  dim3 grid((unsigned)(nsg * nit * other), nfield);
+#define GREEN_CASE(D, KK) hipLaunchKernelGGL((k_green<D, R, KK>), grid, dim3(256), 0, ctx->stream, B, ctx->n1, ctx->n2, ctx->n3)
+  if (dir == 1) { if (KT == 4) GREEN_CASE(1, 4); else if (KT == 8) GREEN_CASE(1, 8); else GREEN_CASE(1, 12); }
+  else { if (KT == 4) GREEN_CASE(2, 4); else if (KT == 8) GREEN_CASE(2, 8); else GREEN_CASE(2, 12); }
+#undef GREEN_CASE
+  MELT_HIP(hipGetLastError());
+  *used = 1;
   return 0;
 }
 
@@ -737,6 +907,11 @@ int euler_launch_explicit_xsolve(melt_ctx* ctx, double bxu, double bxt) {
 int euler_launch_solve(melt_ctx* ctx, int dir, void* const* fields, int nfield, const double* betadx, void* const* qadd, int store_rhs) {
   return ctx->fp32 ? launch_solve<float>(ctx, dir, fields, nfield, betadx, qadd, store_rhs)
                    : launch_solve<double>(ctx, dir, fields, nfield, betadx, qadd, store_rhs);
+}
+int euler_launch_green(melt_ctx* ctx, int dir, void* const* src, void* const* dst, void* const* qadd, const double* betadx, int nfield,
+                       int* used) {
+  return ctx->fp32 ? launch_green<float>(ctx, dir, src, dst, qadd, betadx, nfield, used)
+                   : launch_green<double>(ctx, dir, src, dst, qadd, betadx, nfield, used);
 }
 int euler_launch_divg(melt_ctx* ctx) { return ctx->fp32 ? launch_divg<float>(ctx) : launch_divg<double>(ctx); }
 int euler_launch_project(melt_ctx* ctx, int do_u, int do_p, int use_mean, const int* gate) {
