@@ -145,13 +145,17 @@ def main():
     ap.add_argument("--serve-window", type=float, default=4.0, help="arrival window of the serving leg (s)")
     ap.add_argument("--serve-steps-scale", type=float, default=0.1, help="scale of the 50-500 step counts in the serving leg")
     ap.add_argument("--copy-probe", action="store_true",
-                    help="only run plain device copies of 256 MiB (calibrates the HBM byte counters of a profiler)")
+                    help="only run plain device copies of 1 GiB (calibrates the HBM byte counters of a profiler)")
     args = ap.parse_args()
     if args.copy_probe:
         import torch
-        a = torch.rand(1 << 25, dtype=torch.float64, device="cuda")  # 256 MiB
+        # 1 GiB buffers (4x the 256 MiB last-level cache); the source is written
+        # first and then pushed out of the cache by filling a third buffer
+        a = torch.rand(1 << 27, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)
+        c = torch.empty_like(a)
         for _ in range(5):
+            c.fill_(1.0)
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": a.numel() * 8, "copy_probe_bytes_written": a.numel() * 8, "launches": 5}))
