@@ -131,6 +131,81 @@ def cpu_baseline(cfg, level, rad):
                       f"value = cells / (3 x substep time)"}
 
 
+PROFILES = ROOT / "profiles"
+TRAFFIC_JSON = PROFILES / "hbm_traffic_256cube_melting_sphere.json"
+
+
+def _kernel_short(name: str) -> str:
+    return name.replace("(anonymous namespace)::", "").split("(")[0]
+
+
+def _pmc_avg(db: str, counter: str) -> dict:
+    """Average value per dispatch of `counter` for every kernel in a rocprofv3 --pmc database."""
+    import sqlite3
+    con = sqlite3.connect(db)
+    out = {}
+    for name, val, n in con.execute("select kernel_name, sum(value), count(*) from counters_collection where counter_name = ? "
+                                    "group by kernel_name", (counter,)):
+        out[_kernel_short(name)] = (val / n, n)
+    return out
+
+
+def summarize_pmc(fetch_db, write_db, cfetch_db, cwrite_db, command: str, out_md: Path, out_json: Path) -> None:
+    """HBM bytes per launch from FETCH_SIZE / WRITE_SIZE (KiB per dispatch), each
+    scaled by what the same counter reads for a plain 1 GiB device copy."""
+    f, w = _pmc_avg(fetch_db, "FETCH_SIZE"), _pmc_avg(write_db, "WRITE_SIZE")
+    cf, cw = _pmc_avg(cfetch_db, "FETCH_SIZE"), _pmc_avg(cwrite_db, "WRITE_SIZE")
+    copy_bytes = float(1 << 30)
+    kf = copy_bytes / (cf["__amd_rocclr_copyBuffer"][0] * 1024.0)
+    kw = copy_bytes / (cw["__amd_rocclr_copyBuffer"][0] * 1024.0)
+    rows, data = [], {}
+    for k in sorted(f, key=lambda k: -f[k][0]):
+        if k not in w:
+            continue
+        rd, wr = f[k][0] * 1024.0 * kf, w[k][0] * 1024.0 * kw
+        data[k] = {"read_bytes": rd, "write_bytes": wr, "bytes_per_launch": rd + wr, "dispatches": f[k][1]}
+        if rd + wr > 1e6:
+            rows.append(f"| `{k}` | {f[k][1]} | {rd / 1e9:.3f} | {wr / 1e9:.3f} | {(rd + wr) / 1e9:.3f} |")
+    out_json.write_text(json.dumps({"read_scale": kf, "write_scale": kw, "kernels": data}, indent=1))
+    out_md.write_text("\n".join([
+        "# HBM traffic per launch, 256^3 melting-sphere workload (FP64)", "",
+        "Command (GPU box):", "", "```", command, "```", "",
+        f"Calibration: a plain 1 GiB device copy (`bench.py --copy-probe`, source evicted from the last-level cache) "
+        f"reads {1 / kf:.3f} FETCH_SIZE KiB per KiB and writes {1 / kw:.3f} WRITE_SIZE KiB per KiB, so reads = "
+        f"FETCH_SIZE x {kf:.3f}, writes = WRITE_SIZE x {kw:.3f}.", "",
+        "| kernel | dispatches | read GB | write GB | total GB per launch |", "|---|---|---|---|---|", *rows, ""]))
+
+
+def summarize_trace(db: str, command: str, out_md: Path, top: int = 30) -> None:
+    """Per-kernel average duration from a rocprofv3 --kernel-trace database."""
+    import sqlite3
+    con = sqlite3.connect(db)
+    rows = list(con.execute("select name, count(*), avg(duration), sum(duration) from kernels group by name order by sum(duration) desc"))
+    tot = sum(r[3] for r in rows)
+    lines = ["# Kernel times, 256^3 melting-sphere workload", "", "Command (GPU box):", "", "```", command, "```", "",
+             "| kernel | calls | avg us | total us | share |", "|---|---|---|---|---|"]
+    for name, n, avg, sm in rows[:top]:
+        lines.append(f"| `{_kernel_short(name)}` | {n} | {avg / 1e3:.1f} | {sm / 1e3:.1f} | {100 * sm / tot:.1f}% |")
+    out_md.write_text("\n".join(lines + [""]))
+
+
+KERNEL_SYMBOL = {"explicit_xsolve": "k_explicit_xsolve", "solve_y": "k_green<1", "solve_z_update": "k_green<2",
+                 "divg_fft_x": "k_fft_x_fwd", "fft_y": "k_fft_y", "fft_z_solve": "k_fft_z_solve", "ifft_x": "k_fft_x_bwd",
+                 "project": "k_project"}
+
+
+def measured_traffic(slot: str, n: int):
+    """HBM bytes per launch of the kernel behind `slot` from the committed
+    counter summary (256^3 FP64 workload only), else None."""
+    if n != 256 or not TRAFFIC_JSON.exists():
+        return None
+    data = json.loads(TRAFFIC_JSON.read_text())["kernels"]
+    for k, v in data.items():
+        if k.removeprefix("void ").startswith(KERNEL_SYMBOL.get(slot, "?")) and "double" in k:
+            return round(v["bytes_per_launch"])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,9 +219,19 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-window", type=float, default=4.0, help="arrival window of the serving leg (s)")
     ap.add_argument("--serve-steps-scale", type=float, default=0.1, help="scale of the 50-500 step counts in the serving leg")
+    ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_DB", "WRITE_DB", "COPY_FETCH_DB", "COPY_WRITE_DB"),
+                    help="write the HBM-traffic summary under profiles/ from rocprofv3 --pmc databases")
+    ap.add_argument("--summarize-trace", metavar="TRACE_DB", help="write the kernel-time summary under profiles/")
+    ap.add_argument("--command", default="", help="command line recorded in a summary")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run plain device copies of 1 GiB (calibrates the HBM byte counters of a profiler)")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        summarize_pmc(*args.summarize_pmc, args.command, PROFILES / "hbm_traffic_256cube_melting_sphere.md", TRAFFIC_JSON)
+        return
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace, args.command, PROFILES / "kernel_times_256cube_melting_sphere.md")
+        return
     if args.copy_probe:
         import torch
         # 1 GiB buffers (4x the 256 MiB last-level cache); the source is written
@@ -192,7 +277,7 @@ def main():
                                f"St=2, Newton-Euler on, Re=1000, FP64",
                    "grid": [cfg.n1m, cfg.n2m, cfg.n3m], "faces": 20480, "precision": "fp64"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": measured_traffic(dom, args.n),
                      "bytes_per_launch": bytes_launch, "avg_launch_us": round(avg_s * 1e6, 2),
                      "substep_frac": round(B_ALG_CELL * cells / sub_time / HBM_PEAK, 4),
                      "substep_ms": round(sub_time * 1e3, 4)},

@@ -526,7 +526,9 @@ extern "C" int melt_substep(melt_ctx* ctx, int32_t ns) {
   timing_begin(ctx, SLOT_PH_TAG, &e_ph);
   if (moving) {
     timing_begin(ctx, SLOT_TAG, &e);
-    if ((rc = ibm_tag(ctx))) return rc;
+    ctx->force_window_from_tag = 1;
+    rc = ibm_tag(ctx);
+    if (rc) { ctx->force_window_from_tag = 0; return rc; }
     timing_end(ctx, SLOT_TAG, e);
   }
   timing_end(ctx, SLOT_PH_TAG, e_ph);
@@ -558,9 +560,13 @@ extern "C" int melt_substep(melt_ctx* ctx, int32_t ns) {
   timing_begin(ctx, SLOT_PH_MLS, &e_ph);
   if (ibm) {
     timing_begin(ctx, SLOT_IBM, &e);
-    if ((rc = ibm_find_centroid_indices(ctx)) || (rc = ibm_weights(ctx)) || (rc = ibm_force_scatter(ctx)) || (rc = ibm_probes(ctx)) ||
-        (rc = ibm_force_drain(ctx, 1)))
-      return rc;
+    rc = ibm_find_centroid_indices(ctx);
+    if (!rc) rc = ibm_weights(ctx);
+    if (!rc) rc = ibm_force_scatter(ctx);
+    ctx->force_window_from_tag = 0;
+    if (!rc) rc = ibm_probes(ctx);
+    if (!rc) rc = ibm_force_drain(ctx, 1);
+    if (rc) return rc;
     timing_end(ctx, SLOT_IBM, e);
     timing_begin(ctx, SLOT_LAGR, &e);
     if ((rc = lagr_after_force(ctx))) return rc;
@@ -568,6 +574,7 @@ extern "C" int melt_substep(melt_ctx* ctx, int32_t ns) {
   }
   timing_end(ctx, SLOT_PH_MLS, e_ph);
 
+  ctx->force_window_from_tag = 0;
   timing_begin(ctx, SLOT_PH_PRESSURE, &e_ph);
   // the precision form measures the FP64 divergence of the provisional field
   // (before) and of the projected one (after) for its refinement gate

@@ -125,6 +125,7 @@ struct melt_ctx {
   int* d_err = nullptr;    // device error word (first error wins)
   int* h_err = nullptr;    // pinned mirror
   int defer_mean = 0;      // fused path: mean force applied in the projection
+  int force_window_from_tag = 0;  // fused substep: tagging also places the forcing window
 
   // timing
   int timing = 0;

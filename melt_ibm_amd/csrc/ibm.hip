@@ -278,18 +278,21 @@ __global__ void k_window_reset(melt_window* win, int n) {
   }
 }
 
-// Place the forcing (which = 1: win[0]) or tagging (which = 2: win[1]) box
-// from the reduced face extent, then reset the extent for the next call.
+// Place the forcing (which & 1: win[0]) and / or tagging (which & 2: win[1])
+// box from the face extent reduced into win[0].bmin/bmax, then reset the
+// extent for the next call.
 __global__ void k_window_final(melt_window* win, Win3 ef, Win3 et, int which, int* err) {
-  melt_window& w = win[which == 1 ? 0 : 1];
   int mn[3], mx[3];
-  for (int d = 0; d < 3; ++d) { mn[d] = w.bmin[d]; mx[d] = w.bmax[d]; w.bmin[d] = INT_MAX; w.bmax[d] = INT_MIN; }
-  if (which == 1) {  // forcing window
+  for (int d = 0; d < 3; ++d) { mn[d] = win[0].bmin[d]; mx[d] = win[0].bmax[d]; win[0].bmin[d] = INT_MAX; win[0].bmax[d] = INT_MIN; }
+  if (which & 1) {  // forcing window
+    melt_window& w = win[0];
     for (int d = 0; d < 3; ++d) {
       w.org[d] = mn[d] - 2;
       if (mx[d] + 3 - w.org[d] >= ef.e[d]) raise_error(err, MELT_ERR_WINDOW);
     }
-  } else {  // tagging window
+  }
+  if (which & 2) {  // tagging window
+    melt_window& w = win[1];
     if (w.prev_valid >= 0) {
       for (int d = 0; d < 3; ++d) w.prev_org[d] = w.org[d];
       w.prev_valid = w.pad;
@@ -643,41 +646,46 @@ struct ForceArgs {
 };
 
 // forc1/forc2/forctemp/forc3: Um = sum q phi; spread cfac (V - Um) phi
+// forc1/forc2/forctemp/forc3 (forc1.f90:28-65): Um = sum q phi, spread
+// cfac (V - Um) phi.  32 lanes per (face, component): lane l < 27 loads stencil
+// point l and its weight, every lane forms Um from the 27 products in the
+// reference's point order (shuffles, so Um rounds exactly as the serial loop),
+// and lane l adds its contribution to the exact 64.64 accumulators.
 template <typename R>
-__global__ __launch_bounds__(64) void k_mls_force(Grid g, ForceArgs<R> F, const double* __restrict__ bar, const unsigned char* __restrict__ gf,
-                                                  int nf, const int* __restrict__ pind, const melt_window* __restrict__ win, Win3 ef,
-                                                  unsigned long long* __restrict__ acc, size_t cells, int* err) {
-  int id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= nf * 4) return;
-  int f = id >> 2, comp = id & 3;
-  if (gf[f]) return;
-  const int* p = &pind[6 * f];
-  const double* pos = &bar[3 * f];
-  int i0, j0, kw, ku;
-  int k1u = (int)floor(pos[2] * g.dx3) + 1, kstu = fnint(pos[2] * g.dx3) + 1;
-  if (comp == 0) { i0 = p[3]; j0 = p[1]; kw = p[2]; ku = k1u; }
-  else if (comp == 1) { i0 = p[0]; j0 = p[4]; kw = p[2]; ku = k1u; }
-  else if (comp == 2) { i0 = p[0]; j0 = p[1]; kw = p[2]; ku = k1u; }
-  else { i0 = p[0]; j0 = p[1]; kw = p[5]; ku = kstu; }
-  const double* phi = &F.phi[comp][27 * (size_t)f];
-  const R* q = F.fld[comp];
+__global__ __launch_bounds__(256) void k_mls_force(Grid g, ForceArgs<R> F, const double* __restrict__ bar, const unsigned char* __restrict__ gf,
+                                                   int nf, const int* __restrict__ pind, const melt_window* __restrict__ win, Win3 ef,
+                                                   unsigned long long* __restrict__ acc, size_t cells, int* err) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int item = gid >> 5, l = threadIdx.x & 31, lbase = threadIdx.x & 32;
+  const int f = item >> 2, comp = item & 3;
+  // every lane takes part in the shuffles; dead items / lanes carry zeros
+  const bool live = f < nf && !gf[f];
+  int i0 = 0, j0 = 0, kw = 0, ku = 0;
+  if (live) {
+    const int* p = &pind[6 * f];
+    const double* pos = &bar[3 * f];
+    int k1u = (int)floor(pos[2] * g.dx3) + 1, kstu = fnint(pos[2] * g.dx3) + 1;
+    if (comp == 0) { i0 = p[3]; j0 = p[1]; kw = p[2]; ku = k1u; }
+    else if (comp == 1) { i0 = p[0]; j0 = p[4]; kw = p[2]; ku = k1u; }
+    else if (comp == 2) { i0 = p[0]; j0 = p[1]; kw = p[2]; ku = k1u; }
+    else { i0 = p[0]; j0 = p[1]; kw = p[5]; ku = kstu; }
+  }
+  const bool act = live && l < 27;
+  const int di = l % 3 - 1, dj = (l / 3) % 3 - 1, dk = l / 9 - 1;
+  double ph = 0.0, prod = 0.0;
+  if (act) {
+    ph = F.phi[comp][27 * (size_t)f + l];
+    prod = (double)F.fld[comp][widx(g, i0 + di, j0 + dj, kw + dk)] * ph;
+  }
   double Um = 0.0;
-  int inw = 0;
-  for (int k = kw - 1; k <= kw + 1; ++k)
-    for (int j = j0 - 1; j <= j0 + 1; ++j)
-      for (int i = i0 - 1; i <= i0 + 1; ++i) Um = Um + q[widx(g, i, j, k)] * phi[inw++];
+#pragma unroll
+  for (int q = 0; q < 27; ++q) Um = Um + __shfl(prod, lbase + q, 64);
+  if (!act) return;
   double V = comp == 2 ? F.tmelt : F.vel_tri[3 * f + (comp == 0 ? 0 : comp == 1 ? 1 : 2)];
-  double cf = F.cfac[f];
-  unsigned long long* a = acc + (size_t)comp * cells * 2;
-  inw = 0;
-  for (int dk = -1; dk <= 1; ++dk)
-    for (int j = j0 - 1; j <= j0 + 1; ++j)
-      for (int i = i0 - 1; i <= i0 + 1; ++i) {
-        double v = cf * (V - Um) * phi[inw++];
-        long long s = win_slot(win[0].org, ef.e, i, j, ku + dk);
-        if (s < 0) { raise_error(err, MELT_ERR_WINDOW); return; }
-        acc_add(&a[2 * s], v);
-      }
+  double v = F.cfac[f] * (V - Um) * ph;
+  long long sl = win_slot(win[0].org, ef.e, i0 + di, j0 + dj, ku + dk);
+  if (sl < 0) { raise_error(err, MELT_ERR_WINDOW); return; }
+  acc_add(&acc[((size_t)comp * cells + (size_t)sl) * 2], v);
 }
 
 // Drain the accumulators: mode 0 adds the forces to the fields (velocity
@@ -973,8 +981,11 @@ static int ibm_tag_t(melt_ctx* ctx) {  // tagCells.f90:4-27 (tagType 1) + interp
   }
   Win3 ef{{ctx->fwin[0], ctx->fwin[1], ctx->fwin[2]}}, et{{ctx->twin[0], ctx->twin[1], ctx->twin[2]}};
   hipLaunchKernelGGL(k_window_box, dim3(nb(ctx->nf, 256)), dim3(256), 0, ctx->stream, g, ctx->d_lag[MELT_L_TRI_BAR], ctx->d_gf, ctx->nf,
-                     ctx->d_win + 1);
-  hipLaunchKernelGGL(k_window_final, dim3(1), dim3(1), 0, ctx->stream, ctx->d_win, ef, et, 2, ctx->d_err);
+                     ctx->d_win);
+  // the fused substep places the forcing window from the same extent (the
+  // faces do not move between tagging and forcing)
+  hipLaunchKernelGGL(k_window_final, dim3(1), dim3(1), 0, ctx->stream, ctx->d_win, ef, et, ctx->force_window_from_tag ? 3 : 2,
+                     ctx->d_err);
   hipLaunchKernelGGL(k_tag_reset<R>, dim3(256), dim3(256), 0, ctx->stream, g, fld<R>(ctx, MELT_SDF), ctx->d_win, et);
   hipLaunchKernelGGL(k_tag_init, dim3(256), dim3(256), 0, ctx->stream, ctx->d_tkey, ctx->d_tface, ctx->twin_cells);
   MeshArgs M = mesh_args(ctx);
@@ -1006,15 +1017,17 @@ template <typename R>
 static int ibm_force_scatter_t(melt_ctx* ctx) {
   Grid g = make_grid(ctx);
   Win3 ef{{ctx->fwin[0], ctx->fwin[1], ctx->fwin[2]}}, et{{ctx->twin[0], ctx->twin[1], ctx->twin[2]}};
-  hipLaunchKernelGGL(k_window_box, dim3(nb(ctx->nf, 256)), dim3(256), 0, ctx->stream, g, ctx->d_lag[MELT_L_TRI_BAR], ctx->d_gf, ctx->nf,
-                     ctx->d_win);
-  hipLaunchKernelGGL(k_window_final, dim3(1), dim3(1), 0, ctx->stream, ctx->d_win, ef, et, 1, ctx->d_err);
+  if (!ctx->force_window_from_tag) {
+    hipLaunchKernelGGL(k_window_box, dim3(nb(ctx->nf, 256)), dim3(256), 0, ctx->stream, g, ctx->d_lag[MELT_L_TRI_BAR], ctx->d_gf, ctx->nf,
+                       ctx->d_win);
+    hipLaunchKernelGGL(k_window_final, dim3(1), dim3(1), 0, ctx->stream, ctx->d_win, ef, et, 1, ctx->d_err);
+  }
   ForceArgs<R> F;
   F.fld[0] = fld<R>(ctx, MELT_VX); F.fld[1] = fld<R>(ctx, MELT_VY); F.fld[2] = fld<R>(ctx, MELT_TEMP); F.fld[3] = fld<R>(ctx, MELT_VZ);
   F.phi[0] = ctx->d_lag[MELT_L_PTXAB_Q1]; F.phi[1] = ctx->d_lag[MELT_L_PTXAB_Q2];
   F.phi[2] = ctx->d_lag[MELT_L_PTXAB_TEMP]; F.phi[3] = ctx->d_lag[MELT_L_PTXAB_Q3];
   F.vel_tri = ctx->d_lag[MELT_L_VEL_TRI]; F.cfac = ctx->d_lag[MELT_L_CFAC]; F.tmelt = ctx->cfg.tmelt;
-  hipLaunchKernelGGL(k_mls_force<R>, dim3(nb((size_t)ctx->nf * 4, 64)), dim3(64), 0, ctx->stream, g, F, ctx->d_lag[MELT_L_TRI_BAR], ctx->d_gf,
+  hipLaunchKernelGGL(k_mls_force<R>, dim3(nb((size_t)ctx->nf * 4 * 32, 256)), dim3(256), 0, ctx->stream, g, F, ctx->d_lag[MELT_L_TRI_BAR], ctx->d_gf,
                      ctx->nf, ctx->d_pind, ctx->d_win, ef, ctx->d_acc, ctx->fwin_cells, ctx->d_err);
   MELT_HIP(hipGetLastError());
   return 0;

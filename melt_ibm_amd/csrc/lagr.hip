@@ -102,28 +102,58 @@ __global__ __launch_bounds__(RB) void k_face_sums(LagArgs A, const double* __res
   if ((int)threadIdx.x < ns) part[threadIdx.x * gridDim.x + blockIdx.x] = sm[threadIdx.x][0];
 }
 
-// Stage 2: out[q] = fixed-tree sum of the nb partials of quantity q (nb <= RB).
-__global__ __launch_bounds__(RB) void k_sum_partials(const double* __restrict__ part, int nb, int ns, double* __restrict__ out) {
-  __shared__ double sm[RB];
-  for (int q = 0; q < ns; ++q) {
-    sm[threadIdx.x] = (int)threadIdx.x < nb ? part[q * nb + threadIdx.x] : 0.0;
-    __syncthreads();
-    for (int w = RB / 2; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) sm[threadIdx.x] = sm[threadIdx.x] + sm[threadIdx.x + w];
-      __syncthreads();
+__device__ void rigid_finish(const double* s, double dens_ratio, double com[3], double* vol, double I[9]);
+
+// Body updates that consume a face sum (run by one thread after stage 2).
+enum { FIN_NONE = 0, FIN_LOADS, FIN_VOLUME, FIN_RIGID, FIN_FINAL, FIN_SURFACE };
+
+// Stage 2: out[q] = fixed-pattern sum of the nb <= 64 partials of quantity q
+// (one wave per quantity, butterfly over the lanes), then the body update
+// `fin` on out[].
+__global__ __launch_bounds__(RB) void k_sum_partials(const double* __restrict__ part, int nb, int ns, double* __restrict__ out, int fin,
+                                                     melt_body* body, double dens_ratio, int* err) {
+  __shared__ double sm[16];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = RB / 64;
+  for (int q = wv; q < ns; q += nw) {
+    double v = lane < nb ? part[q * nb + lane] : 0.0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = v + __shfl_xor(v, off, 64);
+    if (lane == 0) sm[q] = v;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < ns) out[threadIdx.x] = sm[threadIdx.x];
+  if (threadIdx.x != 0 || fin == FIN_NONE) return;
+  melt_body* b = body;
+  if (fin == FIN_LOADS) {  // mls_structLoads.f90:22-30, 224-256
+    for (int d = 0; d < 3; ++d) {
+      b->int_prn_dA_m1[d] = b->int_prn_dA[d]; b->int_tau_dA_m1[d] = b->int_tau_dA[d];
+      b->int_r_x_prn_dA_m1[d] = b->int_r_x_prn_dA[d]; b->int_r_x_tau_dA_m1[d] = b->int_r_x_tau_dA[d];
+      b->int_prn_dA[d] = sm[d]; b->int_r_x_prn_dA[d] = sm[3 + d];
+      b->int_tau_dA[d] = sm[6 + d]; b->int_r_x_tau_dA[d] = sm[9 + d];
+      b->fp[d] = b->fp[d] + b->int_prn_dA[d]; b->ftau[d] = b->ftau[d] + b->int_tau_dA[d];
+      b->torq_p[d] = b->torq_p[d] + b->int_r_x_prn_dA[d]; b->torq_tau[d] = b->torq_tau[d] + b->int_r_x_tau_dA[d];
     }
-    if (threadIdx.x == 0) out[q] = sm[0];
-    __syncthreads();
+  } else if (fin == FIN_VOLUME) {
+    b->volume = sm[0] / 6.;
+  } else if (fin == FIN_RIGID) {
+    rigid_finish(sm, dens_ratio, b->pos_cm, &b->volume, b->inert_tensor);
+  } else if (fin == FIN_FINAL) {
+    b->surface = sm[0];
+    b->volume = sm[1] / 6.;
+    rigid_finish(&sm[2], dens_ratio, b->pos_cm, &b->volume, b->inert_tensor);
+    if (b->volume < b->v_thresh) raise_error(err, MELT_ERR_SMALL_VOLUME);
+  } else if (fin == FIN_SURFACE) {
+    b->surface = sm[0];
   }
 }
 
 // Two-stage face sum of `mode` into out[] (d_red scratch holds the partials).
-void face_sums(melt_ctx* ctx, const LagArgs& A, const double* xyz, int mode, double* out) {
+void face_sums(melt_ctx* ctx, const LagArgs& A, const double* xyz, int mode, double* out, int fin = FIN_NONE) {
   int G = std::max(1, std::min(64, (A.nf + RB - 1) / RB));
   int ns = mode == SUM_VOLUME ? 1 : mode == SUM_RIGID ? 10 : 12;
   double* part = ctx->d_red + 8192;
   hipLaunchKernelGGL(k_face_sums, dim3(G), dim3(RB), 0, ctx->stream, A, xyz, ctx->d_rxp, ctx->d_rxt, mode, part);
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(RB), 0, ctx->stream, part, G, ns, out);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(RB), 0, ctx->stream, part, G, ns, out, fin, ctx->d_body, ctx->cfg.dens_ratio, ctx->d_err);
 }
 
 // calc_rigidBody_params epilogue (RigidAuxRoutines.f90:716-740) from the 10 sums
@@ -150,28 +180,11 @@ __global__ void k_body_rigid(melt_body* b, const double* __restrict__ s, double 
   rigid_finish(s, dens_ratio, b->pos_cm, &b->volume, b->inert_tensor);
 }
 
-__global__ void k_body_volume(melt_body* b, const double* __restrict__ s) { b->volume = s[0] / 6.; }
 
-__global__ void k_body_loads(melt_body* b, const double* __restrict__ s) {  // mls_structLoads.f90:22-30, 224-256
-  for (int d = 0; d < 3; ++d) {
-    b->int_prn_dA_m1[d] = b->int_prn_dA[d]; b->int_tau_dA_m1[d] = b->int_tau_dA[d];
-    b->int_r_x_prn_dA_m1[d] = b->int_r_x_prn_dA[d]; b->int_r_x_tau_dA_m1[d] = b->int_r_x_tau_dA[d];
-    b->int_prn_dA[d] = s[d]; b->int_r_x_prn_dA[d] = s[3 + d];
-    b->int_tau_dA[d] = s[6 + d]; b->int_r_x_tau_dA[d] = s[9 + d];
-    b->fp[d] = b->fp[d] + b->int_prn_dA[d]; b->ftau[d] = b->ftau[d] + b->int_tau_dA[d];
-    b->torq_p[d] = b->torq_p[d] + b->int_r_x_prn_dA[d]; b->torq_tau[d] = b->torq_tau[d] + b->int_r_x_tau_dA[d];
-  }
-}
 
 // geometry refresh epilogue: Surface, Volume (calculate_volume, then the
 // rigid-body volume overwrites it as in particle.f90:170-182), COM, inertia,
 // small-volume abort (particle.f90:218-224)
-__global__ void k_body_final(melt_body* b, const double* __restrict__ s, double dens_ratio, int* err) {
-  b->surface = s[0];
-  b->volume = s[1] / 6.;
-  rigid_finish(&s[2], dens_ratio, b->pos_cm, &b->volume, b->inert_tensor);
-  if (b->volume < b->v_thresh) raise_error(err, MELT_ERR_SMALL_VOLUME);
-}
 
 // ---------------------------------------------------------- heat / Stefan
 __global__ __launch_bounds__(256) void k_face_post(LagArgs A, const double* __restrict__ probe, const melt_body* __restrict__ b,
@@ -268,12 +281,16 @@ __global__ __launch_bounds__(256) void k_face_area(LagArgs A, const double* __re
 
 __global__ __launch_bounds__(256) void k_edge_len(LagArgs A, const double* __restrict__ x, melt_body* b, double e_thresh) {  // triaux.f90:217-245
   int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= A.ne || A.ge[e]) return;
-  int v1 = A.voe[2 * e], v2 = A.voe[2 * e + 1];
-  double a = x[3 * v1] - x[3 * v2], c = x[3 * v1 + 1] - x[3 * v2 + 1], d = x[3 * v1 + 2] - x[3 * v2 + 2];
-  double l = sqrt(a * a + c * c + d * d);
-  A.L[MELT_L_ELENGTHS][e] = l;
-  if (l < e_thresh) atomicExch(&b->rm_flag, 1);
+  bool short_edge = false;
+  if (e < A.ne && !A.ge[e]) {
+    int v1 = A.voe[2 * e], v2 = A.voe[2 * e + 1];
+    double a = x[3 * v1] - x[3 * v2], c = x[3 * v1 + 1] - x[3 * v2 + 1], d = x[3 * v1 + 2] - x[3 * v2 + 2];
+    double l = sqrt(a * a + c * c + d * d);
+    A.L[MELT_L_ELENGTHS][e] = l;
+    short_edge = l < e_thresh;
+  }
+  // the remesh flag is a logical OR: one atomic per wave that saw a short edge
+  if (__ballot(short_edge) && (threadIdx.x & 63) == 0) atomicExch(&b->rm_flag, 1);
 }
 
 __global__ void k_clear_rm(melt_body* b) { b->rm_flag = 0; }
@@ -472,8 +489,7 @@ int refresh_geometry(melt_ctx* ctx, LagArgs& A) {
   hipLaunchKernelGGL(k_face_centroid_normal, dim3(nb(ctx->nf, 256)), dim3(256), 0, s, A, x, 1, 1, 1);
   hipLaunchKernelGGL(k_skewness, dim3(nb(ctx->nf, 256)), dim3(256), 0, s, A);
   hipLaunchKernelGGL(k_vert_geom, dim3(nb(ctx->nv, 256)), dim3(256), 0, s, A);
-  face_sums(ctx, A, x, (int)SUM_AREA_VOL_RIGID, ctx->d_red);
-  hipLaunchKernelGGL(k_body_final, dim3(1), dim3(1), 0, s, ctx->d_body, ctx->d_red, ctx->cfg.dens_ratio, ctx->d_err);
+  face_sums(ctx, A, x, (int)SUM_AREA_VOL_RIGID, ctx->d_red, FIN_FINAL);
   if (ctx->cfg.imelt == 1) hipLaunchKernelGGL(k_cfac, dim3(nb(ctx->nf, 256)), dim3(256), 0, s, A, ctx->d_body);
   MELT_HIP(hipGetLastError());
   return 0;
@@ -491,20 +507,17 @@ int lagr_after_force(melt_ctx* ctx) {
     hipLaunchKernelGGL(k_face_post, dim3(nb(ctx->nf, 256)), dim3(256), 0, s, A, ctx->d_probe, ctx->d_body, ctx->d_rxp, ctx->d_rxt, ctx->cfg.ren,
                        heat, loads);
   if (loads) {
-    face_sums(ctx, A, ctx->d_lag[MELT_L_XYZV], (int)SUM_LOADS, ctx->d_red);
-    hipLaunchKernelGGL(k_body_loads, dim3(1), dim3(1), 0, s, ctx->d_body, ctx->d_red);
+    face_sums(ctx, A, ctx->d_lag[MELT_L_XYZV], (int)SUM_LOADS, ctx->d_red, FIN_LOADS);
   }
   if (heat) {
     hipLaunchKernelGGL(k_vert_heat_stefan, dim3(nb(ctx->nv, 256)), dim3(256), 0, s, A, ctx->d_body, ctx->cfg.cpliquid / ctx->cfg.latheat, ctx->dt,
                        ctx->ga, ctx->ro, 1);
-    face_sums(ctx, A, ctx->d_lag[MELT_L_XYZV], (int)SUM_VOLUME, ctx->d_red);
-    hipLaunchKernelGGL(k_body_volume, dim3(1), dim3(1), 0, s, ctx->d_body, ctx->d_red);
+    face_sums(ctx, A, ctx->d_lag[MELT_L_XYZV], (int)SUM_VOLUME, ctx->d_red, FIN_VOLUME);
   }
   if (loads) {
     const double* x = ctx->d_lag[MELT_L_XYZV];
     hipLaunchKernelGGL(k_face_centroid_normal, dim3(nb(ctx->nf, 256)), dim3(256), 0, s, A, x, 1, 1, 1);
-    face_sums(ctx, A, x, (int)SUM_RIGID, ctx->d_red);
-    hipLaunchKernelGGL(k_body_rigid, dim3(1), dim3(1), 0, s, ctx->d_body, ctx->d_red, ctx->cfg.dens_ratio);
+    face_sums(ctx, A, x, (int)SUM_RIGID, ctx->d_red, FIN_RIGID);
     hipLaunchKernelGGL(k_rel, dim3(nb((size_t)ctx->nv + ctx->nf, 256)), dim3(256), 0, s, A, ctx->d_body);
     hipLaunchKernelGGL(k_newton, dim3(1), dim3(1), 0, s, ctx->d_body, ctx->ga, ctx->ro, ctx->al, ctx->dt, ctx->cfg.dens_ratio, ctx->cfg.xlen,
                        ctx->cfg.ylen, ctx->cfg.zlen, ctx->d_red + 32, ctx->d_err);
