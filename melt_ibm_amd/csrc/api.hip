@@ -124,7 +124,7 @@ extern "C" void melt_ctx_destroy(melt_ctx* ctx) {
   for (int i = 0; i < 4; ++i) if (ctx->tmp4[i]) hipFree(ctx->tmp4[i]);
   void* ps[] = {ctx->spec, ctx->ak1, ctx->ak2, ctx->ak3, ctx->tw1, ctx->tw2, ctx->tw3, ctx->d_body, ctx->d_vof, ctx->d_eof, ctx->d_voe,
                 ctx->d_v2f_ptr, ctx->d_v2f, ctx->d_gv, ctx->d_ge, ctx->d_gf, ctx->d_pind, ctx->d_probe, ctx->d_rxp, ctx->d_rxt, ctx->d_red,
-                ctx->d_vbox, ctx->d_acc, ctx->d_fmean, ctx->d_tkey, ctx->d_tface, ctx->d_win, ctx->d_pcount, ctx->d_plist, ctx->d_err, ctx->d_gate, ctx->d_refine_stats};
+                ctx->d_vbox, ctx->d_coord, ctx->d_acc, ctx->d_fmean, ctx->d_tkey, ctx->d_tface, ctx->d_win, ctx->d_pcount, ctx->d_plist, ctx->d_err, ctx->d_gate, ctx->d_refine_stats};
   for (void* p : ps) if (p) hipFree(p);
   for (int i = 0; i < MELT_L_NLAG; ++i) if (ctx->d_lag[i]) hipFree(ctx->d_lag[i]);
   if (ctx->h_err) hipHostFree(ctx->h_err);
@@ -157,7 +157,7 @@ extern "C" int melt_ctx_create(const melt_config* cfg, melt_ctx** out) {
   do {
     if (hipGetDevice(&ctx->device) != hipSuccess) { rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "no HIP device"); break; }
     if (const char* tv = getenv("MELT_TUNE")) {
-      static const char* keys[TUNE_N] = {"xf_w", "xf_nt", "solve_w", "solve_nt"};
+      static const char* keys[TUNE_N] = {"xf_w", "xf_nt", "solve_w", "solve_nt", "xf_z"};
       std::string spec(tv);
       size_t pos = 0;
       while (pos < spec.size()) {
@@ -195,6 +195,21 @@ extern "C" int melt_ctx_create(const melt_config* cfg, melt_ctx** out) {
         hipMemset(ctx->d_refine_stats, 0, sizeof(long long) * 8) != hipSuccess) {
       rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "scratch allocation failed");
       break;
+    }
+    {
+      const Grid g = make_grid(ctx);
+      std::vector<double> tab((size_t)coord_count(g));
+      const int ns[3] = {g.n1, g.n2, g.n3};
+      for (int a = 0; a < 3; ++a)
+        for (int i = 1; i <= ns[a] + 1; ++i) {
+          tab[coord_off(g, a, 0) + i - 1] = a == 0 ? gxc(g, i) : a == 1 ? gyc(g, i) : gzc(g, i);
+          if (i <= ns[a]) tab[coord_off(g, a, 1) + i - 1] = a == 0 ? gxm(g, i) : a == 1 ? gym(g, i) : gzm(g, i);
+        }
+      if (hipMalloc(&ctx->d_coord, sizeof(double) * tab.size()) != hipSuccess ||
+          hipMemcpy(ctx->d_coord, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "coordinate table");
+        break;
+      }
     }
     rc = poisson_init(ctx);
     if (rc) break;

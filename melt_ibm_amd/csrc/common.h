@@ -52,7 +52,7 @@ enum {
 
 // Launch-shape overrides (0 = built-in choice), set from MELT_TUNE at
 // context creation ("xf_w=4,solve_w=16,...") for tile sweeps.
-enum melt_tune { TUNE_XF_W = 0, TUNE_XF_NT, TUNE_SOLVE_W, TUNE_SOLVE_NT, TUNE_N };
+enum melt_tune { TUNE_XF_W = 0, TUNE_XF_NT, TUNE_SOLVE_W, TUNE_SOLVE_NT, TUNE_XF_Z, TUNE_N };
 
 struct melt_ctx {
   melt_config cfg;
@@ -104,6 +104,7 @@ struct melt_ctx {
   double *d_rxp = nullptr, *d_rxt = nullptr;  // r_x_prn_tri, r_x_tau_n (3*nf)
   double* d_red = nullptr;       // scratch for deterministic reductions
   int* d_vbox = nullptr;         // vertex bounding box (6 ints) for ray tagging
+  double* d_coord = nullptr;     // xc, xm, yc, ym, zc, zm tables (coord_off)
 
   // forcing accumulators: 128-bit fixed point (hi:int64, lo:uint64) per
   // component over the MLS window (E1*E2*E3 cells)
@@ -166,6 +167,19 @@ __host__ __device__ inline double gzc(const Grid& g, int k) { double x = (double
 __host__ __device__ inline double gxm(const Grid& g, int i) { return (gxc(g, i) + gxc(g, i + 1)) * 0.5; }
 __host__ __device__ inline double gym(const Grid& g, int j) { return (gyc(g, j) + gyc(g, j + 1)) * 0.5; }
 __host__ __device__ inline double gzm(const Grid& g, int k) { return (gzc(g, k) + gzc(g, k + 1)) * 0.5; }
+
+// The same coordinates as tables (cordin.f90 keeps them as arrays too), for
+// kernels that would otherwise divide per cell: xc[0..n1], xm[0..n1-1],
+// yc[0..n2], ym, zc[0..n3], zm, 0-based (xc[i] = gxc(g, i + 1)), laid out one
+// after the other from coord_off(g, axis, centre).
+__host__ __device__ inline int coord_off(const Grid& g, int axis, int centre) {
+  int o = 0;
+  if (axis > 0) o += 2 * g.n1 + 1;
+  if (axis > 1) o += 2 * g.n2 + 1;
+  const int n = axis == 0 ? g.n1 : axis == 1 ? g.n2 : g.n3;
+  return o + (centre ? n + 1 : 0);
+}
+__host__ __device__ inline int coord_count(const Grid& g) { return 2 * (g.n1 + g.n2 + g.n3) + 3; }
 
 // Non-negative remainder (Fortran MODULO).
 __host__ __device__ inline int pmod(int a, int n) { int r = a % n; return r < 0 ? r + n : r; }

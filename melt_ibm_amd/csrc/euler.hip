@@ -20,7 +20,44 @@ struct Cell {
   __device__ __forceinline__ int at(int a, int b, int d) const {
     return c + (a < 0 ? im : a > 0 ? ip : 0) + (b < 0 ? jm : b > 0 ? jp : 0) + (d < 0 ? km : d > 0 ? kp : 0);
   }
+  template <typename T>
+  __device__ __forceinline__ T ld(const T* p, int a, int b, int d) const { return p[at(a, b, d)]; }
+  template <typename T>
+  __device__ __forceinline__ T& ref(T* p) const { return p[c]; }
 };
+
+// A cell addressed by 32-bit byte offsets from the field base, for fields
+// under 4 GiB: the field pointers stay in scalar registers and every
+// neighbour load is that base plus one vector offset (the 13 distinct
+// neighbour offsets are formed once and shared by all fields).  The wrapped
+// steps are kept modulo 2^32.
+template <typename T>
+struct FCell {
+  int i, j, k;
+  unsigned o, xm, xp, ym, yp, zm, zp;
+  __device__ __forceinline__ unsigned off(int a, int b, int d) const {
+    return o + (a < 0 ? xm : a > 0 ? xp : 0u) + (b < 0 ? ym : b > 0 ? yp : 0u) + (d < 0 ? zm : d > 0 ? zp : 0u);
+  }
+  __device__ __forceinline__ T ld(const T* p, int a, int b, int d) const {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + off(a, b, d));
+  }
+  __device__ __forceinline__ T& ref(T* p) const { return *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + o); }
+};
+
+template <typename T>
+__device__ __forceinline__ FCell<T> fcell_at(const Grid& g, int i, int j, int k) {
+  FCell<T> s;
+  const unsigned E = sizeof(T), n1 = (unsigned)g.n1, n12 = n1 * (unsigned)g.n2;
+  s.i = i; s.j = j; s.k = k;
+  s.o = ((unsigned)i + n1 * (unsigned)j + n12 * (unsigned)k) * E;
+  s.xm = i == 0 ? (n1 - 1) * E : 0u - E;
+  s.xp = i == g.n1 - 1 ? 0u - (n1 - 1) * E : E;
+  s.ym = j == 0 ? (n12 - n1) * E : 0u - n1 * E;
+  s.yp = j == g.n2 - 1 ? 0u - (n12 - n1) * E : n1 * E;
+  s.zm = k == 0 ? (unsigned)(g.n3 - 1) * n12 * E : 0u - n12 * E;
+  s.zp = k == g.n3 - 1 ? 0u - (unsigned)(g.n3 - 1) * n12 * E : n12 * E;
+  return s;
+}
 
 __device__ __forceinline__ Cell cell_at(const Grid& g, int i, int j, int k) {
   Cell s;
@@ -56,134 +93,163 @@ struct EulerArgs {
   const R* __restrict__ forcy;
   const R* __restrict__ forcz;
   const melt_body* __restrict__ body;
+  const double* __restrict__ coord;  // coordinate tables (coord_off)
   Grid g;
+  double rlen[3];                    // 1/xlen, 1/ylen, 1/zlen
   double betagz, tliq;
 };
 
+// The stencil formulas below read field F at offset (a, b, d) from the cell as
+// SV(F, a, b, d); a stencil type supplies the values (straight from the
+// fields, or from registers and LDS in the z-marching pass).
+enum : int { FU, FV, FW, FT, FP, FVOF, FFX, FFY, FFZ, FN };
+#define SV(F, a, b, d) s.template v<F, a, b, d>()
+
+// Reads straight from the fields through a cell addressing C (Cell: element
+// indices, any size; FCell: 32-bit byte offsets).
+template <typename R, typename C>
+struct PtrStencil {
+  const R* p[FN];
+  C c;
+  int i, j, k;
+  template <int F, int a, int b, int d>
+  __device__ __forceinline__ R v() const { return c.ld(p[F], a, b, d); }
+};
+
+template <typename R, typename C>
+__device__ __forceinline__ PtrStencil<R, C> ptr_stencil(const EulerArgs<R>& A, const C& c) {
+  return PtrStencil<R, C>{{A.vx, A.vy, A.vz, A.temp, A.pr, A.vofp, A.forcx, A.forcy, A.forcz}, c, c.i, c.j, c.k};
+}
+
 // hdnl1.f90:32-74
-template <typename R>
-__device__ inline R h1_val(const EulerArgs<R>& A, const Cell& s, R vofx) {
-  const R* vx = A.vx; const R* vy = A.vy; const R* vz = A.vz;
+template <typename R, typename S>
+__device__ inline R h1_val(const EulerArgs<R>& A, const S& s, R vofx) {
   R udx1 = R(A.g.dx1) * R(0.25), udx2 = R(A.g.dx2) * R(0.25), udx3 = R(A.g.dx3) * R(0.25);
-  R h11 = ((vx[s.at(1, 0, 0)] + vx[s.at(0, 0, 0)]) * (vx[s.at(1, 0, 0)] + vx[s.at(0, 0, 0)]) -
-           (vx[s.at(-1, 0, 0)] + vx[s.at(0, 0, 0)]) * (vx[s.at(-1, 0, 0)] + vx[s.at(0, 0, 0)])) * udx1;
-  R h12 = ((vy[s.at(0, 1, 0)] + vy[s.at(-1, 1, 0)]) * (vx[s.at(0, 1, 0)] + vx[s.at(0, 0, 0)]) -
-           (vy[s.at(0, 0, 0)] + vy[s.at(-1, 0, 0)]) * (vx[s.at(0, 0, 0)] + vx[s.at(0, -1, 0)])) * udx2;
-  R h13 = ((vz[s.at(0, 0, 1)] + vz[s.at(-1, 0, 1)]) * (vx[s.at(0, 0, 1)] + vx[s.at(0, 0, 0)]) -
-           (vz[s.at(0, 0, 0)] + vz[s.at(-1, 0, 0)]) * (vx[s.at(0, 0, 0)] + vx[s.at(0, 0, -1)])) * udx3;
-  return -(h11 + h12 + h13) + vofx * A.forcx[s.at(0, 0, 0)] / R(A.g.xlen);
+  R h11 = ((SV(FU, 1, 0, 0) + SV(FU, 0, 0, 0)) * (SV(FU, 1, 0, 0) + SV(FU, 0, 0, 0)) -
+           (SV(FU, -1, 0, 0) + SV(FU, 0, 0, 0)) * (SV(FU, -1, 0, 0) + SV(FU, 0, 0, 0))) * udx1;
+  R h12 = ((SV(FV, 0, 1, 0) + SV(FV, -1, 1, 0)) * (SV(FU, 0, 1, 0) + SV(FU, 0, 0, 0)) -
+           (SV(FV, 0, 0, 0) + SV(FV, -1, 0, 0)) * (SV(FU, 0, 0, 0) + SV(FU, 0, -1, 0))) * udx2;
+  R h13 = ((SV(FW, 0, 0, 1) + SV(FW, -1, 0, 1)) * (SV(FU, 0, 0, 1) + SV(FU, 0, 0, 0)) -
+           (SV(FW, 0, 0, 0) + SV(FW, -1, 0, 0)) * (SV(FU, 0, 0, 0) + SV(FU, 0, 0, -1))) * udx3;
+  return -(h11 + h12 + h13) + vofx * SV(FFX, 0, 0, 0) / R(A.g.xlen);
 }
 
 // hdnl2.f90:33-75
-template <typename R>
-__device__ inline R h2_val(const EulerArgs<R>& A, const Cell& s, R vofy) {
-  const R* vx = A.vx; const R* vy = A.vy; const R* vz = A.vz;
+template <typename R, typename S>
+__device__ inline R h2_val(const EulerArgs<R>& A, const S& s, R vofy) {
   R udx1 = R(A.g.dx1) * R(0.25), udx2 = R(A.g.dx2) * R(0.25), udx3 = R(A.g.dx3) * R(0.25);
-  R h21 = ((vy[s.at(1, 0, 0)] + vy[s.at(0, 0, 0)]) * (vx[s.at(1, 0, 0)] + vx[s.at(1, -1, 0)]) -
-           (vy[s.at(0, 0, 0)] + vy[s.at(-1, 0, 0)]) * (vx[s.at(0, 0, 0)] + vx[s.at(0, -1, 0)])) * udx1;
-  R h22 = ((vy[s.at(0, 1, 0)] + vy[s.at(0, 0, 0)]) * (vy[s.at(0, 1, 0)] + vy[s.at(0, 0, 0)]) -
-           (vy[s.at(0, -1, 0)] + vy[s.at(0, 0, 0)]) * (vy[s.at(0, -1, 0)] + vy[s.at(0, 0, 0)])) * udx2;
-  R h23 = ((vz[s.at(0, 0, 1)] + vz[s.at(0, -1, 1)]) * (vy[s.at(0, 0, 1)] + vy[s.at(0, 0, 0)]) -
-           (vz[s.at(0, 0, 0)] + vz[s.at(0, -1, 0)]) * (vy[s.at(0, 0, 0)] + vy[s.at(0, 0, -1)])) * udx3;
-  return -(h21 + h22 + h23) + vofy * A.forcy[s.at(0, 0, 0)] / R(A.g.ylen);
+  R h21 = ((SV(FV, 1, 0, 0) + SV(FV, 0, 0, 0)) * (SV(FU, 1, 0, 0) + SV(FU, 1, -1, 0)) -
+           (SV(FV, 0, 0, 0) + SV(FV, -1, 0, 0)) * (SV(FU, 0, 0, 0) + SV(FU, 0, -1, 0))) * udx1;
+  R h22 = ((SV(FV, 0, 1, 0) + SV(FV, 0, 0, 0)) * (SV(FV, 0, 1, 0) + SV(FV, 0, 0, 0)) -
+           (SV(FV, 0, -1, 0) + SV(FV, 0, 0, 0)) * (SV(FV, 0, -1, 0) + SV(FV, 0, 0, 0))) * udx2;
+  R h23 = ((SV(FW, 0, 0, 1) + SV(FW, 0, -1, 1)) * (SV(FV, 0, 0, 1) + SV(FV, 0, 0, 0)) -
+           (SV(FW, 0, 0, 0) + SV(FW, 0, -1, 0)) * (SV(FV, 0, 0, 0) + SV(FV, 0, 0, -1))) * udx3;
+  return -(h21 + h22 + h23) + vofy * SV(FFY, 0, 0, 0) / R(A.g.ylen);
 }
 
 // hdnl3.f90:38-82
-template <typename R>
-__device__ inline R h3_val(const EulerArgs<R>& A, const Cell& s, R vofz) {
-  const R* vx = A.vx; const R* vy = A.vy; const R* vz = A.vz;
+template <typename R, typename S>
+__device__ inline R h3_val(const EulerArgs<R>& A, const S& s, R vofz) {
   R udx1 = R(A.g.dx1) * R(0.25), udx2 = R(A.g.dx2) * R(0.25), udx3 = R(A.g.dx3) * R(0.25);
-  R h31 = (((vx[s.at(1, 0, 0)] + vx[s.at(1, 0, -1)]) * (vz[s.at(1, 0, 0)] + vz[s.at(0, 0, 0)])) -
-           ((vx[s.at(0, 0, 0)] + vx[s.at(0, 0, -1)]) * (vz[s.at(0, 0, 0)] + vz[s.at(-1, 0, 0)]))) * udx1;
-  R h32 = (((vy[s.at(0, 1, 0)] + vy[s.at(0, 1, -1)]) * (vz[s.at(0, 1, 0)] + vz[s.at(0, 0, 0)])) -
-           ((vy[s.at(0, 0, 0)] + vy[s.at(0, 0, -1)]) * (vz[s.at(0, 0, 0)] + vz[s.at(0, -1, 0)]))) * udx2;
-  R h33 = (((vz[s.at(0, 0, 1)] + vz[s.at(0, 0, 0)]) * (vz[s.at(0, 0, 1)] + vz[s.at(0, 0, 0)])) -
-           ((vz[s.at(0, 0, 0)] + vz[s.at(0, 0, -1)]) * (vz[s.at(0, 0, 0)] + vz[s.at(0, 0, -1)]))) * udx3;
-  R T_interp = (A.temp[s.at(0, 0, 0)] + A.temp[s.at(0, 0, -1)]) * R(0.5);
+  R h31 = (((SV(FU, 1, 0, 0) + SV(FU, 1, 0, -1)) * (SV(FW, 1, 0, 0) + SV(FW, 0, 0, 0))) -
+           ((SV(FU, 0, 0, 0) + SV(FU, 0, 0, -1)) * (SV(FW, 0, 0, 0) + SV(FW, -1, 0, 0)))) * udx1;
+  R h32 = (((SV(FV, 0, 1, 0) + SV(FV, 0, 1, -1)) * (SV(FW, 0, 1, 0) + SV(FW, 0, 0, 0))) -
+           ((SV(FV, 0, 0, 0) + SV(FV, 0, 0, -1)) * (SV(FW, 0, 0, 0) + SV(FW, 0, -1, 0)))) * udx2;
+  R h33 = (((SV(FW, 0, 0, 1) + SV(FW, 0, 0, 0)) * (SV(FW, 0, 0, 1) + SV(FW, 0, 0, 0))) -
+           ((SV(FW, 0, 0, 0) + SV(FW, 0, 0, -1)) * (SV(FW, 0, 0, 0) + SV(FW, 0, 0, -1)))) * udx3;
+  R T_interp = (SV(FT, 0, 0, 0) + SV(FT, 0, 0, -1)) * R(0.5);
   R fbz = R(A.betagz) * (T_interp - R(A.tliq));
-  return (-(h31 + h32 + h33) + fbz) + vofz * A.forcz[s.at(0, 0, 0)] / R(A.g.zlen);
+  return (-(h31 + h32 + h33) + fbz) + vofz * SV(FFZ, 0, 0, 0) / R(A.g.zlen);
+}
+
+// r - L NINT(r/L), the minimum image of hdnlte.f90:99-101, with NINT of the
+// correctly rounded quotient as there: r * (1/L) is within 2 ulp of r/L and so
+// rounds to the same integer unless it lies within 1e-9 of a half-integer,
+// where the quotient itself is formed.
+__device__ __forceinline__ double min_image(double r, double L, double rL) {
+  const double q = r * rL, a = fabs(q);
+  const int n = fabs(a - floor(a) - 0.5) < 1e-9 ? fnint(r / L) : fnint(q);
+  return r - L * (double)n;
 }
 
 // hdnlte.f90:45-158 (Nparticle = 1)
-template <typename R>
-__device__ inline R ht_val(const EulerArgs<R>& A, const Cell& s, R vofp) {
-  const R* vx = A.vx; const R* vy = A.vy; const R* vz = A.vz; const R* t = A.temp;
+template <typename R, typename S>
+__device__ inline R ht_val(const EulerArgs<R>& A, const S& s, R vofp) {
   const Grid& g = A.g;
   R udx1 = R(g.dx1) * R(0.5), udx2 = R(g.dx2) * R(0.5), udx3 = R(g.dx3) * R(0.5);
-  R T = t[s.at(0, 0, 0)];
-  R h31 = (vx[s.at(1, 0, 0)] * (t[s.at(1, 0, 0)] + T) - vx[s.at(0, 0, 0)] * (T + t[s.at(-1, 0, 0)])) * udx1;
-  R h32 = (vy[s.at(0, 1, 0)] * (t[s.at(0, 1, 0)] + T) - vy[s.at(0, 0, 0)] * (T + t[s.at(0, -1, 0)])) * udx2;
-  R h33 = (vz[s.at(0, 0, 1)] * (t[s.at(0, 0, 1)] + T) - vz[s.at(0, 0, 0)] * (T + t[s.at(0, 0, -1)])) * udx3;
+  R T = SV(FT, 0, 0, 0);
+  R h31 = (SV(FU, 1, 0, 0) * (SV(FT, 1, 0, 0) + T) - SV(FU, 0, 0, 0) * (T + SV(FT, -1, 0, 0))) * udx1;
+  R h32 = (SV(FV, 0, 1, 0) * (SV(FT, 0, 1, 0) + T) - SV(FV, 0, 0, 0) * (T + SV(FT, 0, -1, 0))) * udx2;
+  R h33 = (SV(FW, 0, 0, 1) * (SV(FT, 0, 0, 1) + T) - SV(FW, 0, 0, 0) * (T + SV(FT, 0, 0, -1))) * udx3;
   R ht = -(h31 + h32 + h33) * vofp;
   const double* pc = A.body->pos_cm;
   const double* vc = A.body->vel_cm;
   const double* om = A.body->omega_c;
-  int ic = s.i + 1, jc = s.j + 1, kc = s.k + 1;
-  double r0 = gxc(g, ic) - pc[0], r1 = gym(g, jc) - pc[1], r2 = gzm(g, kc) - pc[2];
-  r0 = r0 - g.xlen * (double)fnint(r0 / g.xlen);
-  r1 = r1 - g.ylen * (double)fnint(r1 / g.ylen);
-  r2 = r2 - g.zlen * (double)fnint(r2 / g.zlen);
+  const double* xy = A.coord;
+  const double xm = xy[coord_off(g, 0, 1) + s.i];
+  const double yc = xy[coord_off(g, 1, 0) + s.j], ym = xy[coord_off(g, 1, 1) + s.j];
+  const double zm = xy[coord_off(g, 2, 1) + s.k];
+  // u at xc(ic): r(1) is not used; w at zc(kc): r(3) is not used and r is not wrapped
+  double r1 = min_image(ym - pc[1], g.ylen, A.rlen[1]), r2 = min_image(zm - pc[2], g.zlen, A.rlen[2]);
   double u_imh = vc[0] + om[1] * r2 - om[2] * r1;
-  h31 = R(u_imh) * (t[s.at(1, 0, 0)] - t[s.at(-1, 0, 0)]) * udx1;
-  r0 = gxm(g, ic) - pc[0]; r1 = gyc(g, jc) - pc[1]; r2 = gzm(g, kc) - pc[2];
-  r0 = r0 - g.xlen * (double)fnint(r0 / g.xlen);
-  r1 = r1 - g.ylen * (double)fnint(r1 / g.ylen);
-  r2 = r2 - g.zlen * (double)fnint(r2 / g.zlen);
+  h31 = R(u_imh) * (SV(FT, 1, 0, 0) - SV(FT, -1, 0, 0)) * udx1;
+  double r0 = min_image(xm - pc[0], g.xlen, A.rlen[0]);
   double v_jmh = vc[1] + om[2] * r0 - om[0] * r2;
-  h32 = R(v_jmh) * (t[s.at(0, 1, 0)] - t[s.at(0, -1, 0)]) * udx2;
-  r0 = gxm(g, ic) - pc[0]; r1 = gym(g, jc) - pc[1]; r2 = gzc(g, kc) - pc[2];
+  h32 = R(v_jmh) * (SV(FT, 0, 1, 0) - SV(FT, 0, -1, 0)) * udx2;
+  r0 = xm - pc[0]; r1 = ym - pc[1];
   double w_kmh = vc[2] + om[0] * r1 - om[1] * r0;
-  h33 = R(w_kmh) * (t[s.at(0, 0, 1)] - t[s.at(0, 0, -1)]) * udx3;
+  h33 = R(w_kmh) * (SV(FT, 0, 0, 1) - SV(FT, 0, 0, -1)) * udx3;
   return ht - (h31 + h32 + h33) * (R(1) - vofp);
 }
 
 template <typename R>
 struct RhsCoef {
   R ga, ro, al, dt, alre, dx1q, dx2q, dx3q, dx1, dx2, dx3, pec;
+  R alpec;   // al / pec, formed once in R
 };
 
 // invtr1.f90:40-60
-template <typename R>
-__device__ inline R rhs_u(const RhsCoef<R>& c, const R* __restrict__ q, const R* __restrict__ pr, const Cell& s, R h, R ru) {
-  R v = q[s.at(0, 0, 0)];
-  R d11 = (q[s.at(1, 0, 0)] - R(2) * v + q[s.at(-1, 0, 0)]) * c.dx1q;
-  R d22 = (q[s.at(0, 1, 0)] - R(2) * v + q[s.at(0, -1, 0)]) * c.dx2q;
-  R d33 = (q[s.at(0, 0, 1)] - R(2) * v + q[s.at(0, 0, -1)]) * c.dx3q;
+template <int FQ, typename R, typename S>
+__device__ inline R rhs_u(const RhsCoef<R>& c, const S& s, R h, R ru) {
+  R v = SV(FQ, 0, 0, 0);
+  R d11 = (SV(FQ, 1, 0, 0) - R(2) * v + SV(FQ, -1, 0, 0)) * c.dx1q;
+  R d22 = (SV(FQ, 0, 1, 0) - R(2) * v + SV(FQ, 0, -1, 0)) * c.dx2q;
+  R d33 = (SV(FQ, 0, 0, 1) - R(2) * v + SV(FQ, 0, 0, -1)) * c.dx3q;
   R dc = d11 + d22 + d33;
-  R dp = (pr[s.at(0, 0, 0)] - pr[s.at(-1, 0, 0)]) * (c.dx1 * c.al);
+  R dp = (SV(FP, 0, 0, 0) - SV(FP, -1, 0, 0)) * (c.dx1 * c.al);
   return (c.ga * h + c.ro * ru + c.alre * dc - dp) * c.dt;
 }
 // invtr2.f90:37-57
-template <typename R>
-__device__ inline R rhs_v(const RhsCoef<R>& c, const R* __restrict__ q, const R* __restrict__ pr, const Cell& s, R h, R ru) {
-  R v = q[s.at(0, 0, 0)];
-  R d11 = (q[s.at(1, 0, 0)] - R(2) * v + q[s.at(-1, 0, 0)]) * c.dx1q;
-  R d22 = (q[s.at(0, 1, 0)] - R(2) * v + q[s.at(0, -1, 0)]) * c.dx2q;
-  R d33 = (q[s.at(0, 0, 1)] - R(2) * v + q[s.at(0, 0, -1)]) * c.dx3q;
+template <int FQ, typename R, typename S>
+__device__ inline R rhs_v(const RhsCoef<R>& c, const S& s, R h, R ru) {
+  R v = SV(FQ, 0, 0, 0);
+  R d11 = (SV(FQ, 1, 0, 0) - R(2) * v + SV(FQ, -1, 0, 0)) * c.dx1q;
+  R d22 = (SV(FQ, 0, 1, 0) - R(2) * v + SV(FQ, 0, -1, 0)) * c.dx2q;
+  R d33 = (SV(FQ, 0, 0, 1) - R(2) * v + SV(FQ, 0, 0, -1)) * c.dx3q;
   R dc = d11 + d22 + d33;
-  R dp = (pr[s.at(0, 0, 0)] - pr[s.at(0, -1, 0)]) * (c.dx2 * c.al);
+  R dp = (SV(FP, 0, 0, 0) - SV(FP, 0, -1, 0)) * (c.dx2 * c.al);
   return (c.ga * h + c.ro * ru + c.alre * dc - dp) * c.dt;
 }
 // invtr3.f90:35-56
-template <typename R>
-__device__ inline R rhs_w(const RhsCoef<R>& c, const R* __restrict__ q, const R* __restrict__ pr, const Cell& s, R h, R ru) {
-  R v = q[s.at(0, 0, 0)];
-  R d1 = (q[s.at(-1, 0, 0)] - R(2) * v + q[s.at(1, 0, 0)]) * c.dx1q;
-  R d2 = (q[s.at(0, -1, 0)] - R(2) * v + q[s.at(0, 1, 0)]) * c.dx2q;
-  R d3 = (q[s.at(0, 0, 1)] - R(2) * v + q[s.at(0, 0, -1)]) * c.dx3q;
+template <int FQ, typename R, typename S>
+__device__ inline R rhs_w(const RhsCoef<R>& c, const S& s, R h, R ru) {
+  R v = SV(FQ, 0, 0, 0);
+  R d1 = (SV(FQ, -1, 0, 0) - R(2) * v + SV(FQ, 1, 0, 0)) * c.dx1q;
+  R d2 = (SV(FQ, 0, -1, 0) - R(2) * v + SV(FQ, 0, 1, 0)) * c.dx2q;
+  R d3 = (SV(FQ, 0, 0, 1) - R(2) * v + SV(FQ, 0, 0, -1)) * c.dx3q;
   R dc = d2 + d3 + d1;
-  R dp = (pr[s.at(0, 0, 0)] - pr[s.at(0, 0, -1)]) * (c.al * c.dx3);
+  R dp = (SV(FP, 0, 0, 0) - SV(FP, 0, 0, -1)) * (c.al * c.dx3);
   return (c.ga * h + c.ro * ru + c.alre * dc - dp) * c.dt;
 }
 // invtrte.f90:32-53
-template <typename R>
-__device__ inline R rhs_t(const RhsCoef<R>& c, const R* __restrict__ q, const Cell& s, R h, R ru) {
-  R v = q[s.at(0, 0, 0)];
-  R d1 = (q[s.at(-1, 0, 0)] - R(2) * v + q[s.at(1, 0, 0)]) * c.dx1q;
-  R d2 = (q[s.at(0, -1, 0)] - R(2) * v + q[s.at(0, 1, 0)]) * c.dx2q;
-  R d3 = (q[s.at(0, 0, 1)] - R(2) * v + q[s.at(0, 0, -1)]) * c.dx3q;
+template <int FQ, typename R, typename S>
+__device__ inline R rhs_t(const RhsCoef<R>& c, const S& s, R h, R ru) {
+  R v = SV(FQ, 0, 0, 0);
+  R d1 = (SV(FQ, -1, 0, 0) - R(2) * v + SV(FQ, 1, 0, 0)) * c.dx1q;
+  R d2 = (SV(FQ, 0, -1, 0) - R(2) * v + SV(FQ, 0, 1, 0)) * c.dx2q;
+  R d3 = (SV(FQ, 0, 0, 1) - R(2) * v + SV(FQ, 0, 0, -1)) * c.dx3q;
   R dc = d2 + d3 + d1;
-  return c.dt * (c.ga * h + c.ro * ru) + (c.al / c.pec) * dc * c.dt;
+  return c.dt * (c.ga * h + c.ro * ru) + c.alpec * dc * c.dt;
 }
 
 template <typename R>
@@ -193,7 +259,9 @@ EulerArgs<R> make_args(melt_ctx* ctx) {
   A.temp = fld<R>(ctx, MELT_TEMP); A.pr = fld<R>(ctx, MELT_PR); A.vofp = fld<R>(ctx, MELT_VOFP);
   A.forcx = fld<R>(ctx, MELT_FORCX); A.forcy = fld<R>(ctx, MELT_FORCY); A.forcz = fld<R>(ctx, MELT_FORCZ);
   A.body = ctx->d_body;
+  A.coord = ctx->d_coord;
   A.g = make_grid(ctx);
+  A.rlen[0] = 1.0 / A.g.xlen; A.rlen[1] = 1.0 / A.g.ylen; A.rlen[2] = 1.0 / A.g.zlen;
   A.betagz = ctx->cfg.betagz; A.tliq = ctx->cfg.tliq;
   return A;
 }
@@ -206,6 +274,7 @@ RhsCoef<R> make_coef(melt_ctx* ctx) {
   c.dx1q = R(ctx->dx1q); c.dx2q = R(ctx->dx2q); c.dx3q = R(ctx->dx3q);
   c.dx1 = R(ctx->dx1); c.dx2 = R(ctx->dx2); c.dx3 = R(ctx->dx3);
   c.pec = R(ctx->cfg.ren * ctx->cfg.prandtl);
+  c.alpec = c.al / c.pec;
   return c;
 }
 
@@ -215,13 +284,12 @@ __global__ __launch_bounds__(256) void k_hdnl(EulerArgs<R> A, R* __restrict__ ou
   unsigned b = xcd_block(blockIdx.x, gridDim.x);
   int c = (int)(b * blockDim.x + threadIdx.x);
   if (c >= nc) return;
-  Cell s = make_cell(A.g, c);
-  const R* vp = A.vofp;
-  R p = vp[s.at(0, 0, 0)];
+  const auto s = ptr_stencil(A, make_cell(A.g, c));
+  R p = SV(FVOF, 0, 0, 0);
   R r;
-  if (which == 0) r = h1_val(A, s, R(0.5) * (vp[s.at(-1, 0, 0)] + p));
-  else if (which == 1) r = h2_val(A, s, R(0.5) * (vp[s.at(0, -1, 0)] + p));
-  else if (which == 2) r = h3_val(A, s, R(0.5) * (vp[s.at(0, 0, -1)] + p));
+  if (which == 0) r = h1_val(A, s, R(0.5) * (SV(FVOF, -1, 0, 0) + p));
+  else if (which == 1) r = h2_val(A, s, R(0.5) * (SV(FVOF, 0, -1, 0) + p));
+  else if (which == 2) r = h3_val(A, s, R(0.5) * (SV(FVOF, 0, 0, -1) + p));
   else r = ht_val(A, s, p);
   out[c] = r;
 }
@@ -232,12 +300,14 @@ __global__ __launch_bounds__(256) void k_invtr_rhs(Grid g, RhsCoef<R> cf, const 
   unsigned b = xcd_block(blockIdx.x, gridDim.x);
   int c = (int)(b * blockDim.x + threadIdx.x);
   if (c >= nc) return;
-  Cell s = make_cell(g, c);
+  const Cell cl = make_cell(g, c);
+  // the transported field q in every slot but the pressure's
+  const PtrStencil<R, Cell> s{{q, q, q, q, pr, q, q, q, q}, cl, cl.i, cl.j, cl.k};
   R hv = h[c], rv = ru[c], r;
-  if (which == 0) r = rhs_u(cf, q, pr, s, hv, rv);
-  else if (which == 1) r = rhs_v(cf, q, pr, s, hv, rv);
-  else if (which == 2) r = rhs_w(cf, q, pr, s, hv, rv);
-  else r = rhs_t(cf, q, s, hv, rv);
+  if (which == 0) r = rhs_u<FU>(cf, s, hv, rv);
+  else if (which == 1) r = rhs_v<FV>(cf, s, hv, rv);
+  else if (which == 2) r = rhs_w<FW>(cf, s, hv, rv);
+  else r = rhs_t<FT>(cf, s, hv, rv);
   rhs[c] = r;
   ru[c] = hv;
 }
@@ -406,14 +476,11 @@ struct Green {
   double g[FIR_KMAX + 1];
 };
 
-// x of element r of an LDS line (elements L[r], r < M, periodic).
-__device__ __forceinline__ double green_lds(const Green& G, const double* L, int r, int M) {
+// x of element r of an LDS line L[0..M-1] padded with its periodic images
+// L[-FIR_KMAX..-1] and L[M..M+FIR_KMAX-1], so no index wraps.
+__device__ __forceinline__ double green_lds(const Green& G, const double* L, int r) {
   double x = G.g[0] * L[r];
-  for (int m = 1; m <= G.K; ++m) {
-    int rm = r - m; if (rm < 0) rm += M;
-    int rp = r + m; if (rp >= M) rp -= M;
-    x += G.g[m] * (L[rm] + L[rp]);
-  }
+  for (int m = 1; m <= G.K; ++m) x += G.g[m] * (L[r - m] + L[r + m]);
   return x;
 }
 
@@ -423,51 +490,64 @@ __device__ __forceinline__ double green_lds(const Green& G, const double* L, int
 // state through L1/L2), keeps the scaled rhs in LDS, solves the four
 // x-systems there and writes the x-solved rhs.  Saves one read and one write
 // of rhs x4 per substep against separate passes.
-template <int E, typename R, bool FIR>
+template <int E, typename R, bool FIR, bool FLAT>
 __global__ __launch_bounds__(256) void k_explicit_xsolve(EulerArgs<R> A, RhsCoef<R> cf, R* __restrict__ ru1, R* __restrict__ ru2,
                                                          R* __restrict__ ru3, R* __restrict__ rut, R* __restrict__ r_u, R* __restrict__ r_v,
-                                                         R* __restrict__ r_w, R* __restrict__ r_t, double bxu, double bxt, int W, Green gu,
-                                                         Green gt) {
+                                                         R* __restrict__ r_w, R* __restrict__ r_t, double bxu, double bxt, double sc_u,
+                                                         double sc_t, int W, Green gu, Green gt) {
   extern __shared__ double lds[];
   const Grid& g = A.g;
-  const int M = g.n1, ldx = M + 1, tid = threadIdx.x, nt = blockDim.x;
+  // LDS line stride: FIR lines carry FIR_KMAX periodic images on each side
+  constexpr int H = FIR ? FIR_KMAX : 0;
+  const int M = g.n1, ldx = FIR ? M + 2 * H : M + 1, tid = threadIdx.x, nt = blockDim.x;
   unsigned t = xcd_block(blockIdx.x, gridDim.x);
   int jt, k;
   tiled_2d(t, g.n2 / W, g.n3, 16, 8, jt, k);
   const int j0 = jt * W;
   const int base = M * (j0 + g.n2 * k);
   const int plane = W * ldx;
-  const double sc_u = 1.0 / (1.0 + 2.0 * bxu), sc_t = 1.0 / (1.0 + 2.0 * bxt);
+  const bool one_line = W == 1;   // then E * nt == M: idx is the x index
 #pragma unroll 1
   for (int q = 0; q < E; ++q) {
     int idx = tid + q * nt;
-    int r = idx % M, c = idx / M;
-    Cell s = cell_at(g, r, j0 + c, k);
-    const int gc = s.c;
-    const R* vp = A.vofp;
-    R p = vp[gc];
-    R hu = h1_val(A, s, R(0.5) * (vp[s.at(-1, 0, 0)] + p));
-    R hv = h2_val(A, s, R(0.5) * (vp[s.at(0, -1, 0)] + p));
-    R hw = h3_val(A, s, R(0.5) * (vp[s.at(0, 0, -1)] + p));
-    R ht = ht_val(A, s, p);
-    R o1 = ru1[gc], o2 = ru2[gc], o3 = ru3[gc], ot = rut[gc];
-    const int l = c * ldx + r;
-    lds[l] = (double)rhs_u(cf, A.vx, A.pr, s, hu, o1) * sc_u;
-    lds[plane + l] = (double)rhs_v(cf, A.vy, A.pr, s, hv, o2) * sc_u;
-    lds[2 * plane + l] = (double)rhs_w(cf, A.vz, A.pr, s, hw, o3) * sc_u;
-    lds[3 * plane + l] = (double)rhs_t(cf, A.temp, s, ht, ot) * sc_t;
-    ru1[gc] = hu; ru2[gc] = hv; ru3[gc] = hw; rut[gc] = ht;
+    int r = one_line ? idx : idx % M, c = one_line ? 0 : idx / M;
+    const int l = c * ldx + H + r;
+    auto cell = [&](const auto& cl) {
+      const auto s = ptr_stencil(A, cl);
+      R p = SV(FVOF, 0, 0, 0);
+      R hu = h1_val(A, s, R(0.5) * (SV(FVOF, -1, 0, 0) + p));
+      R hv = h2_val(A, s, R(0.5) * (SV(FVOF, 0, -1, 0) + p));
+      R hw = h3_val(A, s, R(0.5) * (SV(FVOF, 0, 0, -1) + p));
+      R ht = ht_val(A, s, p);
+      R o1 = cl.ref(ru1), o2 = cl.ref(ru2), o3 = cl.ref(ru3), ot = cl.ref(rut);
+      const double d[4] = {(double)rhs_u<FU>(cf, s, hu, o1) * sc_u, (double)rhs_v<FV>(cf, s, hv, o2) * sc_u,
+                           (double)rhs_w<FW>(cf, s, hw, o3) * sc_u, (double)rhs_t<FT>(cf, s, ht, ot) * sc_t};
+#pragma unroll
+      for (int f = 0; f < 4; ++f) {
+        lds[f * plane + l] = d[f];
+        if constexpr (FIR) {
+          if (r < H) lds[f * plane + l + M] = d[f];
+          if (r >= M - H) lds[f * plane + l - M] = d[f];
+        }
+      }
+      cl.ref(ru1) = hu; cl.ref(ru2) = hv; cl.ref(ru3) = hw; cl.ref(rut) = ht;
+    };
+    if constexpr (FLAT) cell(fcell_at<R>(g, r, j0 + c, k));
+    else cell(cell_at(g, r, j0 + c, k));
   }
   __syncthreads();
   R* outs[4] = {r_u, r_v, r_w, r_t};
   if constexpr (FIR) {
+    const size_t p0 = (size_t)k * (size_t)M * g.n2 * sizeof(R);
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       int idx = tid + q * nt;
-      int r = idx % M, c = idx / M;
-      const double* L = lds + c * ldx;
+      int r = one_line ? idx : idx % M, c = one_line ? 0 : idx / M;
+      const double* L = lds + c * ldx + H;
+      const unsigned off = (unsigned)(r + M * (j0 + c)) * (unsigned)sizeof(R);
 #pragma unroll
-      for (int f = 0; f < 4; ++f) outs[f][base + idx] = (R)green_lds(f == 3 ? gt : gu, L + f * plane, r, M);
+      for (int f = 0; f < 4; ++f)
+        *reinterpret_cast<R*>(reinterpret_cast<char*>(outs[f]) + p0 + off) = (R)green_lds(f == 3 ? gt : gu, L + f * plane, r);
     }
     return;
   }
@@ -486,6 +566,161 @@ __global__ __launch_bounds__(256) void k_explicit_xsolve(EulerArgs<R> A, RhsCoef
       int idx = tid + q * nt;
       outs[f][base + idx] = (R)d[q];
     }
+  }
+}
+
+template <int...>
+constexpr bool no_stencil_value = false;
+
+// Stencil values of one cell of the z-marching pass: the centre column at
+// k-1, k, k+1 rolls in registers, the rows j-1 and j+1 are loaded per plane,
+// and the x-neighbours are read from the plane's rows published in LDS.
+template <typename R>
+struct ZStencil {
+  R cu[3], cv[3], cw[3], ct[3], cp[2], co[2];           // (0, 0, d) at [d + 1]; p, VOFp: k-1, k
+  R ujm, ujp, vjm, vjp, vjp_km, wjm, wjp, wjm_kp, tjm, tjp, pjm, ojm, fx, fy, fz;
+  R u_im, u_ip, ujm_ip, ukm_ip, v_im, v_ip, vjp_im, w_im, w_ip, wkp_im, t_im, t_ip, p_im, o_im;
+  int i, j, k;
+  template <int F, int a, int b, int d>
+  __device__ __forceinline__ R v() const {
+    constexpr int key = (a + 1) * 9 + (b + 1) * 3 + (d + 1);
+    constexpr int C = 13, XM = 4, XP = 22, YM = 10, YP = 16;   // keys of (0,0,0), (-1,0,0), (1,0,0), (0,-1,0), (0,1,0)
+    if constexpr ((F == FU || F == FV || F == FW || F == FT) && a == 0 && b == 0) {
+      return F == FU ? cu[d + 1] : F == FV ? cv[d + 1] : F == FW ? cw[d + 1] : ct[d + 1];
+    } else if constexpr (F == FU) {
+      if constexpr (key == XP) return u_ip;
+      else if constexpr (key == XM) return u_im;
+      else if constexpr (key == YM) return ujm;
+      else if constexpr (key == YP) return ujp;
+      else if constexpr (a == 1 && b == -1 && d == 0) return ujm_ip;
+      else if constexpr (a == 1 && b == 0 && d == -1) return ukm_ip;
+      else static_assert(no_stencil_value<F, a, b, d>);
+    } else if constexpr (F == FV) {
+      if constexpr (key == XP) return v_ip;
+      else if constexpr (key == XM) return v_im;
+      else if constexpr (key == YM) return vjm;
+      else if constexpr (key == YP) return vjp;
+      else if constexpr (a == -1 && b == 1 && d == 0) return vjp_im;
+      else if constexpr (a == 0 && b == 1 && d == -1) return vjp_km;
+      else static_assert(no_stencil_value<F, a, b, d>);
+    } else if constexpr (F == FW) {
+      if constexpr (key == XP) return w_ip;
+      else if constexpr (key == XM) return w_im;
+      else if constexpr (key == YM) return wjm;
+      else if constexpr (key == YP) return wjp;
+      else if constexpr (a == -1 && b == 0 && d == 1) return wkp_im;
+      else if constexpr (a == 0 && b == -1 && d == 1) return wjm_kp;
+      else static_assert(no_stencil_value<F, a, b, d>);
+    } else if constexpr (F == FT) {
+      if constexpr (key == XP) return t_ip;
+      else if constexpr (key == XM) return t_im;
+      else if constexpr (key == YM) return tjm;
+      else if constexpr (key == YP) return tjp;
+      else static_assert(no_stencil_value<F, a, b, d>);
+    } else if constexpr (F == FP || F == FVOF) {
+      if constexpr (key == C) return F == FP ? cp[1] : co[1];
+      else if constexpr (a == 0 && b == 0 && d == -1) return F == FP ? cp[0] : co[0];
+      else if constexpr (key == XM) return F == FP ? p_im : o_im;
+      else if constexpr (key == YM) return F == FP ? pjm : ojm;
+      else static_assert(no_stencil_value<F, a, b, d>);
+    } else if constexpr (key == C) {
+      return F == FFX ? fx : F == FFY ? fy : fz;
+    } else {
+      static_assert(no_stencil_value<F, a, b, d>);
+    }
+  }
+};
+
+// Pass A as a z-march: a block owns one x-line (one thread per i) of row j and
+// walks D consecutive planes.  Per plane a cell loads the centre column at k+1
+// (u, v, w, T) and at k (p, VOFp), the rows j-1 / j+1 it needs (u, v, w, T at
+// k; w at j-1, k+1; p and VOFp at j-1), its forcing and ru: 23 loads instead
+// of the 57 of a free-standing stencil, the rest rolling in registers or
+// coming from LDS rows (two sets, alternating per plane, so one barrier
+// separates writing a set from reading it).  Each plane then runs the x-line
+// filter like k_explicit_xsolve.  Every value and expression is the same as
+// there, so the results are identical.  Off: 32-bit (fields < 4 GiB) or
+// 64-bit byte offsets.
+template <typename R, typename Off>
+__global__ __launch_bounds__(512) void k_explicit_zmarch(EulerArgs<R> A, RhsCoef<R> cf, R* __restrict__ ru1, R* __restrict__ ru2,
+                                                         R* __restrict__ ru3, R* __restrict__ rut, R* __restrict__ r_u, R* __restrict__ r_v,
+                                                         R* __restrict__ r_w, R* __restrict__ r_t, double sc_u, double sc_t, int D, Green gu,
+                                                         Green gt) {
+  extern __shared__ double lds[];
+  constexpr int H = FIR_KMAX;
+  enum { XU, XUJM, XV, XVJP, XWP, XT, XP, XO, NX };   // exchange rows of one set
+  const Grid& g = A.g;
+  const int M = g.n1, i = threadIdx.x, ldx = M + 2 * H;
+  R* xr = reinterpret_cast<R*>(lds + 4 * ldx);
+  auto row = [&](int set, int r) { return xr + (set * NX + r) * M; };
+  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  const int j = (int)(b % (unsigned)g.n2), k0 = (int)(b / (unsigned)g.n2) * D;
+  const int jm = j == 0 ? g.n2 - 1 : j - 1, jp = j == g.n2 - 1 ? 0 : j + 1;
+  const int im = i == 0 ? M - 1 : i - 1, ip = i == M - 1 ? 0 : i + 1;
+  auto off = [&](int jj, int kk) { return ((Off)i + (Off)M * ((Off)jj + (Off)g.n2 * (Off)kk)) * (Off)sizeof(R); };
+  auto ld = [](const R* p, Off o) { return *reinterpret_cast<const R*>(reinterpret_cast<const char*>(p) + o); };
+  auto st = [](R* p, Off o, R v) { *reinterpret_cast<R*>(reinterpret_cast<char*>(p) + o) = v; };
+  ZStencil<R> s;
+  s.i = i; s.j = j;
+  {
+    const int km = k0 == 0 ? g.n3 - 1 : k0 - 1;
+    const Off om = off(j, km), o0 = off(j, k0);
+    s.cu[0] = ld(A.vx, om); s.cu[1] = ld(A.vx, o0);
+    s.cv[0] = ld(A.vy, om); s.cv[1] = ld(A.vy, o0);
+    s.cw[0] = ld(A.vz, om); s.cw[1] = ld(A.vz, o0);
+    s.ct[0] = ld(A.temp, om); s.ct[1] = ld(A.temp, o0);
+    s.cp[0] = ld(A.pr, om); s.co[0] = ld(A.vofp, om);
+    s.vjp_km = ld(A.vy, off(jp, km));
+    s.wjm = ld(A.vz, off(jm, k0));
+    row(1, XU)[i] = s.cu[0];
+    row(1, XWP)[i] = s.cw[1];
+  }
+  R* outs[4] = {r_u, r_v, r_w, r_t};
+#pragma unroll 1
+  for (int t = 0; t < D; ++t) {
+    const int k = k0 + t, kp = k == g.n3 - 1 ? 0 : k + 1;
+    s.k = k;
+    const Off o0 = off(j, k), op = off(j, kp), ojm = off(jm, k), ojp = off(jp, k);
+    s.cu[2] = ld(A.vx, op); s.cv[2] = ld(A.vy, op); s.cw[2] = ld(A.vz, op); s.ct[2] = ld(A.temp, op);
+    s.cp[1] = ld(A.pr, o0); s.co[1] = ld(A.vofp, o0);
+    s.ujm = ld(A.vx, ojm); s.ujp = ld(A.vx, ojp);
+    s.vjm = ld(A.vy, ojm); s.vjp = ld(A.vy, ojp);
+    s.wjp = ld(A.vz, ojp); s.wjm_kp = ld(A.vz, off(jm, kp));
+    s.tjm = ld(A.temp, ojm); s.tjp = ld(A.temp, ojp);
+    s.pjm = ld(A.pr, ojm); s.ojm = ld(A.vofp, ojm);
+    s.fx = ld(A.forcx, o0); s.fy = ld(A.forcy, o0); s.fz = ld(A.forcz, o0);
+    const R o1 = ld(ru1, o0), o2 = ld(ru2, o0), o3 = ld(ru3, o0), ot = ld(rut, o0);
+    const int cur = t & 1, prv = cur ^ 1;
+    row(cur, XU)[i] = s.cu[1]; row(cur, XUJM)[i] = s.ujm; row(cur, XV)[i] = s.cv[1]; row(cur, XVJP)[i] = s.vjp;
+    row(cur, XWP)[i] = s.cw[2]; row(cur, XT)[i] = s.ct[1]; row(cur, XP)[i] = s.cp[1]; row(cur, XO)[i] = s.co[1];
+    __syncthreads();
+    s.u_im = row(cur, XU)[im]; s.u_ip = row(cur, XU)[ip]; s.ujm_ip = row(cur, XUJM)[ip]; s.ukm_ip = row(prv, XU)[ip];
+    s.v_im = row(cur, XV)[im]; s.v_ip = row(cur, XV)[ip]; s.vjp_im = row(cur, XVJP)[im];
+    s.w_im = row(prv, XWP)[im]; s.w_ip = row(prv, XWP)[ip]; s.wkp_im = row(cur, XWP)[im];
+    s.t_im = row(cur, XT)[im]; s.t_ip = row(cur, XT)[ip];
+    s.p_im = row(cur, XP)[im]; s.o_im = row(cur, XO)[im];
+    R p = SV(FVOF, 0, 0, 0);
+    R hu = h1_val(A, s, R(0.5) * (SV(FVOF, -1, 0, 0) + p));
+    R hv = h2_val(A, s, R(0.5) * (SV(FVOF, 0, -1, 0) + p));
+    R hw = h3_val(A, s, R(0.5) * (SV(FVOF, 0, 0, -1) + p));
+    R ht = ht_val(A, s, p);
+    const double d[4] = {(double)rhs_u<FU>(cf, s, hu, o1) * sc_u, (double)rhs_v<FV>(cf, s, hv, o2) * sc_u,
+                         (double)rhs_w<FW>(cf, s, hw, o3) * sc_u, (double)rhs_t<FT>(cf, s, ht, ot) * sc_t};
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      double* L = lds + f * ldx + H;
+      L[i] = d[f];
+      if (i < H) L[i + M] = d[f];
+      if (i >= M - H) L[i - M] = d[f];
+    }
+    st(ru1, o0, hu); st(ru2, o0, hv); st(ru3, o0, hw); st(rut, o0, ht);
+    __syncthreads();
+#pragma unroll
+    for (int f = 0; f < 4; ++f) st(outs[f], o0, (R)green_lds(f == 3 ? gt : gu, lds + f * ldx + H, i));
+    s.cu[0] = s.cu[1]; s.cu[1] = s.cu[2]; s.cv[0] = s.cv[1]; s.cv[1] = s.cv[2];
+    s.cw[0] = s.cw[1]; s.cw[1] = s.cw[2]; s.ct[0] = s.ct[1]; s.ct[1] = s.ct[2];
+    s.cp[0] = s.cp[1]; s.co[0] = s.co[1];
+    s.vjp_km = s.vjp; s.wjm = s.wjm_kp;
   }
 }
 
@@ -721,6 +956,31 @@ template <typename R>
 int launch_explicit_xsolve(melt_ctx* ctx, double bxu, double bxt) {
   EulerArgs<R> A = make_args<R>(ctx);
   RhsCoef<R> cf = make_coef<R>(ctx);
+  const double tol = ctx->fp32 ? 1e-8 : 2e-17;
+  const double sc_u = 1.0 / (1.0 + 2.0 * bxu), sc_t = 1.0 / (1.0 + 2.0 * bxt);   // solxi.f90:33-37
+  Green gu{}, gt{};   // the LDS line holds the scaled rhs, so the filter is the unscaled one
+  const bool fir = !ctx->no_green && green_coeffs(bxu, ctx->n1, tol, 1.0, gu) && green_coeffs(bxt, ctx->n1, tol, 1.0, gt);
+  R* r4[4] = {static_cast<R*>(ctx->rhs4[RHS_U]), static_cast<R*>(ctx->rhs4[RHS_V]), static_cast<R*>(ctx->rhs4[RHS_W]),
+              static_cast<R*>(ctx->rhs4[RHS_T])};
+  // z-march (one x-line per block, D planes) unless line-tile shapes are asked for
+  const size_t zshm = sizeof(double) * 4 * (ctx->n1 + 2 * FIR_KMAX) + sizeof(R) * 16 * (size_t)ctx->n1;
+  if (fir && ctx->tune[TUNE_XF_W] == 0 && ctx->tune[TUNE_XF_NT] == 0 && ctx->n1 % 64 == 0 && ctx->n1 <= 512 && zshm <= 64 * 1024) {
+    int D = ctx->tune[TUNE_XF_Z];
+    if (D <= 0 || ctx->n3 % D) {
+      D = 1;
+      for (int d : {16, 8, 4, 2})
+        if (ctx->n3 % d == 0) { D = d; break; }
+    }
+    const unsigned blocks = (unsigned)((size_t)ctx->n2 * (ctx->n3 / D));
+    if (ctx->nc * sizeof(R) < (size_t(1) << 32))
+      hipLaunchKernelGGL((k_explicit_zmarch<R, unsigned>), dim3(blocks), dim3(ctx->n1), zshm, ctx->stream, A, cf, fld<R>(ctx, MELT_RU1),
+                         fld<R>(ctx, MELT_RU2), fld<R>(ctx, MELT_RU3), fld<R>(ctx, MELT_RUT), r4[0], r4[1], r4[2], r4[3], sc_u, sc_t, D, gu, gt);
+    else
+      hipLaunchKernelGGL((k_explicit_zmarch<R, size_t>), dim3(blocks), dim3(ctx->n1), zshm, ctx->stream, A, cf, fld<R>(ctx, MELT_RU1),
+                         fld<R>(ctx, MELT_RU2), fld<R>(ctx, MELT_RU3), fld<R>(ctx, MELT_RUT), r4[0], r4[1], r4[2], r4[3], sc_u, sc_t, D, gu, gt);
+    MELT_HIP(hipGetLastError());
+    return 0;
+  }
   // W x-lines per block: the fewest that give a 256-cell tile (one cell per
   // thread); small tiles keep the LDS and register footprint low.
   int W = 0;
@@ -733,30 +993,28 @@ int launch_explicit_xsolve(melt_ctx* ctx, double bxu, double bxt) {
   int nt = pick_threads(ncell, 256, ctx->tune[TUNE_XF_NT]);
   if (!nt) return melt_fail(ctx, MELT_ERR_ARG, "explicit_xsolve", "unsupported line length");
   int E = ncell / nt;
-  size_t shm = sizeof(double) * 4 * (size_t)W * (ctx->n1 + 1);
+  size_t shm0 = sizeof(double) * 4 * (size_t)W * (ctx->n1 + 1);
   unsigned blocks = (unsigned)((size_t)(ctx->n2 / W) * ctx->n3);
-  R* r4[4] = {static_cast<R*>(ctx->rhs4[RHS_U]), static_cast<R*>(ctx->rhs4[RHS_V]), static_cast<R*>(ctx->rhs4[RHS_W]),
-              static_cast<R*>(ctx->rhs4[RHS_T])};
-  // the LDS line holds the scaled rhs, so the filter is the unscaled one
-  Green gu{}, gt{};
-  const double tol = ctx->fp32 ? 1e-8 : 2e-17;
-  const bool fir = !ctx->no_green && green_coeffs(bxu, ctx->n1, tol, 1.0, gu) && green_coeffs(bxt, ctx->n1, tol, 1.0, gt);
-#define XF_CASE(EE)                                                                                                                   \
-  case EE:                                                                                                                            \
-    if (fir)                                                                                                                          \
-      hipLaunchKernelGGL((k_explicit_xsolve<EE, R, true>), dim3(blocks), dim3(nt), shm, ctx->stream, A, cf, fld<R>(ctx, MELT_RU1),     \
-                         fld<R>(ctx, MELT_RU2), fld<R>(ctx, MELT_RU3), fld<R>(ctx, MELT_RUT), r4[0], r4[1], r4[2], r4[3], bxu, bxt, W, \
-                         gu, gt);                                                                                                     \
-    else                                                                                                                              \
-      hipLaunchKernelGGL((k_explicit_xsolve<EE, R, false>), dim3(blocks), dim3(nt), shm, ctx->stream, A, cf, fld<R>(ctx, MELT_RU1),    \
-                         fld<R>(ctx, MELT_RU2), fld<R>(ctx, MELT_RU3), fld<R>(ctx, MELT_RUT), r4[0], r4[1], r4[2], r4[3], bxu, bxt, W, \
-                         gu, gt);                                                                                                     \
+  const bool flat = ctx->nc * sizeof(R) < (size_t(1) << 32);   // 32-bit byte offsets reach every element
+  const size_t shm = fir ? sizeof(double) * 4 * (size_t)W * (ctx->n1 + 2 * FIR_KMAX) : shm0;
+  if (shm > 64 * 1024) return melt_fail(ctx, MELT_ERR_ARG, "explicit_xsolve", "tile exceeds LDS");
+#define XF_LAUNCH(EE, FIRV, FLATV)                                                                                      \
+  hipLaunchKernelGGL((k_explicit_xsolve<EE, R, FIRV, FLATV>), dim3(blocks), dim3(nt), shm, ctx->stream, A, cf,           \
+                     fld<R>(ctx, MELT_RU1), fld<R>(ctx, MELT_RU2), fld<R>(ctx, MELT_RU3), fld<R>(ctx, MELT_RUT), r4[0], r4[1], \
+                     r4[2], r4[3], bxu, bxt, sc_u, sc_t, W, gu, gt)
+#define XF_CASE(EE)                                   \
+  case EE:                                            \
+    if (fir && flat) XF_LAUNCH(EE, true, true);       \
+    else if (fir) XF_LAUNCH(EE, true, false);         \
+    else if (flat) XF_LAUNCH(EE, false, true);        \
+    else XF_LAUNCH(EE, false, false);                 \
     break;
   switch (E) {
     XF_CASE(1) XF_CASE(2) XF_CASE(3) XF_CASE(4) XF_CASE(6) XF_CASE(8)
     default: return melt_fail(ctx, MELT_ERR_ARG, "explicit_xsolve", "unsupported tile");
   }
 #undef XF_CASE
+#undef XF_LAUNCH
   MELT_HIP(hipGetLastError());
   return 0;
 }

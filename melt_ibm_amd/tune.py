@@ -1,5 +1,5 @@
-"""Autotuner for the launch shapes of the fused explicit/x-solve pass and of the
-y/z line solves.
+"""Autotuner for the launch shapes of the fused explicit/x-solve pass (z-march
+planes per block, or line-tile shapes) and of the y/z line solves.
 
 For a grid size and precision it times every candidate shape on a fluid-only
 context (seeded solenoidal state, the kernels are the ones a melting run
@@ -22,6 +22,7 @@ _CACHE: dict[tuple[int, int], dict] = {}
 
 XF_W = (1, 2, 4, 8)
 XF_NT = (128, 256)
+XF_Z = (4, 8, 16, 32)
 SOLVE_W = (4, 8, 16, 32)
 SOLVE_NT = (128, 256, 512, 1024)
 
@@ -75,18 +76,22 @@ def autotune(n: int, precision: int = 0, lib=None) -> dict:
             ctx.upload(name, a)
         ctx.set_dt(inputs.cfl_dt(cfg, u, v, w, cfl=0.5))
         best = {}
-        for slots, grid in ((("explicit_xsolve",), [("xf_w", w_, "xf_nt", t) for w_ in XF_W if n % w_ == 0 for t in XF_NT]),
-                            (("solve_y", "solve_z_update"), [("solve_w", w_, "solve_nt", t) for w_ in SOLVE_W if n % w_ == 0 for t in SOLVE_NT])):
+        zero = {k: 0 for k in ("xf_w", "xf_nt", "solve_w", "solve_nt", "xf_z")}
+        # the explicit pass: z-march with D planes per block, or line tiles of W lines x nt threads
+        xf = [{"xf_z": d, "xf_w": 0, "xf_nt": 0} for d in XF_Z if n % d == 0] + \
+             [{"xf_z": 0, "xf_w": w_, "xf_nt": t} for w_ in XF_W if n % w_ == 0 for t in XF_NT]
+        solve = [{"solve_w": w_, "solve_nt": t} for w_ in SOLVE_W if n % w_ == 0 for t in SOLVE_NT]
+        for slots, grid in ((("explicit_xsolve",), xf), (("solve_y", "solve_z_update"), solve)):
             scores = []
-            for ka, va, kb, vb in grid:
+            for cand in grid:
                 try:
-                    ctx.set_tuning(**{ka: va, kb: vb})
-                    scores.append((_time(ctx, slots), {ka: va, kb: vb}))
+                    ctx.set_tuning(**cand)
+                    scores.append((_time(ctx, slots), cand))
                 except MeltError:
                     continue
             if scores:
-                best.update(min(scores, key=lambda s: s[0])[1])
-            ctx.set_tuning(**{k: 0 for k in ("xf_w", "xf_nt", "solve_w", "solve_nt")})
+                best.update({k: v for k, v in min(scores, key=lambda s: s[0])[1].items() if v})
+            ctx.set_tuning(**zero)
     finally:
         ctx.close()
     _CACHE[key] = best

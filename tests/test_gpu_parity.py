@@ -296,3 +296,34 @@ def test_green_line_solves_at_64(gpu_lib, monkeypatch):
         assert e < 1e-10, f"{q}: rel err vs oracle {e}"
         e = rel_err(g.download(q), pcr.download(q))
         assert e < 1e-12, f"{q}: rel err vs cyclic reduction {e}"
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("n3,tunes", [(64, ("xf_w=1", "xf_z=1", "xf_z=4")), (24, ("xf_w=1", "xf_z=3"))])
+def test_zmarch_matches_line_kernel(gpu_lib, monkeypatch, precision, n3, tunes):
+    """The explicit pass as a z-march (one x-line per block walking its planes,
+    default where n1 % 64 == 0) and as one x-line per block (xf_w=1) compute
+    the same values with the same expressions: two RK3 steps with a melting
+    body are bit-identical, whatever the number of planes per block."""
+    n, level, rad = 64, 3, 1.0
+    cfg = make_cfg(n, n3, precision=precision)
+    st, dt = melt_case(make_cfg(n, n3), level, rad, seed=17)
+    from melt_ibm_amd.inputs import inqpr_temperature
+    outs = []
+    for tune in ("",) + tunes:
+        monkeypatch.setenv("MELT_TUNE", tune)
+        c = MeltContext(cfg, lib=gpu_lib)
+        load_state(c, st)
+        with_body(c, level, rad)
+        c.upload("temp", inqpr_temperature(cfg, c.body().pos_cm[:], rad))
+        c.set_dt(dt)
+        c.tagCells()
+        for _ in range(2):
+            c.tschem()
+        c.sync()
+        outs.append([c.download(q) for q in ("vx", "vy", "vz", "temp", "pr", "VOFp")] + [c.lag("xyzv")])
+        c.close()
+    monkeypatch.delenv("MELT_TUNE")
+    for t, other in zip(tunes, outs[1:]):
+        for a, b in zip(outs[0], other):
+            assert np.array_equal(a, b), t
