@@ -117,6 +117,7 @@ struct melt_ctx {
   size_t twin_cells = 0;
   unsigned long long* d_tkey = nullptr;
   int* d_tface = nullptr;
+  double* d_tdist = nullptr;   // the 125 band distances of every face (first tagging pass)
   melt_window* d_win = nullptr;
   // ray tagging scratch (init only)
   int* d_pcount = nullptr;

@@ -384,11 +384,12 @@ struct MeshArgs {
   int nf;
 };
 
-// mode 0: min distance; 1: lowest face among minima; 2: write sdf and VOFp
+// mode 0: distance of every (face, band cell), kept in dist[], and the min
+// per cell; 1: lowest face among the minima; 2: write sdf and VOFp
 template <typename R>
 __global__ __launch_bounds__(256) void k_tag_pass(Grid g, MeshArgs M, const melt_window* __restrict__ win, Win3 et,
-                                                  unsigned long long* __restrict__ key, int* __restrict__ face, R* __restrict__ sdf,
-                                                  R* __restrict__ vofp, int mode, double pi, int* err) {
+                                                  unsigned long long* __restrict__ key, int* __restrict__ face, double* __restrict__ dst,
+                                                  R* __restrict__ sdf, R* __restrict__ vofp, int mode, double pi, int* err) {
   size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= (size_t)M.nf * 125) return;
   int f = (int)(id / 125), q = (int)(id % 125);
@@ -397,19 +398,23 @@ __global__ __launch_bounds__(256) void k_tag_pass(Grid g, MeshArgs M, const melt
   int i = (int)floor(pos[0] * g.dx1) + 1 + (q % 5) - 2;
   int j = (int)floor(pos[1] * g.dx2) + 1 + (q / 5) % 5 - 2;
   int k = (int)floor(pos[2] * g.dx3) + 1 + q / 25 - 2;
-  const double* V0 = &M.xyzv[3 * M.vof[3 * f + 0]];
-  const double* V1 = &M.xyzv[3 * M.vof[3 * f + 1]];
-  const double* V2 = &M.xyzv[3 * M.vof[3 * f + 2]];
-  double P[3] = {gxm(g, i), gym(g, j), gzm(g, k)};
-  double dist = dist_point_tri(P, V0, V1, V2);
   long long s = win_slot(win[1].org, et.e, i, j, k);
   if (s < 0) { raise_error(err, MELT_ERR_WINDOW); return; }
-  unsigned long long bits = (unsigned long long)__double_as_longlong(dist);
   if (mode == 0) {
-    atomicMin(&key[s], bits);
-  } else if (mode == 1) {
-    if (key[s] == bits) atomicMin(&face[s], f);
+    const double* V0 = &M.xyzv[3 * M.vof[3 * f + 0]];
+    const double* V1 = &M.xyzv[3 * M.vof[3 * f + 1]];
+    const double* V2 = &M.xyzv[3 * M.vof[3 * f + 2]];
+    double P[3] = {gxm(g, i), gym(g, j), gzm(g, k)};
+    double dist = dist_point_tri(P, V0, V1, V2);
+    dst[id] = dist;
+    atomicMin(&key[s], (unsigned long long)__double_as_longlong(dist));
+    return;
+  }
+  const double dist = dst[id];
+  if (mode == 1) {
+    if (key[s] == (unsigned long long)__double_as_longlong(dist)) atomicMin(&face[s], f);
   } else if (face[s] == f) {
+    double P[3] = {gxm(g, i), gym(g, j), gzm(g, k)};
     const double* n = &M.nor[3 * f];
     double dot = n[0] * (pos[0] - P[0]) + n[1] * (pos[1] - P[1]) + n[2] * (pos[2] - P[2]);
     double sgn = signbit(dot) ? -1.0 : 1.0;
@@ -991,8 +996,8 @@ static int ibm_tag_t(melt_ctx* ctx) {  // tagCells.f90:4-27 (tagType 1) + interp
   MeshArgs M = mesh_args(ctx);
   unsigned blocks = nb((size_t)ctx->nf * 125, 256);
   for (int mode = 0; mode < 3; ++mode)
-    hipLaunchKernelGGL(k_tag_pass<R>, dim3(blocks), dim3(256), 0, ctx->stream, g, M, ctx->d_win, et, ctx->d_tkey, ctx->d_tface, fld<R>(ctx, MELT_SDF),
-                       fld<R>(ctx, MELT_VOFP), mode, ctx->pi, ctx->d_err);
+    hipLaunchKernelGGL(k_tag_pass<R>, dim3(blocks), dim3(256), 0, ctx->stream, g, M, ctx->d_win, et, ctx->d_tkey, ctx->d_tface, ctx->d_tdist,
+                       fld<R>(ctx, MELT_SDF), fld<R>(ctx, MELT_VOFP), mode, ctx->pi, ctx->d_err);
   MELT_HIP(hipGetLastError());
   return 0;
 }
