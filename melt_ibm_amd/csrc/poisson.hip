@@ -279,6 +279,196 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_x_bwd(SpecArgs<T> S, FftPlan P, 
   }
 }
 
+// ---------------------------------------------------------------- register FFTs
+// For lines of N = P*P points (P = 8 or 16) the y and z passes use the
+// two-level form X[k1 + P k2] = sum_b W_P^(b k2) W_N^(b k1) sum_a x[b + P a] W_P^(a k1):
+// thread b of a line loads x[b + P a] straight from HBM, does a P-point DFT in
+// registers, scales by W_N^(b k1) (twiddles of N staged in LDS), and one LDS
+// transpose hands it the P values of its k1 for the second P-point DFT, after
+// which it holds X[k1 + P k2] for every k2.  The inverse runs the same steps
+// from that layout in reverse.  One exchange (two barriers) per transform
+// instead of one per radix stage.
+
+// cos(pi m / 8) and sin(pi m / 8), m = 0..15 (W_16^m = cos - i sin forward)
+template <typename T>
+__device__ __forceinline__ cx<T> w16(int m, bool inv) {
+  const T c1 = T(0.92387953251128675613), s1 = T(0.38268343236508977173), r = T(0.70710678118654752440);
+  const T cs[16] = {T(1), c1, r, s1, T(0), -s1, -r, -c1, T(-1), -c1, -r, -s1, T(0), s1, r, c1};
+  const T sn[16] = {T(0), s1, r, c1, T(1), c1, r, s1, T(0), -s1, -r, -c1, T(-1), -c1, -r, -s1};
+  return mkc<T>(cs[m], inv ? sn[m] : -sn[m]);
+}
+
+// v * W_16^M with the exact forms for M = 0, 4, 8, 12
+template <typename T, int M>
+__device__ __forceinline__ cx<T> mul_w16(cx<T> v, bool inv) {
+  constexpr int m = M % 16;
+  if constexpr (m == 0) return v;
+  else if constexpr (m == 8) return mkc<T>(-v.x, -v.y);
+  else if constexpr (m == 4) return inv ? mkc<T>(-v.y, v.x) : mkc<T>(v.y, -v.x);    // +i v or -i v
+  else if constexpr (m == 12) return inv ? mkc<T>(v.y, -v.x) : mkc<T>(-v.y, v.x);
+  else return cmul(v, w16<T>(m, inv));
+}
+
+// 4-point DFT of (v[o], v[o+s], v[o+2s], v[o+3s]) in place, natural order.
+template <typename T>
+__device__ __forceinline__ void dft4(cx<T>& x0, cx<T>& x1, cx<T>& x2, cx<T>& x3, bool inv) {
+  cx<T> a0 = cadd(x0, x2), a1 = csub(x0, x2), a2 = cadd(x1, x3), d = csub(x1, x3);
+  cx<T> a3 = inv ? mkc<T>(-d.y, d.x) : mkc<T>(d.y, -d.x);
+  x0 = cadd(a0, a2); x1 = cadd(a1, a3); x2 = csub(a0, a2); x3 = csub(a1, a3);
+}
+
+// P-point DFT (P = 8, 16) of v[0..P-1] in registers; result in natural order.
+template <typename T, int P>
+__device__ __forceinline__ void dft_reg(cx<T> (&v)[P], bool inv) {
+  if constexpr (P == 16) {
+    // a = 4 a1 + a0: DFT4 over a1, twiddle W_16^(a0 k0), DFT4 over a0
+#pragma unroll
+    for (int a0 = 0; a0 < 4; ++a0) dft4(v[a0], v[a0 + 4], v[a0 + 8], v[a0 + 12], inv);   // v[a0 + 4 k0]
+    v[5] = mul_w16<T, 1>(v[5], inv); v[9] = mul_w16<T, 2>(v[9], inv); v[13] = mul_w16<T, 3>(v[13], inv);
+    v[6] = mul_w16<T, 2>(v[6], inv); v[10] = mul_w16<T, 4>(v[10], inv); v[14] = mul_w16<T, 6>(v[14], inv);
+    v[7] = mul_w16<T, 3>(v[7], inv); v[11] = mul_w16<T, 6>(v[11], inv); v[15] = mul_w16<T, 9>(v[15], inv);
+    cx<T> o[16];
+#pragma unroll
+    for (int k0 = 0; k0 < 4; ++k0) {
+      cx<T> x0 = v[4 * k0], x1 = v[4 * k0 + 1], x2 = v[4 * k0 + 2], x3 = v[4 * k0 + 3];
+      dft4(x0, x1, x2, x3, inv);
+      o[k0] = x0; o[k0 + 4] = x1; o[k0 + 8] = x2; o[k0 + 12] = x3;
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = o[q];
+  } else {
+    static_assert(P == 8, "register DFT of 8 or 16 points");
+    // a = 2 a1 + a0: DFT4 over a1, twiddle W_8^(a0 k0) = W_16^(2 a0 k0), DFT2 over a0
+    dft4(v[0], v[2], v[4], v[6], inv);
+    dft4(v[1], v[3], v[5], v[7], inv);
+    v[3] = mul_w16<T, 2>(v[3], inv); v[5] = mul_w16<T, 4>(v[5], inv); v[7] = mul_w16<T, 6>(v[7], inv);
+    cx<T> o[8];
+#pragma unroll
+    for (int k0 = 0; k0 < 4; ++k0) {
+      cx<T> e = v[2 * k0], f = v[2 * k0 + 1];
+      o[k0] = cadd(e, f); o[k0 + 4] = csub(e, f);
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = o[q];
+  }
+}
+
+constexpr int RF_NC = 8;   // kx columns per register-FFT block (8 x 16 B = one 128-byte line per row)
+
+// LDS image of one block: the exchange tile (columns x P rows of pitch P+1)
+// and the N twiddles.
+template <typename T, int P>
+struct RegFftLds {
+  cx<T> ex[RF_NC * P * (P + 1)];
+  cx<T> tw[P * P];
+};
+
+// One transform of the lines (column c, thread b) held as v[a] = x[b + P a]:
+// on return v[k2] = X[k1 + P k2] for k1 = b.
+template <typename T, int P>
+__device__ __forceinline__ void reg_fft_fwd(cx<T> (&v)[P], RegFftLds<T, P>& L, int c, int b, bool inv) {
+  constexpr int N = P * P;
+  dft_reg<T, P>(v, inv);
+#pragma unroll
+  for (int k1 = 1; k1 < P; ++k1) {
+    cx<T> w = L.tw[(b * k1) % N];
+    if (inv) w.y = -w.y;
+    v[k1] = cmul(v[k1], w);
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < P; ++k1) L.ex[(c * P + k1) * (P + 1) + b] = v[k1];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < P; ++q) v[q] = L.ex[(c * P + b) * (P + 1) + q];
+  dft_reg<T, P>(v, inv);
+}
+
+// Inverse of that layout: v[k2] = X[k1 + P k2] with k1 = b in, v[n2] =
+// x[b + P n2] out (unnormalised, like FFTW_BACKWARD when inv, else forward).
+template <typename T, int P>
+__device__ __forceinline__ void reg_fft_back(cx<T> (&v)[P], RegFftLds<T, P>& L, int c, int b, bool inv) {
+  constexpr int N = P * P;
+  dft_reg<T, P>(v, inv);   // over k2: v[n1] = Z[k1 = b][n1]
+#pragma unroll
+  for (int n1 = 1; n1 < P; ++n1) {
+    cx<T> w = L.tw[(b * n1) % N];
+    if (inv) w.y = -w.y;
+    v[n1] = cmul(v[n1], w);
+  }
+  __syncthreads();   // every thread has read its forward exchange
+#pragma unroll
+  for (int n1 = 0; n1 < P; ++n1) L.ex[(c * P + n1) * (P + 1) + b] = v[n1];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < P; ++q) v[q] = L.ex[(c * P + b) * (P + 1) + q];
+  dft_reg<T, P>(v, inv);   // over k1: v[n2] = x[b + P n2]
+}
+
+// y pass (N2 = P*P) with register FFTs: RF_NC kx columns x P threads per block.
+template <typename T, int P>
+__global__ __launch_bounds__(RF_NC * P) void k_fft_y_reg(SpecArgs<T> S, const cx<T>* __restrict__ tw, int inv, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  __shared__ RegFftLds<T, P> L;
+  constexpr int N = P * P;
+  const int t = threadIdx.x, c = t % RF_NC, b = t / RF_NC;
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const int ntm = S.n1hp / RF_NC;
+  const int m0 = (int)(blk % (unsigned)ntm) * RF_NC, k = (int)(blk / (unsigned)ntm);
+  for (int q = t; q < N; q += RF_NC * P) L.tw[q] = tw[q];
+  cx<T>* base = S.spec + (size_t)m0 + c + (size_t)S.n1hp * N * k;
+  const size_t rs = (size_t)S.n1hp;
+  cx<T> v[P];
+#pragma unroll
+  for (int a = 0; a < P; ++a) v[a] = base[rs * (b + P * a)];
+  __syncthreads();   // twiddles
+  reg_fft_fwd<T, P>(v, L, c, b, inv != 0);
+#pragma unroll
+  for (int k2 = 0; k2 < P; ++k2) base[rs * (b + P * k2)] = v[k2];
+}
+
+// z pass with register FFTs: forward FFT, 1/(N1 N2 N3), division by
+// -(ak3+ak1+ak2) with the zero mode set to 0 (phcalc.f90:40-73), inverse FFT.
+template <typename T, int P>
+__global__ __launch_bounds__(RF_NC * P) void k_fft_z_solve_reg(SpecArgs<T> S, const cx<T>* __restrict__ tw, const double* __restrict__ ak1,
+                                                               const double* __restrict__ ak2, const double* __restrict__ ak3, double nrm,
+                                                               const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  __shared__ RegFftLds<T, P> L;
+  constexpr int N = P * P;
+  const int t = threadIdx.x, c = t % RF_NC, b = t / RF_NC;
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const int ntm = S.n1hp / RF_NC;
+  const int m0 = (int)(blk % (unsigned)ntm) * RF_NC, j = (int)(blk / (unsigned)ntm);
+  for (int q = t; q < N; q += RF_NC * P) L.tw[q] = tw[q];
+  cx<T>* base = S.spec + (size_t)m0 + c + (size_t)S.n1hp * j;
+  const size_t rs = (size_t)S.n1hp * S.g.n2;
+  cx<T> v[P];
+#pragma unroll
+  for (int a = 0; a < P; ++a) v[a] = base[rs * (b + P * a)];
+  __syncthreads();
+  reg_fft_fwd<T, P>(v, L, c, b, false);
+  const int m = m0 + c;
+#pragma unroll
+  for (int k2 = 0; k2 < P; ++k2) {
+    const int p = b + P * k2;
+    if (m < S.n1h) {
+      double ev = -ak3[p] - ak1[m] - ak2[j];
+      double vx = (double)v[k2].x / nrm, vy = (double)v[k2].y / nrm;
+      if (m == 0 && j == 0 && p == 0) { vx = 0.0; vy = 0.0; }
+      else { vx = vx / ev; vy = vy / ev; }
+      v[k2] = mkc<T>((T)vx, (T)vy);
+    } else {
+      v[k2] = mkc<T>(0, 0);
+    }
+  }
+  reg_fft_back<T, P>(v, L, c, b, true);
+#pragma unroll
+  for (int n2 = 0; n2 < P; ++n2) base[rs * (b + P * n2)] = v[n2];
+}
+
+// P with P*P = n, or 0
+int reg_fft_p(int n) { return n == 256 ? 16 : n == 64 ? 8 : 0; }
+
 bool make_plan(int N, FftPlan& P) {
   P.N = N;
   P.nst = 0;
@@ -370,8 +560,16 @@ int solve_t(melt_ctx* ctx, int fused_divg, const int* gate) {
   int nly = std::max(1, std::min(ctx->n1hp, (FFT_EPT * FFT_NT) / ctx->n2));
   size_t shy = sizeof(cx<T>) * (size_t)nly * ctx->n2;
   unsigned gy = (unsigned)(((ctx->n1hp + nly - 1) / nly) * ctx->n3);
+  // y and z lines of 64 or 256 points: register FFTs (RF_NC columns per block)
+  const int py = reg_fft_p(ctx->n2), pz = reg_fft_p(ctx->n3);
+  const unsigned gry = (unsigned)((ctx->n1hp / RF_NC) * ctx->n3), grz = (unsigned)((ctx->n1hp / RF_NC) * ctx->n2);
+  auto fft_y = [&](int inv) {
+    if (py == 16) hipLaunchKernelGGL((k_fft_y_reg<T, 16>), dim3(gry), dim3(RF_NC * 16), 0, ctx->stream, S, twy, inv, gate);
+    else if (py == 8) hipLaunchKernelGGL((k_fft_y_reg<T, 8>), dim3(gry), dim3(RF_NC * 8), 0, ctx->stream, S, twy, inv, gate);
+    else hipLaunchKernelGGL((k_fft_y<T>), dim3(gy), dim3(FFT_NT), shy, ctx->stream, S, Py, twy, nly, inv, gate);
+  };
   if (!gate) timing_begin(ctx, SLOT_FFTY, &ev);
-  hipLaunchKernelGGL((k_fft_y<T>), dim3(gy), dim3(FFT_NT), shy, ctx->stream, S, Py, twy, nly, 0, gate);
+  fft_y(0);
   MELT_HIP(hipGetLastError());
   if (!gate) timing_end(ctx, SLOT_FFTY, ev);
 
@@ -380,12 +578,17 @@ int solve_t(melt_ctx* ctx, int fused_divg, const int* gate) {
   unsigned gz = (unsigned)(((ctx->n1hp + nlz - 1) / nlz) * ctx->n2);
   double nrm = (double)((long long)ctx->n1 * ctx->n2 * ctx->n3);
   if (!gate) timing_begin(ctx, SLOT_FFTZ_SOLVE, &ev);
-  hipLaunchKernelGGL((k_fft_z_solve<T>), dim3(gz), dim3(FFT_NT), shz, ctx->stream, S, Pz, twz, nlz, ctx->ak1, ctx->ak2, ctx->ak3, nrm, gate);
+  if (pz == 16)
+    hipLaunchKernelGGL((k_fft_z_solve_reg<T, 16>), dim3(grz), dim3(RF_NC * 16), 0, ctx->stream, S, twz, ctx->ak1, ctx->ak2, ctx->ak3, nrm, gate);
+  else if (pz == 8)
+    hipLaunchKernelGGL((k_fft_z_solve_reg<T, 8>), dim3(grz), dim3(RF_NC * 8), 0, ctx->stream, S, twz, ctx->ak1, ctx->ak2, ctx->ak3, nrm, gate);
+  else
+    hipLaunchKernelGGL((k_fft_z_solve<T>), dim3(gz), dim3(FFT_NT), shz, ctx->stream, S, Pz, twz, nlz, ctx->ak1, ctx->ak2, ctx->ak3, nrm, gate);
   MELT_HIP(hipGetLastError());
   if (!gate) timing_end(ctx, SLOT_FFTZ_SOLVE, ev);
 
   if (!gate) timing_begin(ctx, SLOT_IFFTY, &ev);
-  hipLaunchKernelGGL((k_fft_y<T>), dim3(gy), dim3(FFT_NT), shy, ctx->stream, S, Py, twy, nly, 1, gate);
+  fft_y(1);
   MELT_HIP(hipGetLastError());
   if (!gate) timing_end(ctx, SLOT_IFFTY, ev);
 

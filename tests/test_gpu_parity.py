@@ -82,7 +82,8 @@ def test_invtr_and_line_solves(gpu_lib, n, n3):
         assert e < 1e-13, f"solx* betadx={betadx}: rel err {e}"
 
 
-@pytest.mark.parametrize("n,n3", SIZES)
+# 64 and 256-point y / z lines take the register FFTs, other lengths the LDS Stockham ones
+@pytest.mark.parametrize("n,n3", SIZES + [(64, 64), (256, 16), (16, 256)])
 def test_pressure_solve_and_projection(gpu_lib, n, n3):
     cfg = make_cfg(n, n3)
     g, o = pair(cfg, gpu_lib)
