@@ -222,14 +222,17 @@ __device__ int mls_grad_lanes(const Grid& g, const R* __restrict__ fld, const do
 
 // ---------------------------------------------------- 128-bit accumulation
 // acc_split / acc_value (fixed128.h) hold each contribution exactly as a
-// 64.64 fixed-point integer; the atomics add the two words with carry.
+// 64.64 fixed-point integer; the atomics add the two words with carry.  The
+// high word changes only when the running sum crosses a multiple of 2^64
+// (for a small negative contribution its all-ones high word and the carry
+// cancel), so its atomic is issued only when hi + carry is not zero.
 __device__ inline void acc_add(unsigned long long* w /* [lo, hi] */, double v) {
   unsigned long long lo, hi;
   acc_split(v, lo, hi);
   if ((lo | hi) == 0) return;
   unsigned long long old = atomicAdd(&w[0], lo);
   unsigned long long carry = (old + lo < old) ? 1ull : 0ull;
-  atomicAdd(&w[1], hi + carry);
+  if (hi + carry) atomicAdd(&w[1], hi + carry);
 }
 
 __device__ inline long long win_slot(const int org[3], const int E[3], int iu, int ju, int ku) {
