@@ -124,7 +124,7 @@ extern "C" void melt_ctx_destroy(melt_ctx* ctx) {
   for (int i = 0; i < 4; ++i) if (ctx->tmp4[i]) hipFree(ctx->tmp4[i]);
   void* ps[] = {ctx->spec, ctx->ak1, ctx->ak2, ctx->ak3, ctx->tw1, ctx->tw2, ctx->tw3, ctx->d_body, ctx->d_vof, ctx->d_eof, ctx->d_voe,
                 ctx->d_v2f_ptr, ctx->d_v2f, ctx->d_gv, ctx->d_ge, ctx->d_gf, ctx->d_pind, ctx->d_probe, ctx->d_rxp, ctx->d_rxt, ctx->d_red,
-                ctx->d_vbox, ctx->d_coord, ctx->d_acc, ctx->d_fmean, ctx->d_tkey, ctx->d_tface, ctx->d_tdist, ctx->d_win, ctx->d_pcount, ctx->d_plist, ctx->d_err, ctx->d_gate, ctx->d_refine_stats};
+                ctx->d_vbox, ctx->d_slots, ctx->d_coord, ctx->d_acc, ctx->d_fmean, ctx->d_tkey, ctx->d_tface, ctx->d_tdist, ctx->d_win, ctx->d_pcount, ctx->d_plist, ctx->d_err, ctx->d_gate, ctx->d_refine_stats};
   for (void* p : ps) if (p) hipFree(p);
   for (int i = 0; i < MELT_L_NLAG; ++i) if (ctx->d_lag[i]) hipFree(ctx->d_lag[i]);
   if (ctx->h_err) hipHostFree(ctx->h_err);
@@ -192,6 +192,8 @@ extern "C" int melt_ctx_create(const melt_config* cfg, melt_ctx** out) {
         hipMalloc(&ctx->d_win, sizeof(melt_window) * 2) != hipSuccess || hipMemset(ctx->d_win, 0, sizeof(melt_window) * 2) != hipSuccess ||
         hipMalloc(&ctx->d_gate, sizeof(int)) != hipSuccess || hipMemset(ctx->d_gate, 0, sizeof(int)) != hipSuccess ||
         hipMalloc(&ctx->d_refine_stats, sizeof(long long) * 8) != hipSuccess ||
+        hipMalloc(&ctx->d_slots, sizeof(unsigned long long) * REDUCE_SLOTS) != hipSuccess ||
+        hipMemset(ctx->d_slots, 0, sizeof(unsigned long long) * REDUCE_SLOTS) != hipSuccess ||
         hipMemset(ctx->d_refine_stats, 0, sizeof(long long) * 8) != hipSuccess) {
       rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "scratch allocation failed");
       break;
@@ -595,8 +597,8 @@ extern "C" int melt_substep(melt_ctx* ctx, int32_t ns) {
   // (before) and of the projected one (after) for its refinement gate
   unsigned long long* dmax = reinterpret_cast<unsigned long long*>(ctx->d_refine_stats + 4);
   const bool refine = ctx->fp32 && ctx->refine_max > 0;
-  if (refine && (rc = euler_launch_reduce_max(ctx, 1, dmax))) return rc;
-  if ((rc = poisson_solve(ctx, 1))) return rc;
+  if ((rc = poisson_solve(ctx, 1, nullptr, refine ? ctx->d_slots : nullptr))) return rc;
+  if (refine && (rc = euler_fold_slots(ctx, ctx->d_slots, dmax))) return rc;
   timing_begin(ctx, SLOT_PROJECT, &e);
   if ((rc = euler_launch_project(ctx, 1, 1, ibm ? 1 : 0))) return rc;
   timing_end(ctx, SLOT_PROJECT, e);

@@ -104,6 +104,7 @@ struct melt_ctx {
   double *d_rxp = nullptr, *d_rxt = nullptr;  // r_x_prn_tri, r_x_tau_n (3*nf)
   double* d_red = nullptr;       // scratch for deterministic reductions
   int* d_vbox = nullptr;         // vertex bounding box (6 ints) for ray tagging
+  unsigned long long* d_slots = nullptr;  // REDUCE_SLOTS partial maxima (zero between uses)
   double* d_coord = nullptr;     // xc, xm, yc, ym, zc, zm tables (coord_off)
 
   // forcing accumulators: 128-bit fixed point (hi:int64, lo:uint64) per
@@ -379,6 +380,24 @@ __device__ inline int lu_inverse(double a[N * N]) {
 // Device error word: first error code wins.
 __device__ inline void raise_error(int* err, int code) { atomicCAS(err, 0, code); }
 
+// Max reductions of non-negative doubles: their bit patterns order like the
+// values, so partial maxima combine with integer atomicMax.  Each block folds
+// its maximum into slot blockIdx % REDUCE_SLOTS (few atomics per address);
+// euler_fold_slots turns the slots into one value and clears them.
+constexpr int REDUCE_SLOTS = 1024;
+__device__ inline void block_max_to_slots(double v, unsigned long long* slots) {
+  __shared__ double wm[16];
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  if (lane == 0) wm[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = wm[0];
+    for (int q = 1; q < nw; ++q) m = fmax(m, wm[q]);
+    atomicMax(&slots[blockIdx.x % REDUCE_SLOTS], (unsigned long long)__double_as_longlong(m));
+  }
+}
+
 // XCD-aware block remap: consecutive logical blocks land on the same XCD (the
 // hardware deals workgroups round-robin over the 8 XCDs), so neighbouring tiles
 // share an L2.
@@ -420,7 +439,10 @@ int euler_launch_interp_vof(melt_ctx* ctx);
 int euler_reduce_max(melt_ctx* ctx, int what, double* out);
 int euler_launch_inqpr(melt_ctx* ctx);
 int poisson_init(melt_ctx* ctx);
-int poisson_solve(melt_ctx* ctx, int fused_divg, const int* gate = nullptr);
+// div_slots != null: the forward x pass also folds max |div u| (FP64, as
+// divgck) into those REDUCE_SLOTS slots.
+int poisson_solve(melt_ctx* ctx, int fused_divg, const int* gate = nullptr, unsigned long long* div_slots = nullptr);
+int euler_fold_slots(melt_ctx* ctx, unsigned long long* slots, unsigned long long* out);
 int ibm_tag(melt_ctx* ctx);
 int ibm_find_centroid_indices(melt_ctx* ctx);
 int ibm_weights(melt_ctx* ctx);

@@ -874,34 +874,50 @@ __global__ __launch_bounds__(256) void k_interp_vof(Grid g, const R* __restrict_
   vz[c] = R(0.5) * (p[s.at(0, 0, -1)] + v);
 }
 
-__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
-
 // Max reductions of cfl.f90:9-25 (what = 0) and divgck divg.f90:36-50
 // (what = 1), in FP64 arithmetic whatever the storage type; the result
 // (non-negative) is combined by atomicMax on its bit pattern.
 template <typename R>
 __global__ __launch_bounds__(256) void k_reduce_max(Grid g, const R* __restrict__ vx, const R* __restrict__ vy, const R* __restrict__ vz,
-                                                    int what, int nc, unsigned long long* out) {
-  __shared__ double sm[256];
+                                                    int what, int nc, unsigned long long* slots) {
+  // V consecutive x cells per thread through 16-byte loads (n1 % V == 0)
+  constexpr int V = 16 / sizeof(R);
+  struct alignas(16) Vec { R v[V]; };
   double m = what == 0 ? 0.00000001 : 0.0;
-  for (int c = (int)(blockIdx.x * blockDim.x + threadIdx.x); c < nc; c += (int)(gridDim.x * blockDim.x)) {
-    Cell s = make_cell(g, c);
-    double q;
-    double x0 = vx[c], x1 = vx[s.at(1, 0, 0)], y0 = vy[c], y1 = vy[s.at(0, 1, 0)], z0 = vz[c], z1 = vz[s.at(0, 0, 1)];
-    if (what == 0)
-      q = fabs((x0 + x1) * 0.5 * g.dx1) + fabs((y0 + y1) * 0.5 * g.dx2) + fabs((z0 + z1) * 0.5 * g.dx3);
-    else
-      q = fabs((x1 - x0) * g.dx1 + (y1 - y0) * g.dx2 + (z1 - z0) * g.dx3);
-    m = fmax(m, q);
+  const int c = (int)(xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) * V;
+  if (c < nc) {
+    const Cell s = make_cell(g, c);
+    const Vec X = *reinterpret_cast<const Vec*>(vx + c), Y0 = *reinterpret_cast<const Vec*>(vy + c);
+    const Vec Y1 = *reinterpret_cast<const Vec*>(vy + s.at(0, 1, 0)), Z0 = *reinterpret_cast<const Vec*>(vz + c);
+    const Vec Z1 = *reinterpret_cast<const Vec*>(vz + s.at(0, 0, 1));
+    const R xl = s.i + V == g.n1 ? vx[c + V - g.n1] : vx[c + V];   // x neighbour of the last cell
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      double x0 = X.v[e], x1 = e + 1 < V ? X.v[e + 1] : xl, y0 = Y0.v[e], y1 = Y1.v[e], z0 = Z0.v[e], z1 = Z1.v[e];
+      double q;
+      if (what == 0)
+        q = fabs((x0 + x1) * 0.5 * g.dx1) + fabs((y0 + y1) * 0.5 * g.dx2) + fabs((z0 + z1) * 0.5 * g.dx3);
+      else
+        q = fabs((x1 - x0) * g.dx1 + (y1 - y0) * g.dx2 + (z1 - z0) * g.dx3);
+      m = fmax(m, q);
+    }
   }
-  sm[threadIdx.x] = m;
+  block_max_to_slots(m, slots);
+}
+
+// The REDUCE_SLOTS partial maxima into *out; the slots are cleared for the
+// next reduction.  One block of REDUCE_SLOTS threads.
+__global__ __launch_bounds__(REDUCE_SLOTS) void k_fold_slots(unsigned long long* __restrict__ slots, unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long sm[REDUCE_SLOTS];
+  const int t = threadIdx.x;
+  sm[t] = slots[t];
+  slots[t] = 0ull;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + w]);
+  for (int w = REDUCE_SLOTS / 2; w > 0; w >>= 1) {
+    if (t < w) sm[t] = sm[t] > sm[t + w] ? sm[t] : sm[t + w];
     __syncthreads();
   }
-  // non-negative doubles order like their bit patterns
-  if (threadIdx.x == 0) atomicMax(out, (unsigned long long)__double_as_longlong(sm[0]));
+  if (t == 0) *out = sm[0];
 }
 
 // Refinement gate of the precision form: refine while the FP64 divergence of
@@ -1131,16 +1147,17 @@ int launch_interp_vof(melt_ctx* ctx) {
   return 0;
 }
 
-// Launch the max reduction into d (zeroed by a fill kernel first, so the
-// sequence may be captured into a graph).
+// Launch the max reduction into d: one cell per thread into the partial
+// slots, then one block folds them (kernels only, so the sequence may be
+// captured into a graph).
 template <typename R>
 int launch_reduce_max(melt_ctx* ctx, int what, unsigned long long* d) {
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, ctx->stream, d);
-  unsigned nb = (unsigned)std::min<size_t>(1024, nblocks(ctx->nc, 256));
-  hipLaunchKernelGGL(k_reduce_max<R>, dim3(nb), dim3(256), 0, ctx->stream, make_grid(ctx), fld<R>(ctx, MELT_VX), fld<R>(ctx, MELT_VY),
-                     fld<R>(ctx, MELT_VZ), what, (int)ctx->nc, d);
+  constexpr int V = 16 / sizeof(R);
+  if (ctx->n1 % V) return melt_fail(ctx, MELT_ERR_ARG, "reduce_max", "n1m is not a multiple of the vector width");
+  hipLaunchKernelGGL(k_reduce_max<R>, dim3(nblocks(ctx->nc / V, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), fld<R>(ctx, MELT_VX),
+                     fld<R>(ctx, MELT_VY), fld<R>(ctx, MELT_VZ), what, (int)ctx->nc, ctx->d_slots);
   MELT_HIP(hipGetLastError());
-  return 0;
+  return euler_fold_slots(ctx, ctx->d_slots, d);
 }
 
 template <typename R>
@@ -1178,6 +1195,11 @@ int euler_launch_project(melt_ctx* ctx, int do_u, int do_p, int use_mean, const 
 int euler_launch_interp_vof(melt_ctx* ctx) { return ctx->fp32 ? launch_interp_vof<float>(ctx) : launch_interp_vof<double>(ctx); }
 int euler_launch_reduce_max(melt_ctx* ctx, int what, unsigned long long* d) {
   return ctx->fp32 ? launch_reduce_max<float>(ctx, what, d) : launch_reduce_max<double>(ctx, what, d);
+}
+int euler_fold_slots(melt_ctx* ctx, unsigned long long* slots, unsigned long long* out) {
+  hipLaunchKernelGGL(k_fold_slots, dim3(1), dim3(REDUCE_SLOTS), 0, ctx->stream, slots, out);
+  MELT_HIP(hipGetLastError());
+  return 0;
 }
 int euler_launch_refine_gate(melt_ctx* ctx, const unsigned long long* dmax, int* gate) {
   hipLaunchKernelGGL(k_refine_gate, dim3(1), dim3(1), 0, ctx->stream, dmax, ctx->refine_rtol, gate, ctx->d_refine_stats);

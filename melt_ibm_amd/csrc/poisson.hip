@@ -129,11 +129,13 @@ struct SpecArgs {
 };
 
 // x pass, forward: nl real lines (divergence divg.f90:17-20, or a stored field)
-// -> bins 0..N1/2 of each, two lines per complex transform.
+// -> bins 0..N1/2 of each, two lines per complex transform.  div_slots !=
+// null: max |div u| in FP64 (divg.f90:36-50) is folded into those slots too.
 template <typename Rf, typename T>
 __global__ __launch_bounds__(FFT_NT) void k_fft_x_fwd(SpecArgs<T> S, FftPlan P, const cx<T>* __restrict__ tw, int nl,
                                                       const Rf* __restrict__ vx, const Rf* __restrict__ vy, const Rf* __restrict__ vz,
-                                                      const Rf* __restrict__ src, T usdtal, const int* __restrict__ gate) {
+                                                      const Rf* __restrict__ src, T usdtal, const int* __restrict__ gate,
+                                                      unsigned long long* __restrict__ div_slots) {
   if (gate && *gate == 0) return;
   extern __shared__ unsigned char smem[];
   cx<T>* sh = reinterpret_cast<cx<T>*>(smem);
@@ -146,6 +148,7 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_x_fwd(SpecArgs<T> S, FftPlan P, 
   const int j0 = jt * nl;
   const T dx1 = (T)g.dx1, dx2 = (T)g.dx2, dx3 = (T)g.dx3;
   const int kp = k == g.n3 - 1 ? 0 : k + 1;
+  double dmax = 0.0;
   for (int c = threadIdx.x; c < nl * N; c += blockDim.x) {
     int l = c / N, i = c % N, j = j0 + l;
     size_t idx = (size_t)i + (size_t)N * ((size_t)j + (size_t)g.n2 * k);
@@ -154,12 +157,16 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_x_fwd(SpecArgs<T> S, FftPlan P, 
       v = (T)src[idx];
     } else {
       int ip = i == N - 1 ? 0 : i + 1, jp = j == g.n2 - 1 ? 0 : j + 1;
-      T dq = ((T)vx[idx - i + ip] - (T)vx[idx]) * dx1 + ((T)vy[(size_t)i + (size_t)N * ((size_t)jp + (size_t)g.n2 * k)] - (T)vy[idx]) * dx2 +
-             ((T)vz[(size_t)i + (size_t)N * ((size_t)j + (size_t)g.n2 * kp)] - (T)vz[idx]) * dx3;
+      const Rf x0 = vx[idx], x1 = vx[idx - i + ip], y0 = vy[idx], y1 = vy[(size_t)i + (size_t)N * ((size_t)jp + (size_t)g.n2 * k)];
+      const Rf z0 = vz[idx], z1 = vz[(size_t)i + (size_t)N * ((size_t)j + (size_t)g.n2 * kp)];
+      T dq = ((T)x1 - (T)x0) * dx1 + ((T)y1 - (T)y0) * dx2 + ((T)z1 - (T)z0) * dx3;
       v = dq * usdtal;
+      if (div_slots)
+        dmax = fmax(dmax, fabs(((double)x1 - (double)x0) * g.dx1 + ((double)y1 - (double)y0) * g.dx2 + ((double)z1 - (double)z0) * g.dx3));
     }
     shr[2 * ((l >> 1) * ls + i) + (l & 1)] = v;  // line 2p -> real part, 2p+1 -> imaginary part
   }
+  if (div_slots) block_max_to_slots(dmax, div_slots);
   __syncthreads();
   lds_fft<T, false>(sh, nl / 2, ls, 1, P, tw, false);
   const int np = nl / 2;
@@ -536,7 +543,7 @@ int poisson_init(melt_ctx* ctx) {
 // gate != null: every kernel returns at once unless *gate != 0 (the
 // refinement solve of the precision form).
 template <typename T>
-int solve_t(melt_ctx* ctx, int fused_divg, const int* gate) {
+int solve_t(melt_ctx* ctx, int fused_divg, const int* gate, unsigned long long* div_slots) {
   SpecArgs<T> S;
   S.g = make_grid(ctx);
   S.n1h = ctx->n1h; S.n1hp = ctx->n1hp; S.spec = static_cast<cx<T>*>(ctx->spec);
@@ -553,7 +560,7 @@ int solve_t(melt_ctx* ctx, int fused_divg, const int* gate) {
   unsigned gx = (unsigned)((ctx->n2 / nlx) * ctx->n3);
   if (!gate) timing_begin(ctx, SLOT_DIVG_FFTX, &ev);
   hipLaunchKernelGGL((k_fft_x_fwd<T, T>), dim3(gx), dim3(FFT_NT), shx, ctx->stream, S, Px, twx, nlx, fld<T>(ctx, MELT_VX), fld<T>(ctx, MELT_VY),
-                     fld<T>(ctx, MELT_VZ), fused_divg ? nullptr : fld<T>(ctx, MELT_DPH), usdtal, gate);
+                     fld<T>(ctx, MELT_VZ), fused_divg ? nullptr : fld<T>(ctx, MELT_DPH), usdtal, gate, fused_divg ? div_slots : nullptr);
   MELT_HIP(hipGetLastError());
   if (!gate) timing_end(ctx, SLOT_DIVG_FFTX, ev);
 
@@ -599,6 +606,6 @@ int solve_t(melt_ctx* ctx, int fused_divg, const int* gate) {
   return 0;
 }
 
-int poisson_solve(melt_ctx* ctx, int fused_divg, const int* gate) {
-  return ctx->fp32 ? solve_t<float>(ctx, fused_divg, gate) : solve_t<double>(ctx, fused_divg, gate);
+int poisson_solve(melt_ctx* ctx, int fused_divg, const int* gate, unsigned long long* div_slots) {
+  return ctx->fp32 ? solve_t<float>(ctx, fused_divg, gate, div_slots) : solve_t<double>(ctx, fused_divg, gate, div_slots);
 }
