@@ -362,18 +362,19 @@ __device__ __forceinline__ void dft_reg(cx<T> (&v)[P], bool inv) {
 
 constexpr int RF_NC = 8;   // kx columns per register-FFT block (8 x 16 B = one 128-byte line per row)
 
-// LDS image of one block: the exchange tile (columns x P rows of pitch P+1)
-// and the N twiddles.
-template <typename T, int P>
+// LDS image of one block of NC lines: the exchange tile (NC x P rows of
+// pitch P+1) and the N twiddles.
+template <typename T, int P, int NC>
 struct RegFftLds {
-  cx<T> ex[RF_NC * P * (P + 1)];
+  static constexpr int lines = NC;
+  cx<T> ex[NC * P * (P + 1)];
   cx<T> tw[P * P];
 };
 
 // One transform of the lines (column c, thread b) held as v[a] = x[b + P a]:
 // on return v[k2] = X[k1 + P k2] for k1 = b.
-template <typename T, int P>
-__device__ __forceinline__ void reg_fft_fwd(cx<T> (&v)[P], RegFftLds<T, P>& L, int c, int b, bool inv) {
+template <typename T, int P, int NC>
+__device__ __forceinline__ void reg_fft_fwd(cx<T> (&v)[P], RegFftLds<T, P, NC>& L, int c, int b, bool inv) {
   constexpr int N = P * P;
   dft_reg<T, P>(v, inv);
 #pragma unroll
@@ -392,8 +393,8 @@ __device__ __forceinline__ void reg_fft_fwd(cx<T> (&v)[P], RegFftLds<T, P>& L, i
 
 // Inverse of that layout: v[k2] = X[k1 + P k2] with k1 = b in, v[n2] =
 // x[b + P n2] out (unnormalised, like FFTW_BACKWARD when inv, else forward).
-template <typename T, int P>
-__device__ __forceinline__ void reg_fft_back(cx<T> (&v)[P], RegFftLds<T, P>& L, int c, int b, bool inv) {
+template <typename T, int P, int NC>
+__device__ __forceinline__ void reg_fft_back(cx<T> (&v)[P], RegFftLds<T, P, NC>& L, int c, int b, bool inv) {
   constexpr int N = P * P;
   dft_reg<T, P>(v, inv);   // over k2: v[n1] = Z[k1 = b][n1]
 #pragma unroll
@@ -415,7 +416,7 @@ __device__ __forceinline__ void reg_fft_back(cx<T> (&v)[P], RegFftLds<T, P>& L, 
 template <typename T, int P>
 __global__ __launch_bounds__(RF_NC * P) void k_fft_y_reg(SpecArgs<T> S, const cx<T>* __restrict__ tw, int inv, const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
-  __shared__ RegFftLds<T, P> L;
+  __shared__ RegFftLds<T, P, RF_NC> L;
   constexpr int N = P * P;
   const int t = threadIdx.x, c = t % RF_NC, b = t / RF_NC;
   const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
@@ -440,7 +441,7 @@ __global__ __launch_bounds__(RF_NC * P) void k_fft_z_solve_reg(SpecArgs<T> S, co
                                                                const double* __restrict__ ak2, const double* __restrict__ ak3, double nrm,
                                                                const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
-  __shared__ RegFftLds<T, P> L;
+  __shared__ RegFftLds<T, P, RF_NC> L;
   constexpr int N = P * P;
   const int t = threadIdx.x, c = t % RF_NC, b = t / RF_NC;
   const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
@@ -471,6 +472,117 @@ __global__ __launch_bounds__(RF_NC * P) void k_fft_z_solve_reg(SpecArgs<T> S, co
   reg_fft_back<T, P>(v, L, c, b, true);
 #pragma unroll
   for (int n2 = 0; n2 < P; ++n2) base[rs * (b + P * n2)] = v[n2];
+}
+
+// x pass, forward, with register FFTs (N1 = P*P): RF_XT threads per block,
+// thread (c, b) owns line pair c of the block (lines j, j+1 packed as real and
+// imaginary part, divergence formed at load as k_fft_x_fwd) and points
+// b + P a.  After the transform the two half spectra are separated with the
+// mirror bins N-m, exchanged through LDS.
+constexpr int RF_XT = 128;
+
+template <typename Rf, typename T, int P>
+__global__ __launch_bounds__(RF_XT) void k_fft_x_fwd_reg(SpecArgs<T> S, const cx<T>* __restrict__ tw, const Rf* __restrict__ vx,
+                                                         const Rf* __restrict__ vy, const Rf* __restrict__ vz, const Rf* __restrict__ src,
+                                                         T usdtal, const int* __restrict__ gate, unsigned long long* __restrict__ div_slots) {
+  if (gate && *gate == 0) return;
+  constexpr int N = P * P, NPB = RF_XT / P;   // line pairs per block
+  __shared__ RegFftLds<T, P, NPB> L;
+  static_assert(NPB * N <= (int)(sizeof(L.ex) / sizeof(L.ex[0])), "mirror exchange fits the transpose tile");
+  const Grid& g = S.g;
+  const int t = threadIdx.x, b = t % P, c = t / P;
+  int jt, k;
+  tiled_2d(xcd_block(blockIdx.x, gridDim.x), g.n2 / (2 * NPB), g.n3, 16, 8, jt, k);  // the divergence reads rows j+1 and planes k+1
+  const int j = jt * 2 * NPB + 2 * c, kp = k == g.n3 - 1 ? 0 : k + 1;
+  for (int q = t; q < N; q += RF_XT) L.tw[q] = tw[q];
+  const T dx1 = (T)g.dx1, dx2 = (T)g.dx2, dx3 = (T)g.dx3;
+  double dmax = 0.0;
+  cx<T> v[P];
+#pragma unroll
+  for (int a = 0; a < P; ++a) {
+    const int i = b + P * a, ip = i == N - 1 ? 0 : i + 1;
+    T z[2];
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+      const int jj = j + l, jp = jj == g.n2 - 1 ? 0 : jj + 1;
+      const size_t idx = (size_t)i + (size_t)N * ((size_t)jj + (size_t)g.n2 * k);
+      if (src) {
+        z[l] = (T)src[idx];
+      } else {
+        const Rf x0 = vx[idx], x1 = vx[idx - i + ip], y0 = vy[idx], y1 = vy[(size_t)i + (size_t)N * ((size_t)jp + (size_t)g.n2 * k)];
+        const Rf z0 = vz[idx], z1 = vz[(size_t)i + (size_t)N * ((size_t)jj + (size_t)g.n2 * kp)];
+        T dq = ((T)x1 - (T)x0) * dx1 + ((T)y1 - (T)y0) * dx2 + ((T)z1 - (T)z0) * dx3;
+        z[l] = dq * usdtal;
+        if (div_slots)
+          dmax = fmax(dmax, fabs(((double)x1 - (double)x0) * g.dx1 + ((double)y1 - (double)y0) * g.dx2 + ((double)z1 - (double)z0) * g.dx3));
+      }
+    }
+    v[a] = mkc<T>(z[0], z[1]);
+  }
+  if (div_slots) block_max_to_slots(dmax, div_slots);
+  __syncthreads();   // twiddles
+  reg_fft_fwd<T, P>(v, L, c, b, false);   // v[k2] = Z[b + P k2]
+  cx<T>* zs = L.ex + c * N;
+  __syncthreads();   // the transpose tile is read
+#pragma unroll
+  for (int k2 = 0; k2 < P; ++k2) zs[b + P * k2] = v[k2];
+  __syncthreads();
+  const size_t row = (size_t)S.n1hp * ((size_t)j + (size_t)g.n2 * k);
+#pragma unroll
+  for (int k2 = 0; k2 < P; ++k2) {
+    const int m = b + P * k2;
+    if (m >= S.n1hp) break;
+    cx<T> A = mkc<T>(0, 0), B = mkc<T>(0, 0);
+    if (m < S.n1h) {
+      const cx<T> z = v[k2], zr = zs[m == 0 ? 0 : N - m];
+      // A = (Z[m] + conj Z[N-m]) / 2,  B = (Z[m] - conj Z[N-m]) / (2i)
+      A = mkc<T>((z.x + zr.x) * T(0.5), (z.y - zr.y) * T(0.5));
+      B = mkc<T>((z.y + zr.y) * T(0.5), (zr.x - z.x) * T(0.5));
+    }
+    S.spec[row + m] = A;
+    S.spec[row + S.n1hp + m] = B;
+  }
+}
+
+// x pass, backward, with register FFTs (FFTW c2r semantics as k_fft_x_bwd):
+// thread (c, b) packs Z = A + iB of line pair c for bins b + P k2 (the bins
+// above N/2 as conj(A) + i conj(B) of the mirror bin), transforms, and writes
+// line j from the real and line j+1 from the imaginary part.
+template <typename Rf, typename T, int P>
+__global__ __launch_bounds__(RF_XT) void k_fft_x_bwd_reg(SpecArgs<T> S, const cx<T>* __restrict__ tw, Rf* __restrict__ dst,
+                                                         const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  constexpr int N = P * P, NPB = RF_XT / P;
+  __shared__ RegFftLds<T, P, NPB> L;
+  const Grid& g = S.g;
+  const int t = threadIdx.x, b = t % P, c = t / P;
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const int ntj = g.n2 / (2 * NPB);
+  const int j = (int)(blk % (unsigned)ntj) * 2 * NPB + 2 * c, k = (int)(blk / (unsigned)ntj);
+  for (int q = t; q < N; q += RF_XT) L.tw[q] = tw[q];
+  const size_t row = (size_t)S.n1hp * ((size_t)j + (size_t)g.n2 * k);
+  cx<T> v[P];
+#pragma unroll
+  for (int k2 = 0; k2 < P; ++k2) {
+    const int m = b + P * k2;
+    if (m < S.n1h) {
+      cx<T> A = S.spec[row + m], B = S.spec[row + S.n1hp + m];
+      if (m == 0 || 2 * m == N) { A.y = T(0); B.y = T(0); }
+      v[k2] = mkc<T>(A.x - B.y, A.y + B.x);   // A + iB
+    } else {
+      const cx<T> A = S.spec[row + (N - m)], B = S.spec[row + S.n1hp + (N - m)];
+      v[k2] = mkc<T>(A.x + B.y, B.x - A.y);   // conj(A) + i conj(B)
+    }
+  }
+  __syncthreads();   // twiddles
+  reg_fft_back<T, P>(v, L, c, b, true);   // v[n2] = z[b + P n2]
+  const size_t o0 = (size_t)N * ((size_t)j + (size_t)g.n2 * k);
+#pragma unroll
+  for (int n2 = 0; n2 < P; ++n2) {
+    const int i = b + P * n2;
+    dst[o0 + i] = (Rf)v[n2].x;
+    dst[o0 + N + i] = (Rf)v[n2].y;
+  }
 }
 
 // P with P*P = n, or 0
@@ -558,9 +670,21 @@ int solve_t(melt_ctx* ctx, int fused_divg, const int* gate, unsigned long long* 
   size_t shx = sizeof(cx<T>) * (size_t)(nlx / 2) * (ctx->n1 + 1);
   T usdtal = T(1.0 / (ctx->dt * ctx->al));
   unsigned gx = (unsigned)((ctx->n2 / nlx) * ctx->n3);
+  // x lines of 64 or 256 points: register FFTs (RF_XT threads, RF_XT / P line pairs per block)
+  const int px = reg_fft_p(ctx->n1);
+  const unsigned grx = px ? (unsigned)((ctx->n2 / (2 * RF_XT / px)) * ctx->n3) : 0;
+  const T* src = fused_divg ? nullptr : fld<T>(ctx, MELT_DPH);
+  unsigned long long* dsl = fused_divg ? div_slots : nullptr;
   if (!gate) timing_begin(ctx, SLOT_DIVG_FFTX, &ev);
-  hipLaunchKernelGGL((k_fft_x_fwd<T, T>), dim3(gx), dim3(FFT_NT), shx, ctx->stream, S, Px, twx, nlx, fld<T>(ctx, MELT_VX), fld<T>(ctx, MELT_VY),
-                     fld<T>(ctx, MELT_VZ), fused_divg ? nullptr : fld<T>(ctx, MELT_DPH), usdtal, gate, fused_divg ? div_slots : nullptr);
+  if (px == 16)
+    hipLaunchKernelGGL((k_fft_x_fwd_reg<T, T, 16>), dim3(grx), dim3(RF_XT), 0, ctx->stream, S, twx, fld<T>(ctx, MELT_VX), fld<T>(ctx, MELT_VY),
+                       fld<T>(ctx, MELT_VZ), src, usdtal, gate, dsl);
+  else if (px == 8)
+    hipLaunchKernelGGL((k_fft_x_fwd_reg<T, T, 8>), dim3(grx), dim3(RF_XT), 0, ctx->stream, S, twx, fld<T>(ctx, MELT_VX), fld<T>(ctx, MELT_VY),
+                       fld<T>(ctx, MELT_VZ), src, usdtal, gate, dsl);
+  else
+    hipLaunchKernelGGL((k_fft_x_fwd<T, T>), dim3(gx), dim3(FFT_NT), shx, ctx->stream, S, Px, twx, nlx, fld<T>(ctx, MELT_VX), fld<T>(ctx, MELT_VY),
+                       fld<T>(ctx, MELT_VZ), src, usdtal, gate, dsl);
   MELT_HIP(hipGetLastError());
   if (!gate) timing_end(ctx, SLOT_DIVG_FFTX, ev);
 
@@ -600,7 +724,12 @@ int solve_t(melt_ctx* ctx, int fused_divg, const int* gate, unsigned long long* 
   if (!gate) timing_end(ctx, SLOT_IFFTY, ev);
 
   if (!gate) timing_begin(ctx, SLOT_IFFTX, &ev);
-  hipLaunchKernelGGL((k_fft_x_bwd<T, T>), dim3(gx), dim3(FFT_NT), shx, ctx->stream, S, Px, twx, nlx, fld<T>(ctx, MELT_DPH), gate);
+  if (px == 16)
+    hipLaunchKernelGGL((k_fft_x_bwd_reg<T, T, 16>), dim3(grx), dim3(RF_XT), 0, ctx->stream, S, twx, fld<T>(ctx, MELT_DPH), gate);
+  else if (px == 8)
+    hipLaunchKernelGGL((k_fft_x_bwd_reg<T, T, 8>), dim3(grx), dim3(RF_XT), 0, ctx->stream, S, twx, fld<T>(ctx, MELT_DPH), gate);
+  else
+    hipLaunchKernelGGL((k_fft_x_bwd<T, T>), dim3(gx), dim3(FFT_NT), shx, ctx->stream, S, Px, twx, nlx, fld<T>(ctx, MELT_DPH), gate);
   MELT_HIP(hipGetLastError());
   if (!gate) timing_end(ctx, SLOT_IFFTX, ev);
   return 0;
