@@ -129,6 +129,9 @@ extern "C" void melt_ctx_destroy(melt_ctx* ctx) {
   for (int i = 0; i < MELT_L_NLAG; ++i) if (ctx->d_lag[i]) hipFree(ctx->d_lag[i]);
   if (ctx->h_err) hipHostFree(ctx->h_err);
   for (auto e : ctx->ev_pool) hipEventDestroy(e);
+  if (ctx->side) hipStreamSynchronize(ctx->side);
+  for (hipEvent_t e : {ctx->ev_fork, ctx->ev_join}) if (e) hipEventDestroy(e);
+  if (ctx->side) hipStreamDestroy(ctx->side);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -173,7 +176,13 @@ extern "C" int melt_ctx_create(const melt_config* cfg, melt_ctx** out) {
         pos = end + 1;
       }
     }
-    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { rc = MELT_ERR_HIP; break; }
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
+      rc = MELT_ERR_HIP;
+      break;
+    }
     for (int i = 0; i < MELT_NFIELDS && !rc; ++i) {
       if (hipMalloc(&ctx->f[i], ctx->esz * ctx->nc) != hipSuccess || hipMemset(ctx->f[i], 0, ctx->esz * ctx->nc) != hipSuccess)
         rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "field allocation failed");
@@ -585,9 +594,21 @@ extern "C" int melt_substep(melt_ctx* ctx, int32_t ns) {
     if (!rc) rc = ibm_force_drain(ctx, 1);
     if (rc) return rc;
     timing_end(ctx, SLOT_IBM, e);
+    // The Lagrangian update after the forcing (loads, heat flux, Stefan
+    // melting, Newton-Euler motion, geometry refresh) reads only the probe
+    // results and the body arrays, which the pressure phase does not touch:
+    // it runs on the side stream, concurrently with the pressure solve, and
+    // rejoins before the substep ends (the next tagging needs the geometry).
+    MELT_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+    MELT_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+    hipStream_t main_stream = ctx->stream;
+    ctx->stream = ctx->side;
     timing_begin(ctx, SLOT_LAGR, &e);
-    if ((rc = lagr_after_force(ctx))) return rc;
+    rc = lagr_after_force(ctx);
     timing_end(ctx, SLOT_LAGR, e);
+    ctx->stream = main_stream;
+    if (rc) return rc;
+    MELT_HIP(hipEventRecord(ctx->ev_join, ctx->side));
   }
   timing_end(ctx, SLOT_PH_MLS, e_ph);
 
@@ -615,6 +636,7 @@ extern "C" int melt_substep(melt_ctx* ctx, int32_t ns) {
     timing_end(ctx, SLOT_REFINE, e);
   }
   timing_end(ctx, SLOT_PH_PRESSURE, e_ph);
+  if (ibm) MELT_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
   timing_end(ctx, SLOT_SUBSTEP, e_sub);
   if (ctx->timing && ctx->ev_pending.size() > 4096) timing_flush(ctx);
   return 0;

@@ -65,6 +65,10 @@ struct melt_ctx {
   double al = 0, ga = 0, ro = 0, dt = 0, beta = 0;
   int stage = 0;
   hipStream_t stream = nullptr;
+  // second stream of the fused substep: the Lagrangian update runs there,
+  // concurrently with the pressure phase (fork / join events)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // graph replay of one RK3 step (melt_graph_enable): captured on first use,
   // dropped whenever a host-side launch parameter (dt, refinement) changes
   int use_graph = 0;
