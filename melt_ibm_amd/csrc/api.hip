@@ -123,7 +123,7 @@ extern "C" void melt_ctx_destroy(melt_ctx* ctx) {
   for (int i = 1; i < 4; ++i) if (ctx->rhs4[i]) hipFree(ctx->rhs4[i]);
   for (int i = 0; i < 4; ++i) if (ctx->tmp4[i]) hipFree(ctx->tmp4[i]);
   void* ps[] = {ctx->spec, ctx->ak1, ctx->ak2, ctx->ak3, ctx->tw1, ctx->tw2, ctx->tw3, ctx->d_body, ctx->d_vof, ctx->d_eof, ctx->d_voe,
-                ctx->d_v2f_ptr, ctx->d_v2f, ctx->d_gv, ctx->d_ge, ctx->d_gf, ctx->d_pind, ctx->d_probe, ctx->d_rxp, ctx->d_rxt, ctx->d_red,
+                ctx->d_v2f_ptr, ctx->d_v2f, ctx->d_gv, ctx->d_ge, ctx->d_gf, ctx->d_pind, ctx->d_pind2, ctx->d_probe, ctx->d_rxp, ctx->d_rxt, ctx->d_red,
                 ctx->d_vbox, ctx->d_slots, ctx->d_coord, ctx->d_acc, ctx->d_fmean, ctx->d_tkey, ctx->d_tface, ctx->d_tdist, ctx->d_win, ctx->d_pcount, ctx->d_plist, ctx->d_err, ctx->d_gate, ctx->d_refine_stats};
   for (void* p : ps) if (p) hipFree(p);
   for (int i = 0; i < MELT_L_NLAG; ++i) if (ctx->d_lag[i]) hipFree(ctx->d_lag[i]);
@@ -539,6 +539,27 @@ extern "C" int melt_particle(melt_ctx* ctx) {
 }
 
 // ------------------------------------------------------------ fused path
+// Queue `work` (launches on ctx->stream) on the side stream after everything
+// queued on the main stream so far; side_join makes the main stream wait for
+// it.  Fork and join are stream events, so a captured step keeps them as
+// graph dependencies.
+template <class F>
+static int side_run(melt_ctx* ctx, F&& work) {
+  MELT_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+  MELT_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+  hipStream_t main_stream = ctx->stream;
+  ctx->stream = ctx->side;
+  int rc = work();
+  ctx->stream = main_stream;
+  if (rc) return rc;
+  MELT_HIP(hipEventRecord(ctx->ev_join, ctx->side));
+  return 0;
+}
+static int side_join(melt_ctx* ctx) {
+  MELT_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+  return 0;
+}
+
 extern "C" int melt_substep(melt_ctx* ctx, int32_t ns) {
   if (check_ctx(ctx)) return MELT_ERR_ARG;
   int rc = melt_set_stage(ctx, ns);
@@ -586,29 +607,28 @@ extern "C" int melt_substep(melt_ctx* ctx, int32_t ns) {
   timing_begin(ctx, SLOT_PH_MLS, &e_ph);
   if (ibm) {
     timing_begin(ctx, SLOT_IBM, &e);
-    rc = ibm_find_centroid_indices(ctx);
-    if (!rc) rc = ibm_weights(ctx);
-    if (!rc) rc = ibm_force_scatter(ctx);
+    if ((rc = ibm_find_centroid_indices(ctx)) || (rc = ibm_weights(ctx))) return rc;
+    // the gradient probes read the same pre-force fields as the spreading and
+    // write only their own outputs (their cell indices into d_pind2, since the
+    // spreading still reads d_pind): side stream, alongside the spreading
+    if ((rc = side_run(ctx, [&] { return ibm_probes(ctx, ctx->d_pind2); }))) return rc;
+    rc = ibm_force_scatter(ctx);
     ctx->force_window_from_tag = 0;
-    if (!rc) rc = ibm_probes(ctx);
-    if (!rc) rc = ibm_force_drain(ctx, 1);
-    if (rc) return rc;
+    if (rc || (rc = side_join(ctx)) || (rc = ibm_force_drain(ctx, 1))) return rc;
     timing_end(ctx, SLOT_IBM, e);
     // The Lagrangian update after the forcing (loads, heat flux, Stefan
     // melting, Newton-Euler motion, geometry refresh) reads only the probe
     // results and the body arrays, which the pressure phase does not touch:
     // it runs on the side stream, concurrently with the pressure solve, and
     // rejoins before the substep ends (the next tagging needs the geometry).
-    MELT_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
-    MELT_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
-    hipStream_t main_stream = ctx->stream;
-    ctx->stream = ctx->side;
-    timing_begin(ctx, SLOT_LAGR, &e);
-    rc = lagr_after_force(ctx);
-    timing_end(ctx, SLOT_LAGR, e);
-    ctx->stream = main_stream;
-    if (rc) return rc;
-    MELT_HIP(hipEventRecord(ctx->ev_join, ctx->side));
+    if ((rc = side_run(ctx, [&] {
+          timing_begin(ctx, SLOT_LAGR, &e);
+          int r = ibm_commit_probe_indices(ctx);
+          if (!r) r = lagr_after_force(ctx);
+          timing_end(ctx, SLOT_LAGR, e);
+          return r;
+        })))
+      return rc;
   }
   timing_end(ctx, SLOT_PH_MLS, e_ph);
 
@@ -636,7 +656,7 @@ extern "C" int melt_substep(melt_ctx* ctx, int32_t ns) {
     timing_end(ctx, SLOT_REFINE, e);
   }
   timing_end(ctx, SLOT_PH_PRESSURE, e_ph);
-  if (ibm) MELT_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+  if (ibm && (rc = side_join(ctx))) return rc;
   timing_end(ctx, SLOT_SUBSTEP, e_sub);
   if (ctx->timing && ctx->ev_pending.size() > 4096) timing_flush(ctx);
   return 0;

@@ -104,6 +104,7 @@ struct melt_ctx {
   unsigned char *d_gv = nullptr, *d_ge = nullptr, *d_gf = nullptr;
   double* d_lag[MELT_L_NLAG] = {};
   int* d_pind = nullptr;
+  int* d_pind2 = nullptr;        // probe indices of the fused substep, copied into d_pind after the spreading
   double* d_probe = nullptr;     // PRB_N * nf
   double *d_rxp = nullptr, *d_rxt = nullptr;  // r_x_prn_tri, r_x_tau_n (3*nf)
   double* d_red = nullptr;       // scratch for deterministic reductions
@@ -455,7 +456,10 @@ int ibm_force_drain(melt_ctx* ctx, int apply_now);
 int ibm_sub_mean(melt_ctx* ctx);
 int ibm_velforce(melt_ctx* ctx, int defer);
 int ibm_tempforce(melt_ctx* ctx);
-int ibm_probes(melt_ctx* ctx);
+// pind_out: where the heat-flux probes' cell indices go (null: d_pind, as
+// mls_heatFlux does); ibm_commit_probe_indices copies d_pind2 into d_pind.
+int ibm_probes(melt_ctx* ctx, int* pind_out = nullptr);
+int ibm_commit_probe_indices(melt_ctx* ctx);
 int ibm_window_reset(melt_ctx* ctx);
 int lagr_after_force(melt_ctx* ctx);
 int lagr_alloc(melt_ctx* ctx, const double* xyz0, const int* voe, const int* eof, const int* vof, double rad_p);

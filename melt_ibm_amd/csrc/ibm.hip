@@ -1101,7 +1101,7 @@ static int ibm_tempforce_t(melt_ctx* ctx) {  // tempforce.f90
 }
 
 template <typename R>
-static int ibm_probes_t(melt_ctx* ctx) {
+static int ibm_probes_t(melt_ctx* ctx, int* pind_out) {
   ProbeArgs<R> P;
   P.temp = fld<R>(ctx, MELT_TEMP); P.pr = fld<R>(ctx, MELT_PR); P.vx = fld<R>(ctx, MELT_VX); P.vy = fld<R>(ctx, MELT_VY); P.vz = fld<R>(ctx, MELT_VZ);
   P.bar = ctx->d_lag[MELT_L_TRI_BAR]; P.nor = ctx->d_lag[MELT_L_TRI_NOR]; P.gf = ctx->d_gf; P.nf = ctx->nf;
@@ -1111,11 +1111,28 @@ static int ibm_probes_t(melt_ctx* ctx) {
   int klo = P.heat ? 0 : 2, khi = P.loads ? 6 : 2;
   if (khi > klo) {
     size_t threads = (size_t)ctx->nf * (khi - klo) * 4;
-    hipLaunchKernelGGL(k_probes_grad<R>, dim3(nb(threads, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), P, ctx->d_probe, ctx->d_pind, klo,
+    hipLaunchKernelGGL(k_probes_grad<R>, dim3(nb(threads, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), P, ctx->d_probe,
+                       pind_out ? pind_out : ctx->d_pind, klo,
                        khi - klo, ctx->d_err);
   }
   if (P.heat || P.loads)
     hipLaunchKernelGGL(k_probes_point<R>, dim3(nb(ctx->nf, 64)), dim3(64), 0, ctx->stream, make_grid(ctx), P, ctx->d_probe, ctx->d_err);
+  MELT_HIP(hipGetLastError());
+  return 0;
+}
+
+// The heat-flux probes' indices of every live face (the 6 words the probes
+// wrote) into d_pind.
+__global__ __launch_bounds__(256) void k_commit_probe_idx(int* __restrict__ pind, const int* __restrict__ pind2, const unsigned char* __restrict__ gf,
+                                                          int nf) {
+  int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t < 6 * nf && !gf[t / 6]) pind[t] = pind2[t];
+}
+
+int ibm_commit_probe_indices(melt_ctx* ctx) {
+  if (ctx->cfg.imelt != 1) return 0;   // only the heat-flux probes (kinds 0, 1) write indices
+  hipLaunchKernelGGL(k_commit_probe_idx, dim3(nb((size_t)6 * ctx->nf, 256)), dim3(256), 0, ctx->stream, ctx->d_pind, ctx->d_pind2, ctx->d_gf,
+                     ctx->nf);
   MELT_HIP(hipGetLastError());
   return 0;
 }
@@ -1133,4 +1150,4 @@ int ibm_force_drain(melt_ctx* ctx, int apply_now) { return ctx->fp32 ? ibm_force
 int ibm_sub_mean(melt_ctx* ctx) { return ctx->fp32 ? ibm_sub_mean_t<float>(ctx) : ibm_sub_mean_t<double>(ctx); }
 int ibm_velforce(melt_ctx* ctx, int defer) { return ctx->fp32 ? ibm_velforce_t<float>(ctx, defer) : ibm_velforce_t<double>(ctx, defer); }
 int ibm_tempforce(melt_ctx* ctx) { return ctx->fp32 ? ibm_tempforce_t<float>(ctx) : ibm_tempforce_t<double>(ctx); }
-int ibm_probes(melt_ctx* ctx) { return ctx->fp32 ? ibm_probes_t<float>(ctx) : ibm_probes_t<double>(ctx); }
+int ibm_probes(melt_ctx* ctx, int* pind_out) { return ctx->fp32 ? ibm_probes_t<float>(ctx, pind_out) : ibm_probes_t<double>(ctx, pind_out); }

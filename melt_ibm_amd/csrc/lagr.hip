@@ -636,6 +636,7 @@ int lagr_alloc(melt_ctx* ctx, const double* xyz0, const int* voe, const int* eof
   rc |= dalloc((void**)&ctx->d_ge, ne);
   rc |= dalloc((void**)&ctx->d_gf, nf);
   rc |= dalloc((void**)&ctx->d_pind, sizeof(int) * 6 * nf);
+  rc |= dalloc((void**)&ctx->d_pind2, sizeof(int) * 6 * nf);
   rc |= dalloc((void**)&ctx->d_probe, sizeof(double) * PRB_N * nf);
   rc |= dalloc((void**)&ctx->d_rxp, sizeof(double) * 3 * nf);
   rc |= dalloc((void**)&ctx->d_rxt, sizeof(double) * 3 * nf);
