@@ -206,6 +206,90 @@ def measured_traffic(slot: str, n: int):
     return None
 
 
+def precision_study(args):
+    """The precision-form case of the benchmark configurations: a prolate
+    spheroid (semi-axes 0.3, 0.3, 0.6, icosphere frequency 55: 60500 faces) in
+    a 512^3 periodic cube, melting (St 2) with Newton-Euler coupling, seeded
+    solenoidal velocity and the low-wavenumber forcing field of the FP64 leg
+    (HIT synthesis and remeshing are not part of this path).  FP64 and the
+    precision form advance side by side from the same state; at every
+    checkpoint the max-norm relative error of each field is reported over the
+    whole field and away from interface cells whose tag differs (see the
+    mixed leg), with the melted-volume error and the time per substep."""
+    from scipy.ndimage import binary_dilation
+
+    from melt_ibm_amd import geometry, inputs
+    from melt_ibm_amd.api import MeltContext
+    n = args.n if args.n != 256 else 512
+    cfg, _, _ = workload(n)
+    r_eq = (0.3 * 0.3 * 0.6) ** (1.0 / 3.0)   # volume-equivalent radius: the mesh is scaled by it, inqpr uses it
+    mesh = geometry.spheroid(0.3 / r_eq, 0.6 / r_eq, 55)
+    print(f"# building the {n}^3 initial state", file=sys.stderr, flush=True)
+    init = dict(zip(("vx", "vy", "vz"), inputs.solenoidal_velocity(cfg, seed=0, kmax=4.0, rms=0.1)))
+    init.update(zip(("forcx", "forcy", "forcz"), inputs.low_mode_forcing(cfg, seed=1, kf=2.5, amplitude=args.study_forcing)))
+    dt = inputs.cfl_dt(cfg, init["vx"], init["vy"], init["vz"], cfl=args.study_cfl)
+    ctxs = []
+    for prec in (0, 1):
+        cfg.precision = prec
+        c = MeltContext(cfg)
+        c.set_geometry(mesh.xyz, mesh.vert_of_edge, mesh.edge_of_face, r_eq)
+        c.inqpr()
+        for name, a in init.items():
+            c.upload(name, a)
+        c.set_dt(dt)
+        c.tagCells()
+        c.sync()
+        ctxs.append(c)
+        print(f"# precision {prec} set up: N={n} faces={mesh.edge_of_face.shape[0]} dt={dt:.4e}", file=sys.stderr, flush=True)
+    del init
+    qs = ("vx", "vy", "vz", "temp", "pr")
+    t_run = [0.0, 0.0]
+    checkpoints = []
+    done = 0
+    from melt_ibm_amd.api import MeltError
+    stopped = None
+    while done < args.study_steps:
+        k = min(args.study_every, args.study_steps - done)
+        try:
+            for p_, c in enumerate(ctxs):
+                c.sync()
+                t0 = time.perf_counter()
+                for _ in range(k):
+                    c.tschem()
+                c.sync()
+                t_run[p_] += time.perf_counter() - t0
+        except MeltError as e:   # a terminal condition of the reference (e.g. the body left its window)
+            stopped = {"after_step": done, "precision": ("fp64", "mixed")[p_], "error": str(e)}
+            break
+        done += k
+        f = [{q: c.download(q) for q in qs + ("VOFp",)} for c in ctxs]
+        flip = np.abs(f[1]["VOFp"] - f[0]["VOFp"]) > 1e-6
+        near = binary_dilation(flip, iterations=2, border_value=0) if flip.any() else flip
+        cp = {"step": done, "tag_flip_cells": int(flip.sum()), "field_rel_err": {}, "field_rel_err_off_tag_flips": {}}
+        for q in qs:
+            d = np.abs(f[1][q] - f[0][q])
+            ref = max(float(np.max(np.abs(f[0][q]))), 1e-300)
+            cp["field_rel_err"][q] = float(f"{float(d.max()) / ref:.3e}")
+            cp["field_rel_err_off_tag_flips"][q] = float(f"{float(np.where(near, 0.0, d).max()) / ref:.3e}")
+        v0, v1 = ctxs[0].body().volume, ctxs[1].body().volume
+        cp["volume_rel_err"] = float(f"{abs(v1 - v0) / v0:.3e}")
+        cp["fp64_umax"] = float(f"{max(float(np.max(np.abs(f[0][q]))) for q in ('vx', 'vy', 'vz')):.4f}")
+        cp["fp64_cfl"] = float(f"{ctxs[0].cfl() * dt:.3f}")
+        checkpoints.append(cp)
+        del f, flip, near
+        print(f"# step {done}: {cp}", file=sys.stderr, flush=True)
+    stats = ctxs[1].refinement_stats()
+    for c in ctxs:
+        c.close()
+    sub = [t / (3 * max(done, 1)) * 1e3 for t in t_run]
+    print(json.dumps({"study": "precision form vs FP64", "grid": [n, n, n], "faces": int(mesh.edge_of_face.shape[0]),
+                      "forcing_amplitude": args.study_forcing, "cfl": args.study_cfl, "dt": dt,
+                      "solid": "prolate spheroid, semi-axes 0.3/0.3/0.6, icosphere frequency 55, melting St 2, Newton-Euler on, no remeshing",
+                      "steps": done, "substep_ms": {"fp64": round(sub[0], 4), "mixed": round(sub[1], 4)},
+                      "speedup": round(sub[0] / sub[1], 3), "refinement": {"checks": stats[0], "refined": stats[1], "last_ratio": stats[2]},
+                      "checkpoints": checkpoints, "stopped": stopped}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -225,7 +309,16 @@ def main():
     ap.add_argument("--command", default="", help="command line recorded in a summary")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run plain device copies of 1 GiB (calibrates the HBM byte counters of a profiler)")
+    ap.add_argument("--precision-study", action="store_true",
+                    help="precision form vs FP64 side by side on the 512^3 spheroid case (errors per field over --study-steps)")
+    ap.add_argument("--study-steps", type=int, default=100)
+    ap.add_argument("--study-every", type=int, default=20, help="steps between error checkpoints")
+    ap.add_argument("--study-forcing", type=float, default=0.5, help="amplitude of the low-wavenumber forcing field")
+    ap.add_argument("--study-cfl", type=float, default=0.5, help="CFL number of the fixed dt")
     args = ap.parse_args()
+    if args.precision_study:
+        precision_study(args)
+        return
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc, args.command, PROFILES / "hbm_traffic_256cube_melting_sphere.md", TRAFFIC_JSON)
         return
