@@ -750,45 +750,56 @@ struct GreenBatch {
   Green gr[2];
 };
 
-// A block is 64 consecutive i (coalesced rows) x 4 segments of GQ points of
-// the line; each thread slides a register window of GQ + 2 KT values along
-// its segment, so every input is read about (GQ + 2 KT) / GQ times (the halo
-// rows of neighbouring segments mostly from L2).  Coefficients beyond the
-// system's K are zero.  Needs n1 % 64 == 0 and N % (4 GQ) == 0.
+// A block is 64 consecutive i (coalesced rows) x 4 segments of NCH chunks of
+// GQ points of the line; each thread slides a register window of GQ + 2 KT
+// values along its segment, GQ new values per chunk, so every input is read
+// about (NCH GQ + 2 KT) / (NCH GQ) times (the halo rows of neighbouring
+// segments mostly from L2).  Coefficients beyond the system's K are zero.
+// Needs n1 % 64 == 0 and N % (4 GQ NCH) == 0.
 constexpr int GQ = 16;
 
-template <int DIR, typename R, int KT>
+template <int DIR, typename R, int KT, int NCH>
 __global__ __launch_bounds__(256) void k_green(GreenBatch<R> B, int n1, int n2, int n3) {
   const int f = blockIdx.y;
   const int N = DIR == 1 ? n2 : n3;
   const int s = DIR == 1 ? n1 : n1 * n2;
-  const int nsg = N / (4 * GQ), nit = n1 / 64;
+  const int nsg = N / (4 * GQ * NCH), nit = n1 / 64;
   unsigned b = xcd_block(blockIdx.x, gridDim.x);
   const int sg = (int)(b % nsg), rest = (int)(b / nsg);
   const int it = rest % nit, other = rest / nit;   // other: k (DIR 1) or j (DIR 2)
   const int i = it * 64 + (threadIdx.x & 63);
-  const int p0 = (sg * 4 + (threadIdx.x >> 6)) * GQ;
+  const int p0 = (sg * 4 + (threadIdx.x >> 6)) * GQ * NCH;
   const int base = i + (DIR == 1 ? n1 * n2 * other : n1 * other);
   const Green& G = B.gr[B.which[f]];
   double g[KT + 1];
 #pragma unroll
   for (int m = 0; m <= KT; ++m) g[m] = m <= G.K ? G.g[m] : 0.0;
   const R* __restrict__ d = B.src[f];
-  double v[GQ + 2 * KT];
-#pragma unroll
-  for (int q = 0; q < GQ + 2 * KT; ++q) {
-    int p = p0 - KT + q;
+  auto at = [&](int p) {
     p = p < 0 ? p + N : (p >= N ? p - N : p);
-    v[q] = (double)d[base + p * s];
-  }
+    return (double)d[base + p * s];
+  };
+  double v[GQ + 2 * KT];   // window of positions pc - KT .. pc + GQ + KT - 1
 #pragma unroll
-  for (int q = 0; q < GQ; ++q) {
-    double x = g[0] * v[q + KT];
+  for (int q = 0; q < GQ + 2 * KT; ++q) v[q] = at(p0 - KT + q);
 #pragma unroll
-    for (int m = 1; m <= KT; ++m) x += g[m] * (v[q + KT - m] + v[q + KT + m]);
-    const int c = base + (p0 + q) * s;
-    if (B.qadd[f]) B.qadd[f][c] = B.qadd[f][c] + (R)x;
-    else B.dst[f][c] = (R)x;
+  for (int ch = 0; ch < NCH; ++ch) {
+    const int pc = p0 + ch * GQ;
+#pragma unroll
+    for (int q = 0; q < GQ; ++q) {
+      double x = g[0] * v[q + KT];
+#pragma unroll
+      for (int m = 1; m <= KT; ++m) x += g[m] * (v[q + KT - m] + v[q + KT + m]);
+      const int c = base + (pc + q) * s;
+      if (B.qadd[f]) B.qadd[f][c] = B.qadd[f][c] + (R)x;
+      else B.dst[f][c] = (R)x;
+    }
+    if (ch + 1 < NCH) {   // slide by GQ: the last 2 KT values become the head
+#pragma unroll
+      for (int q = 0; q < 2 * KT; ++q) v[q] = v[q + GQ];
+#pragma unroll
+      for (int q = 0; q < GQ; ++q) v[2 * KT + q] = at(pc + GQ + KT + q);
+    }
   }
 }
 
@@ -1107,11 +1118,17 @@ int launch_green(melt_ctx* ctx, int dir, void* const* src, void* const* dst, voi
   int KT = kmax <= 4 ? 4 : kmax <= 8 ? 8 : 12;
   // KT < N / 2 keeps every window inside one period
   if (ctx->n1 % 64 || N % (4 * GQ) || 2 * KT >= N) return 0;
-  const int nsg = N / (4 * GQ), nit = ctx->n1 / 64, other = dir == 1 ? ctx->n3 : ctx->n2;
+  // chunks per thread along z (the longest segment that tiles the line; along
+  // y single chunks measured faster, the extra blocks outweigh the halo)
+  const int nch = dir == 1 ? 1 : N % (16 * GQ) == 0 ? 4 : N % (8 * GQ) == 0 ? 2 : 1;
+  const int nsg = N / (4 * GQ * nch), nit = ctx->n1 / 64, other = dir == 1 ? ctx->n3 : ctx->n2;
   dim3 grid((unsigned)(nsg * nit * other), nfield);
-#define GREEN_CASE(D, KK) hipLaunchKernelGGL((k_green<D, R, KK>), grid, dim3(256), 0, ctx->stream, B, ctx->n1, ctx->n2, ctx->n3)
-  if (dir == 1) { if (KT == 4) GREEN_CASE(1, 4); else if (KT == 8) GREEN_CASE(1, 8); else GREEN_CASE(1, 12); }
-  else { if (KT == 4) GREEN_CASE(2, 4); else if (KT == 8) GREEN_CASE(2, 8); else GREEN_CASE(2, 12); }
+#define GREEN_CASE(D, KK, NC) hipLaunchKernelGGL((k_green<D, R, KK, NC>), grid, dim3(256), 0, ctx->stream, B, ctx->n1, ctx->n2, ctx->n3)
+#define GREEN_NCH(D, KK) \
+  do { if (nch == 4) GREEN_CASE(D, KK, 4); else if (nch == 2) GREEN_CASE(D, KK, 2); else GREEN_CASE(D, KK, 1); } while (0)
+  if (dir == 1) { if (KT == 4) GREEN_NCH(1, 4); else if (KT == 8) GREEN_NCH(1, 8); else GREEN_NCH(1, 12); }
+  else { if (KT == 4) GREEN_NCH(2, 4); else if (KT == 8) GREEN_NCH(2, 8); else GREEN_NCH(2, 12); }
+#undef GREEN_NCH
 #undef GREEN_CASE
   MELT_HIP(hipGetLastError());
   *used = 1;

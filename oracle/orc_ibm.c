@@ -435,6 +435,9 @@ int orc_tagcells(orc_ctx* o) {  /* tagCells.f90:4-27 (tagType 1) */
     if (rc) return rc;
   }
   narrowBandTagging(o);
+  /* gcurv.f90:61-63: the first call (ray tagging of every cell) happens with
+     initial_tag set, every later one tags the narrow band only */
+  o->body.initial_tag = 0;
   return o->status;
 }
 

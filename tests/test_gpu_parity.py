@@ -111,9 +111,13 @@ def test_pressure_solve_and_projection(gpu_lib, n, n3):
     assert g.divgck() < 1e-11 and o.divgck() < 1e-11
 
 
-@pytest.mark.parametrize("n,level,rad", [(32, 2, 1.2), (48, 3, 1.2), (64, 3, 0.6)])
-def test_tagging_bit_exact(gpu_lib, n, level, rad):
-    cfg = make_cfg(n)
+@pytest.mark.parametrize("n,n3,level,rad", [(32, 32, 2, 1.2), (48, 48, 3, 1.2), (64, 64, 3, 0.6), (32, 64, 2, 1.0)])
+def test_tagging_bit_exact(gpu_lib, n, n3, level, rad):
+    """Initial tagging (ray casting + narrow band), then a move and a re-tag:
+    after the first call only the narrow band is re-tagged and VOFp keeps its
+    history elsewhere (tagCells.f90:18-22 with initial_tag cleared after the
+    first call, gcurv.f90:61-63); on anisotropic cells too."""
+    cfg = make_cfg(n, n3)
     g, o = pair(cfg, gpu_lib)
     for c in (g, o):
         with_body(c, level, rad)
@@ -267,13 +271,15 @@ def test_pencil_overflow_is_reported(gpu_lib):
         assert ei.value.code == 5
 
 
-def test_green_line_solves_at_64(gpu_lib, monkeypatch):
-    """At 64^3 the fused substep solves y and z (and x inside the explicit
-    pass) with the truncated Green's function; it agrees with the oracle's
-    Thomas solves, and with the library's own cyclic-reduction path
-    (MELT_TUNE green=0), to round-off."""
-    n, level, rad = 64, 3, 1.0
-    cfg = make_cfg(n)
+@pytest.mark.parametrize("n,n3", [(64, 64), (128, 256)])
+def test_green_line_solves(gpu_lib, monkeypatch, n, n3):
+    """The fused substep solves y and z (and x inside the explicit pass) with
+    the truncated Green's function (at 128 x 128 x 256 the z filter runs 4
+    window chunks per thread); it agrees with the oracle's Thomas
+    solves, and with the library's own cyclic-reduction path (MELT_TUNE
+    green=0), to round-off."""
+    level, rad = 3, 1.0
+    cfg = make_cfg(n, n3)
     st, dt = melt_case(cfg, level, rad, seed=9)
     from melt_ibm_amd.inputs import inqpr_temperature
     ctxs = []
