@@ -14,6 +14,9 @@ term, a wrong sign or neighbour index, a transposed operand) fails it:
  - tagging: narrow-band |sdf| equals a brute-force point-triangle distance over
    all faces; ray tags agree with an inside test of the polyhedron;
  - MLS: partition of unity, linear reproduction, zero force for linear fields;
+ - probes: structural loads, heat flux and the Stefan update against closed
+   forms for linear fields (divergence theorem, n.grad T / Pe, area-weighted
+   vertex means) and the latent-heat energy balance of a melting body;
  - geometry / rigid body: closed-form polyhedron volume, area and inertia;
    Newton-Euler invariants.
 """
@@ -382,3 +385,121 @@ def test_solenoidal_inputs():
     div = (np.roll(u, -1, 2) - u) * d1 + (np.roll(v, -1, 1) - v) * d2 + (np.roll(w, -1, 0) - w) * d3
     assert np.max(np.abs(div)) < 1e-13
     assert abs(np.sqrt((u ** 2 + v ** 2 + w ** 2).mean() / 3) - 0.1) < 1e-12
+
+
+# ------------------------------------------------ probe chain: loads, heat flux, Stefan
+# The gradient probes (mls_heatFlux.f90:131-141, mls_structLoads.f90:457-465) take
+# the coordinates of their third z layer from the second layer (k_inds =
+# [k1-1, k1, k1]) while reading field values from the third, so they reproduce
+# exactly only fields that do not vary along z; the pressure value probe
+# (mls_structLoads.f90:319-323) uses the true coordinates.  The closed forms
+# below therefore use fields linear in x and y.  The MLS moment matrix is built
+# on absolute coordinates ([1, x, y, z] with x near 3 and a 0.13 spacing) and
+# the gradient applies its inverse twice, so probe values carry round-off of
+# about 5e-8 relative (zero-mean over the faces, whatever the field's constant
+# part); the tolerances are 1e-6 relative, far below any slip in a term.
+def _close(a, b, rel=1e-6):
+    return np.max(np.abs(a - b)) <= rel * np.max(np.abs(b))
+
+
+def _xy_fields(cfg):
+    (xm, ym, zm), (xc, yc, zc) = inputs.grid_coords(cfg)
+    sh = cfg.shape
+
+    def lin(c, xs, ys):  # c0 + c1 x + c2 y on the [k, j, i] grid
+        return np.broadcast_to(c[0] + c[1] * xs[None, None, :] + c[2] * ys[None, :, None], sh).copy()
+    return (xm, ym, xc, yc), lin
+
+
+def _probe_case(**kw):
+    cfg = make_cfg(48, **kw)
+    o = oracle.context(cfg)
+    m = with_body(o, 3, 1.0)
+    o.set_dt(1e-3)
+    o.set_stage(1)
+    z = np.zeros(cfg.shape)
+    load_state(o, {"vx": z, "vy": z, "vz": z, "pr": z, "temp": z})
+    return cfg, o, m
+
+
+def test_struct_loads_closed_forms(lib):
+    cfg, o, m = _probe_case(imelt=0, imlsstr=1)
+    (xm, ym, xc, yc), lin = _xy_fields(cfg)
+    G = (0.7, -1.3)
+    A = np.array([[0.4, -0.9], [1.1, 0.25], [-0.6, 0.8]])  # d(u,v,w)/d(x,y)
+    o.upload("pr", lin((0.2, *G), xm, ym))
+    o.upload("vx", lin((0.1, *A[0]), xc, ym))
+    o.upload("vy", lin((-0.2, *A[1]), xm, yc))
+    o.upload("vz", lin((0.3, *A[2]), xm, ym))
+    vol0 = o.body().volume
+    nor, bar, sur = o.lag("tri_nor"), o.lag("tri_bar"), o.lag("sur")
+    o.particle()
+    # pressure extrapolated back from the probe to the face: p(bar) n dA
+    p_bar = 0.2 + G[0] * bar[:, 0] + G[1] * bar[:, 1]
+    assert _close(o.lag("press_n_tri"), (p_bar * sur)[:, None] * nor)
+    # divergence theorem: sum p n dA = grad(p) V for a linear p (exact on flat faces)
+    b = o.body()
+    assert np.max(np.abs(np.array(b.int_prn_dA[:]) - np.array([G[0], G[1], 0.0]) * vol0)) < 1e-6 * vol0
+    # viscous traction (S + S^T) n dA / Re with the constant velocity gradient S
+    S = np.zeros((3, 3))
+    S[:, :2] = A
+    tau = (sur / cfg.ren)[:, None] * (nor @ (S + S.T).T)
+    assert _close(o.lag("tau_n"), tau)
+    assert np.max(np.abs(np.array(b.int_tau_dA[:]))) < 1e-6 * np.sum(np.abs(tau))  # closed surface: sum n dA = 0
+
+
+def test_heat_flux_linear_temperature(lib):
+    cfg, o, m = _probe_case(imelt=1, imlsstr=0)
+    (xm, ym, _, _), lin = _xy_fields(cfg)
+    g = (0.8, -0.45)
+    o.upload("temp", lin((0.3, *g), xm, ym))
+    nor, sur, avert = o.lag("tri_nor"), o.lag("sur"), o.lag("Avert")
+    f = m.faces() - 1
+    # vertex areas: a third of each adjacent face, summing to the surface
+    a3 = np.zeros(o.nv)
+    np.add.at(a3, f.ravel(), np.repeat(sur / 3.0, 3))
+    assert np.max(np.abs(avert - a3)) < 1e-15 and abs(avert.sum() - sur.sum()) < 1e-12
+    o.particle()
+    pec = cfg.ren * cfg.prandtl
+    q = (g[0] * nor[:, 0] + g[1] * nor[:, 1]) / pec  # n . grad T / Pe on both sides
+    assert _close(o.lag("qw_o"), q) and _close(o.lag("qw_i"), q)
+    # face -> vertex: the area-weighted mean of the adjacent faces' fluxes
+    num = np.zeros(o.nv)
+    np.add.at(num, f.ravel(), np.repeat(q * sur, 3))
+    qv = num / (3.0 * a3)
+    assert _close(o.lag("qw_oVert"), qv) and _close(o.lag("qw_iVert"), qv)
+
+
+def test_stefan_time_integration_and_sign(lib):
+    # equal fluxes on both sides: the new melt velocity vanishes and the vertices
+    # move by dt * rho_2 * (previous melt velocity) at the second RK3 stage
+    cfg, o, m = _probe_case(imelt=1, imlsstr=0)
+    (xm, ym, _, _), lin = _xy_fields(cfg)
+    o.upload("temp", lin((0.3, 0.8, -0.45), xm, ym))
+    o.set_stage(2)
+    rng = np.random.default_rng(5)
+    vprev = rng.normal(0, 0.1, (o.nv, 3))
+    o.set_lag("vmelt", vprev)
+    x0 = o.lag("xyzv")
+    o.particle()
+    ro = -17.0 / 60.0
+    qscale = 0.8 / (cfg.ren * cfg.prandtl) * cfg.cpliquid / cfg.latheat
+    assert np.max(np.abs(o.lag("vmelt"))) < 1e-6 * qscale
+    assert np.array_equal(o.lag("vmelt_m1"), vprev)
+    assert np.max(np.abs(o.lag("xyzv") - (x0 + 1e-3 * ro * vprev))) < 1e-3 * 1e-6 * qscale + 1e-15
+
+    # a body in fluid that warms away from it melts: the volume lost over one
+    # stage matches the latent heat of the net flux into the body,
+    # dV = dt gamma_1 (cp / L) sum_v Avert (qw_i - qw_o)
+    cfg, o, m = _probe_case(imelt=1, imlsstr=0)
+    (xm, ym, zm), _ = inputs.grid_coords(cfg)
+    r = np.sqrt((xm[None, None, :] - np.pi) ** 2 + (ym[None, :, None] - np.pi) ** 2 + (zm[:, None, None] - np.pi) ** 2)
+    o.upload("temp", 2.0 * np.maximum(r - 1.0, 0.0))
+    v0 = o.body().volume
+    avert = o.lag("Avert")
+    o.particle()
+    dv = o.body().volume - v0
+    net = np.sum(avert * (o.lag("qw_iVert") - o.lag("qw_oVert")))
+    assert net < 0 and dv < 0
+    expect = 1e-3 * (8.0 / 15.0) * cfg.cpliquid / cfg.latheat * net
+    assert abs(dv - expect) < 0.02 * abs(expect)
