@@ -37,8 +37,8 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E bytes/s
 # pass: each field it must read or write, once.
 KERNEL_BYTES = {
     "explicit_xsolve": (13 + 8) * 8,   # u,v,w,T,p, ru x4, VOFp, forc x3 -> x-solved rhs x4, ru x4
-    "solve_y": (4 + 4) * 8,
-    "solve_z_update": (4 + 4 + 4) * 8,  # rhs x4, q x4 -> q x4
+    "solve_y": (4 + 4 + 4) * 8,         # y and z solves in one pass: rhs x4, q x4 -> q x4
+    "solve_z_update": (4 + 4 + 4) * 8,  # (two-pass form only) tmp x4, q x4 -> q x4
     "divg_fft_x": (3 + 1) * 8,
     "fft_y": 2 * 8,
     "fft_z_solve": 2 * 8,
@@ -189,7 +189,7 @@ def summarize_trace(db: str, command: str, out_md: Path, top: int = 30) -> None:
     out_md.write_text("\n".join(lines + [""]))
 
 
-KERNEL_SYMBOL = {"explicit_xsolve": "k_explicit_zmarch", "solve_y": "k_green<1", "solve_z_update": "k_green<2",
+KERNEL_SYMBOL = {"explicit_xsolve": "k_explicit_zmarch", "solve_y": "k_green_yz", "solve_z_update": "k_green<2",
                  "divg_fft_x": "k_fft_x_fwd_reg", "fft_y": "k_fft_y_reg", "fft_z_solve": "k_fft_z_solve_reg", "ifft_x": "k_fft_x_bwd_reg",
                  "project": "k_project"}
 

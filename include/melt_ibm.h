@@ -224,8 +224,10 @@ int melt_graph_enable(melt_ctx* ctx, int32_t on);
  * x-lines per block of the fused explicit/x-solve pass, 1 = its threads per
  * block (either one selects the line-tile form of that pass instead of the
  * z-march), 2 = lines per block of the y/z line solves, 3 = their threads per
- * block, 4 = planes per block of the z-marching explicit pass; value 0 =
- * built-in choice.  Results do not depend on it. */
+ * block, 4 = planes per block of the z-marching explicit pass, 5 = j-rows per
+ * block (8 or 16) of the pass that does the y and z solves together, 1 there
+ * = two separate passes; value 0 = built-in choice.  Results do not depend on
+ * it. */
 int melt_set_tuning(melt_ctx* ctx, int32_t key, int32_t value);
 void* melt_stream(melt_ctx* ctx);
 int melt_get_refinement_stats(melt_ctx* ctx, int64_t* checks, int64_t* refined, double* last_ratio);

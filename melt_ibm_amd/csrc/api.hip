@@ -160,7 +160,7 @@ extern "C" int melt_ctx_create(const melt_config* cfg, melt_ctx** out) {
   do {
     if (hipGetDevice(&ctx->device) != hipSuccess) { rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "no HIP device"); break; }
     if (const char* tv = getenv("MELT_TUNE")) {
-      static const char* keys[TUNE_N] = {"xf_w", "xf_nt", "solve_w", "solve_nt", "xf_z"};
+      static const char* keys[TUNE_N] = {"xf_w", "xf_nt", "solve_w", "solve_nt", "xf_z", "yz_jb"};
       std::string spec(tv);
       size_t pos = 0;
       while (pos < spec.size()) {
@@ -586,21 +586,24 @@ extern "C" int melt_substep(melt_ctx* ctx, int32_t ns) {
   timing_end(ctx, SLOT_EXPLICIT, e);
   void* r[4] = {ctx->rhs4[RHS_U], ctx->rhs4[RHS_V], ctx->rhs4[RHS_W], ctx->rhs4[RHS_T]};
   void* q[4] = {ctx->f[MELT_VX], ctx->f[MELT_VY], ctx->f[MELT_VZ], ctx->f[MELT_TEMP]};
-  double b[4];
-  // y: Green filter rhs -> tmp (or cyclic reduction in place)
+  double b[4], bz[4];
   for (int m = 0; m < 4; ++m) b[m] = betadx_q(ctx, m, 1);
+  for (int m = 0; m < 4; ++m) bz[m] = betadx_q(ctx, m, 2);
+  // y and z in one pass: q += Gz Gy rhs
   timing_begin(ctx, SLOT_SOLVE_Y, &e);
+  int used_yz = 0;
+  if ((rc = euler_launch_green_yz(ctx, r, q, b, bz, 4, &used_yz))) return rc;
+  // otherwise y: Green filter rhs -> tmp (or cyclic reduction in place)
   int used_y = 0;
-  if ((rc = euler_launch_green(ctx, 1, r, ctx->tmp4, nullptr, b, 4, &used_y))) return rc;
-  if (!used_y && (rc = euler_launch_solve(ctx, 1, r, 4, b, nullptr, 0))) return rc;
+  if (!used_yz && (rc = euler_launch_green(ctx, 1, r, ctx->tmp4, nullptr, b, 4, &used_y))) return rc;
+  if (!used_yz && !used_y && (rc = euler_launch_solve(ctx, 1, r, 4, b, nullptr, 0))) return rc;
   timing_end(ctx, SLOT_SOLVE_Y, e);
   // z: q += solution (solxk.f90:44-50)
   void* const* zsrc = used_y ? ctx->tmp4 : r;
-  for (int m = 0; m < 4; ++m) b[m] = betadx_q(ctx, m, 2);
   timing_begin(ctx, SLOT_SOLVE_Z, &e);
   int used_z = 0;
-  if ((rc = euler_launch_green(ctx, 2, zsrc, nullptr, q, b, 4, &used_z))) return rc;
-  if (!used_z && (rc = euler_launch_solve(ctx, 2, zsrc, 4, b, q, 0))) return rc;
+  if (!used_yz && (rc = euler_launch_green(ctx, 2, zsrc, nullptr, q, bz, 4, &used_z))) return rc;
+  if (!used_yz && !used_z && (rc = euler_launch_solve(ctx, 2, zsrc, 4, bz, q, 0))) return rc;
   timing_end(ctx, SLOT_SOLVE_Z, e);
   timing_end(ctx, SLOT_PH_EULER, e_ph);
 

@@ -52,7 +52,7 @@ enum {
 
 // Launch-shape overrides (0 = built-in choice), set from MELT_TUNE at
 // context creation ("xf_w=4,solve_w=16,...") for tile sweeps.
-enum melt_tune { TUNE_XF_W = 0, TUNE_XF_NT, TUNE_SOLVE_W, TUNE_SOLVE_NT, TUNE_XF_Z, TUNE_N };
+enum melt_tune { TUNE_XF_W = 0, TUNE_XF_NT, TUNE_SOLVE_W, TUNE_SOLVE_NT, TUNE_XF_Z, TUNE_YZ_JB, TUNE_N };
 
 struct melt_ctx {
   melt_config cfg;
@@ -437,6 +437,7 @@ int euler_launch_explicit_xsolve(melt_ctx* ctx, double bxu, double bxt);
 int euler_launch_divg(melt_ctx* ctx);
 int euler_launch_green(melt_ctx* ctx, int dir, void* const* src, void* const* dst, void* const* qadd, const double* betadx, int nfield,
                        int* used);
+int euler_launch_green_yz(melt_ctx* ctx, void* const* src, void* const* qadd, const double* by, const double* bz, int nfield, int* used);
 int euler_launch_project(melt_ctx* ctx, int do_u, int do_p, int use_mean, const int* gate = nullptr);
 int euler_launch_reduce_max(melt_ctx* ctx, int what, unsigned long long* d);
 int euler_launch_refine_gate(melt_ctx* ctx, const unsigned long long* dmax, int* gate);

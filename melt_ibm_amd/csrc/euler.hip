@@ -803,6 +803,116 @@ __global__ __launch_bounds__(256) void k_green(GreenBatch<R> B, int n1, int n2, 
   }
 }
 
+// y and z solves of one field in one pass, q += Gz * (Gy * rhs) (solxj.f90,
+// solxk.f90:44-50): the y filter of every plane is taken from an LDS tile and
+// fed to the z filter's register window, so the y result never goes to HBM.
+template <typename R>
+struct GreenYZ {
+  const R* src[4];
+  R* qadd[4];
+  int which[4];
+  Green gy[2], gz[2];
+};
+
+// A block is 64 consecutive i x JB rows j, one (i, j) column per thread, and
+// marches through one z segment of GQ NCH planes (plus KT halo planes each
+// side).  Per plane the block stages rows j0-KT .. j0+JB+KT-1 in one of two
+// LDS tiles (one barrier per plane), each thread filters its column along y,
+// rounds to R (the value the separate y pass would store) and shifts it into a
+// register window of 2 KT + 1 planes, on which the z filter runs as in
+// k_green.  The loads run PF planes ahead of the plane being filtered (a ring
+// of PF register sets, indexed statically by unrolling PF planes at a time).
+// Needs n1 % 64 == 0, n2 % JB == 0, n3 % seg == 0, seg % PF == 0.
+template <typename R, int KT, int JB>
+__global__ __launch_bounds__(64 * JB) void k_green_yz(GreenYZ<R> B, int n1, int n2, int n3, int seg) {
+  constexpr int ROWS = JB + 2 * KT, RPT = (ROWS + JB - 1) / JB, W = 2 * KT + 1, PF = 2;
+  static_assert((2 * KT) % PF == 0, "prefetch ring must tile the halo planes");
+  __shared__ double tile[2][ROWS * 64];
+  const int f = blockIdx.y;
+  const int nsg = n3 / seg, njt = n2 / JB;
+  unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  const int sg = (int)(b % nsg), rest = (int)(b / nsg);
+  const int jt = rest % njt, it = rest / njt;
+  const int lane = threadIdx.x & 63, jl = threadIdx.x >> 6;
+  const int i = it * 64 + lane, j0 = jt * JB, p0 = sg * seg;
+  const int wy = B.which[f];
+  // the coefficients are uniform over the block: kept in scalar registers
+  auto uni = [](double v) {
+    long long l = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readfirstlane((int)(l & 0xffffffff)), hi = __builtin_amdgcn_readfirstlane((int)(l >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+  };
+  double gy[KT + 1], gz[KT + 1];
+#pragma unroll
+  for (int m = 0; m <= KT; ++m) {
+    gy[m] = uni(m <= B.gy[wy].K ? B.gy[wy].g[m] : 0.0);
+    gz[m] = uni(m <= B.gz[wy].K ? B.gz[wy].g[m] : 0.0);
+  }
+  // no __restrict__: loads of src must stay behind the barriers, or the
+  // compiler hoists the prefetches of many planes and runs out of registers
+  const R* d = B.src[f];
+  R* q = B.qadd[f];
+  const size_t plane = (size_t)n1 * n2;
+  int roff[RPT];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    int jj = j0 - KT + jl + r * JB;
+    jj = jj < 0 ? jj + n2 : (jj >= n2 ? jj - n2 : jj);
+    roff[r] = jj * n1 + i;
+  }
+  R nx[PF][RPT];
+  auto fetch = [&](int slot, int p) {
+    p = p < 0 ? p + n3 : (p >= n3 ? p - n3 : p);
+    const R* dp = d + (size_t)p * plane;
+#pragma unroll
+    for (int r = 0; r < RPT; ++r)
+      if (jl + r * JB < ROWS) nx[slot][r] = dp[roff[r]];
+  };
+  int buf = 0;
+  int pn = p0 - KT;   // next plane to filter; its values sit in ring slot (pn - p0 + KT) % PF
+  const int pend = p0 + seg + KT;
+#pragma unroll
+  for (int s = 0; s < PF; ++s) fetch(s, pn + s);
+  double w[W];   // y-filtered planes p - KT .. p + KT around the output plane p
+  // shifts the window by one plane and appends the y filter of plane pn for
+  // this thread's column, then refills the ring slot with plane pn + PF
+  auto advance = [&](int slot) {
+#pragma unroll
+    for (int m = 0; m + 1 < W; ++m) w[m] = w[m + 1];
+    double* T = tile[buf];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r)
+      if (jl + r * JB < ROWS) T[(jl + r * JB) * 64 + lane] = (double)nx[slot][r];
+    __syncthreads();
+    if (pn + PF < pend) fetch(slot, pn + PF);
+    const double* L = T + (jl + KT) * 64 + lane;
+    double x = gy[0] * L[0];
+#pragma unroll
+    for (int m = 1; m <= KT; ++m) x += gy[m] * (L[-m * 64] + L[m * 64]);
+    w[W - 1] = (double)(R)x;
+    buf ^= 1;
+    ++pn;
+  };
+#pragma unroll 1
+  for (int s = 0; s < 2 * KT; s += PF) {
+#pragma unroll
+    for (int u = 0; u < PF; ++u) advance(u);
+  }
+  const size_t col = (size_t)j0 * n1 + (size_t)jl * n1 + i;
+#pragma unroll 1
+  for (int p = p0; p < p0 + seg; p += PF) {
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      advance(u);
+      double x = gz[0] * w[KT];
+#pragma unroll
+      for (int m = 1; m <= KT; ++m) x += gz[m] * (w[KT - m] + w[KT + m]);
+      const size_t c = (size_t)(p + u) * plane + col;
+      q[c] = q[c] + (R)x;
+    }
+  }
+}
+
 // Coefficients of the truncated Green's function of the normalised system
 // with off-diagonal ratio G = betadx on N points, scaled by `scale`; false
 // when more than FIR_KMAX terms are needed for relative truncation `tol`.
@@ -1135,6 +1245,59 @@ int launch_green(melt_ctx* ctx, int dir, void* const* src, void* const* dst, voi
   return 0;
 }
 
+// The y and z solves of the fused substep as one pass (k_green_yz): returns 0
+// with *used = 0 (launching nothing) when the shapes or the systems do not fit,
+// and the caller runs the two passes instead.  Tuning key yz_jb: 0 default
+// (8 rows per block), 8 or 16 rows, 1 = always two passes.
+constexpr int YZ_SEG_MAX = 256;   // whole lines up to 256 planes measured fastest
+
+template <typename R>
+int launch_green_yz(melt_ctx* ctx, void* const* src, void* const* qadd, const double* by, const double* bz, int nfield, int* used) {
+  *used = 0;
+  const int jb = ctx->tune[TUNE_YZ_JB] == 0 ? 8 : ctx->tune[TUNE_YZ_JB];
+  if (ctx->no_green || (jb != 8 && jb != 16)) return 0;
+  const int n1 = ctx->n1, n2 = ctx->n2, n3 = ctx->n3;
+  const double tol = ctx->fp32 ? 1e-8 : 2e-17;
+  GreenYZ<R> B{};
+  int ng = 0;
+  double seen[2][2] = {};
+  for (int f = 0; f < nfield; ++f) {
+    int w = -1;
+    for (int g = 0; g < ng; ++g)
+      if (seen[g][0] == by[f] && seen[g][1] == bz[f]) w = g;
+    if (w < 0) {
+      if (ng == 2) return 0;
+      if (!green_coeffs(by[f], n2, tol, 1.0 / (1.0 + 2.0 * by[f]), B.gy[ng]) ||
+          !green_coeffs(bz[f], n3, tol, 1.0 / (1.0 + 2.0 * bz[f]), B.gz[ng]))
+        return 0;
+      seen[ng][0] = by[f];
+      seen[ng][1] = bz[f];
+      w = ng++;
+    }
+    B.which[f] = w;
+    B.src[f] = static_cast<const R*>(src[f]);
+    B.qadd[f] = static_cast<R*>(qadd[f]);
+  }
+  int kmax = 0;
+  for (int g = 0; g < ng; ++g) kmax = std::max(kmax, std::max(B.gy[g].K, B.gz[g].K));
+  const int KT = kmax <= 4 ? 4 : kmax <= 8 ? 8 : 12;
+  if (n1 % 64 || n2 % jb || n3 % GQ || 2 * KT >= n2 || 2 * KT >= n3) return 0;
+  // planes per z segment: the longest of YZ_SEG_MAX, 64, 32, 16 that tiles the line
+  int seg = GQ;
+  for (int s : {YZ_SEG_MAX, 64, 32})
+    if (n3 % s == 0) { seg = s; break; }
+  dim3 grid((unsigned)((n3 / seg) * (n2 / jb) * (n1 / 64)), nfield);
+#define YZ_CASE(KK, JB) hipLaunchKernelGGL((k_green_yz<R, KK, JB>), grid, dim3(64 * JB), 0, ctx->stream, B, n1, n2, n3, seg)
+#define YZ_JB(KK) \
+  do { if (jb == 16) YZ_CASE(KK, 16); else YZ_CASE(KK, 8); } while (0)
+  if (KT == 4) YZ_JB(4); else if (KT == 8) YZ_JB(8); else YZ_JB(12);
+#undef YZ_JB
+#undef YZ_CASE
+  MELT_HIP(hipGetLastError());
+  *used = 1;
+  return 0;
+}
+
 template <typename R>
 int launch_divg(melt_ctx* ctx) {
   R usdtal = R(1.0 / (ctx->dt * ctx->al));
@@ -1204,6 +1367,10 @@ int euler_launch_green(melt_ctx* ctx, int dir, void* const* src, void* const* ds
                        int* used) {
   return ctx->fp32 ? launch_green<float>(ctx, dir, src, dst, qadd, betadx, nfield, used)
                    : launch_green<double>(ctx, dir, src, dst, qadd, betadx, nfield, used);
+}
+int euler_launch_green_yz(melt_ctx* ctx, void* const* src, void* const* qadd, const double* by, const double* bz, int nfield, int* used) {
+  return ctx->fp32 ? launch_green_yz<float>(ctx, src, qadd, by, bz, nfield, used)
+                   : launch_green_yz<double>(ctx, src, qadd, by, bz, nfield, used);
 }
 int euler_launch_divg(melt_ctx* ctx) { return ctx->fp32 ? launch_divg<float>(ctx) : launch_divg<double>(ctx); }
 int euler_launch_project(melt_ctx* ctx, int do_u, int do_p, int use_mean, const int* gate) {

@@ -76,12 +76,15 @@ def autotune(n: int, precision: int = 0, lib=None) -> dict:
             ctx.upload(name, a)
         ctx.set_dt(inputs.cfl_dt(cfg, u, v, w, cfl=0.5))
         best = {}
-        zero = {k: 0 for k in ("xf_w", "xf_nt", "solve_w", "solve_nt", "xf_z")}
+        zero = {k: 0 for k in ("xf_w", "xf_nt", "solve_w", "solve_nt", "xf_z", "yz_jb")}
         # the explicit pass: z-march with D planes per block, or line tiles of W lines x nt threads
         xf = [{"xf_z": d, "xf_w": 0, "xf_nt": 0} for d in XF_Z if n % d == 0] + \
              [{"xf_z": 0, "xf_w": w_, "xf_nt": t} for w_ in XF_W if n % w_ == 0 for t in XF_NT]
         solve = [{"solve_w": w_, "solve_nt": t} for w_ in SOLVE_W if n % w_ == 0 for t in SOLVE_NT]
-        for slots, grid in ((("explicit_xsolve",), xf), (("solve_y", "solve_z_update"), solve)):
+        # the y/z solves: one fused pass with 16 or 8 rows per block, or two passes
+        yz = [{"yz_jb": 0}, {"yz_jb": 16}, {"yz_jb": 1}]
+        for slots, grid in ((("explicit_xsolve",), xf), (("solve_y", "solve_z_update"), yz),
+                            (("solve_y", "solve_z_update"), solve)):
             scores = []
             for cand in grid:
                 try:

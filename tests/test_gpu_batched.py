@@ -53,7 +53,7 @@ def test_ensemble_members_match_alone(gpu_lib, precision):
 
 def test_autotuned_shapes_do_not_change_results(gpu_lib):
     tuning = autotune(48, 0, lib=gpu_lib)
-    assert set(tuning) <= {"xf_w", "xf_nt", "solve_w", "solve_nt", "xf_z"} and tuning
+    assert set(tuning) <= {"xf_w", "xf_nt", "solve_w", "solve_nt", "xf_z", "yz_jb"} and tuning
     m = small_members()[1]
     a = alone(m, 2, 0, gpu_lib)
     b = alone(m, 2, 0, gpu_lib, tuning=tuning, graph=True)

@@ -306,12 +306,14 @@ def test_green_line_solves(gpu_lib, monkeypatch, n, n3):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-@pytest.mark.parametrize("n3,tunes", [(64, ("xf_w=1", "xf_z=1", "xf_z=4")), (24, ("xf_w=1", "xf_z=3"))])
+@pytest.mark.parametrize("n3,tunes", [(64, ("xf_w=1", "xf_z=1", "xf_z=4", "yz_jb=1", "yz_jb=16")),
+                                      (24, ("xf_w=1", "xf_z=3", "yz_jb=1"))])
 def test_zmarch_matches_line_kernel(gpu_lib, monkeypatch, precision, n3, tunes):
     """The explicit pass as a z-march (one x-line per block walking its planes,
     default where n1 % 64 == 0) and as one x-line per block (xf_w=1) compute
-    the same values with the same expressions: two RK3 steps with a melting
-    body are bit-identical, whatever the number of planes per block."""
+    the same values with the same expressions, and so do the y and z solves as
+    one pass (default where n3 % 16 == 0) and as two (yz_jb=1): two RK3 steps
+    with a melting body are bit-identical, whatever the launch shape."""
     n, level, rad = 64, 3, 1.0
     cfg = make_cfg(n, n3, precision=precision)
     st, dt = melt_case(make_cfg(n, n3), level, rad, seed=17)
