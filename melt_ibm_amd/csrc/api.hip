@@ -560,7 +560,13 @@ static int side_join(melt_ctx* ctx) {
   return 0;
 }
 
-extern "C" int melt_substep(melt_ctx* ctx, int32_t ns) {
+// One fused substep.  With tag_next (tschem's first two substeps, where no
+// caller code runs before the next one) the tagging of the next substep is
+// queued on the side stream right after this substep's Lagrangian update: it
+// reads only the geometry that update writes and writes only VOFp, sdf, its
+// own scratch and the window box, none of which the pressure phase touches,
+// so it overlaps the pressure solve.
+static int substep_impl(melt_ctx* ctx, int32_t ns, bool tag_next) {
   if (check_ctx(ctx)) return MELT_ERR_ARG;
   int rc = melt_set_stage(ctx, ns);
   if (rc || (rc = need_stage(ctx))) return rc;
@@ -571,13 +577,16 @@ extern "C" int melt_substep(melt_ctx* ctx, int32_t ns) {
   timing_begin(ctx, SLOT_SUBSTEP, &e_sub);
 
   timing_begin(ctx, SLOT_PH_TAG, &e_ph);
-  if (moving) {
+  if (moving && ctx->tag_ahead) {
+    ctx->force_window_from_tag = 1;   // placed by the tagging queued last substep
+  } else if (moving) {
     timing_begin(ctx, SLOT_TAG, &e);
     ctx->force_window_from_tag = 1;
     rc = ibm_tag(ctx);
     if (rc) { ctx->force_window_from_tag = 0; return rc; }
     timing_end(ctx, SLOT_TAG, e);
   }
+  ctx->tag_ahead = 0;
   timing_end(ctx, SLOT_PH_TAG, e_ph);
 
   timing_begin(ctx, SLOT_PH_EULER, &e_ph);
@@ -629,6 +638,14 @@ extern "C" int melt_substep(melt_ctx* ctx, int32_t ns) {
           int r = ibm_commit_probe_indices(ctx);
           if (!r) r = lagr_after_force(ctx);
           timing_end(ctx, SLOT_LAGR, e);
+          if (!r && tag_next && moving) {
+            timing_begin(ctx, SLOT_TAG, &e);
+            ctx->force_window_from_tag = 1;
+            r = ibm_tag(ctx);
+            ctx->force_window_from_tag = 0;
+            timing_end(ctx, SLOT_TAG, e);
+            if (!r) ctx->tag_ahead = 1;
+          }
           return r;
         })))
       return rc;
@@ -665,14 +682,16 @@ extern "C" int melt_substep(melt_ctx* ctx, int32_t ns) {
   return 0;
 }
 
+extern "C" int melt_substep(melt_ctx* ctx, int32_t ns) { return substep_impl(ctx, ns, false); }
+
 static int tschem_launch(melt_ctx* ctx) {
   if (ctx->has_geom) {
     hipLaunchKernelGGL(k_reset_loads, dim3(1), dim3(1), 0, ctx->stream, ctx->d_body);
     MELT_HIP(hipGetLastError());
   }
   for (int ns = 1; ns <= 3; ++ns) {
-    int rc = melt_substep(ctx, ns);
-    if (rc) return rc;
+    int rc = substep_impl(ctx, ns, ns < 3);
+    if (rc) { ctx->tag_ahead = 0; return rc; }
   }
   return 0;
 }

@@ -134,6 +134,7 @@ struct melt_ctx {
   int* h_err = nullptr;    // pinned mirror
   int defer_mean = 0;      // fused path: mean force applied in the projection
   int force_window_from_tag = 0;  // fused substep: tagging also places the forcing window
+  int tag_ahead = 0;              // the next substep's tagging is already queued (side stream)
 
   // timing
   int timing = 0;
