@@ -367,7 +367,11 @@ constexpr int RF_NC = 8;   // kx columns per register-FFT block (8 x 16 B = one 
 template <typename T, int P, int NC>
 struct RegFftLds {
   static constexpr int lines = NC;
-  cx<T> ex[NC * P * (P + 1)];
+  // column c's P x (P+1) transpose tile starts at c * CS; the odd stride puts
+  // the 8 columns that one 8-lane group of the y/z passes touches in different
+  // LDS banks (a stride of P (P+1) = 272 elements maps them all to one bank)
+  static constexpr int CS = P * (P + 1) + 1;
+  cx<T> ex[NC * CS];
   cx<T> tw[P * P];
 };
 
@@ -384,10 +388,10 @@ __device__ __forceinline__ void reg_fft_fwd(cx<T> (&v)[P], RegFftLds<T, P, NC>& 
     v[k1] = cmul(v[k1], w);
   }
 #pragma unroll
-  for (int k1 = 0; k1 < P; ++k1) L.ex[(c * P + k1) * (P + 1) + b] = v[k1];
+  for (int k1 = 0; k1 < P; ++k1) L.ex[c * L.CS + k1 * (P + 1) + b] = v[k1];
   __syncthreads();
 #pragma unroll
-  for (int q = 0; q < P; ++q) v[q] = L.ex[(c * P + b) * (P + 1) + q];
+  for (int q = 0; q < P; ++q) v[q] = L.ex[c * L.CS + b * (P + 1) + q];
   dft_reg<T, P>(v, inv);
 }
 
@@ -405,10 +409,10 @@ __device__ __forceinline__ void reg_fft_back(cx<T> (&v)[P], RegFftLds<T, P, NC>&
   }
   __syncthreads();   // every thread has read its forward exchange
 #pragma unroll
-  for (int n1 = 0; n1 < P; ++n1) L.ex[(c * P + n1) * (P + 1) + b] = v[n1];
+  for (int n1 = 0; n1 < P; ++n1) L.ex[c * L.CS + n1 * (P + 1) + b] = v[n1];
   __syncthreads();
 #pragma unroll
-  for (int q = 0; q < P; ++q) v[q] = L.ex[(c * P + b) * (P + 1) + q];
+  for (int q = 0; q < P; ++q) v[q] = L.ex[c * L.CS + b * (P + 1) + q];
   dft_reg<T, P>(v, inv);   // over k1: v[n2] = x[b + P n2]
 }
 
