@@ -226,8 +226,10 @@ int melt_graph_enable(melt_ctx* ctx, int32_t on);
  * z-march), 2 = lines per block of the y/z line solves, 3 = their threads per
  * block, 4 = planes per block of the z-marching explicit pass, 5 = j-rows per
  * block (8 or 16) of the pass that does the y and z solves together, 1 there
- * = two separate passes; value 0 = built-in choice.  Results do not depend on
- * it. */
+ * = two separate passes, 6 = streams: 1 keeps all of the context's work on its
+ * one stream (for many contexts sharing a GPU; default: the Lagrangian update,
+ * probes and next tagging run on a second stream); value 0 = built-in choice.
+ * Results do not depend on it. */
 int melt_set_tuning(melt_ctx* ctx, int32_t key, int32_t value);
 void* melt_stream(melt_ctx* ctx);
 int melt_get_refinement_stats(melt_ctx* ctx, int64_t* checks, int64_t* refined, double* last_ratio);

@@ -160,7 +160,7 @@ extern "C" int melt_ctx_create(const melt_config* cfg, melt_ctx** out) {
   do {
     if (hipGetDevice(&ctx->device) != hipSuccess) { rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "no HIP device"); break; }
     if (const char* tv = getenv("MELT_TUNE")) {
-      static const char* keys[TUNE_N] = {"xf_w", "xf_nt", "solve_w", "solve_nt", "xf_z", "yz_jb"};
+      static const char* keys[TUNE_N] = {"xf_w", "xf_nt", "solve_w", "solve_nt", "xf_z", "yz_jb", "streams"};
       std::string spec(tv);
       size_t pos = 0;
       while (pos < spec.size()) {
@@ -176,8 +176,11 @@ extern "C" int melt_ctx_create(const melt_config* cfg, melt_ctx** out) {
         pos = end + 1;
       }
     }
+    // the side stream is created on first use (one_stream() contexts never
+    // create it): HIP places streams on its hardware queues round-robin in
+    // creation order, so many contexts on one GPU spread their main streams
+    // over all queues only if no side streams are interleaved among them
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
       rc = MELT_ERR_HIP;
@@ -543,8 +546,17 @@ extern "C" int melt_particle(melt_ctx* ctx) {
 // queued on the main stream so far; side_join makes the main stream wait for
 // it.  Fork and join are stream events, so a captured step keeps them as
 // graph dependencies.
+// Tuning key streams = 1 keeps all work on the main stream (many contexts
+// sharing one GPU get their concurrency from each other).
+static bool one_stream(const melt_ctx* ctx) { return ctx->tune[TUNE_STREAMS] == 1; }
+static int ensure_side(melt_ctx* ctx) {
+  if (!ctx->side && !one_stream(ctx)) MELT_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+  return 0;
+}
 template <class F>
 static int side_run(melt_ctx* ctx, F&& work) {
+  if (one_stream(ctx)) return work();
+  if (int rc = ensure_side(ctx)) return rc;
   MELT_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
   MELT_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
   hipStream_t main_stream = ctx->stream;
@@ -556,6 +568,7 @@ static int side_run(melt_ctx* ctx, F&& work) {
   return 0;
 }
 static int side_join(melt_ctx* ctx) {
+  if (one_stream(ctx)) return 0;
   MELT_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
   return 0;
 }
@@ -690,7 +703,7 @@ static int tschem_launch(melt_ctx* ctx) {
     MELT_HIP(hipGetLastError());
   }
   for (int ns = 1; ns <= 3; ++ns) {
-    int rc = substep_impl(ctx, ns, ns < 3);
+    int rc = substep_impl(ctx, ns, ns < 3 && !one_stream(ctx));
     if (rc) { ctx->tag_ahead = 0; return rc; }
   }
   return 0;
@@ -703,6 +716,7 @@ extern "C" int melt_tschem(melt_ctx* ctx) {
   bool graph = ctx->use_graph && !ctx->timing && !(ctx->has_geom && ctx->h_body.initial_tag);
   if (!graph) return tschem_launch(ctx);
   if (!ctx->graph_exec) {
+    if (int rc = ensure_side(ctx)) return rc;   // not created inside the capture
     MELT_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     int rc = tschem_launch(ctx);
     hipGraph_t g = nullptr;

@@ -109,8 +109,9 @@ class Simulation:
                 ctx.upload(name, a)
             dt = inputs.cfl_dt(cfg, u, v, w, cfl=0.5)
             ctx.set_dt(dt)
-            if tuning:
-                ctx.set_tuning(**tuning)
+            # many contexts share the GPU: each keeps its work on one stream
+            # (their streams then spread over the device's hardware queues)
+            ctx.set_tuning(streams=1, **(tuning or {}))
             ctx.tagCells()
             ctx.sync()
             if graph:
