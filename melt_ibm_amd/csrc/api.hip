@@ -124,7 +124,7 @@ extern "C" void melt_ctx_destroy(melt_ctx* ctx) {
   for (int i = 0; i < 4; ++i) if (ctx->tmp4[i]) hipFree(ctx->tmp4[i]);
   void* ps[] = {ctx->spec, ctx->ak1, ctx->ak2, ctx->ak3, ctx->tw1, ctx->tw2, ctx->tw3, ctx->d_body, ctx->d_vof, ctx->d_eof, ctx->d_voe,
                 ctx->d_v2f_ptr, ctx->d_v2f, ctx->d_gv, ctx->d_ge, ctx->d_gf, ctx->d_pind, ctx->d_pind2, ctx->d_probe, ctx->d_rxp, ctx->d_rxt, ctx->d_red,
-                ctx->d_vbox, ctx->d_slots, ctx->d_coord, ctx->d_acc, ctx->d_fmean, ctx->d_tkey, ctx->d_tface, ctx->d_tdist, ctx->d_win, ctx->d_pcount, ctx->d_plist, ctx->d_err, ctx->d_gate, ctx->d_refine_stats};
+                ctx->d_vbox, ctx->d_slots, ctx->d_coord, ctx->d_acc, ctx->d_fmean, ctx->d_tkey, ctx->d_tface, ctx->d_tdist, ctx->d_pcoef, ctx->d_win, ctx->d_pcount, ctx->d_plist, ctx->d_err, ctx->d_gate, ctx->d_refine_stats};
   for (void* p : ps) if (p) hipFree(p);
   for (int i = 0; i < MELT_L_NLAG; ++i) if (ctx->d_lag[i]) hipFree(ctx->d_lag[i]);
   if (ctx->h_err) hipHostFree(ctx->h_err);
@@ -578,7 +578,8 @@ static int side_join(melt_ctx* ctx) {
 // queued on the side stream right after this substep's Lagrangian update: it
 // reads only the geometry that update writes and writes only VOFp, sdf, its
 // own scratch and the window box, none of which the pressure phase touches,
-// so it overlaps the pressure solve.
+// so it overlaps the pressure solve; so does the geometry part of the next
+// substep's gradient probes (it writes only its coefficient buffer).
 static int substep_impl(melt_ctx* ctx, int32_t ns, bool tag_next) {
   if (check_ctx(ctx)) return MELT_ERR_ARG;
   int rc = melt_set_stage(ctx, ns);
@@ -601,6 +602,12 @@ static int substep_impl(melt_ctx* ctx, int32_t ns, bool tag_next) {
   }
   ctx->tag_ahead = 0;
   timing_end(ctx, SLOT_PH_TAG, e_ph);
+  // the geometry part of the gradient probes reads only the faces (final
+  // since the last Lagrangian update) and the grid: queued with the tagging
+  // ahead of time, else on the side stream beside the explicit pass
+  const bool probes = ibm && (c.imelt == 1 || c.imlsstr == 1);
+  if (probes && !ctx->coef_ahead && (rc = side_run(ctx, [&] { return ibm_probe_coefs(ctx); }))) return rc;
+  ctx->coef_ahead = 0;
 
   timing_begin(ctx, SLOT_PH_EULER, &e_ph);
   timing_begin(ctx, SLOT_EXPLICIT, &e);
@@ -636,7 +643,7 @@ static int substep_impl(melt_ctx* ctx, int32_t ns, bool tag_next) {
     // the gradient probes read the same pre-force fields as the spreading and
     // write only their own outputs (their cell indices into d_pind2, since the
     // spreading still reads d_pind): side stream, alongside the spreading
-    if ((rc = side_run(ctx, [&] { return ibm_probes(ctx, ctx->d_pind2); }))) return rc;
+    if ((rc = side_run(ctx, [&] { return ibm_probes(ctx, ctx->d_pind2, probes); }))) return rc;
     rc = ibm_force_scatter(ctx);
     ctx->force_window_from_tag = 0;
     if (rc || (rc = side_join(ctx)) || (rc = ibm_force_drain(ctx, 1))) return rc;
@@ -658,6 +665,10 @@ static int substep_impl(melt_ctx* ctx, int32_t ns, bool tag_next) {
             ctx->force_window_from_tag = 0;
             timing_end(ctx, SLOT_TAG, e);
             if (!r) ctx->tag_ahead = 1;
+            if (!r && probes) {
+              r = ibm_probe_coefs(ctx);
+              if (!r) ctx->coef_ahead = 1;
+            }
           }
           return r;
         })))
@@ -704,7 +715,7 @@ static int tschem_launch(melt_ctx* ctx) {
   }
   for (int ns = 1; ns <= 3; ++ns) {
     int rc = substep_impl(ctx, ns, ns < 3 && !one_stream(ctx));
-    if (rc) { ctx->tag_ahead = 0; return rc; }
+    if (rc) { ctx->tag_ahead = 0; ctx->coef_ahead = 0; return rc; }
   }
   return 0;
 }

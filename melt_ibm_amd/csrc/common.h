@@ -124,6 +124,7 @@ struct melt_ctx {
   unsigned long long* d_tkey = nullptr;
   int* d_tface = nullptr;
   double* d_tdist = nullptr;   // the 125 band distances of every face (first tagging pass)
+  double* d_pcoef = nullptr;   // gradient probes' shape-function derivatives: 27 x 3 x 6 per face
   melt_window* d_win = nullptr;
   // ray tagging scratch (init only)
   int* d_pcount = nullptr;
@@ -135,6 +136,7 @@ struct melt_ctx {
   int defer_mean = 0;      // fused path: mean force applied in the projection
   int force_window_from_tag = 0;  // fused substep: tagging also places the forcing window
   int tag_ahead = 0;              // the next substep's tagging is already queued (side stream)
+  int coef_ahead = 0;             // and the geometry part of its gradient probes
 
   // timing
   int timing = 0;
@@ -460,7 +462,9 @@ int ibm_velforce(melt_ctx* ctx, int defer);
 int ibm_tempforce(melt_ctx* ctx);
 // pind_out: where the heat-flux probes' cell indices go (null: d_pind, as
 // mls_heatFlux does); ibm_commit_probe_indices copies d_pind2 into d_pind.
-int ibm_probes(melt_ctx* ctx, int* pind_out = nullptr);
+int ibm_probes(melt_ctx* ctx, int* pind_out = nullptr, bool coefs_ready = false);
+// geometry part of the gradient probes into d_pcoef (then ibm_probes(..., true))
+int ibm_probe_coefs(melt_ctx* ctx);
 int ibm_commit_probe_indices(melt_ctx* ctx);
 int ibm_window_reset(melt_ctx* ctx);
 int lagr_after_force(melt_ctx* ctx);

@@ -127,7 +127,8 @@ __device__ inline double dweight(double pc, double gcrd, double nrm, double scal
 // on lanes 1..3 (0 elsewhere); *grad_d receives component lane-1.
 template <typename R>
 __device__ int mls_grad_lanes(const Grid& g, const R* __restrict__ fld, const double pos[3], int ic, int jc, const int kz[3],
-                              const int ke[3], const int c[3], int lane, bool active, double* shA, double* grad_d) {
+                              const int ke[3], const int c[3], int lane, bool active, double* shA, double* grad_d,
+                              double* coef = nullptr, size_t cstride = 0) {
   const double sx = WSCL / g.dx1, sy = WSCL / g.dx2, sz = WSCL / g.dx3;
   double xs[3], ys[3], zs[3], w1[3], w2[3], w3[3], dw1[3], dw2[3], dw3[3];
   double M[16];
@@ -208,13 +209,17 @@ __device__ int mls_grad_lanes(const Grid& g, const R* __restrict__ fld, const do
         double wtx = w1[iq] * (w2[jq] * w3[nk]);
         double p[4] = {1.0, xs[iq], ys[jq], zs[nk]};
         double dw = d == 0 ? dw1[iq] * w2[jq] * w3[nk] : d == 1 ? w1[iq] * dw2[jq] * w3[nk] : w1[iq] * w2[jq] * dw3[nk];
-        double val = fld[widx(g, ic - 1 + iq, jc - 1 + jq, ke[nk])];
         double acc = 0.0;
 #pragma unroll
         for (int l = 0; l < 4; ++l) acc = acc + (1.0 * (wtx * p[l])) * dg[l];
 #pragma unroll
         for (int l = 0; l < 4; ++l) acc = acc + (1.0 * (dw * p[l])) * gam[l];
-        s = s + acc * val;
+        if (coef) {   // geometry only: the point's shape-function derivative
+          coef[(size_t)(9 * nk + 3 * jq + iq) * cstride] = acc;
+        } else {
+          double val = fld[widx(g, ic - 1 + iq, jc - 1 + jq, ke[nk])];
+          s = s + acc * val;
+        }
       }
   *grad_d = s;
   return 0;
@@ -832,65 +837,117 @@ __device__ inline void probe_pos(const ProbeArgs<R>& P, int f, int kind, double 
   for (int d = 0; d < 3; ++d) pos[d] = sg > 0 ? bar[d] + P.h * nor[d] / s : bar[d] - P.h * nor[d] / s;
 }
 
-// Gradient probes: kinds 0/1 (temperature, fluid/solid side), 3 (pressure),
-// 4/5/6 (u, v, w); four lanes per (face, kind), kinds major so that a wave
-// runs one kind.
+// Where gradient probe `kind` of face f samples: its position, the stencil
+// origin (ic, jc), the z coordinates' grid indices kz and field planes ke
+// (findProbeIndices.f90:9-76), the staggering of the field and the field.
+template <typename R>
+struct ProbeSite {
+  double pos[3];
+  int ic, jc, kz[3], ke[3], cc[3];
+  int icent, jcent, kw;
+  const R* fld;
+};
+template <typename R>
+__device__ inline void probe_site(const Grid& g, const ProbeArgs<R>& P, int f, int kind, ProbeSite<R>& S) {
+  probe_pos(P, f, kind, S.pos);
+  const double* pos = S.pos;
+  int icent = (int)floor(pos[0] * g.dx1) + 1, jcent = (int)floor(pos[1] * g.dx2) + 1;
+  int istag = fnint(pos[0] * g.dx1) + 1, jstag = fnint(pos[1] * g.dx2) + 1;
+  int k1 = (int)floor(pos[2] * g.dx3) + 1, kst = fnint(pos[2] * g.dx3) + 1;
+  int kw = pmod(k1 - 1, g.n3) + 1, kws = pmod(kst - 1, g.n3) + 1;
+  S.icent = icent; S.jcent = jcent; S.kw = kw;
+  S.ic = icent; S.jc = jcent;
+  S.cc[0] = S.cc[1] = S.cc[2] = 1;
+  S.kz[0] = k1 - 1; S.kz[1] = k1; S.kz[2] = k1;
+  S.ke[0] = kw - 1; S.ke[1] = kw; S.ke[2] = kw + 1;
+  if (kind <= 1) {
+    S.fld = P.temp;
+  } else if (kind == 3) {
+    S.fld = P.pr;
+  } else if (kind == 4) {
+    S.fld = P.vx; S.ic = istag; S.cc[0] = 0;
+  } else if (kind == 5) {
+    S.fld = P.vy; S.jc = jstag; S.cc[1] = 0;
+  } else {
+    S.fld = P.vz; S.cc[2] = 0;
+    S.kz[0] = kst - 1; S.kz[1] = kst; S.kz[2] = kst;
+    S.ke[0] = kws - 1; S.ke[1] = kws; S.ke[2] = kws + 1;
+  }
+}
+
 __device__ __forceinline__ int grad_kind(int q) { return q < 2 ? q : q + 1; }  // 0, 1, 3, 4, 5, 6
 
+__device__ inline double* probe_out(double* out, int f, int kind, int d) {
+  double* o = &out[(size_t)PRB_N * f];
+  return &o[(kind == 0 ? PRB_GT0 : kind == 1 ? PRB_GT1 : kind == 3 ? PRB_GP : kind == 4 ? PRB_GU : kind == 5 ? PRB_GV : PRB_GW) + d];
+}
+
+// Gradient probes: kinds 0/1 (temperature, fluid/solid side), 3 (pressure),
+// 4/5/6 (u, v, w); item = kind slot x nf + face, 4 lanes per item.  With
+// coef (the fused substep) only the geometry part runs: lane 1+d stores the
+// 27 shape-function derivatives of component d at coef[q * 3 nitems + 3 item
+// + d], and k_probes_apply later sums them against the field in the same
+// order; without it the lanes sum against the field directly.  Either way
+// the values round as in the single-thread formulation.
 template <typename R>
 __global__ __launch_bounds__(256) void k_probes_grad(Grid g, ProbeArgs<R> P, double* __restrict__ out, int* __restrict__ pind, int klo,
-                                                     int nkinds, int* err) {
+                                                     int nkinds, double* __restrict__ coef, int* err) {
   __shared__ double shA[64][16];
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
   const int item = id >> 2, lane = id & 3, slot = threadIdx.x >> 2;
-  const bool inrange = item < P.nf * nkinds;
+  const int nitem = P.nf * nkinds;
+  const bool inrange = item < nitem;
   const int f = inrange ? item % P.nf : 0;
   const int kind = grad_kind(klo + (inrange ? item / P.nf : 0));
   const bool active = inrange && !P.gf[f];
-  double pos[3] = {0.0, 0.0, 0.0};
-  int ic = 1, jc = 1, kz[3] = {1, 1, 1}, ke[3] = {1, 1, 1}, cc[3] = {1, 1, 1};
-  const R* fld = P.temp;
+  ProbeSite<R> S;
+  S.pos[0] = S.pos[1] = S.pos[2] = 0.0;
+  S.ic = S.jc = 1;
+  for (int q = 0; q < 3; ++q) { S.kz[q] = 1; S.ke[q] = 1; S.cc[q] = 1; }
+  S.fld = P.temp;
   if (active) {
-    probe_pos(P, f, kind, pos);
-    int icent = (int)floor(pos[0] * g.dx1) + 1, jcent = (int)floor(pos[1] * g.dx2) + 1;
-    int istag = fnint(pos[0] * g.dx1) + 1, jstag = fnint(pos[1] * g.dx2) + 1;
-    int k1 = (int)floor(pos[2] * g.dx3) + 1, kst = fnint(pos[2] * g.dx3) + 1;
-    int kw = pmod(k1 - 1, g.n3) + 1, kws = pmod(kst - 1, g.n3) + 1;
-    ic = icent; jc = jcent;
-    kz[0] = k1 - 1; kz[1] = k1; kz[2] = k1;
-    ke[0] = kw - 1; ke[1] = kw; ke[2] = kw + 1;
-    if (kind <= 1) {
-      fld = P.temp;
-      if (lane == 0) { int* pi = &pind[6 * f + 3 * kind]; pi[0] = icent; pi[1] = jcent; pi[2] = kw; }
-    } else if (kind == 3) {
-      fld = P.pr;
-    } else if (kind == 4) {
-      fld = P.vx; ic = istag; cc[0] = 0;
-    } else if (kind == 5) {
-      fld = P.vy; jc = jstag; cc[1] = 0;
-    } else {
-      fld = P.vz; cc[2] = 0;
-      kz[0] = kst - 1; kz[1] = kst; kz[2] = kst;
-      ke[0] = kws - 1; ke[1] = kws; ke[2] = kws + 1;
-    }
+    probe_site(g, P, f, kind, S);
+    if (!coef && kind <= 1 && lane == 0) { int* pi = &pind[6 * f + 3 * kind]; pi[0] = S.icent; pi[1] = S.jcent; pi[2] = S.kw; }
   }
   double gd = 0.0;
-  int info = mls_grad_lanes(g, fld, pos, ic, jc, kz, ke, cc, lane, active, shA[slot], &gd);
+  const size_t cs = (size_t)3 * nitem;
+  double* cf = coef && lane > 0 ? coef + (size_t)3 * item + (lane - 1) : nullptr;
+  int info = mls_grad_lanes(g, S.fld, S.pos, S.ic, S.jc, S.kz, S.ke, S.cc, lane, active, shA[slot], &gd, cf, cs);
   if (!active || lane == 0) return;
   if (info) { raise_error(err, MELT_ERR_SINGULAR_MLS); return; }
-  double* o = &out[(size_t)PRB_N * f];
-  const int d = lane - 1;
-  if (kind <= 1) {
-    o[(kind == 0 ? PRB_GT0 : PRB_GT1) + d] = gd;
-  } else if (kind == 3) {
-    o[PRB_GP + d] = gd;
-  } else if (kind == 4) {
-    o[PRB_GU + d] = gd;
-  } else if (kind == 5) {
-    o[PRB_GV + d] = gd;
-  } else {
-    o[PRB_GW + d] = gd;
-  }
+  if (!coef) *probe_out(out, f, kind, lane - 1) = gd;
+}
+
+// The field part of the gradient probes of the fused substep: one thread per
+// (item, component) sums the stored shape-function derivatives against the
+// field values in the reference's point order (and records the heat-flux
+// probes' cell indices, as the probes do).
+template <typename R>
+__global__ __launch_bounds__(256) void k_probes_apply(Grid g, ProbeArgs<R> P, const double* __restrict__ coef, double* __restrict__ out,
+                                                      int* __restrict__ pind, int klo, int nkinds) {
+  const int nitem = P.nf * nkinds;
+  const int id = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (id >= 3 * nitem) return;
+  const int item = id / 3, d = id - 3 * item;
+  const int f = item % P.nf, kind = grad_kind(klo + item / P.nf);
+  if (P.gf[f]) return;
+  ProbeSite<R> S;
+  probe_site(g, P, f, kind, S);
+  if (kind <= 1 && d == 0) { int* pi = &pind[6 * f + 3 * kind]; pi[0] = S.icent; pi[1] = S.jcent; pi[2] = S.kw; }
+  const size_t cs = (size_t)3 * nitem;
+  const double* c = coef + id;
+  double s = 0.0;
+#pragma unroll 1
+  for (int nk = 0; nk < 3; ++nk)
+#pragma unroll
+    for (int jq = 0; jq < 3; ++jq)
+#pragma unroll
+      for (int iq = 0; iq < 3; ++iq) {
+        double acc = c[(size_t)(9 * nk + 3 * jq + iq) * cs];
+        double val = S.fld[widx(g, S.ic - 1 + iq, S.jc - 1 + jq, S.ke[nk])];
+        s = s + acc * val;
+      }
+  *probe_out(out, f, kind, d) = s;
 }
 
 // Pressure interpolation probe (kind 2) and the heat-flux dot products of
@@ -1101,19 +1158,45 @@ static int ibm_tempforce_t(melt_ctx* ctx) {  // tempforce.f90
 }
 
 template <typename R>
-static int ibm_probes_t(melt_ctx* ctx, int* pind_out) {
+static ProbeArgs<R> probe_args(melt_ctx* ctx) {
   ProbeArgs<R> P;
   P.temp = fld<R>(ctx, MELT_TEMP); P.pr = fld<R>(ctx, MELT_PR); P.vx = fld<R>(ctx, MELT_VX); P.vy = fld<R>(ctx, MELT_VY); P.vz = fld<R>(ctx, MELT_VZ);
   P.bar = ctx->d_lag[MELT_L_TRI_BAR]; P.nor = ctx->d_lag[MELT_L_TRI_NOR]; P.gf = ctx->d_gf; P.nf = ctx->nf;
   P.h = ctx->h_body.h_eulerian; P.pec = ctx->cfg.ren * ctx->cfg.prandtl;
   P.heat = ctx->cfg.imelt == 1;
   P.loads = ctx->cfg.imlsstr == 1;
+  return P;
+}
+
+// The geometry part of the gradient probes (shape-function derivatives into
+// d_pcoef): reads only the faces and the grid, so the fused substep runs it
+// beside the explicit pass.
+template <typename R>
+static int ibm_probe_coefs_t(melt_ctx* ctx) {
+  ProbeArgs<R> P = probe_args<R>(ctx);
   int klo = P.heat ? 0 : 2, khi = P.loads ? 6 : 2;
   if (khi > klo) {
     size_t threads = (size_t)ctx->nf * (khi - klo) * 4;
-    hipLaunchKernelGGL(k_probes_grad<R>, dim3(nb(threads, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), P, ctx->d_probe,
-                       pind_out ? pind_out : ctx->d_pind, klo,
-                       khi - klo, ctx->d_err);
+    hipLaunchKernelGGL(k_probes_grad<R>, dim3(nb(threads, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), P, ctx->d_probe, ctx->d_pind,
+                       klo, khi - klo, ctx->d_pcoef, ctx->d_err);
+  }
+  MELT_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename R>
+static int ibm_probes_t(melt_ctx* ctx, int* pind_out, bool coefs_ready) {
+  ProbeArgs<R> P = probe_args<R>(ctx);
+  int klo = P.heat ? 0 : 2, khi = P.loads ? 6 : 2;
+  int* pind = pind_out ? pind_out : ctx->d_pind;
+  if (khi > klo && coefs_ready) {
+    size_t threads = (size_t)ctx->nf * (khi - klo) * 3;
+    hipLaunchKernelGGL(k_probes_apply<R>, dim3(nb(threads, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), P, ctx->d_pcoef, ctx->d_probe,
+                       pind, klo, khi - klo);
+  } else if (khi > klo) {
+    size_t threads = (size_t)ctx->nf * (khi - klo) * 4;
+    hipLaunchKernelGGL(k_probes_grad<R>, dim3(nb(threads, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), P, ctx->d_probe, pind, klo,
+                       khi - klo, nullptr, ctx->d_err);
   }
   if (P.heat || P.loads)
     hipLaunchKernelGGL(k_probes_point<R>, dim3(nb(ctx->nf, 64)), dim3(64), 0, ctx->stream, make_grid(ctx), P, ctx->d_probe, ctx->d_err);
@@ -1150,4 +1233,7 @@ int ibm_force_drain(melt_ctx* ctx, int apply_now) { return ctx->fp32 ? ibm_force
 int ibm_sub_mean(melt_ctx* ctx) { return ctx->fp32 ? ibm_sub_mean_t<float>(ctx) : ibm_sub_mean_t<double>(ctx); }
 int ibm_velforce(melt_ctx* ctx, int defer) { return ctx->fp32 ? ibm_velforce_t<float>(ctx, defer) : ibm_velforce_t<double>(ctx, defer); }
 int ibm_tempforce(melt_ctx* ctx) { return ctx->fp32 ? ibm_tempforce_t<float>(ctx) : ibm_tempforce_t<double>(ctx); }
-int ibm_probes(melt_ctx* ctx, int* pind_out) { return ctx->fp32 ? ibm_probes_t<float>(ctx, pind_out) : ibm_probes_t<double>(ctx, pind_out); }
+int ibm_probes(melt_ctx* ctx, int* pind_out, bool coefs_ready) {
+  return ctx->fp32 ? ibm_probes_t<float>(ctx, pind_out, coefs_ready) : ibm_probes_t<double>(ctx, pind_out, coefs_ready);
+}
+int ibm_probe_coefs(melt_ctx* ctx) { return ctx->fp32 ? ibm_probe_coefs_t<float>(ctx) : ibm_probe_coefs_t<double>(ctx); }

@@ -644,6 +644,7 @@ int lagr_alloc(melt_ctx* ctx, const double* xyz0, const int* voe, const int* eof
   rc |= dalloc((void**)&ctx->d_tkey, sizeof(unsigned long long) * ctx->twin_cells);
   rc |= dalloc((void**)&ctx->d_tface, sizeof(int) * ctx->twin_cells);
   rc |= dalloc((void**)&ctx->d_tdist, sizeof(double) * 125 * nf);
+  rc |= dalloc((void**)&ctx->d_pcoef, sizeof(double) * 27 * 3 * 6 * nf);
   if (rc) return MELT_ERR_HIP;
   MELT_HIP(hipDeviceSynchronize());  // the zero fills above are not ordered with the context's stream
   if (int rc2 = ibm_window_reset(ctx)) return rc2;

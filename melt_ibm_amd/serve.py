@@ -36,8 +36,8 @@ FREQ_FOR_GRID = {64: 10, 128: 20, 256: 35, 384: 55}   # 2000, 8000, 24500, 60500
 def request_bytes(m: Member, precision: int) -> int:
     """Device bytes of one simulation (what the library allocates, with a
     margin): 26 fields + 7 solve buffers, the half spectrum, the Lagrangian
-    arrays (with the 125 tagging-band distances per face) and the
-    immersed-boundary windows."""
+    arrays (with the 125 tagging-band distances and the 486 gradient-probe
+    coefficients per face) and the immersed-boundary windows."""
     esz = 4 if precision else 8
     n = m.n
     n1hp = ((n // 2 + 1) + 7) & ~7
@@ -45,7 +45,7 @@ def request_bytes(m: Member, precision: int) -> int:
     nf = 20 * f * f
     nv = nf // 2 + 2
     win = min(n, int(np.ceil(2.0 * m.rad * max(1.0, m.aspect) * n / (2 * np.pi))) + 10) ** 3
-    b = esz * n ** 3 * 33 + 2 * esz * n1hp * n * n + 8 * (nf * (4 * 27 + 40 + 125) + nv * 24) + win * (4 * 16 + 12)
+    b = esz * n ** 3 * 33 + 2 * esz * n1hp * n * n + 8 * (nf * (4 * 27 + 40 + 125 + 486) + nv * 24) + win * (4 * 16 + 12)
     return int(b * 1.10) + (64 << 20)
 
 
