@@ -367,13 +367,34 @@ constexpr int RF_NC = 8;   // kx columns per register-FFT block (8 x 16 B = one 
 template <typename T, int P, int NC>
 struct RegFftLds {
   static constexpr int lines = NC;
-  // column c's P x (P+1) transpose tile starts at c * CS; the odd stride puts
-  // the 8 columns that one 8-lane group of the y/z passes touches in different
-  // LDS banks (a stride of P (P+1) = 272 elements maps them all to one bank)
-  static constexpr int CS = P * (P + 1) + 1;
-  cx<T> ex[NC * CS];
+  // The transposes move the real and then the imaginary parts through one
+  // tile of T (half the LDS of a complex tile, so more blocks fit per CU).
+  // Column c's P x (P+1) tile starts at c * CS; CS = 2 mod 16 puts the 16
+  // lanes of a 64-bit access in distinct bank pairs in both thread layouts
+  // (y/z: 8 columns x 2 threads per lane group; x: 16 threads of one column).
+  static constexpr int CS = P * (P + 1) + ((2 - P * (P + 1) % 16) + 16) % 16;
+  T ex[NC * CS];
   cx<T> tw[P * P];
 };
+
+// v[q] of thread b of column c becomes v[b] of thread q (a P x P transpose
+// within the column), one component at a time.  The tile must be free on
+// entry (callers put a barrier after any earlier read of it).
+template <typename T, int P, int NC>
+__device__ __forceinline__ void reg_transpose(cx<T> (&v)[P], RegFftLds<T, P, NC>& L, int c, int b) {
+  T* e = L.ex + c * L.CS;
+#pragma unroll
+  for (int k1 = 0; k1 < P; ++k1) e[k1 * (P + 1) + b] = v[k1].x;
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < P; ++q) v[q].x = e[b * (P + 1) + q];
+  __syncthreads();
+#pragma unroll
+  for (int k1 = 0; k1 < P; ++k1) e[k1 * (P + 1) + b] = v[k1].y;
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < P; ++q) v[q].y = e[b * (P + 1) + q];
+}
 
 // One transform of the lines (column c, thread b) held as v[a] = x[b + P a]:
 // on return v[k2] = X[k1 + P k2] for k1 = b.
@@ -387,11 +408,7 @@ __device__ __forceinline__ void reg_fft_fwd(cx<T> (&v)[P], RegFftLds<T, P, NC>& 
     if (inv) w.y = -w.y;
     v[k1] = cmul(v[k1], w);
   }
-#pragma unroll
-  for (int k1 = 0; k1 < P; ++k1) L.ex[c * L.CS + k1 * (P + 1) + b] = v[k1];
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < P; ++q) v[q] = L.ex[c * L.CS + b * (P + 1) + q];
+  reg_transpose<T, P, NC>(v, L, c, b);
   dft_reg<T, P>(v, inv);
 }
 
@@ -408,11 +425,7 @@ __device__ __forceinline__ void reg_fft_back(cx<T> (&v)[P], RegFftLds<T, P, NC>&
     v[n1] = cmul(v[n1], w);
   }
   __syncthreads();   // every thread has read its forward exchange
-#pragma unroll
-  for (int n1 = 0; n1 < P; ++n1) L.ex[c * L.CS + n1 * (P + 1) + b] = v[n1];
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < P; ++q) v[q] = L.ex[c * L.CS + b * (P + 1) + q];
+  reg_transpose<T, P, NC>(v, L, c, b);
   dft_reg<T, P>(v, inv);   // over k1: v[n2] = x[b + P n2]
 }
 
@@ -526,10 +539,21 @@ __global__ __launch_bounds__(RF_XT) void k_fft_x_fwd_reg(SpecArgs<T> S, const cx
   if (div_slots) block_max_to_slots(dmax, div_slots);
   __syncthreads();   // twiddles
   reg_fft_fwd<T, P>(v, L, c, b, false);   // v[k2] = Z[b + P k2]
-  cx<T>* zs = L.ex + c * N;
+  // mirror bins Z[N-m] of the line pair through LDS, one component at a time
+  T* zs = L.ex + c * N;
+  T zrx[P];
   __syncthreads();   // the transpose tile is read
 #pragma unroll
-  for (int k2 = 0; k2 < P; ++k2) zs[b + P * k2] = v[k2];
+  for (int k2 = 0; k2 < P; ++k2) zs[b + P * k2] = v[k2].x;
+  __syncthreads();
+#pragma unroll
+  for (int k2 = 0; k2 < P; ++k2) {
+    const int m = b + P * k2;
+    zrx[k2] = zs[m == 0 ? 0 : N - m];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k2 = 0; k2 < P; ++k2) zs[b + P * k2] = v[k2].y;
   __syncthreads();
   const size_t row = (size_t)S.n1hp * ((size_t)j + (size_t)g.n2 * k);
 #pragma unroll
@@ -538,7 +562,7 @@ __global__ __launch_bounds__(RF_XT) void k_fft_x_fwd_reg(SpecArgs<T> S, const cx
     if (m >= S.n1hp) break;
     cx<T> A = mkc<T>(0, 0), B = mkc<T>(0, 0);
     if (m < S.n1h) {
-      const cx<T> z = v[k2], zr = zs[m == 0 ? 0 : N - m];
+      const cx<T> z = v[k2], zr = mkc<T>(zrx[k2], zs[m == 0 ? 0 : N - m]);
       // A = (Z[m] + conj Z[N-m]) / 2,  B = (Z[m] - conj Z[N-m]) / (2i)
       A = mkc<T>((z.x + zr.x) * T(0.5), (z.y - zr.y) * T(0.5));
       B = mkc<T>((z.y + zr.y) * T(0.5), (zr.x - z.x) * T(0.5));
