@@ -814,39 +814,55 @@ struct GreenYZ {
   Green gy[2], gz[2];
 };
 
-// A block is 64 consecutive i x JB rows j, one (i, j) column per thread, and
-// marches through one z segment of GQ NCH planes (plus KT halo planes each
-// side).  Per plane the block stages rows j0-KT .. j0+JB+KT-1 in one of two
-// LDS tiles (one barrier per plane), each thread filters its column along y,
-// rounds to R (the value the separate y pass would store) and shifts it into a
-// register window of 2 KT + 1 planes, on which the z filter runs as in
-// k_green.  The loads run PF planes ahead of the plane being filtered (a ring
-// of PF register sets, indexed statically by unrolling PF planes at a time).
-// Needs n1 % 64 == 0, n2 % JB == 0, n3 % seg == 0, seg % PF == 0.
-template <typename R, int KT, int JB>
+// C consecutive values of a field (C = 1, or 2 as one 8-byte access).
+template <typename R, int C>
+__device__ __forceinline__ void ld_c(const R* p, R (&o)[C]) {
+  if constexpr (C == 2) {
+    const HIP_vector_type<R, 2> t = *reinterpret_cast<const HIP_vector_type<R, 2>*>(p);
+    o[0] = t.x; o[1] = t.y;
+  } else {
+    o[0] = *p;
+  }
+}
+template <typename R, int C>
+__device__ __forceinline__ void st_c(R* p, const R (&o)[C]) {
+  if constexpr (C == 2) {
+    HIP_vector_type<R, 2> t;
+    t.x = o[0]; t.y = o[1];
+    *reinterpret_cast<HIP_vector_type<R, 2>*>(p) = t;
+  } else {
+    *p = o[0];
+  }
+}
+
+// A block is 64 C consecutive i x JB rows j, C (i, j) columns per thread (C =
+// 2 for 4-byte values, so a wave still moves 512-byte rows), and marches
+// through one z segment of seg planes (plus KT halo planes each side).  Per
+// plane the block stages rows j0-KT .. j0+JB+KT-1 in one of two LDS tiles
+// (one barrier per plane), each thread filters its columns along y, rounds to
+// R (the value the separate y pass would store) and shifts the result into a
+// register window of 2 KT + 1 planes per column, on which the z filter runs
+// as in k_green.  The loads run PF planes ahead of the plane being filtered
+// (a ring of PF register sets, indexed statically by unrolling PF planes at a
+// time).  Needs n1 % (64 C) == 0, n2 % JB == 0, n3 % seg == 0, seg % PF == 0.
+template <typename R, int KT, int JB, int C>
 __global__ __launch_bounds__(64 * JB) void k_green_yz(GreenYZ<R> B, int n1, int n2, int n3, int seg) {
-  constexpr int ROWS = JB + 2 * KT, RPT = (ROWS + JB - 1) / JB, W = 2 * KT + 1, PF = 2;
+  constexpr int ROWS = JB + 2 * KT, RPT = (ROWS + JB - 1) / JB, W = 2 * KT + 1, PF = 2, RW = 64 * C;
   static_assert((2 * KT) % PF == 0, "prefetch ring must tile the halo planes");
-  __shared__ double tile[2][ROWS * 64];
+  __shared__ R tile[2][ROWS * RW];
   const int f = blockIdx.y;
   const int nsg = n3 / seg, njt = n2 / JB;
   unsigned b = xcd_block(blockIdx.x, gridDim.x);
   const int sg = (int)(b % nsg), rest = (int)(b / nsg);
   const int jt = rest % njt, it = rest / njt;
   const int lane = threadIdx.x & 63, jl = threadIdx.x >> 6;
-  const int i = it * 64 + lane, j0 = jt * JB, p0 = sg * seg;
+  const int i = it * RW + C * lane, j0 = jt * JB, p0 = sg * seg;
   const int wy = B.which[f];
-  // the coefficients are uniform over the block: kept in scalar registers
-  auto uni = [](double v) {
-    long long l = __double_as_longlong(v);
-    int lo = __builtin_amdgcn_readfirstlane((int)(l & 0xffffffff)), hi = __builtin_amdgcn_readfirstlane((int)(l >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-  };
   double gy[KT + 1], gz[KT + 1];
 #pragma unroll
   for (int m = 0; m <= KT; ++m) {
-    gy[m] = uni(m <= B.gy[wy].K ? B.gy[wy].g[m] : 0.0);
-    gz[m] = uni(m <= B.gz[wy].K ? B.gz[wy].g[m] : 0.0);
+    gy[m] = m <= B.gy[wy].K ? B.gy[wy].g[m] : 0.0;
+    gz[m] = m <= B.gz[wy].K ? B.gz[wy].g[m] : 0.0;
   }
   // no __restrict__: loads of src must stay behind the barriers, or the
   // compiler hoists the prefetches of many planes and runs out of registers
@@ -860,36 +876,41 @@ __global__ __launch_bounds__(64 * JB) void k_green_yz(GreenYZ<R> B, int n1, int 
     jj = jj < 0 ? jj + n2 : (jj >= n2 ? jj - n2 : jj);
     roff[r] = jj * n1 + i;
   }
-  R nx[PF][RPT];
+  R nx[PF][RPT][C];
   auto fetch = [&](int slot, int p) {
     p = p < 0 ? p + n3 : (p >= n3 ? p - n3 : p);
     const R* dp = d + (size_t)p * plane;
 #pragma unroll
     for (int r = 0; r < RPT; ++r)
-      if (jl + r * JB < ROWS) nx[slot][r] = dp[roff[r]];
+      if (jl + r * JB < ROWS) ld_c<R, C>(dp + roff[r], nx[slot][r]);
   };
   int buf = 0;
   int pn = p0 - KT;   // next plane to filter; its values sit in ring slot (pn - p0 + KT) % PF
   const int pend = p0 + seg + KT;
 #pragma unroll
   for (int s = 0; s < PF; ++s) fetch(s, pn + s);
-  double w[W];   // y-filtered planes p - KT .. p + KT around the output plane p
-  // shifts the window by one plane and appends the y filter of plane pn for
-  // this thread's column, then refills the ring slot with plane pn + PF
+  double w[C][W];   // per column: y-filtered planes p - KT .. p + KT around the output plane p
+  // shifts the windows by one plane and appends the y filter of plane pn for
+  // this thread's columns, then refills the ring slot with plane pn + PF
   auto advance = [&](int slot) {
 #pragma unroll
-    for (int m = 0; m + 1 < W; ++m) w[m] = w[m + 1];
-    double* T = tile[buf];
+    for (int cc = 0; cc < C; ++cc)
+#pragma unroll
+      for (int m = 0; m + 1 < W; ++m) w[cc][m] = w[cc][m + 1];
+    R* T = tile[buf];
 #pragma unroll
     for (int r = 0; r < RPT; ++r)
-      if (jl + r * JB < ROWS) T[(jl + r * JB) * 64 + lane] = (double)nx[slot][r];
+      if (jl + r * JB < ROWS) st_c<R, C>(T + (jl + r * JB) * RW + C * lane, nx[slot][r]);
     __syncthreads();
     if (pn + PF < pend) fetch(slot, pn + PF);
-    const double* L = T + (jl + KT) * 64 + lane;
-    double x = gy[0] * L[0];
 #pragma unroll
-    for (int m = 1; m <= KT; ++m) x += gy[m] * (L[-m * 64] + L[m * 64]);
-    w[W - 1] = (double)(R)x;
+    for (int cc = 0; cc < C; ++cc) {
+      const R* L = T + (jl + KT) * RW + C * lane + cc;
+      double x = gy[0] * (double)L[0];
+#pragma unroll
+      for (int m = 1; m <= KT; ++m) x += gy[m] * ((double)L[-m * RW] + (double)L[m * RW]);
+      w[cc][W - 1] = (double)(R)x;
+    }
     buf ^= 1;
     ++pn;
   };
@@ -904,11 +925,17 @@ __global__ __launch_bounds__(64 * JB) void k_green_yz(GreenYZ<R> B, int n1, int 
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
       advance(u);
-      double x = gz[0] * w[KT];
-#pragma unroll
-      for (int m = 1; m <= KT; ++m) x += gz[m] * (w[KT - m] + w[KT + m]);
       const size_t c = (size_t)(p + u) * plane + col;
-      q[c] = q[c] + (R)x;
+      R qv[C];
+      ld_c<R, C>(q + c, qv);
+#pragma unroll
+      for (int cc = 0; cc < C; ++cc) {
+        double x = gz[0] * w[cc][KT];
+#pragma unroll
+        for (int m = 1; m <= KT; ++m) x += gz[m] * (w[cc][KT - m] + w[cc][KT + m]);
+        qv[cc] = qv[cc] + (R)x;
+      }
+      st_c<R, C>(q + c, qv);
     }
   }
 }
@@ -1249,7 +1276,10 @@ int launch_green(melt_ctx* ctx, int dir, void* const* src, void* const* dst, voi
 // with *used = 0 (launching nothing) when the shapes or the systems do not fit,
 // and the caller runs the two passes instead.  Tuning key yz_jb: 0 default
 // (8 rows per block), 8 or 16 rows, 1 = always two passes.
-constexpr int YZ_SEG_MAX = 256;   // whole lines up to 256 planes measured fastest
+// planes per z segment at most: whole 256-plane lines measured fastest for
+// 8-byte values; 4-byte values (two columns per thread, half the blocks) need
+// the extra blocks of 128-plane segments (0.243 -> 0.169 ms at 256^3)
+constexpr int YZ_SEG_MAX8 = 256, YZ_SEG_MAX4 = 128;
 
 template <typename R>
 int launch_green_yz(melt_ctx* ctx, void* const* src, void* const* qadd, const double* by, const double* bz, int nfield, int* used) {
@@ -1281,15 +1311,21 @@ int launch_green_yz(melt_ctx* ctx, void* const* src, void* const* qadd, const do
   int kmax = 0;
   for (int g = 0; g < ng; ++g) kmax = std::max(kmax, std::max(B.gy[g].K, B.gz[g].K));
   const int KT = kmax <= 4 ? 4 : kmax <= 8 ? 8 : 12;
+  // two columns per thread for 4-byte values where the rows allow it
+  constexpr int C2 = sizeof(R) == 4 ? 2 : 1;   // (no 2-column instantiation for 8-byte values)
+  const int C = C2 == 2 && n1 % 128 == 0 ? 2 : 1;
   if (n1 % 64 || n2 % jb || n3 % GQ || 2 * KT >= n2 || 2 * KT >= n3) return 0;
-  // planes per z segment: the longest of YZ_SEG_MAX, 64, 32, 16 that tiles the line
+  // planes per z segment: the longest of the maximum above, 64, 32, 16 that tiles the line
   int seg = GQ;
-  for (int s : {YZ_SEG_MAX, 64, 32})
+  for (int s : {sizeof(R) == 4 ? YZ_SEG_MAX4 : YZ_SEG_MAX8, 64, 32})
     if (n3 % s == 0) { seg = s; break; }
-  dim3 grid((unsigned)((n3 / seg) * (n2 / jb) * (n1 / 64)), nfield);
-#define YZ_CASE(KK, JB) hipLaunchKernelGGL((k_green_yz<R, KK, JB>), grid, dim3(64 * JB), 0, ctx->stream, B, n1, n2, n3, seg)
+  dim3 grid((unsigned)((n3 / seg) * (n2 / jb) * (n1 / (64 * C))), nfield);
+#define YZ_CASE(KK, JB, CC) hipLaunchKernelGGL((k_green_yz<R, KK, JB, CC>), grid, dim3(64 * JB), 0, ctx->stream, B, n1, n2, n3, seg)
 #define YZ_JB(KK) \
-  do { if (jb == 16) YZ_CASE(KK, 16); else YZ_CASE(KK, 8); } while (0)
+  do {                                                         \
+    if (C == 2) { if (jb == 16) YZ_CASE(KK, 16, C2); else YZ_CASE(KK, 8, C2); } \
+    else { if (jb == 16) YZ_CASE(KK, 16, 1); else YZ_CASE(KK, 8, 1); }       \
+  } while (0)
   if (KT == 4) YZ_JB(4); else if (KT == 8) YZ_JB(8); else YZ_JB(12);
 #undef YZ_JB
 #undef YZ_CASE
