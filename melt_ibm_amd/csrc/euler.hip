@@ -1389,7 +1389,7 @@ template <typename R>
 int launch_project(melt_ctx* ctx, int do_u, int do_p, int use_mean, const int* gate) {
   R udx1 = R(ctx->al * ctx->dt * ctx->dx1), udx2 = R(ctx->al * ctx->dt * ctx->dx2), udx3 = R(ctx->al * ctx->dt * ctx->dx3);
   R be = R(ctx->al * ctx->beta);
-  if (sizeof(R) == 4 && ctx->n1 % 2 == 0) {
+  if (ctx->n1 % 2 == 0) {
     hipLaunchKernelGGL(k_project2<R>, dim3(nblocks(ctx->nc / 2, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), fld<R>(ctx, MELT_VX),
                        fld<R>(ctx, MELT_VY), fld<R>(ctx, MELT_VZ), fld<R>(ctx, MELT_PR), fld<R>(ctx, MELT_DPH),
                        use_mean ? ctx->d_fmean : nullptr, udx1, udx2, udx3, be, R(ctx->dx1q), R(ctx->dx2q), R(ctx->dx3q), do_u, do_p,
