@@ -877,6 +877,17 @@ __device__ inline void probe_site(const Grid& g, const ProbeArgs<R>& P, int f, i
 
 __device__ __forceinline__ int grad_kind(int q) { return q < 2 ? q : q + 1; }  // 0, 1, 3, 4, 5, 6
 
+// Slots of the stored shape-function derivatives.  The pressure-gradient
+// probe (kind 3) samples the same point as the fluid-side temperature probe
+// (kind 0, bar + h n) on the same cell-centred stencil, so with both heat flux
+// and loads on it shares kind 0's slot: kinds 0, 1, 4, 5, 6.
+__device__ __forceinline__ int coef_kind(int cs, int heat, int loads) {
+  return heat && loads ? (cs < 2 ? cs : cs + 2) : heat ? cs : cs + 3;
+}
+__device__ __forceinline__ int coef_slot(int kind, int heat, int loads) {
+  return heat && loads ? (kind < 2 ? kind : kind == 3 ? 0 : kind - 2) : heat ? kind : kind - 3;
+}
+
 __device__ inline double* probe_out(double* out, int f, int kind, int d) {
   double* o = &out[(size_t)PRB_N * f];
   return &o[(kind == 0 ? PRB_GT0 : kind == 1 ? PRB_GT1 : kind == 3 ? PRB_GP : kind == 4 ? PRB_GU : kind == 5 ? PRB_GV : PRB_GW) + d];
@@ -895,10 +906,11 @@ __global__ __launch_bounds__(256) void k_probes_grad(Grid g, ProbeArgs<R> P, dou
   __shared__ double shA[64][16];
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
   const int item = id >> 2, lane = id & 3, slot = threadIdx.x >> 2;
-  const int nitem = P.nf * nkinds;
+  const int nitem = P.nf * nkinds;   // with coef: items over the coefficient slots
   const bool inrange = item < nitem;
   const int f = inrange ? item % P.nf : 0;
-  const int kind = grad_kind(klo + (inrange ? item / P.nf : 0));
+  const int q = inrange ? item / P.nf : 0;
+  const int kind = coef ? coef_kind(q, P.heat, P.loads) : grad_kind(klo + q);
   const bool active = inrange && !P.gf[f];
   ProbeSite<R> S;
   S.pos[0] = S.pos[1] = S.pos[2] = 0.0;
@@ -924,7 +936,7 @@ __global__ __launch_bounds__(256) void k_probes_grad(Grid g, ProbeArgs<R> P, dou
 // probes' cell indices, as the probes do).
 template <typename R>
 __global__ __launch_bounds__(256) void k_probes_apply(Grid g, ProbeArgs<R> P, const double* __restrict__ coef, double* __restrict__ out,
-                                                      int* __restrict__ pind, int klo, int nkinds) {
+                                                      int* __restrict__ pind, int klo, int nkinds, int ncoef) {
   const int nitem = P.nf * nkinds;
   const int id = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (id >= 3 * nitem) return;
@@ -934,8 +946,8 @@ __global__ __launch_bounds__(256) void k_probes_apply(Grid g, ProbeArgs<R> P, co
   ProbeSite<R> S;
   probe_site(g, P, f, kind, S);
   if (kind <= 1 && d == 0) { int* pi = &pind[6 * f + 3 * kind]; pi[0] = S.icent; pi[1] = S.jcent; pi[2] = S.kw; }
-  const size_t cs = (size_t)3 * nitem;
-  const double* c = coef + id;
+  const size_t cs = (size_t)3 * P.nf * ncoef;
+  const double* c = coef + (size_t)3 * ((size_t)coef_slot(kind, P.heat, P.loads) * P.nf + f) + d;
   double s = 0.0;
 #pragma unroll 1
   for (int nk = 0; nk < 3; ++nk)
@@ -1168,6 +1180,9 @@ static ProbeArgs<R> probe_args(melt_ctx* ctx) {
   return P;
 }
 
+template <typename R>
+static int probe_coef_slots(const ProbeArgs<R>& P) { return P.heat && P.loads ? 5 : P.heat ? 2 : P.loads ? 4 : 0; }
+
 // The geometry part of the gradient probes (shape-function derivatives into
 // d_pcoef): reads only the faces and the grid, so the fused substep runs it
 // beside the explicit pass.
@@ -1175,10 +1190,11 @@ template <typename R>
 static int ibm_probe_coefs_t(melt_ctx* ctx) {
   ProbeArgs<R> P = probe_args<R>(ctx);
   int klo = P.heat ? 0 : 2, khi = P.loads ? 6 : 2;
+  const int ncoef = probe_coef_slots(P);
   if (khi > klo) {
-    size_t threads = (size_t)ctx->nf * (khi - klo) * 4;
+    size_t threads = (size_t)ctx->nf * ncoef * 4;
     hipLaunchKernelGGL(k_probes_grad<R>, dim3(nb(threads, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), P, ctx->d_probe, ctx->d_pind,
-                       klo, khi - klo, ctx->d_pcoef, ctx->d_err);
+                       klo, ncoef, ctx->d_pcoef, ctx->d_err);
   }
   MELT_HIP(hipGetLastError());
   return 0;
@@ -1192,7 +1208,7 @@ static int ibm_probes_t(melt_ctx* ctx, int* pind_out, bool coefs_ready) {
   if (khi > klo && coefs_ready) {
     size_t threads = (size_t)ctx->nf * (khi - klo) * 3;
     hipLaunchKernelGGL(k_probes_apply<R>, dim3(nb(threads, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), P, ctx->d_pcoef, ctx->d_probe,
-                       pind, klo, khi - klo);
+                       pind, klo, khi - klo, probe_coef_slots(P));
   } else if (khi > klo) {
     size_t threads = (size_t)ctx->nf * (khi - klo) * 4;
     hipLaunchKernelGGL(k_probes_grad<R>, dim3(nb(threads, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), P, ctx->d_probe, pind, klo,
