@@ -187,7 +187,7 @@ int melt_velforce(melt_ctx* ctx);     /* velforce.f90:1-59                      
 int melt_tempforce(melt_ctx* ctx);    /* tempforce.f90:1-46                         */
 int melt_particle(melt_ctx* ctx);     /* particle.f90:1-229 (IREMESH=0)             */
 
-/* Diagnostics used by the driver around tschem. */
+/* Diagnostics the reference's time loop (gcurv.f90) calls around tschem. */
 int melt_cfl(melt_ctx* ctx, double* cflm);     /* cfl.f90:1-30 (max over cells)     */
 int melt_divgck(melt_ctx* ctx, double* qmax);  /* divg.f90:26-56                    */
 int melt_inqpr(melt_ctx* ctx);                 /* inqpr.f90:3-44                    */

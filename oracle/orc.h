@@ -14,7 +14,7 @@
  * Fortran intrinsics restated: NINT rounds half away from zero, MODULO is the
  * non-negative remainder, SIGN(1,x) takes the sign bit of x.
  *
- * Deviations (each also listed with its check in the oracle section of DESIGN.md):
+ * Deviations (each pinned by the check named with it in tests/test_oracle.py):
  *  - FFTW (phcalc.f90) is replaced by an own mixed-radix FFT; the Poisson result is
  *    pinned by its defining property (discrete Laplacian of phi == rhs, mean 0).
  *  - LAPACK dgetrf/dgetri and BLAS dgemm/dgemv are restated from the reference
