@@ -66,6 +66,21 @@ void graph_drop(melt_ctx* ctx) {
   ctx->graph = nullptr;
 }
 
+// tschem queues the next call's first tagging and probe geometry stage at the
+// end of its last substep.  A call that changes what those read (the faces,
+// the body) or what the tagging writes (VOFp, sdf) cancels them, and drops a
+// captured step (whose first substep relies on them); the next substep then
+// does both itself.
+void cancel_ahead(melt_ctx* ctx) {
+  if (!ctx->tag_ahead && !ctx->coef_ahead) return;
+  ctx->tag_ahead = 0;
+  ctx->coef_ahead = 0;
+  graph_drop(ctx);
+}
+bool tag_field(int field) {
+  return field == MELT_VOFP || field == MELT_VOFX || field == MELT_VOFY || field == MELT_VOFZ || field == MELT_SDF;
+}
+
 }  // namespace
 
 // --------------------------------------------------------------- timing
@@ -292,6 +307,7 @@ static int copy_from_device(melt_ctx* ctx, int field, double* dst) {
 
 extern "C" int melt_upload_field(melt_ctx* ctx, int field, const double* host, const melt_layout* lay) {
   if (check_ctx(ctx) || field < 0 || field >= MELT_NFIELDS || !host) return MELT_ERR_ARG;
+  if (tag_field(field)) cancel_ahead(ctx);
   melt_layout L = lay ? *lay : melt_layout{ctx->n1, ctx->n2, 0, 0};
   if (L.ld1 < ctx->n1 || L.ld2 < ctx->n2) return melt_fail(ctx, MELT_ERR_ARG, "upload", "leading dimension too small");
   MELT_HIP(hipStreamSynchronize(ctx->stream));
@@ -325,6 +341,7 @@ extern "C" int melt_download_field(melt_ctx* ctx, int field, double* host, const
 
 extern "C" int melt_fill_field(melt_ctx* ctx, int field, double value) {
   if (check_ctx(ctx) || field < 0 || field >= MELT_NFIELDS) return MELT_ERR_ARG;
+  if (tag_field(field)) cancel_ahead(ctx);
   fill_field(ctx, field, value);
   MELT_HIP(hipGetLastError());
   return 0;
@@ -358,6 +375,7 @@ extern "C" int melt_get_body(melt_ctx* ctx, melt_body* out) {
 
 extern "C" int melt_set_body(melt_ctx* ctx, const melt_body* in) {
   if (check_ctx(ctx) || !in) return MELT_ERR_ARG;
+  cancel_ahead(ctx);
   ctx->h_body = *in;
   MELT_HIP(hipMemcpyAsync(ctx->d_body, &ctx->h_body, sizeof(melt_body), hipMemcpyHostToDevice, ctx->stream));
   MELT_HIP(hipStreamSynchronize(ctx->stream));
@@ -379,6 +397,7 @@ extern "C" int melt_get_lag(melt_ctx* ctx, int id, double* host) {
 
 extern "C" int melt_set_lag(melt_ctx* ctx, int id, const double* host) {
   if (check_ctx(ctx) || id < 0 || id >= MELT_L_NLAG || !host || !ctx->has_geom) return MELT_ERR_ARG;
+  cancel_ahead(ctx);
   MELT_HIP(hipStreamSynchronize(ctx->stream));
   MELT_HIP(hipMemcpy(ctx->d_lag[id], host, sizeof(double) * lag_count(ctx, id), hipMemcpyHostToDevice));
   return 0;
@@ -423,6 +442,7 @@ static int need_stage(melt_ctx* ctx) {
 // ---------------------------------------------------- per-routine entries
 extern "C" int melt_tagcells(melt_ctx* ctx) {
   if (check_ctx(ctx)) return MELT_ERR_ARG;
+  cancel_ahead(ctx);
   int rc = ibm_tag(ctx);
   if (rc) return rc;
   return euler_launch_interp_vof(ctx);
@@ -536,6 +556,7 @@ static int particle_impl(melt_ctx* ctx, int defer) {
 
 extern "C" int melt_particle(melt_ctx* ctx) {
   if (check_ctx(ctx)) return MELT_ERR_ARG;
+  cancel_ahead(ctx);
   int rc = need_geom(ctx);
   if (rc || (rc = need_stage(ctx))) return rc;
   return particle_impl(ctx, 0);
@@ -578,9 +599,13 @@ static int side_join(melt_ctx* ctx) {
 // queued on the side stream right after this substep's Lagrangian update: it
 // reads only the geometry that update writes and writes only VOFp, sdf, its
 // own scratch and the window box, none of which the pressure phase touches,
-// so it overlaps the pressure solve; so does the geometry part of the next
-// substep's gradient probes (it writes only its coefficient buffer).
-static int substep_impl(melt_ctx* ctx, int32_t ns, bool tag_next) {
+// so it overlaps the pressure solve; so does, with coef_next (every substep
+// of tschem, also for the next call's first substep: a caller's change of the
+// faces or body in between cancels it), the geometry part of the next
+// substep's gradient probes (it writes only its coefficient buffer).  The
+// tagging itself is not queued across calls: VOFp and sdf after tschem hold
+// the last substep's tagging, as in the reference.
+static int substep_impl(melt_ctx* ctx, int32_t ns, bool tag_next, bool coef_next) {
   if (check_ctx(ctx)) return MELT_ERR_ARG;
   int rc = melt_set_stage(ctx, ns);
   if (rc || (rc = need_stage(ctx))) return rc;
@@ -665,10 +690,10 @@ static int substep_impl(melt_ctx* ctx, int32_t ns, bool tag_next) {
             ctx->force_window_from_tag = 0;
             timing_end(ctx, SLOT_TAG, e);
             if (!r) ctx->tag_ahead = 1;
-            if (!r && probes) {
-              r = ibm_probe_coefs(ctx);
-              if (!r) ctx->coef_ahead = 1;
-            }
+          }
+          if (!r && coef_next && probes) {
+            r = ibm_probe_coefs(ctx);
+            if (!r) ctx->coef_ahead = 1;
           }
           return r;
         })))
@@ -706,7 +731,7 @@ static int substep_impl(melt_ctx* ctx, int32_t ns, bool tag_next) {
   return 0;
 }
 
-extern "C" int melt_substep(melt_ctx* ctx, int32_t ns) { return substep_impl(ctx, ns, false); }
+extern "C" int melt_substep(melt_ctx* ctx, int32_t ns) { return substep_impl(ctx, ns, false, false); }
 
 static int tschem_launch(melt_ctx* ctx) {
   if (ctx->has_geom) {
@@ -714,7 +739,7 @@ static int tschem_launch(melt_ctx* ctx) {
     MELT_HIP(hipGetLastError());
   }
   for (int ns = 1; ns <= 3; ++ns) {
-    int rc = substep_impl(ctx, ns, ns < 3 && !one_stream(ctx));
+    int rc = substep_impl(ctx, ns, ns < 3 && !one_stream(ctx), !one_stream(ctx));
     if (rc) { ctx->tag_ahead = 0; ctx->coef_ahead = 0; return rc; }
   }
   return 0;
