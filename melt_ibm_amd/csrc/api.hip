@@ -680,8 +680,7 @@ static int substep_impl(melt_ctx* ctx, int32_t ns, bool tag_next, bool coef_next
     // rejoins before the substep ends (the next tagging needs the geometry).
     if ((rc = side_run(ctx, [&] {
           timing_begin(ctx, SLOT_LAGR, &e);
-          int r = ibm_commit_probe_indices(ctx);
-          if (!r) r = lagr_after_force(ctx);
+          int r = lagr_after_force(ctx, true);
           timing_end(ctx, SLOT_LAGR, e);
           if (!r && tag_next && moving) {
             timing_begin(ctx, SLOT_TAG, &e);

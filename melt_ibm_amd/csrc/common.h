@@ -461,13 +461,13 @@ int ibm_sub_mean(melt_ctx* ctx);
 int ibm_velforce(melt_ctx* ctx, int defer);
 int ibm_tempforce(melt_ctx* ctx);
 // pind_out: where the heat-flux probes' cell indices go (null: d_pind, as
-// mls_heatFlux does); ibm_commit_probe_indices copies d_pind2 into d_pind.
+// mls_heatFlux does); lagr_after_force(ctx, true) copies d_pind2 into d_pind.
 int ibm_probes(melt_ctx* ctx, int* pind_out = nullptr, bool coefs_ready = false);
 // geometry part of the gradient probes into d_pcoef (then ibm_probes(..., true))
 int ibm_probe_coefs(melt_ctx* ctx);
-int ibm_commit_probe_indices(melt_ctx* ctx);
 int ibm_window_reset(melt_ctx* ctx);
-int lagr_after_force(melt_ctx* ctx);
+// with commit_pind (the fused substep) it also commits the staged heat-flux probe indices
+int lagr_after_force(melt_ctx* ctx, bool commit_pind = false);
 int lagr_alloc(melt_ctx* ctx, const double* xyz0, const int* voe, const int* eof, const int* vof, double rad_p);
 void timing_begin(melt_ctx* ctx, int slot, hipEvent_t* ev);
 void timing_end(melt_ctx* ctx, int slot, hipEvent_t ev);

@@ -1220,22 +1220,6 @@ static int ibm_probes_t(melt_ctx* ctx, int* pind_out, bool coefs_ready) {
   return 0;
 }
 
-// The heat-flux probes' indices of every live face (the 6 words the probes
-// wrote) into d_pind.
-__global__ __launch_bounds__(256) void k_commit_probe_idx(int* __restrict__ pind, const int* __restrict__ pind2, const unsigned char* __restrict__ gf,
-                                                          int nf) {
-  int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (t < 6 * nf && !gf[t / 6]) pind[t] = pind2[t];
-}
-
-int ibm_commit_probe_indices(melt_ctx* ctx) {
-  if (ctx->cfg.imelt != 1) return 0;   // only the heat-flux probes (kinds 0, 1) write indices
-  hipLaunchKernelGGL(k_commit_probe_idx, dim3(nb((size_t)6 * ctx->nf, 256)), dim3(256), 0, ctx->stream, ctx->d_pind, ctx->d_pind2, ctx->d_gf,
-                     ctx->nf);
-  MELT_HIP(hipGetLastError());
-  return 0;
-}
-
 int ibm_window_reset(melt_ctx* ctx) {
   hipLaunchKernelGGL(k_window_reset, dim3(1), dim3(1), 0, ctx->stream, ctx->d_win, 2);
   MELT_HIP(hipGetLastError());

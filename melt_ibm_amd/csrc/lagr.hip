@@ -57,10 +57,13 @@ __device__ inline void face_rigid(const double* xyz, const int* vof, int f, doub
   o[9] = v * (z1 * y1 + z2 * y2 + z3 * y3 + z4 * y4);
 }
 
+
 // Stage 1 of a deterministic two-stage sum: block b's threads walk faces
 // b*RB + tid, + gridDim*RB, ... in order, then a fixed tree writes the block's
 // partial to part[q * gridDim + b]; k_sum_partials adds the partials in a
-// fixed tree.  Output layout per mode:
+// fixed tree (a second launch: finishing in the last block would need a
+// device-scope fence per block, which writes back the L2 of every XCD and
+// measured slower).  Output layout per mode:
 //  SUM_VOLUME: [vol]; SUM_RIGID: [10 rigid terms]; SUM_LOADS: [prn 3, rxp 3, tau 3, rxt 3];
 //  SUM_AREA_VOL_RIGID: [area, vol, 10 rigid terms].
 __global__ __launch_bounds__(RB) void k_face_sums(LagArgs A, const double* __restrict__ xyz, const double* __restrict__ rxp,
@@ -187,11 +190,16 @@ __global__ void k_body_rigid(melt_body* b, const double* __restrict__ s, double 
 // small-volume abort (particle.f90:218-224)
 
 // ---------------------------------------------------------- heat / Stefan
+// With pind2, also commits the heat-flux probes' cell indices that the fused
+// substep staged there (the spreading had still read the old ones).
 __global__ __launch_bounds__(256) void k_face_post(LagArgs A, const double* __restrict__ probe, const melt_body* __restrict__ b,
-                                                   double* __restrict__ rxp, double* __restrict__ rxt, double ren, int heat, int loads) {
+                                                   double* __restrict__ rxp, double* __restrict__ rxt, double ren, int heat, int loads,
+                                                   int* __restrict__ pind, const int* __restrict__ pind2) {
   int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= A.nf || A.gf[f]) return;
   const double* pr = &probe[(size_t)PRB_N * f];
+  if (pind2)
+    for (int q = 0; q < 6; ++q) pind[6 * f + q] = pind2[6 * f + q];
   if (heat) { A.L[MELT_L_QW_O][f] = pr[PRB_QO]; A.L[MELT_L_QW_I][f] = pr[PRB_QI]; }
   if (!loads) return;
   const double* nor = &A.L[MELT_L_TRI_NOR][3 * f];
@@ -295,6 +303,34 @@ __global__ __launch_bounds__(256) void k_edge_len(LagArgs A, const double* __res
 
 __global__ void k_clear_rm(melt_body* b) { b->rm_flag = 0; }
 
+// calculate_area (triaux.f90:53-91), calc_centroids_from_vert (:247-262) and
+// update_tri_normal of every live face in one pass (the same expressions as
+// k_face_area and k_face_centroid_normal); thread 0 also clears the remesh
+// flag that k_edge_len then sets.
+__global__ __launch_bounds__(256) void k_face_geom(LagArgs A, const double* __restrict__ x, melt_body* b) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f == 0) b->rm_flag = 0;
+  if (f >= A.nf || A.gf[f]) return;
+  int v[3] = {A.vof[3 * f], A.vof[3 * f + 1], A.vof[3 * f + 2]};
+  double dd[3];
+  for (int e = 0; e < 3; ++e) {
+    int a = v[e], c = v[(e + 1) % 3];
+    double q0 = x[3 * a] - x[3 * c], q1 = x[3 * a + 1] - x[3 * c + 1], q2 = x[3 * a + 2] - x[3 * c + 2];
+    dd[e] = sqrt(q0 * q0 + q1 * q1 + q2 * q2);
+  }
+  double sp = (dd[0] + dd[1] + dd[2]) / 2.;
+  A.L[MELT_L_SUR][f] = sqrt(sp * (sp - dd[0]) * (sp - dd[1]) * (sp - dd[2]));
+  const int v1 = v[0], v2 = v[1], v3 = v[2];
+  for (int d = 0; d < 3; ++d) A.L[MELT_L_TRI_BAR][3 * f + d] = (A.L[MELT_L_XYZV][3 * v1 + d] + A.L[MELT_L_XYZV][3 * v2 + d] + A.L[MELT_L_XYZV][3 * v3 + d]) / 3.0;
+  double e1[3], e2[3], n[3];
+  for (int d = 0; d < 3; ++d) { e1[d] = x[3 * v2 + d] - x[3 * v1 + d]; e2[d] = x[3 * v3 + d] - x[3 * v1 + d]; }
+  n[0] = e1[1] * e2[2] - e1[2] * e2[1];
+  n[1] = e1[2] * e2[0] - e1[0] * e2[2];
+  n[2] = e1[0] * e2[1] - e1[1] * e2[0];
+  double s = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  for (int d = 0; d < 3; ++d) A.L[MELT_L_TRI_NOR][3 * f + d] = n[d] / s;
+}
+
 __global__ __launch_bounds__(256) void k_skewness(LagArgs A) {  // triaux.f90:264-290
   int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= A.nf || A.gf[f]) return;
@@ -330,18 +366,39 @@ __global__ __launch_bounds__(256) void k_cfac(LagArgs A, const melt_body* __rest
   A.L[MELT_L_CFAC][f] = (A.L[MELT_L_SUR][f] * b->h_eulerian) / b->celvol;
 }
 
-// ---------------------------------------------------------- rigid body
-__global__ __launch_bounds__(256) void k_rel(LagArgs A, const melt_body* __restrict__ b) {  // RigidAuxRoutines.f90:20-35
+// k_skewness and (do_cfac) k_cfac over the faces and k_vert_geom over the
+// vertices in one launch of nf + nv threads (the same expressions).
+__global__ __launch_bounds__(256) void k_face_vert_post(LagArgs A, const melt_body* __restrict__ b, int do_cfac) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < A.nv) {
-    if (!A.gv[t])
-      for (int d = 0; d < 3; ++d) A.L[MELT_L_DXYZV_S][3 * t + d] = A.L[MELT_L_XYZV][3 * t + d] - b->pos_cm[d];
-  } else if (t < A.nv + A.nf) {
-    int f = t - A.nv;
-    if (!A.gf[f])
-      for (int d = 0; d < 3; ++d) A.L[MELT_L_DXYZ_S][3 * f + d] = A.L[MELT_L_TRI_BAR][3 * f + d] - b->pos_cm[d];
+  if (t < A.nf) {
+    const int f = t;
+    if (do_cfac) A.L[MELT_L_CFAC][f] = (A.L[MELT_L_SUR][f] * b->h_eulerian) / b->celvol;
+    if (A.gf[f]) return;
+    const double* el = A.L[MELT_L_ELENGTHS];
+    double perim = el[A.eof[3 * f]] + el[A.eof[3 * f + 1]] + el[A.eof[3 * f + 2]];
+    double q = perim / 3.0;
+    double sopt = sqrt(3.0) / 4.0 * (q * q);
+    A.L[MELT_L_SKEWNESS][f] = (sopt - A.L[MELT_L_SUR][f]) / sopt;
+    return;
   }
+  const int v = t - A.nf;
+  if (v >= A.nv) return;
+  double av = 0.0, n[3] = {0.0, 0.0, 0.0};
+  for (int p = A.v2f_ptr[v]; p < A.v2f_ptr[v + 1]; ++p) {
+    int f = A.v2f[p];
+    double s = A.L[MELT_L_SUR][f];
+    av = av + s / 3.0;
+    for (int d = 0; d < 3; ++d) n[d] = n[d] + s * A.L[MELT_L_TRI_NOR][3 * f + d];
+  }
+  A.L[MELT_L_AVERT][v] = av;
+  if (!A.gv[v]) {
+    double m = fnorm2(n, 3);
+    for (int d = 0; d < 3; ++d) n[d] = n[d] / m;
+  }
+  for (int d = 0; d < 3; ++d) A.L[MELT_L_VERT_NOR][3 * v + d] = n[d];
 }
+
+// ---------------------------------------------------------- rigid body
 
 __device__ void rot_matrix(double q[4], double AA[9]) {  // calc_rot_matrix RigidAuxRoutines.f90:620-651 (column-major)
   double n = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
@@ -375,8 +432,8 @@ __device__ void mv3(const double* A, const double* x, double* y) {
 
 // newton_euler (RigidAuxRoutines.f90:140-310), the periodic COM wrap of
 // update_part_pos (:44-51) and the rotation matrix update_xyz uses (:855).
-__global__ void k_newton(melt_body* b, double ga, double ro, double al, double dt, double dens_ratio, double xlen, double ylen, double zlen,
-                         double* __restrict__ aa_out, int* err) {
+__device__ void newton_update(melt_body* b, double ga, double ro, double al, double dt, double dens_ratio, double xlen, double ylen,
+                              double zlen, double* __restrict__ aa_out, int* err) {
   double vel_m1[3], pos_m1[3], om_m1[3];
   for (int d = 0; d < 3; ++d) { vel_m1[d] = b->vel_cm[d]; pos_m1[d] = b->pos_cm[d]; om_m1[d] = b->omega_c[d]; }
   double e_z[3] = {0., 0., -1.0};
@@ -425,6 +482,24 @@ __global__ void k_newton(melt_body* b, double ga, double ro, double al, double d
   }
   rot_matrix(b->quat, AA);
   for (int i = 0; i < 9; ++i) aa_out[i] = AA[i];
+}
+
+// Relative vertex and face positions (RigidAuxRoutines.f90:20-35).
+__global__ __launch_bounds__(256) void k_rel(LagArgs A, const melt_body* __restrict__ b) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < A.nv) {
+    if (!A.gv[t])
+      for (int d = 0; d < 3; ++d) A.L[MELT_L_DXYZV_S][3 * t + d] = A.L[MELT_L_XYZV][3 * t + d] - b->pos_cm[d];
+  } else if (t < A.nv + A.nf) {
+    int f = t - A.nv;
+    if (!A.gf[f])
+      for (int d = 0; d < 3; ++d) A.L[MELT_L_DXYZ_S][3 * f + d] = A.L[MELT_L_TRI_BAR][3 * f + d] - b->pos_cm[d];
+  }
+}
+
+__global__ void k_newton(melt_body* b, double ga, double ro, double al, double dt, double dens_ratio, double xlen, double ylen, double zlen,
+                         double* __restrict__ aa_out, int* err) {
+  newton_update(b, ga, ro, al, dt, dens_ratio, xlen, ylen, zlen, aa_out, err);
 }
 
 __global__ __launch_bounds__(256) void k_update_xyz(LagArgs A, const melt_body* __restrict__ b, const double* __restrict__ AA) {
@@ -483,14 +558,10 @@ LagArgs lag_args(melt_ctx* ctx) {
 int refresh_geometry(melt_ctx* ctx, LagArgs& A) {
   const double* x = ctx->d_lag[MELT_L_XYZV];
   hipStream_t s = ctx->stream;
-  hipLaunchKernelGGL(k_face_area, dim3(nb(ctx->nf, 256)), dim3(256), 0, s, A, x);
-  hipLaunchKernelGGL(k_clear_rm, dim3(1), dim3(1), 0, s, ctx->d_body);
+  hipLaunchKernelGGL(k_face_geom, dim3(nb(ctx->nf, 256)), dim3(256), 0, s, A, x, ctx->d_body);
   hipLaunchKernelGGL(k_edge_len, dim3(nb(ctx->ne, 256)), dim3(256), 0, s, A, x, ctx->d_body, ctx->h_body.e_thresh);
-  hipLaunchKernelGGL(k_face_centroid_normal, dim3(nb(ctx->nf, 256)), dim3(256), 0, s, A, x, 1, 1, 1);
-  hipLaunchKernelGGL(k_skewness, dim3(nb(ctx->nf, 256)), dim3(256), 0, s, A);
-  hipLaunchKernelGGL(k_vert_geom, dim3(nb(ctx->nv, 256)), dim3(256), 0, s, A);
+  hipLaunchKernelGGL(k_face_vert_post, dim3(nb((size_t)ctx->nf + ctx->nv, 256)), dim3(256), 0, s, A, ctx->d_body, ctx->cfg.imelt == 1 ? 1 : 0);
   face_sums(ctx, A, x, (int)SUM_AREA_VOL_RIGID, ctx->d_red, FIN_FINAL);
-  if (ctx->cfg.imelt == 1) hipLaunchKernelGGL(k_cfac, dim3(nb(ctx->nf, 256)), dim3(256), 0, s, A, ctx->d_body);
   MELT_HIP(hipGetLastError());
   return 0;
 }
@@ -499,13 +570,13 @@ int refresh_geometry(melt_ctx* ctx, LagArgs& A) {
 
 // particle.f90:62-193 after the MLS interpolation: heat flux -> vertices,
 // loads, Stefan update, Newton-Euler motion, geometry refresh.
-int lagr_after_force(melt_ctx* ctx) {
+int lagr_after_force(melt_ctx* ctx, bool commit_pind) {
   LagArgs A = lag_args(ctx);
   hipStream_t s = ctx->stream;
   int heat = ctx->cfg.imelt == 1, loads = ctx->cfg.imlsstr == 1;
   if (heat || loads)
     hipLaunchKernelGGL(k_face_post, dim3(nb(ctx->nf, 256)), dim3(256), 0, s, A, ctx->d_probe, ctx->d_body, ctx->d_rxp, ctx->d_rxt, ctx->cfg.ren,
-                       heat, loads);
+                       heat, loads, ctx->d_pind, commit_pind && heat ? ctx->d_pind2 : nullptr);
   if (loads) {
     face_sums(ctx, A, ctx->d_lag[MELT_L_XYZV], (int)SUM_LOADS, ctx->d_red, FIN_LOADS);
   }
