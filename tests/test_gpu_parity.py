@@ -378,3 +378,31 @@ def test_cpu_reference_case_100_steps(gpu_lib):
     om = np.array(bo.omega_c[:])
     assert np.max(np.abs(np.array(bg.omega_c[:]) - om)) < 1e-9 * max(np.max(np.abs(om)), 1e-3)
     assert g.divgck() < 1e-12
+
+
+def test_fp32_two_column_yz_pass_matches_two_passes(gpu_lib, monkeypatch):
+    """In the precision form with n1 % 128 == 0 the fused y/z pass takes two
+    columns per thread (8-byte accesses) and 128-plane segments; it computes
+    the same values as the separate y and z passes (yz_jb=1): two RK3 steps
+    with a melting body are bit-identical."""
+    n, level, rad = 128, 3, 1.0
+    cfg = make_cfg(n, precision=1)
+    st, dt = melt_case(make_cfg(n), level, rad, seed=21)
+    from melt_ibm_amd.inputs import inqpr_temperature
+    outs = []
+    for tune in ("", "yz_jb=1"):
+        monkeypatch.setenv("MELT_TUNE", tune)
+        c = MeltContext(cfg, lib=gpu_lib)
+        load_state(c, st)
+        with_body(c, level, rad)
+        c.upload("temp", inqpr_temperature(cfg, c.body().pos_cm[:], rad))
+        c.set_dt(dt)
+        c.tagCells()
+        for _ in range(2):
+            c.tschem()
+        c.sync()
+        outs.append([c.download(q) for q in ("vx", "vy", "vz", "temp", "pr")])
+        c.close()
+    monkeypatch.delenv("MELT_TUNE")
+    for q, a, b in zip(("vx", "vy", "vz", "temp", "pr"), *outs):
+        assert np.array_equal(a, b), q
