@@ -18,7 +18,7 @@ term, a wrong sign or neighbour index, a transposed operand) fails it:
    forms for linear fields (divergence theorem, n.grad T / Pe, area-weighted
    vertex means) and the latent-heat energy balance of a melting body;
  - geometry / rigid body: closed-form polyhedron volume, area and inertia;
-   Newton-Euler invariants.
+   Newton-Euler invariants and the rigid rotation of a spinning sphere.
 """
 import ctypes as C
 
@@ -338,6 +338,32 @@ def test_newton_euler_free_body(lib):
     al = 8.0 / 15.0
     assert abs(b.vel_cm[2] - (-al * 0.01 * (1 - 1 / 2.0))) < 1e-15
     assert max(abs(x) for x in b.omega_c[:]) < 1e-14
+
+
+def test_newton_euler_spinning_sphere_rotates_rigidly(lib):
+    # no loads, neutral buoyancy, isotropic inertia: omega is conserved and the
+    # vertices turn about the centroid by omega * alpha_1 * dt (counter-clockwise
+    # for omega along +z), which pins the quaternion and rotation-matrix path
+    cfg = make_cfg(32, imelt=0, dens_ratio=1.0, forcing=0)
+    o = oracle.context(cfg)
+    with_body(o, 2, 1.2)
+    dt, w = 0.01, 0.7
+    o.set_dt(dt)
+    o.set_stage(1)
+    z = np.zeros(cfg.shape)
+    load_state(o, {"vx": z, "vy": z, "vz": z, "pr": z, "temp": z})
+    b = o.body()
+    b.omega_c[2] = w
+    o.set_body(b)
+    c0 = np.array(o.body().pos_cm[:])
+    x0 = o.lag("xyzv")
+    o.particle()
+    b = o.body()
+    assert abs(b.omega_c[2] - w) < 1e-14 and max(abs(b.omega_c[0]), abs(b.omega_c[1])) < 1e-12
+    assert max(abs(v) for v in b.vel_cm[:]) == 0.0
+    th = w * (8.0 / 15.0) * dt
+    R = np.array([[np.cos(th), -np.sin(th), 0.0], [np.sin(th), np.cos(th), 0.0], [0.0, 0.0, 1.0]])
+    assert np.max(np.abs(o.lag("xyzv") - ((x0 - c0) @ R.T + np.array(b.pos_cm[:])))) < 1e-13
 
 
 def test_gts_reader_and_icosphere():
