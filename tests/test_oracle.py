@@ -9,7 +9,8 @@ term, a wrong sign or neighbour index, a transposed operand) fails it:
  - Poisson: the discrete Laplacian of phi reproduces the mean-free right-hand
    side; single Fourier modes divide by the closed-form eigenvalue;
  - explicit terms: axis-permutation symmetry (hdnl1 <-> hdnl2 <-> hdnl3,
-   bit-exact), telescoping of the flux form, closed-form viscous eigenvalues;
+   bit-exact), telescoping of the flux form, closed-form viscous eigenvalues,
+   hdnlte's solid-velocity term for a linear T in a solid domain;
  - projection: divergence-free to round-off;
  - tagging: narrow-band |sdf| equals a brute-force point-triangle distance over
    all faces; ray tags agree with an inside test of the polyhedron;
@@ -188,6 +189,40 @@ def test_advection_telescopes_and_shear_vanishes(lib):
     load_state(o, {**st, "vx": u, "vy": z, "vz": z})
     o.hdnl1()
     assert np.max(np.abs(o.download("dq"))) == 0.0
+
+
+def test_hdnlte_solid_velocity_term_closed_form(lib):
+    # all solid (VOFp = 0), T linear: htemp = -(U_s . grad T) with the rigid
+    # velocity U_s = V + omega x r taken at the cell's u, v, w faces
+    # (hdnlte.f90:99-153); the central differences of a linear T are exact
+    cfg = make_cfg(24, 32)
+    o = oracle.context(cfg)
+    m = with_body(o, 2, 0.8)
+    (xm, ym, zm), (xc, yc, zc) = inputs.grid_coords(cfg)
+    g = np.array([0.3, -0.7, 0.45])
+    T = g[0] * xm[None, None, :] + g[1] * ym[None, :, None] + g[2] * zm[:, None, None]
+    z = np.zeros(cfg.shape)
+    load_state(o, {"vx": z, "vy": z, "vz": z, "temp": T, "VOFp": z})
+    b = o.body()
+    V, W = np.array([0.2, -0.1, 0.05]), np.array([0.3, 0.5, -0.4])
+    for d in range(3):
+        b.vel_cm[d] = V[d]
+        b.omega_c[d] = W[d]
+    o.set_body(b)
+    pc = np.array(o.body().pos_cm[:])
+    o.hdnlte()
+    ht = o.download("htemp")
+    K, J, I = np.meshgrid(np.arange(cfg.n3m), np.arange(cfg.n2m), np.arange(cfg.n1m), indexing="ij")
+
+    def us(x, y, zz, comp):  # rigid velocity component at a point (no wrap needed near the centre)
+        r = np.stack([x - pc[0], y - pc[1], zz - pc[2]])
+        return V[comp] + np.cross(W, r, axis=0)[comp]
+    u = us(xc[I], ym[J], zm[K], 0)
+    v = us(xm[I], yc[J], zm[K], 1)
+    w = us(xm[I], ym[J], zc[K], 2)
+    expect = -(u * g[0] + v * g[1] + w * g[2])
+    inner = (slice(8, -8), slice(6, -6), slice(6, -6))   # away from T's periodic seam and the min-image switch
+    assert np.max(np.abs(ht[inner] - expect[inner])) < 1e-12
 
 
 def test_viscous_term_closed_form(lib):
