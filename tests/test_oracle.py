@@ -14,7 +14,8 @@ term, a wrong sign or neighbour index, a transposed operand) fails it:
  - projection: divergence-free to round-off;
  - tagging: narrow-band |sdf| equals a brute-force point-triangle distance over
    all faces; ray tags agree with an inside test of the polyhedron;
- - MLS: partition of unity, linear reproduction, zero force for linear fields;
+ - MLS: partition of unity, linear reproduction, zero force for linear fields,
+   the spread force totalling sum cfac (V - U~);
  - probes: structural loads, heat flux and the Stefan update against closed
    forms for linear fields (divergence theorem, n.grad T / Pe, area-weighted
    vertex means) and the latent-heat energy balance of a melting body;
@@ -340,6 +341,27 @@ def test_mls_partition_of_unity_and_linear_exactness(lib):
     o.mlsForce()
     o.velforce()
     assert np.max(np.abs(o.download("vx") - u)) < 1e-10
+
+
+def test_mls_spreading_conserves_the_face_force(lib):
+    # mlsForce spreads cfac (V - U~) phi over 27 cells per face; the shape
+    # functions sum to one, so the force added to the grid totals
+    # sum_f cfac_f (V_f - U~_f): with a fluid at rest and every face moving
+    # with V, V * sum cfac (and Tmelt * sum cfac for a fluid at T = 0)
+    cfg = make_cfg(32, tmelt=0.25)
+    o = oracle.context(cfg)
+    with_body(o, 2, 1.2)
+    z = np.zeros(cfg.shape)
+    load_state(o, {"vx": z, "vy": z, "vz": z, "temp": z})
+    o.findCentroidIndices()
+    o.mlsWeight()
+    V = np.array([0.3, -0.2, 0.15])
+    o.set_lag("vel_tri", np.tile(V, (o.nf, 1)))
+    o.mlsForce()
+    cf = o.lag("cfac").sum()
+    for q, v in (("for_xc", V[0]), ("for_yc", V[1]), ("for_zc", V[2]), ("for_temp", 0.25)):
+        tot = o.download(q).sum()
+        assert abs(tot - v * cf) < 1e-12 * abs(v * cf), q
 
 
 # ------------------------------------------------------- geometry / body
