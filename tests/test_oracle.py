@@ -364,6 +364,28 @@ def test_mls_spreading_conserves_the_face_force(lib):
         assert abs(tot - v * cf) < 1e-12 * abs(v * cf), q
 
 
+def test_velforce_removes_the_mean_tempforce_does_not(lib):
+    # velforce.f90:21-51: u += f - mean(f) (a mean-free momentum source);
+    # tempforce.f90: T += f_T as it is
+    cfg = make_cfg(32, tmelt=0.25)
+    o = oracle.context(cfg)
+    with_body(o, 2, 1.2)
+    rng = np.random.default_rng(4)
+    st = {q: rng.normal(0.0, 0.1, cfg.shape) for q in ("vx", "vy", "vz", "temp")}
+    load_state(o, st)
+    o.findCentroidIndices()
+    o.mlsWeight()
+    o.set_lag("vel_tri", np.tile([0.3, -0.2, 0.15], (o.nf, 1)))
+    o.mlsForce()
+    f = {q: o.download(fq) for q, fq in (("vx", "for_xc"), ("vy", "for_yc"), ("vz", "for_zc"), ("temp", "for_temp"))}
+    o.velforce()
+    o.tempforce()
+    for q in ("vx", "vy", "vz"):
+        du = o.download(q) - st[q]
+        assert abs(du.mean()) < 1e-15 and np.max(np.abs(du - (f[q] - f[q].mean()))) < 1e-14, q
+    assert np.max(np.abs(o.download("temp") - st["temp"] - f["temp"])) < 1e-15
+
+
 # ------------------------------------------------------- geometry / body
 def test_geometry_closed_forms(lib):
     cfg = make_cfg(64)
