@@ -385,8 +385,13 @@ __device__ inline int lu_inverse(double a[N * N]) {
 #undef A_
 }
 
-// Device error word: first error code wins.
-__device__ inline void raise_error(int* err, int code) { atomicCAS(err, 0, code); }
+// Device error word: first error code wins.  A blown-up body sends every
+// thread of a tagging or forcing pass here; reading the word first keeps them
+// from queueing compare-and-swaps on one address once it is set (3M of them
+// cost 35 ms per pass).
+__device__ inline void raise_error(int* err, int code) {
+  if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) atomicCAS(err, 0, code);
+}
 
 // Max reductions of non-negative doubles: their bit patterns order like the
 // values, so partial maxima combine with integer atomicMax.  Each block folds

@@ -128,8 +128,13 @@ class Server:
                     r.sim.ctx.sync()
                 except MeltError as e:
                     # the reference's terminal conditions end the request, not the server:
-                    # code 4 is a body melted below V_thresh (particle.f90:218-224)
-                    r.status = "melted" if e.code == 4 else f"error {e.code}"
+                    # code 4 is a volume below V_thresh (particle.f90:218-224), which a
+                    # tangled surface (melting front faster than a cell per step, no
+                    # remeshing) also trips, with a negative or non-finite volume
+                    r.status = f"error {e.code}"
+                    if e.code == 4:
+                        vol = r.sim.ctx.body().volume
+                        r.status = "melted" if np.isfinite(vol) and vol > 0.0 else "diverged"
                 r.step_ms.append((time.perf_counter() - ts) * 1e3)
                 r.done_steps += 1
                 cell_steps += r.member.cells
