@@ -228,7 +228,10 @@ int melt_graph_enable(melt_ctx* ctx, int32_t on);
  * block (8 or 16) of the pass that does the y and z solves together, 1 there
  * = two separate passes, 6 = streams: 1 keeps all of the context's work on its
  * one stream (for many contexts sharing a GPU; default: the Lagrangian update,
- * probes and next tagging run on a second stream); value 0 = built-in choice.
+ * probes and next tagging run on a second stream), 7 = where the geometry
+ * stage of the gradient probes runs: 1 after the previous substep's Lagrangian
+ * update (beside the pressure solve), 2 at the start of its own substep
+ * (beside the explicit pass); value 0 = built-in choice.
  * Results do not depend on it. */
 int melt_set_tuning(melt_ctx* ctx, int32_t key, int32_t value);
 void* melt_stream(melt_ctx* ctx);

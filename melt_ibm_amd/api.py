@@ -389,10 +389,10 @@ class MeltContext:
         """Replay tschem as one captured HIP graph."""
         self._chk(self._f["graph_enable"](self._ctx, 1 if on else 0))
 
-    TUNE_KEYS = {"xf_w": 0, "xf_nt": 1, "solve_w": 2, "solve_nt": 3, "xf_z": 4, "yz_jb": 5, "streams": 6}
+    TUNE_KEYS = {"xf_w": 0, "xf_nt": 1, "solve_w": 2, "solve_nt": 3, "xf_z": 4, "yz_jb": 5, "streams": 6, "coef": 7}
 
     def set_tuning(self, **kw) -> None:
-        """Launch-shape overrides (xf_w, xf_nt, solve_w, solve_nt, xf_z, yz_jb, streams; 0 = default)."""
+        """Launch-shape overrides (xf_w, xf_nt, solve_w, solve_nt, xf_z, yz_jb, streams, coef; 0 = default)."""
         for k, v in kw.items():
             self._chk(self._f["set_tuning"](self._ctx, self.TUNE_KEYS[k], int(v)))
 

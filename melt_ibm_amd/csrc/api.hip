@@ -175,7 +175,7 @@ extern "C" int melt_ctx_create(const melt_config* cfg, melt_ctx** out) {
   do {
     if (hipGetDevice(&ctx->device) != hipSuccess) { rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "no HIP device"); break; }
     if (const char* tv = getenv("MELT_TUNE")) {
-      static const char* keys[TUNE_N] = {"xf_w", "xf_nt", "solve_w", "solve_nt", "xf_z", "yz_jb", "streams"};
+      static const char* keys[TUNE_N] = {"xf_w", "xf_nt", "solve_w", "solve_nt", "xf_z", "yz_jb", "streams", "coef"};
       std::string spec(tv);
       size_t pos = 0;
       while (pos < spec.size()) {
@@ -570,6 +570,11 @@ extern "C" int melt_particle(melt_ctx* ctx) {
 // Tuning key streams = 1 keeps all work on the main stream (many contexts
 // sharing one GPU get their concurrency from each other).
 static bool one_stream(const melt_ctx* ctx) { return ctx->tune[TUNE_STREAMS] == 1; }
+// Where the geometry stage of the gradient probes runs: 1 = queued on the side
+// stream after the previous substep's Lagrangian update and next tagging
+// (beside the pressure solve), 2 = on the side stream at the start of the
+// substep that uses it (beside the explicit pass).
+static int coef_place(const melt_ctx* ctx) { return ctx->tune[TUNE_COEF] ? ctx->tune[TUNE_COEF] : 1; }
 static int ensure_side(melt_ctx* ctx) {
   if (!ctx->side && !one_stream(ctx)) MELT_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
   return 0;
@@ -738,7 +743,7 @@ static int tschem_launch(melt_ctx* ctx) {
     MELT_HIP(hipGetLastError());
   }
   for (int ns = 1; ns <= 3; ++ns) {
-    int rc = substep_impl(ctx, ns, ns < 3 && !one_stream(ctx), !one_stream(ctx));
+    int rc = substep_impl(ctx, ns, ns < 3 && !one_stream(ctx), !one_stream(ctx) && coef_place(ctx) == 1);
     if (rc) { ctx->tag_ahead = 0; ctx->coef_ahead = 0; return rc; }
   }
   return 0;

@@ -306,7 +306,7 @@ def test_green_line_solves(gpu_lib, monkeypatch, n, n3):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-@pytest.mark.parametrize("n3,tunes", [(64, ("xf_w=1", "xf_z=1", "xf_z=4", "yz_jb=1", "yz_jb=16", "streams=1")),
+@pytest.mark.parametrize("n3,tunes", [(64, ("xf_w=1", "xf_z=1", "xf_z=4", "yz_jb=1", "yz_jb=16", "streams=1", "coef=2")),
                                       (24, ("xf_w=1", "xf_z=3", "yz_jb=1"))])
 def test_zmarch_matches_line_kernel(gpu_lib, monkeypatch, precision, n3, tunes):
     """The explicit pass as a z-march (one x-line per block walking its planes,
@@ -314,8 +314,10 @@ def test_zmarch_matches_line_kernel(gpu_lib, monkeypatch, precision, n3, tunes):
     the same values with the same expressions, and so do the y and z solves as
     one pass (default where n3 % 16 == 0) and as two (yz_jb=1), with the
     Lagrangian update, probes and next tagging on a second stream (default) or
-    all on one (streams=1): two RK3 steps with a melting body are
-    bit-identical, whatever the launch shape."""
+    all on one (streams=1), and with the probes' geometry stage queued after
+    the previous Lagrangian update (default) or at the start of its own
+    substep (coef=2): two RK3 steps with a melting body are bit-identical,
+    whatever the launch shape."""
     n, level, rad = 64, 3, 1.0
     cfg = make_cfg(n, n3, precision=precision)
     st, dt = melt_case(make_cfg(n, n3), level, rad, seed=17)
