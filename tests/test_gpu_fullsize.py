@@ -1,8 +1,8 @@
 """Parity at the benchmark's full size: the 256^3 melting-sphere workload of
 bench.py (level-5 icosahedron, 20480 faces, low-wavenumber forcing, St 2,
-Newton-Euler on) on the GPU against the oracle from the same seeded state.
-The oracle runs with OpenMP on the host (about 1.5 s per substep on 16
-threads)."""
+Newton-Euler on) on the GPU, in FP64 against the oracle and in the precision
+form against the FP64 path, from the same seeded state.  The oracle runs
+with OpenMP on the host (about 1.5 s per substep on 16 threads)."""
 import numpy as np
 import pytest
 
