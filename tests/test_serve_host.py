@@ -1,0 +1,99 @@
+"""Host logic of the serving loop, on the CPU: first-come first-served
+admission under the byte budget, and how a terminal device error ends one
+request without ending the server.  The simulations are stand-ins that count
+steps and raise the device's small-volume error (particle.f90:218-224) at a
+chosen step, with the volume the body then has."""
+import math
+
+import pytest
+
+from melt_ibm_amd import serve
+from melt_ibm_amd.api import MeltError
+from melt_ibm_amd.ensemble import Member
+
+
+class _Body:
+    def __init__(self, volume):
+        self.volume = volume
+
+
+class _Ctx:
+    def __init__(self, fail_at, volume_after):
+        self.steps, self.fail_at, self.volume_after = 0, fail_at, volume_after
+
+    def tschem(self):
+        self.steps += 1
+
+    def sync(self):
+        if self.fail_at and self.steps == self.fail_at:
+            raise MeltError(4, "device: Geometry V/VE is smaller than threshold")
+
+    def body(self):
+        failed = self.fail_at and self.steps >= self.fail_at
+        return _Body(self.volume_after if failed else 0.5)
+
+
+class _Sim:
+    behaviour = {}     # member seed -> (fail_at step or 0, volume after the failure)
+    built = []
+
+    def __init__(self, m):
+        self.ctx = _Ctx(*self.behaviour.get(m.seed, (0, 0.0)))
+        self.closed = False
+
+    @classmethod
+    def build(cls, m, precision, lib, tuning, graph=True):
+        s = cls(m)
+        cls.built.append((m.seed, s))
+        return s
+
+    def close(self):
+        self.closed = True
+
+
+@pytest.fixture
+def fake_sims(monkeypatch):
+    _Sim.behaviour, _Sim.built = {}, []
+    monkeypatch.setattr(serve, "Simulation", _Sim)
+    return _Sim
+
+
+def _member(seed, steps, n=64):
+    return Member(n=n, rad=0.5, st=1.0, re=500.0, seed=seed, level=3, steps=steps)
+
+
+def test_terminal_errors_end_only_their_request(fake_sims):
+    fake_sims.behaviour = {1: (2, 1e-7), 2: (3, -7.4e33), 3: (2, math.nan)}
+    reqs = [serve.Request(i, 0.0, _member(i, 5)) for i in range(4)]
+    srv = serve.Server(budget_bytes=1e12, precision=1)
+    rep = srv.run(reqs)
+    status = {r.rid: r.status for r in srv.done}
+    assert status == {0: "ok", 1: "melted", 2: "diverged", 3: "diverged"}
+    steps = {r.rid: r.done_steps for r in srv.done}
+    assert steps == {0: 5, 1: 2, 2: 3, 3: 2}
+    assert rep["status"] == {"ok": 1, "melted": 1, "diverged": 2}
+    assert srv.used == 0 and not srv.active
+    assert all(s.closed for _, s in fake_sims.built)
+
+
+def test_admission_is_first_come_first_served_within_the_budget(fake_sims):
+    big, small = _member(0, 3, n=128), _member(1, 1, n=64)
+    later = _member(2, 1, n=64)
+    b_big, b_small = serve.request_bytes(big, 1), serve.request_bytes(small, 1)
+    budget = b_big + b_small - 1          # the big request and a small one never fit together
+    reqs = [serve.Request(0, 0.0, big), serve.Request(1, 0.0, small), serve.Request(2, 0.0, later)]
+    srv = serve.Server(budget_bytes=budget, precision=1)
+    rep = srv.run(reqs)
+    assert [seed for seed, _ in fake_sims.built] == [0, 1, 2]
+    admitted = {r.rid: r.admitted for r in srv.done}
+    finished = {r.rid: r.finished for r in srv.done}
+    # the small requests wait behind the big one, although they arrived at the same time
+    assert admitted[1] >= finished[0] and admitted[2] >= finished[0]
+    assert rep["peak_budget_gb"] * 1e9 <= budget
+    assert rep["requests"] == 3 and rep["status"] == {"ok": 3}
+
+
+def test_request_larger_than_the_budget_is_refused(fake_sims):
+    m = _member(0, 1, n=256)
+    with pytest.raises(ValueError, match="exceeds the HBM budget"):
+        serve.Server(budget_bytes=serve.request_bytes(m, 1) - 1, precision=1).run([serve.Request(0, 0.0, m)])
