@@ -1,4 +1,4 @@
-"""Host logic of the serving loop, on the CPU: first-come first-served
+"""Host logic of the serving loop and the ensemble, on the CPU: first-come first-served
 admission under the byte budget, and how a terminal device error ends one
 request without ending the server.  The simulations are stand-ins that count
 steps and raise the device's small-volume error (particle.f90:218-224) at a
@@ -97,3 +97,33 @@ def test_request_larger_than_the_budget_is_refused(fake_sims):
     m = _member(0, 1, n=256)
     with pytest.raises(ValueError, match="exceeds the HBM budget"):
         serve.Server(budget_bytes=serve.request_bytes(m, 1) - 1, precision=1).run([serve.Request(0, 0.0, m)])
+
+
+def test_pencil_overflow_steps_the_mesh_down(monkeypatch):
+    """A member whose mesh puts more than n1m faces in one ray-tagging pencil
+    (pencil_rayTag.f90:126-130, device error 5) is rebuilt one icosahedron
+    level coarser (or at 3/4 of its frequency); other errors propagate."""
+    from melt_ibm_amd import ensemble
+    tried = []
+
+    def fake_build(cls, m, precision, lib, tuning, graph):
+        f = m.freq or 2 ** m.level
+        tried.append(f)
+        if f > 8:
+            raise MeltError(5, "device: pencil buffer overflow")
+        return ("built", f)
+
+    monkeypatch.setattr(ensemble.Simulation, "_build", classmethod(fake_build))
+    assert ensemble.Simulation.build(Member(n=32, rad=1.0, st=1.0, re=500.0, seed=0, level=5)) == ("built", 8)
+    assert tried == [32, 16, 8]
+    tried.clear()
+    assert ensemble.Simulation.build(Member(n=32, rad=1.0, st=1.0, re=500.0, seed=0, freq=20)) == ("built", 8)
+    assert tried == [20, 15, 11, 8]
+
+    def other_error(cls, m, precision, lib, tuning, graph):
+        raise MeltError(3, "device: window overflow")
+
+    monkeypatch.setattr(ensemble.Simulation, "_build", classmethod(other_error))
+    with pytest.raises(MeltError) as ei:
+        ensemble.Simulation.build(Member(n=32, rad=1.0, st=1.0, re=500.0, seed=0, level=4))
+    assert ei.value.code == 3
