@@ -133,7 +133,10 @@ class Server:
                     # remeshing) also trips, with a negative or non-finite volume
                     r.status = f"error {e.code}"
                     if e.code == 4:
-                        vol = r.sim.ctx.body().volume
+                        try:
+                            vol = r.sim.ctx.body().volume
+                        except MeltError:
+                            vol = float("nan")
                         r.status = "melted" if np.isfinite(vol) and vol > 0.0 else "diverged"
                 r.step_ms.append((time.perf_counter() - ts) * 1e3)
                 r.done_steps += 1

@@ -30,6 +30,8 @@ class _Ctx:
 
     def body(self):
         failed = self.fail_at and self.steps >= self.fail_at
+        if failed and self.volume_after is None:
+            raise MeltError(1, "hipMemcpy: unspecified launch failure")
         return _Body(self.volume_after if failed else 0.5)
 
 
@@ -63,15 +65,16 @@ def _member(seed, steps, n=64):
 
 
 def test_terminal_errors_end_only_their_request(fake_sims):
-    fake_sims.behaviour = {1: (2, 1e-7), 2: (3, -7.4e33), 3: (2, math.nan)}
-    reqs = [serve.Request(i, 0.0, _member(i, 5)) for i in range(4)]
+    # 4: the body cannot even be read back after the error
+    fake_sims.behaviour = {1: (2, 1e-7), 2: (3, -7.4e33), 3: (2, math.nan), 4: (1, None)}
+    reqs = [serve.Request(i, 0.0, _member(i, 5)) for i in range(5)]
     srv = serve.Server(budget_bytes=1e12, precision=1)
     rep = srv.run(reqs)
     status = {r.rid: r.status for r in srv.done}
-    assert status == {0: "ok", 1: "melted", 2: "diverged", 3: "diverged"}
+    assert status == {0: "ok", 1: "melted", 2: "diverged", 3: "diverged", 4: "diverged"}
     steps = {r.rid: r.done_steps for r in srv.done}
-    assert steps == {0: 5, 1: 2, 2: 3, 3: 2}
-    assert rep["status"] == {"ok": 1, "melted": 1, "diverged": 2}
+    assert steps == {0: 5, 1: 2, 2: 3, 3: 2, 4: 1}
+    assert rep["status"] == {"ok": 1, "melted": 1, "diverged": 3}
     assert srv.used == 0 and not srv.active
     assert all(s.closed for _, s in fake_sims.built)
 
