@@ -176,13 +176,25 @@ def summarize_pmc(fetch_db, write_db, cfetch_db, cwrite_db, command: str, out_md
         "| kernel | dispatches | read GB | write GB | total GB per launch |", "|---|---|---|---|---|", *rows, ""]))
 
 
-def summarize_trace(db: str, command: str, out_md: Path, top: int = 30) -> None:
-    """Per-kernel average duration from a rocprofv3 --kernel-trace database."""
+def summarize_trace(db: str, command: str, out_md: Path, top: int = 30, leg: str = "fp64") -> None:
+    """Per-kernel average duration from a rocprofv3 --kernel-trace database.
+    The FP64 leg runs before the precision-form leg in one process, so the
+    first dispatch of an FP32 kernel splits the trace: leg "fp64" keeps what
+    comes before it, leg "mixed" what comes after (its FP64 immersed-boundary
+    and Lagrangian kernels included)."""
     import sqlite3
     con = sqlite3.connect(db)
-    rows = list(con.execute("select name, count(*), avg(duration), sum(duration) from kernels group by name order by sum(duration) desc"))
+    split = con.execute("select min(start) from kernels where name like '%<float%'").fetchone()[0]
+    where = ""
+    if split is not None:
+        where = f"where start < {split}" if leg == "fp64" else f"where start >= {split}"
+    elif leg != "fp64":
+        raise SystemExit("no precision-form kernels in this trace")
+    rows = list(con.execute(f"select name, count(*), avg(duration), sum(duration) from kernels {where} "
+                            "group by name order by sum(duration) desc"))
     tot = sum(r[3] for r in rows)
-    lines = ["# Kernel times, 256^3 melting-sphere workload", "", "Command (GPU box):", "", "```", command, "```", "",
+    title = "# Kernel times, 256^3 melting-sphere workload" + (", precision form" if leg != "fp64" else "")
+    lines = [title, "", "Command (GPU box):", "", "```", command, "```", "",
              "| kernel | calls | avg us | total us | share |", "|---|---|---|---|---|"]
     for name, n, avg, sm in rows[:top]:
         lines.append(f"| `{_kernel_short(name)}` | {n} | {avg / 1e3:.1f} | {sm / 1e3:.1f} | {100 * sm / tot:.1f}% |")
@@ -306,6 +318,7 @@ def main():
     ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_DB", "WRITE_DB", "COPY_FETCH_DB", "COPY_WRITE_DB"),
                     help="write the HBM-traffic summary under profiles/ from rocprofv3 --pmc databases")
     ap.add_argument("--summarize-trace", metavar="TRACE_DB", help="write the kernel-time summary under profiles/")
+    ap.add_argument("--trace-leg", choices=("fp64", "mixed"), default="fp64", help="which leg of the trace to summarize")
     ap.add_argument("--command", default="", help="command line recorded in a summary")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run plain device copies of 1 GiB (calibrates the HBM byte counters of a profiler)")
@@ -323,7 +336,8 @@ def main():
         summarize_pmc(*args.summarize_pmc, args.command, PROFILES / "hbm_traffic_256cube_melting_sphere.md", TRAFFIC_JSON)
         return
     if args.summarize_trace:
-        summarize_trace(args.summarize_trace, args.command, PROFILES / "kernel_times_256cube_melting_sphere.md")
+        name = "kernel_times_256cube_melting_sphere" + ("_precision_form" if args.trace_leg == "mixed" else "")
+        summarize_trace(args.summarize_trace, args.command, PROFILES / f"{name}.md", leg=args.trace_leg)
         return
     if args.copy_probe:
         import torch
