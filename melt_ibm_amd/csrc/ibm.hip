@@ -126,7 +126,7 @@ __device__ inline double dweight(double pc, double gcrd, double nrm, double scal
 // active = 0 lanes only take part in the barrier.  Returns the LU info of A
 // on lanes 1..3 (0 elsewhere); *grad_d receives component lane-1.
 template <typename R>
-__device__ int mls_grad_lanes(const Grid& g, const R* __restrict__ fld, const double pos[3], int ic, int jc, const int kz[3],
+__device__ __forceinline__ int mls_grad_lanes(const Grid& g, const R* __restrict__ fld, const double pos[3], int ic, int jc, const int kz[3],
                               const int ke[3], const int c[3], int lane, bool active, double* shA, double* grad_d,
                               double* coef = nullptr, size_t cstride = 0) {
   const double sx = WSCL / g.dx1, sy = WSCL / g.dx2, sz = WSCL / g.dx3;
@@ -149,19 +149,25 @@ __device__ int mls_grad_lanes(const Grid& g, const R* __restrict__ fld, const do
       w3[q] = mls_gaussian(n3); dw3[q] = dweight(pos[2], zs[q], n3, sz);
     }
 #pragma unroll 1
-    for (int nk = 0; nk < 3; ++nk)
+    for (int nk = 0; nk < 3; ++nk) {
+      // the z loop stays rolled (code size); selecting its plane's values
+      // keeps the arrays in registers (an index by nk would put them in scratch)
+      const double zk = nk == 0 ? zs[0] : nk == 1 ? zs[1] : zs[2];
+      const double w3k = nk == 0 ? w3[0] : nk == 1 ? w3[1] : w3[2];
+      const double dw3k = nk == 0 ? dw3[0] : nk == 1 ? dw3[1] : dw3[2];
 #pragma unroll
       for (int jq = 0; jq < 3; ++jq)
 #pragma unroll
         for (int iq = 0; iq < 3; ++iq) {
-          double p[4] = {1.0, xs[iq], ys[jq], zs[nk]};
+          double p[4] = {1.0, xs[iq], ys[jq], zk};
           double wgt;
-          if (lane == 0) wgt = w1[iq] * (w2[jq] * w3[nk]);
-          else if (lane == 1) wgt = dw1[iq] * w2[jq] * w3[nk];
-          else if (lane == 2) wgt = w1[iq] * dw2[jq] * w3[nk];
-          else wgt = w1[iq] * w2[jq] * dw3[nk];
+          if (lane == 0) wgt = w1[iq] * (w2[jq] * w3k);
+          else if (lane == 1) wgt = dw1[iq] * w2[jq] * w3k;
+          else if (lane == 2) wgt = w1[iq] * dw2[jq] * w3k;
+          else wgt = w1[iq] * w2[jq] * dw3k;
           rank1(M, wgt, p);
         }
+    }
     if (lane == 0)
 #pragma unroll
       for (int q = 0; q < 16; ++q) shA[q] = M[q];
@@ -183,8 +189,9 @@ __device__ int mls_grad_lanes(const Grid& g, const R* __restrict__ fld, const do
     for (int i = 0; i < 4; ++i) gam[i] = gam[i] + t * A[i + 4 * j];
   }
   const int d = lane - 1;
-  double pv[4] = {0.0, 0.0, 0.0, 0.0};
-  pv[d + 1] = 1.0;
+  double pv[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) pv[i] = i == d + 1 ? 1.0 : 0.0;   // unit vector e_(d+1), in registers
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     double t = -1.0 * gam[j];
@@ -201,14 +208,18 @@ __device__ int mls_grad_lanes(const Grid& g, const R* __restrict__ fld, const do
   }
   double s = 0.0;
 #pragma unroll 1
-  for (int nk = 0; nk < 3; ++nk)
+  for (int nk = 0; nk < 3; ++nk) {
+    const double zk = nk == 0 ? zs[0] : nk == 1 ? zs[1] : zs[2];
+    const double w3k = nk == 0 ? w3[0] : nk == 1 ? w3[1] : w3[2];
+    const double dw3k = nk == 0 ? dw3[0] : nk == 1 ? dw3[1] : dw3[2];
+    const int kk = nk == 0 ? ke[0] : nk == 1 ? ke[1] : ke[2];
 #pragma unroll
     for (int jq = 0; jq < 3; ++jq)
 #pragma unroll
       for (int iq = 0; iq < 3; ++iq) {
-        double wtx = w1[iq] * (w2[jq] * w3[nk]);
-        double p[4] = {1.0, xs[iq], ys[jq], zs[nk]};
-        double dw = d == 0 ? dw1[iq] * w2[jq] * w3[nk] : d == 1 ? w1[iq] * dw2[jq] * w3[nk] : w1[iq] * w2[jq] * dw3[nk];
+        double wtx = w1[iq] * (w2[jq] * w3k);
+        double p[4] = {1.0, xs[iq], ys[jq], zk};
+        double dw = d == 0 ? dw1[iq] * w2[jq] * w3k : d == 1 ? w1[iq] * dw2[jq] * w3k : w1[iq] * w2[jq] * dw3k;
         double acc = 0.0;
 #pragma unroll
         for (int l = 0; l < 4; ++l) acc = acc + (1.0 * (wtx * p[l])) * dg[l];
@@ -217,10 +228,11 @@ __device__ int mls_grad_lanes(const Grid& g, const R* __restrict__ fld, const do
         if (coef) {   // geometry only: the point's shape-function derivative
           coef[(size_t)(9 * nk + 3 * jq + iq) * cstride] = acc;
         } else {
-          double val = fld[widx(g, ic - 1 + iq, jc - 1 + jq, ke[nk])];
+          double val = fld[widx(g, ic - 1 + iq, jc - 1 + jq, kk)];
           s = s + acc * val;
         }
       }
+  }
   *grad_d = s;
   return 0;
 }
@@ -950,15 +962,17 @@ __global__ __launch_bounds__(256) void k_probes_apply(Grid g, ProbeArgs<R> P, co
   const double* c = coef + (size_t)3 * ((size_t)coef_slot(kind, P.heat, P.loads) * P.nf + f) + d;
   double s = 0.0;
 #pragma unroll 1
-  for (int nk = 0; nk < 3; ++nk)
+  for (int nk = 0; nk < 3; ++nk) {
+    const int kk = nk == 0 ? S.ke[0] : nk == 1 ? S.ke[1] : S.ke[2];   // registers, not scratch
 #pragma unroll
     for (int jq = 0; jq < 3; ++jq)
 #pragma unroll
       for (int iq = 0; iq < 3; ++iq) {
         double acc = c[(size_t)(9 * nk + 3 * jq + iq) * cs];
-        double val = S.fld[widx(g, S.ic - 1 + iq, S.jc - 1 + jq, S.ke[nk])];
+        double val = S.fld[widx(g, S.ic - 1 + iq, S.jc - 1 + jq, kk)];
         s = s + acc * val;
       }
+  }
   *probe_out(out, f, kind, d) = s;
 }
 
