@@ -130,6 +130,10 @@ __device__ __forceinline__ int mls_grad_lanes(const Grid& g, const R* __restrict
                               const int ke[3], const int c[3], int lane, bool active, double* shA, double* grad_d,
                               double* coef = nullptr, size_t cstride = 0) {
   const double sx = WSCL / g.dx1, sy = WSCL / g.dx2, sz = WSCL / g.dx3;
+  // the field planes, read once: a per-plane pick from ke[] inside the rolled
+  // loop is folded into one load at a selected offset, which keeps the
+  // caller's probe-site record in scratch memory
+  const int ke0 = ke[0], ke1 = ke[1], ke2 = ke[2];
   double xs[3], ys[3], zs[3], w1[3], w2[3], w3[3], dw1[3], dw2[3], dw3[3];
   double M[16];
 #pragma unroll
@@ -212,7 +216,7 @@ __device__ __forceinline__ int mls_grad_lanes(const Grid& g, const R* __restrict
     const double zk = nk == 0 ? zs[0] : nk == 1 ? zs[1] : zs[2];
     const double w3k = nk == 0 ? w3[0] : nk == 1 ? w3[1] : w3[2];
     const double dw3k = nk == 0 ? dw3[0] : nk == 1 ? dw3[1] : dw3[2];
-    const int kk = nk == 0 ? ke[0] : nk == 1 ? ke[1] : ke[2];
+    const int kk = nk == 0 ? ke0 : nk == 1 ? ke1 : ke2;
 #pragma unroll
     for (int jq = 0; jq < 3; ++jq)
 #pragma unroll
