@@ -64,18 +64,19 @@ void graph_drop(melt_ctx* ctx) {
   if (ctx->graph) hipGraphDestroy(ctx->graph);
   ctx->graph_exec = nullptr;
   ctx->graph = nullptr;
+  ctx->graph_uses_ahead = 0;
 }
 
 // tschem queues the next call's first tagging and probe geometry stage at the
 // end of its last substep.  A call that changes what those read (the faces,
 // the body) or what the tagging writes (VOFp, sdf) cancels them, and drops a
-// captured step (whose first substep relies on them); the next substep then
-// does both itself.
+// captured step whose first substep relies on them (also when the queue was
+// already consumed outside the graph); the next substep then does both itself.
 void cancel_ahead(melt_ctx* ctx) {
-  if (!ctx->tag_ahead && !ctx->coef_ahead) return;
+  const bool queued = ctx->tag_ahead || ctx->coef_ahead;
   ctx->tag_ahead = 0;
   ctx->coef_ahead = 0;
-  graph_drop(ctx);
+  if (queued || ctx->graph_uses_ahead) graph_drop(ctx);
 }
 bool tag_field(int field) {
   return field == MELT_VOFP || field == MELT_VOFX || field == MELT_VOFY || field == MELT_VOFZ || field == MELT_SDF;
@@ -735,7 +736,14 @@ static int substep_impl(melt_ctx* ctx, int32_t ns, bool tag_next, bool coef_next
   return 0;
 }
 
-extern "C" int melt_substep(melt_ctx* ctx, int32_t ns) { return substep_impl(ctx, ns, false, false); }
+// A substep outside tschem consumes the queued probe coefficients and moves the
+// faces without queueing new ones, so a captured step that starts from queued
+// coefficients no longer holds.
+extern "C" int melt_substep(melt_ctx* ctx, int32_t ns) {
+  if (check_ctx(ctx)) return MELT_ERR_ARG;
+  if (ctx->graph_uses_ahead) graph_drop(ctx);
+  return substep_impl(ctx, ns, false, false);
+}
 
 static int tschem_launch(melt_ctx* ctx) {
   if (ctx->has_geom) {
@@ -744,7 +752,12 @@ static int tschem_launch(melt_ctx* ctx) {
   }
   for (int ns = 1; ns <= 3; ++ns) {
     int rc = substep_impl(ctx, ns, ns < 3 && !one_stream(ctx), !one_stream(ctx) && coef_place(ctx) == 1);
-    if (rc) { ctx->tag_ahead = 0; ctx->coef_ahead = 0; return rc; }
+    if (rc) {
+      ctx->tag_ahead = 0;
+      ctx->coef_ahead = 0;
+      if (ctx->graph_uses_ahead) graph_drop(ctx);
+      return rc;
+    }
   }
   return 0;
 }
@@ -757,6 +770,7 @@ extern "C" int melt_tschem(melt_ctx* ctx) {
   if (!graph) return tschem_launch(ctx);
   if (!ctx->graph_exec) {
     if (int rc = ensure_side(ctx)) return rc;   // not created inside the capture
+    const int uses_ahead = ctx->coef_ahead;
     MELT_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     int rc = tschem_launch(ctx);
     hipGraph_t g = nullptr;
@@ -765,6 +779,7 @@ extern "C" int melt_tschem(melt_ctx* ctx) {
     if (e != hipSuccess) return melt_fail(ctx, MELT_ERR_HIP, "tschem graph capture", hipGetErrorString(e));
     ctx->graph = g;
     MELT_HIP(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
+    ctx->graph_uses_ahead = uses_ahead;
   }
   MELT_HIP(hipGraphLaunch(ctx->graph_exec, ctx->stream));
   return 0;

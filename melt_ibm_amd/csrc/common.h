@@ -137,6 +137,7 @@ struct melt_ctx {
   int force_window_from_tag = 0;  // fused substep: tagging also places the forcing window
   int tag_ahead = 0;              // the next substep's tagging is already queued (side stream)
   int coef_ahead = 0;             // and the geometry part of its gradient probes
+  int graph_uses_ahead = 0;       // the captured step's first substep reads coefficients queued before it
 
   // timing
   int timing = 0;

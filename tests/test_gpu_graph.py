@@ -70,3 +70,37 @@ def test_queued_probe_geometry_matches_recomputed(gpu_lib, use_graph):
         assert np.array_equal(plain.lag("xyzv"), cancelled.lag("xyzv")), f"step {step}: xyzv"
     plain.close()
     cancelled.close()
+
+
+def test_substeps_between_graph_replays_match_eager(gpu_lib):
+    """A captured step whose first substep starts from the probe coefficients
+    queued by the previous call must not be replayed after melt_substep calls
+    have moved the faces: graph tschem, substeps 1-3, graph tschem again (and
+    a set_body in between) give the eager context's bits."""
+    n, level, rad = 32, 2, 1.2
+    cfg = make_cfg(n)
+    st, dt = melt_case(cfg, level, rad, seed=11)
+    eager, graph = MeltContext(cfg, lib=gpu_lib), MeltContext(cfg, lib=gpu_lib)
+    for c in (eager, graph):
+        load_state(c, st)
+        with_body(c, level, rad)
+        c.upload("temp", inqpr_temperature(cfg, c.body().pos_cm[:], rad))
+        c.set_dt(dt)
+        c.tagCells()
+    graph.graph(True)
+    script = ["tschem", "tschem", "sub", "tschem", "tschem", "sub", "body", "tschem", "tschem"]
+    for i, op in enumerate(script):
+        for c in (eager, graph):
+            if op == "tschem":
+                c.tschem()
+            elif op == "sub":
+                for ns in (1, 2, 3):
+                    c.substep(ns)
+            else:
+                c.set_body(c.body())
+            c.sync()
+        for q in FIELDS:
+            assert np.array_equal(eager.download(q), graph.download(q)), f"op {i} ({op}): {q}"
+        assert np.array_equal(eager.lag("xyzv"), graph.lag("xyzv")), f"op {i} ({op}): xyzv"
+    eager.close()
+    graph.close()
