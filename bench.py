@@ -14,7 +14,7 @@ Prints one JSON line: value = Mcell-steps/s (whole job), plus
    launch time (HIP events on the library's stream, over the timed region),
    against the 8 TB/s HBM3E peak, and the same for the whole substep;
  - cpu_baseline: the FP64 C restatement of the reference path on this host's
-   cores for one substep of the same workload.
+   cores, median substep time at the 64^3 reference case and at this workload.
 """
 from __future__ import annotations
 
@@ -103,32 +103,78 @@ def run_gpu(args, precision=0, keep_fields=False):
     return cfg, (t1 - t0), tm, qmax, body, fields, stats
 
 
-def cpu_baseline(cfg, level, rad):
-    """One substep of the same workload through the oracle (OpenMP threads)."""
+def workload_c1():
+    """The CPU-runnable reference case: 64^3, sphere of radius 0.6 (level-3
+    icosahedron, 1280 faces), Re 500, Pr 1, St 1, no forcing."""
+    from melt_ibm_amd.api import MeltConfig
+    cfg = MeltConfig(n1m=64, n2m=64, n3m=64, ren=500.0, prandtl=1.0, betagz=0.0, tmelt=0.0, tliq=1.0, tsol=0.0, cpliquid=1.0,
+                     latheat=1.0, dens_ratio=1.0, imlsfor=1, imlsstr=1, imelt=1, iremesh=0, forcing=0)
+    return cfg, 3, 0.6
+
+
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_substeps(cfg, level, rad, forcing: bool, warm: int, timed: int) -> list[float]:
+    """Wall time of each of `timed` oracle substeps after `warm` untimed ones
+    (substeps cycle 1, 2, 3 as in tschem), excluding set-up and I/O."""
     import oracle
     from melt_ibm_amd import geometry, inputs
-    lib = oracle.load()
     o = oracle.context(cfg)
     m = geometry.sphere_level(level)
     o.set_geometry(m.xyz, m.vert_of_edge, m.edge_of_face, rad)
     o.inqpr()
     u, v, w = inputs.solenoidal_velocity(cfg, seed=0, kmax=4.0, rms=0.1)
-    fx, fy, fz = inputs.low_mode_forcing(cfg, seed=1, kf=2.5, amplitude=0.5)
-    for name, a in (("vx", u), ("vy", v), ("vz", w), ("forcx", fx), ("forcy", fy), ("forcz", fz)):
+    fields = [("vx", u), ("vy", v), ("vz", w)]
+    if forcing:
+        fields += list(zip(("forcx", "forcy", "forcz"), inputs.low_mode_forcing(cfg, seed=1, kf=2.5, amplitude=0.5)))
+    for name, a in fields:
         o.upload(name, a)
     o.set_dt(inputs.cfl_dt(cfg, u, v, w, cfl=0.5))
-    del u, v, w, fx, fy, fz
+    del u, v, w, fields
     o.tagCells()
-    t0 = time.perf_counter()
-    o.substep(1)
-    t1 = time.perf_counter()
-    cores = int(lib.orc_threads())
+    times = []
+    for i in range(warm + timed):
+        t0 = time.perf_counter()
+        o.substep(i % 3 + 1)
+        if i >= warm:
+            times.append(time.perf_counter() - t0)
     o.close()
-    cells = cfg.n1m * cfg.n2m * cfg.n3m
-    sub_s = t1 - t0
-    return {"value": cells / (3.0 * sub_s) / 1e6, "unit": "Mcell-steps/s", "cores": cores, "kind": "port",
-            "sample": f"1 RK3 substep of the {cfg.n1m}^3 workload (incl. IBM) on {cores} OpenMP threads: {sub_s:.2f} s; "
-                      f"value = cells / (3 x substep time)"}
+    return times
+
+
+def cpu_baseline(n: int, warm: int = 3, timed: int = 20):
+    """The oracle (the FP64 C restatement of the reference path, OpenMP over
+    this host's cores) on the reference case (64^3) and on the benchmark
+    workload (n^3): median substep time of `timed` substeps after `warm`."""
+    import oracle
+    lib = oracle.load()
+    threads = int(lib.orc_threads())
+    out = {}
+    for tag, (cfg, level, rad), forcing in (("c1_64", workload_c1(), False), (f"c2_{n}", workload(n), True)):
+        print(f"# cpu baseline {tag}: {warm} + {timed} oracle substeps", file=sys.stderr, flush=True)
+        t = np.array(_oracle_substeps(cfg, level, rad, forcing, warm, timed))
+        cells = cfg.n1m * cfg.n2m * cfg.n3m
+        med = float(np.median(t))
+        out[tag] = {"median_substep_s": round(med, 5), "min_substep_s": round(float(t.min()), 5),
+                    "max_substep_s": round(float(t.max()), 5), "mcell_steps_per_s": round(cells / (3.0 * med) / 1e6, 3)}
+    main_tag = f"c2_{n}"
+    return {"value": out[main_tag]["mcell_steps_per_s"], "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
+            "implementation": "oracle/ (the FP64 C restatement of the reference path; the reference's MPI Fortran does not "
+                              "build without MPI/HDF5/FFTW modules), gcc -O2 -ffp-contract=off -fopenmp",
+            "host": {"cpu_model": _cpu_model(), "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+                     "omp_threads": threads},
+            "sample": f"median of {timed} substeps after {warm} warm-up substeps (immersed boundary included, set-up and I/O "
+                      f"excluded) on {threads} OpenMP threads, at the 64^3 reference case and at the {n}^3 workload; "
+                      f"value = {n}^3 cells / (3 x median substep)",
+            "runs": out}
 
 
 PROFILES = ROOT / "profiles"
@@ -445,9 +491,8 @@ def main():
                            f"200 GB budget; FP32 bulk with FP64 refinement")
         out["serving"] = rep
     if not args.no_cpu:
-        cfg2, level, rad = workload(args.n)
         try:
-            out["cpu_baseline"] = cpu_baseline(cfg2, level, rad)
+            out["cpu_baseline"] = cpu_baseline(args.n)
         except Exception as e:  # the baseline is reported, never required
             out["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(out))

@@ -130,12 +130,22 @@ int  melt_ctx_create(const melt_config* cfg, melt_ctx** out);
 void melt_ctx_destroy(melt_ctx* ctx);
 const char* melt_last_error(const melt_ctx* ctx);
 int  melt_sync(melt_ctx* ctx);
+/* No counterpart in the reference: *done = 1 when all work queued on the
+ * context has finished, 0 while some is pending (never blocks), for a caller
+ * that schedules several contexts itself; device errors are then read by
+ * melt_sync, which returns at once. */
+int  melt_poll(melt_ctx* ctx, int32_t* done);
 int  melt_device_ok(void);   /* 1 if a HIP device is present */
 
 /* ------------------------------------------------------- data movement */
 int melt_upload_field(melt_ctx* ctx, int field, const double* host, const melt_layout* lay);
 int melt_download_field(melt_ctx* ctx, int field, double* host, const melt_layout* lay);
 int melt_fill_field(melt_ctx* ctx, int field, double value);
+/* No counterpart in the reference (its fields live in host memory): upload
+ * from device memory, src holding N1*N2*N3 doubles (i fastest) that are
+ * complete before the call, for callers that synthesise initial states on the
+ * GPU.  Returns once the copy is done. */
+int melt_upload_field_device(melt_ctx* ctx, int field, const double* src);
 
 /* Mesh import: set_connectivity + setup_particles + init_geomCoords + the
  * geometry set-up of tri_geo + init_mlsForce (partaux.f90:160-267,
@@ -175,7 +185,12 @@ int melt_invtr3(melt_ctx* ctx);       /* invtr3.f90:1-65                        
 int melt_invtrte(melt_ctx* ctx);      /* invtrte.f90:1-67                           */
 int melt_solxi(melt_ctx* ctx, double betadx);              /* solxi.f90:1-52 on MELT_RHS */
 int melt_solxj(melt_ctx* ctx, double betadx);              /* solxj.f90:1-40 on MELT_RHS */
-int melt_solxk(melt_ctx* ctx, int field, double betadx);   /* solxk.f90:1-60: q += solve(rhs) */
+int melt_solxk(melt_ctx* ctx, int field, double betadx);   /* solxk.f90:1-60: q += solve(rhs), q device-resident */
+/* solxk.f90:1-60 in the reference's own form solxk(q, betadx) (call sites
+ * invtr1.f90:68, invtr2.f90:64, invtr3.f90:62, invtrte.f90:63): the z solves run
+ * on the device rhs, which keeps the solution, and q += rhs is applied to the
+ * caller's host array q(1:ld1,1:ld2,...) described by lay (null: n1m x n2m x n3m). */
+int melt_solxk_array(melt_ctx* ctx, double* q, const melt_layout* lay, double betadx);
 int melt_divg(melt_ctx* ctx);         /* divg.f90:1-22   -> MELT_DPH                */
 int melt_phcalc(melt_ctx* ctx);       /* phcalc.f90:11-97 in place on MELT_DPH      */
 int melt_updvp(melt_ctx* ctx);        /* updvp.f90:8-35                             */
@@ -191,6 +206,33 @@ int melt_particle(melt_ctx* ctx);     /* particle.f90:1-229 (IREMESH=0)         
 int melt_cfl(melt_ctx* ctx, double* cflm);     /* cfl.f90:1-30 (max over cells)     */
 int melt_divgck(melt_ctx* ctx, double* qmax);  /* divg.f90:26-56                    */
 int melt_inqpr(melt_ctx* ctx);                 /* inqpr.f90:3-44                    */
+/* get_dt (cfl.f90:33-77): *dt_out = dt * cflfix / max(max over cells of
+ * |u| dt dx1 + |v| dt dx2 + |w| dt dx3, 6 dt dx1^2 / Re) at the context's dt
+ * (melt_set_dt); the caller sets the new dt, as gcurv.f90:151-157 does. */
+int melt_get_dt(melt_ctx* ctx, double cflfix, double* dt_out);
+/* CalcDissipation (diss.f90:11-201): re-tags a moving body's cells, then the
+ * fluid-cell (VOFp == 1) averages and the statistics diss.txt holds:
+ * out[11] = {diss_volAvg, kenerg, urms, eta, lambda_t, Re_L, re_lam, L_int,
+ *            chi_volAvg, kmax_eta, sumVOF}. */
+int melt_dissipation(melt_ctx* ctx, double* out);
+
+/* HIT forcing of the time loop (hit.f90), outside tschem as in gcurv.f90:163-170.
+ * melt_hit_init = InitRandomForce (hit.f90:128-232) with HITForcing.in's TL,
+ * EPSSTAR and KF/KMIN (nmodes = 2 floor(KF/KMIN) + 1 <= 9); melt_hit_force =
+ * CalcHITRandomForce (hit.f90:1-126) at the context's dt: one Ornstein-Uhlenbeck
+ * step of bhat for the modes with 0 < |k| < kf, the solenoidal projection and
+ * forcx/forcy/forcz on the device (kernels only: capturable).  The Gaussian
+ * numbers are Box-Muller pairs from SplitMix64 at a counter (seed, step) in the
+ * reference's draw order, in place of gfortran's random_number stream.
+ * melt_hit_state gets (set = 0) or sets (set = 1) bhat(3,nm,nm,nm) as (re,im)
+ * pairs and the number of steps taken (WriteRandForcCoef / hdf_read_bhat,
+ * hit.f90:267-356, plus the generator position).  melt_abc_force =
+ * CalcABC_HITForce (hit.f90:234-265). */
+int melt_hit_init(melt_ctx* ctx, double tl, double epsstar, double kf_on_kmin, uint64_t seed);
+int melt_hit_force(melt_ctx* ctx);
+int melt_hit_state(melt_ctx* ctx, double* bhat, uint64_t* steps, int32_t set);
+int melt_hit_nmodes(melt_ctx* ctx, int32_t* nmodes, int32_t* nforced);
+int melt_abc_force(melt_ctx* ctx, double c_hit);
 
 /* Phase timing with HIP events around the four TIMED buckets of tsch.f90:37-84
  * (tagging, Eulerian solve, MLS/particle, pressure) and around each fused-path
@@ -231,8 +273,10 @@ int melt_graph_enable(melt_ctx* ctx, int32_t on);
  * probes and next tagging run on a second stream), 7 = where the geometry
  * stage of the gradient probes runs: 1 after the previous substep's Lagrangian
  * update (beside the pressure solve), 2 at the start of its own substep
- * (beside the explicit pass); value 0 = built-in choice.
- * Results do not depend on it. */
+ * (beside the explicit pass), 8 = faces a ray-tagging pencil may hold
+ * (BUFF_SIZE of pencil_rayTag.f90:84, n1m there; more faces raise
+ * MELT_ERR_PENCIL_OVERFLOW as the reference aborts); value 0 = built-in choice.
+ * Results do not depend on it (key 8: as long as no pencil overflows). */
 int melt_set_tuning(melt_ctx* ctx, int32_t key, int32_t value);
 void* melt_stream(melt_ctx* ctx);
 int melt_get_refinement_stats(melt_ctx* ctx, int64_t* checks, int64_t* refined, double* last_ratio);

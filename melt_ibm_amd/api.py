@@ -174,11 +174,19 @@ _SIGS = {
     "hdnl1": (C.c_int, [_P]), "hdnl2": (C.c_int, [_P]), "hdnl3": (C.c_int, [_P]), "hdnlte": (C.c_int, [_P]),
     "invtr1": (C.c_int, [_P]), "invtr2": (C.c_int, [_P]), "invtr3": (C.c_int, [_P]), "invtrte": (C.c_int, [_P]),
     "solxi": (C.c_int, [_P, C.c_double]), "solxj": (C.c_int, [_P, C.c_double]), "solxk": (C.c_int, [_P, C.c_int, C.c_double]),
+    "solxk_array": (C.c_int, [_P, _P, C.POINTER(_Layout), C.c_double]),
     "divg": (C.c_int, [_P]), "phcalc": (C.c_int, [_P]), "updvp": (C.c_int, [_P]), "prcalc": (C.c_int, [_P]),
     "find_centroid_indices": (C.c_int, [_P]), "mlsweight": (C.c_int, [_P]), "mlsforce": (C.c_int, [_P]),
     "velforce": (C.c_int, [_P]), "tempforce": (C.c_int, [_P]), "particle": (C.c_int, [_P]),
     "cfl": (C.c_int, [_P, C.POINTER(C.c_double)]), "divgck": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "inqpr": (C.c_int, [_P]),
+    "get_dt": (C.c_int, [_P, C.c_double, C.POINTER(C.c_double)]),
+    "hit_init": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.c_uint64]),
+    "hit_force": (C.c_int, [_P]),
+    "hit_state": (C.c_int, [_P, _P, C.POINTER(C.c_uint64), C.c_int32]),
+    "hit_nmodes": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "abc_force": (C.c_int, [_P, C.c_double]),
+    "dissipation": (C.c_int, [_P, _P]),
     "timing_enable": (C.c_int, [_P, C.c_int32]), "timing_reset": (C.c_int, [_P]),
     "timing_count": (C.c_int, []), "timing_name": (C.c_char_p, [C.c_int32]),
     "timing_read": (C.c_int, [_P, _P, _P]),
@@ -187,10 +195,12 @@ _SIGS = {
     "graph_enable": (C.c_int, [_P, C.c_int32]),
     "set_tuning": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "stream": (C.c_void_p, [_P]),
+    "poll": (C.c_int, [_P, C.POINTER(C.c_int32)]),
+    "upload_field_device": (C.c_int, [_P, C.c_int, C.c_void_p]),
 }
 OPTIONAL = {"timing_enable", "timing_reset", "timing_count", "timing_name", "timing_read", "device_ok", "set_refinement",
             "get_refinement_stats", "graph_enable", "stream",
-            "set_tuning"}
+            "set_tuning", "poll", "upload_field_device"}
 
 
 def bind(lib: C.CDLL, prefix: str) -> dict:
@@ -248,6 +258,16 @@ class MeltContext:
 
     def sync(self) -> None:
         self._chk(self._f["sync"](self._ctx))
+
+    def poll(self) -> bool:
+        """True when all queued work has finished (never blocks)."""
+        d = C.c_int32(0)
+        self._chk(self._f["poll"](self._ctx, C.byref(d)))
+        return bool(d.value)
+
+    def upload_device(self, name: str, ptr: int) -> None:
+        """Upload from device memory at ptr (N1*N2*N3 doubles, i fastest, complete)."""
+        self._chk(self._f["upload_field_device"](self._ctx, FIELD_ID[name], C.c_void_p(ptr)))
 
     # ------------------------------------------------------------ fields
     def upload(self, name: str, arr: np.ndarray) -> None:
@@ -345,6 +365,13 @@ class MeltContext:
     def solxj(self, betadx: float): self._chk(self._f["solxj"](self._ctx, float(betadx)))
     def solxk(self, q: str, betadx: float): self._chk(self._f["solxk"](self._ctx, FIELD_ID[q], float(betadx)))
 
+    def solxk_array(self, q: np.ndarray, betadx: float, kghost: int = 0) -> None:
+        """solxk(q, betadx) on a host array q[k, j, i] (Fortran q(i, j, k)), updated in place."""
+        if q.dtype != np.float64 or not q.flags.c_contiguous:
+            raise ValueError("q must be a C-contiguous float64 array")
+        lay = _Layout(q.shape[2], q.shape[1], kghost, 0)
+        self._chk(self._f["solxk_array"](self._ctx, q.ctypes.data, C.byref(lay), float(betadx)))
+
     def divg(self): self._chk(self._f["divg"](self._ctx))
     def phcalc(self): self._chk(self._f["phcalc"](self._ctx))
     def updvp(self): self._chk(self._f["updvp"](self._ctx))
@@ -361,6 +388,53 @@ class MeltContext:
         v = C.c_double()
         self._chk(self._f["cfl"](self._ctx, C.byref(v)))
         return v.value
+
+    def get_dt(self, cflfix: float) -> float:
+        """get_dt (cfl.f90:33-77): the dt that brings the current CFL to cflfix."""
+        d = C.c_double(0.0)
+        self._chk(self._f["get_dt"](self._ctx, float(cflfix), C.byref(d)))
+        return d.value
+
+    # ------------------------------------------------------------ HIT forcing
+    def hit_init(self, tl: float, epsstar: float, kf_on_kmin: float, seed: int = 0) -> None:
+        """InitRandomForce (hit.f90:128-232) with HITForcing.in's TL, EPSSTAR, KF/KMIN."""
+        self._chk(self._f["hit_init"](self._ctx, float(tl), float(epsstar), float(kf_on_kmin), int(seed)))
+
+    def hit_force(self) -> None:
+        """CalcHITRandomForce (hit.f90:1-126) at the current dt: forcx/forcy/forcz."""
+        self._chk(self._f["hit_force"](self._ctx))
+
+    def hit_nmodes(self) -> tuple[int, int]:
+        nm, na = C.c_int32(0), C.c_int32(0)
+        self._chk(self._f["hit_nmodes"](self._ctx, C.byref(nm), C.byref(na)))
+        return nm.value, na.value
+
+    def hit_state(self) -> tuple[np.ndarray, int]:
+        """bhat as complex [n, m, l, dir] (Fortran bhat(dir, l, m, n)) and the steps taken."""
+        nm, _ = self.hit_nmodes()
+        a = np.zeros(2 * 3 * nm ** 3)
+        st = C.c_uint64(0)
+        self._chk(self._f["hit_state"](self._ctx, a.ctypes.data, C.byref(st), 0))
+        return (a[0::2] + 1j * a[1::2]).reshape(nm, nm, nm, 3), int(st.value)
+
+    def set_hit_state(self, bhat: np.ndarray, steps: int) -> None:
+        b = np.ascontiguousarray(bhat, dtype=np.complex128).reshape(-1)
+        a = np.empty(2 * b.size)
+        a[0::2], a[1::2] = b.real, b.imag
+        st = C.c_uint64(int(steps))
+        self._chk(self._f["hit_state"](self._ctx, a.ctypes.data, C.byref(st), 1))
+
+    def abc_force(self, c_hit: float) -> None:
+        """CalcABC_HITForce (hit.f90:234-265)."""
+        self._chk(self._f["abc_force"](self._ctx, float(c_hit)))
+
+    DISS_KEYS = ("diss", "kenerg", "urms", "eta", "lambda_t", "re_l", "re_lambda", "l_int", "chi", "kmax_eta", "sumvof")
+
+    def dissipation(self) -> dict:
+        """CalcDissipation (diss.f90:11-201): fluid averages and turbulence statistics."""
+        a = np.zeros(11)
+        self._chk(self._f["dissipation"](self._ctx, a.ctypes.data))
+        return dict(zip(self.DISS_KEYS, (float(x) for x in a)))
 
     def divgck(self) -> float:
         v = C.c_double()
@@ -389,10 +463,11 @@ class MeltContext:
         """Replay tschem as one captured HIP graph."""
         self._chk(self._f["graph_enable"](self._ctx, 1 if on else 0))
 
-    TUNE_KEYS = {"xf_w": 0, "xf_nt": 1, "solve_w": 2, "solve_nt": 3, "xf_z": 4, "yz_jb": 5, "streams": 6, "coef": 7}
+    TUNE_KEYS = {"xf_w": 0, "xf_nt": 1, "solve_w": 2, "solve_nt": 3, "xf_z": 4, "yz_jb": 5, "streams": 6, "coef": 7, "pbuf": 8}
 
     def set_tuning(self, **kw) -> None:
-        """Launch-shape overrides (xf_w, xf_nt, solve_w, solve_nt, xf_z, yz_jb, streams, coef; 0 = default)."""
+        """Launch-shape overrides (xf_w, xf_nt, solve_w, solve_nt, xf_z, yz_jb, streams, coef) and the ray-tagging
+        pencil capacity (pbuf, BUFF_SIZE of pencil_rayTag.f90:84); 0 = default."""
         for k, v in kw.items():
             self._chk(self._f["set_tuning"](self._ctx, self.TUNE_KEYS[k], int(v)))
 

@@ -19,7 +19,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libmeltgpu.so"
 OBJDIR = PKG / "_build"
-SOURCES = ["api.hip", "euler.hip", "poisson.hip", "ibm.hip", "lagr.hip"]
+SOURCES = ["api.hip", "euler.hip", "poisson.hip", "ibm.hip", "lagr.hip", "diag.hip", "hit.hip"]
 ARCH = os.environ.get("MELT_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -71,6 +71,21 @@ def build_fortran(verbose: bool = False) -> Path | None:
     out = OBJDIR / "melt_ibm_f.o"
     OBJDIR.mkdir(exist_ok=True)
     _run([fc, "-c", "-fdefault-real-8", "-fdefault-double-8", "-O2", "-fPIC", f"-J{OBJDIR}", str(src), "-o", str(out)])
+    if verbose:
+        print(f"built {out}")
+    return out
+
+
+def build_fortran_program(src: Path, out: Path, verbose: bool = False) -> Path | None:
+    """Link a Fortran program that uses the binding module against the library
+    (run-time search path relative to the program, so the tree can move)."""
+    fc = shutil.which("amdflang") or ("/opt/rocm/bin/amdflang" if Path("/opt/rocm/bin/amdflang").exists() else None)
+    mod = build_fortran()
+    if not fc or mod is None:
+        return None
+    rel = os.path.relpath(PKG, out.parent)
+    _run([fc, "-O2", "-fdefault-real-8", "-fdefault-double-8", f"-I{OBJDIR}", str(src), str(mod), f"-L{PKG}", "-lmeltgpu",
+          f"-Wl,-rpath,$ORIGIN/{rel}", "-Wl,-rpath,/opt/rocm/lib", "-o", str(out)])
     if verbose:
         print(f"built {out}")
     return out

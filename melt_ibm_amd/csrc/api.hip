@@ -143,6 +143,7 @@ extern "C" void melt_ctx_destroy(melt_ctx* ctx) {
                 ctx->d_vbox, ctx->d_slots, ctx->d_coord, ctx->d_acc, ctx->d_fmean, ctx->d_tkey, ctx->d_tface, ctx->d_tdist, ctx->d_pcoef, ctx->d_win, ctx->d_pcount, ctx->d_plist, ctx->d_err, ctx->d_gate, ctx->d_refine_stats};
   for (void* p : ps) if (p) hipFree(p);
   for (int i = 0; i < MELT_L_NLAG; ++i) if (ctx->d_lag[i]) hipFree(ctx->d_lag[i]);
+  hit_free(ctx);
   if (ctx->h_err) hipHostFree(ctx->h_err);
   for (auto e : ctx->ev_pool) hipEventDestroy(e);
   if (ctx->side) hipStreamSynchronize(ctx->side);
@@ -176,7 +177,7 @@ extern "C" int melt_ctx_create(const melt_config* cfg, melt_ctx** out) {
   do {
     if (hipGetDevice(&ctx->device) != hipSuccess) { rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "no HIP device"); break; }
     if (const char* tv = getenv("MELT_TUNE")) {
-      static const char* keys[TUNE_N] = {"xf_w", "xf_nt", "solve_w", "solve_nt", "xf_z", "yz_jb", "streams", "coef"};
+      static const char* keys[TUNE_N] = {"xf_w", "xf_nt", "solve_w", "solve_nt", "xf_z", "yz_jb", "streams", "coef", "pbuf"};
       std::string spec(tv);
       size_t pos = 0;
       while (pos < spec.size()) {
@@ -340,6 +341,34 @@ extern "C" int melt_download_field(melt_ctx* ctx, int field, double* host, const
   return 0;
 }
 
+template <typename R>
+__global__ void k_from_double(R* __restrict__ dst, const double* __restrict__ src, size_t n) {
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) dst[c] = (R)src[c];
+}
+
+// Device-to-device upload: src is device memory holding N1*N2*N3 doubles
+// (i fastest) that is complete before the call; the precision form stores
+// them rounded to float.  Returns after the copy has finished.
+extern "C" int melt_upload_field_device(melt_ctx* ctx, int field, const double* src) {
+  if (check_ctx(ctx) || field < 0 || field >= MELT_NFIELDS || !src) return MELT_ERR_ARG;
+  if (tag_field(field)) cancel_ahead(ctx);
+  if (ctx->fp32) hipLaunchKernelGGL(k_from_double<float>, dim3(2048), dim3(256), 0, ctx->stream, fld<float>(ctx, field), src, ctx->nc);
+  else hipLaunchKernelGGL(k_from_double<double>, dim3(2048), dim3(256), 0, ctx->stream, fld<double>(ctx, field), src, ctx->nc);
+  MELT_HIP(hipGetLastError());
+  MELT_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+// Non-blocking completion test of the work queued on the context (its side
+// stream joins the main stream before every substep ends).
+extern "C" int melt_poll(melt_ctx* ctx, int32_t* done) {
+  if (check_ctx(ctx) || !done) return MELT_ERR_ARG;
+  hipError_t e = hipStreamQuery(ctx->stream);
+  if (e == hipSuccess) { *done = 1; return 0; }
+  if (e == hipErrorNotReady) { *done = 0; return 0; }
+  return melt_fail(ctx, MELT_ERR_HIP, "hipStreamQuery", hipGetErrorString(e));
+}
+
 extern "C" int melt_fill_field(melt_ctx* ctx, int field, double value) {
   if (check_ctx(ctx) || field < 0 || field >= MELT_NFIELDS) return MELT_ERR_ARG;
   if (tag_field(field)) cancel_ahead(ctx);
@@ -500,6 +529,29 @@ extern "C" int melt_solxk(melt_ctx* ctx, int field, double betadx) {
   void* r[1] = {ctx->f[MELT_RHS]};
   void* q[1] = {ctx->f[field]};
   return euler_launch_solve(ctx, 2, r, 1, &betadx, q, 1);
+}
+
+// The reference's own form, solxk(q, betadx) with q an array section
+// (solxk.f90:1,7; invtr1.f90:68): the z-line solves run on the device rhs
+// (left holding the solution, as the reference's rhs is) and q += rhs is
+// applied to the caller's array through its layout (solxk.f90:44-50).
+extern "C" int melt_solxk_array(melt_ctx* ctx, double* q, const melt_layout* lay, double betadx) {
+  if (check_ctx(ctx) || !q) return MELT_ERR_ARG;
+  melt_layout L = lay ? *lay : melt_layout{ctx->n1, ctx->n2, 0, 0};
+  if (L.ld1 < ctx->n1 || L.ld2 < ctx->n2 || L.kghost < 0) return melt_fail(ctx, MELT_ERR_ARG, "solxk", "bad layout");
+  void* r[1] = {ctx->f[MELT_RHS]};
+  int rc = euler_launch_solve(ctx, 2, r, 1, &betadx, nullptr, 1);
+  if (rc) return rc;
+  MELT_HIP(hipStreamSynchronize(ctx->stream));
+  std::vector<double> tmp(ctx->nc);
+  if ((rc = copy_from_device(ctx, MELT_RHS, tmp.data()))) return rc;
+  for (int k = 0; k < ctx->n3; ++k)
+    for (int j = 0; j < ctx->n2; ++j) {
+      double* row = q + (size_t)L.ld1 * (j + (size_t)L.ld2 * (k + L.kghost));
+      const double* src = &tmp[(size_t)ctx->n1 * (j + (size_t)ctx->n2 * k)];
+      for (int i = 0; i < ctx->n1; ++i) row[i] = row[i] + src[i];
+    }
+  return 0;
 }
 
 extern "C" int melt_divg(melt_ctx* ctx) {

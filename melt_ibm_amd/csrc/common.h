@@ -52,7 +52,7 @@ enum {
 
 // Launch-shape overrides (0 = built-in choice), set from MELT_TUNE at
 // context creation ("xf_w=4,solve_w=16,...") for tile sweeps.
-enum melt_tune { TUNE_XF_W = 0, TUNE_XF_NT, TUNE_SOLVE_W, TUNE_SOLVE_NT, TUNE_XF_Z, TUNE_YZ_JB, TUNE_STREAMS, TUNE_COEF, TUNE_N };
+enum melt_tune { TUNE_XF_W = 0, TUNE_XF_NT, TUNE_SOLVE_W, TUNE_SOLVE_NT, TUNE_XF_Z, TUNE_YZ_JB, TUNE_STREAMS, TUNE_COEF, TUNE_PBUF, TUNE_N };
 
 struct melt_ctx {
   melt_config cfg;
@@ -130,6 +130,15 @@ struct melt_ctx {
   int* d_pcount = nullptr;
   int* d_plist = nullptr;
   size_t plist_cap = 0;
+
+  // HIT forcing (hit.f90): bhat and Fhat (3 x nm^3 complex), exp(i k x) of the
+  // staggered coordinates, the forced modes, the generator's step counter
+  int hit_nm = 0, hit_nact = 0;
+  double hit_tl = 0, hit_eps = 0;
+  unsigned long long hit_seed = 0;
+  double *d_hit_bhat = nullptr, *d_hit_fhat = nullptr, *d_hit_tab = nullptr;
+  int* d_hit_act = nullptr;
+  unsigned long long* d_hit_count = nullptr;
 
   int* d_err = nullptr;    // device error word (first error wins)
   int* h_err = nullptr;    // pinned mirror
@@ -448,10 +457,12 @@ int euler_launch_green(melt_ctx* ctx, int dir, void* const* src, void* const* ds
                        int* used);
 int euler_launch_green_yz(melt_ctx* ctx, void* const* src, void* const* qadd, const double* by, const double* bz, int nfield, int* used);
 int euler_launch_project(melt_ctx* ctx, int do_u, int do_p, int use_mean, const int* gate = nullptr);
-int euler_launch_reduce_max(melt_ctx* ctx, int what, unsigned long long* d);
+int euler_launch_reduce_max(melt_ctx* ctx, int what, unsigned long long* d, double dt = 0.0);
 int euler_launch_refine_gate(melt_ctx* ctx, const unsigned long long* dmax, int* gate);
 int euler_launch_interp_vof(melt_ctx* ctx);
-int euler_reduce_max(melt_ctx* ctx, int what, double* out);
+int euler_reduce_max(melt_ctx* ctx, int what, double* out, double dt = 0.0);
+int diag_dissipation(melt_ctx* ctx, double* sums);
+int hit_free(melt_ctx* ctx);   // fluid-cell sums of tke, diss, chi and the cell count
 int euler_launch_inqpr(melt_ctx* ctx);
 int poisson_init(melt_ctx* ctx);
 // div_slots != null: the forward x pass also folds max |div u| (FP64, as

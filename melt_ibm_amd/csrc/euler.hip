@@ -1064,12 +1064,13 @@ __global__ __launch_bounds__(256) void k_interp_vof(Grid g, const R* __restrict_
   vz[c] = R(0.5) * (p[s.at(0, 0, -1)] + v);
 }
 
-// Max reductions of cfl.f90:9-25 (what = 0) and divgck divg.f90:36-50
-// (what = 1), in FP64 arithmetic whatever the storage type; the result
-// (non-negative) is combined by atomicMax on its bit pattern.
+// Max reductions of cfl.f90:9-25 (what = 0), divgck divg.f90:36-50
+// (what = 1) and get_dt's advective measure cfl.f90:48-51 (what = 2, with dt),
+// in FP64 arithmetic whatever the storage type; the result (non-negative) is
+// combined by atomicMax on its bit pattern.
 template <typename R>
 __global__ __launch_bounds__(256) void k_reduce_max(Grid g, const R* __restrict__ vx, const R* __restrict__ vy, const R* __restrict__ vz,
-                                                    int what, int nc, unsigned long long* slots) {
+                                                    int what, double dt, int nc, unsigned long long* slots) {
   // V consecutive x cells per thread through 16-byte loads (n1 % V == 0)
   constexpr int V = 16 / sizeof(R);
   struct alignas(16) Vec { R v[V]; };
@@ -1087,6 +1088,8 @@ __global__ __launch_bounds__(256) void k_reduce_max(Grid g, const R* __restrict_
       double q;
       if (what == 0)
         q = fabs((x0 + x1) * 0.5 * g.dx1) + fabs((y0 + y1) * 0.5 * g.dx2) + fabs((z0 + z1) * 0.5 * g.dx3);
+      else if (what == 2)
+        q = fabs(x0) * dt * g.dx1 + fabs(y0) * dt * g.dx2 + fabs(z0) * dt * g.dx3;
       else
         q = fabs((x1 - x0) * g.dx1 + (y1 - y0) * g.dx2 + (z1 - z0) * g.dx3);
       m = fmax(m, q);
@@ -1417,11 +1420,11 @@ int launch_interp_vof(melt_ctx* ctx) {
 // slots, then one block folds them (kernels only, so the sequence may be
 // captured into a graph).
 template <typename R>
-int launch_reduce_max(melt_ctx* ctx, int what, unsigned long long* d) {
+int launch_reduce_max(melt_ctx* ctx, int what, unsigned long long* d, double dt) {
   constexpr int V = 16 / sizeof(R);
   if (ctx->n1 % V) return melt_fail(ctx, MELT_ERR_ARG, "reduce_max", "n1m is not a multiple of the vector width");
   hipLaunchKernelGGL(k_reduce_max<R>, dim3(nblocks(ctx->nc / V, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), fld<R>(ctx, MELT_VX),
-                     fld<R>(ctx, MELT_VY), fld<R>(ctx, MELT_VZ), what, (int)ctx->nc, ctx->d_slots);
+                     fld<R>(ctx, MELT_VY), fld<R>(ctx, MELT_VZ), what, dt, (int)ctx->nc, ctx->d_slots);
   MELT_HIP(hipGetLastError());
   return euler_fold_slots(ctx, ctx->d_slots, d);
 }
@@ -1463,8 +1466,8 @@ int euler_launch_project(melt_ctx* ctx, int do_u, int do_p, int use_mean, const 
   return ctx->fp32 ? launch_project<float>(ctx, do_u, do_p, use_mean, gate) : launch_project<double>(ctx, do_u, do_p, use_mean, gate);
 }
 int euler_launch_interp_vof(melt_ctx* ctx) { return ctx->fp32 ? launch_interp_vof<float>(ctx) : launch_interp_vof<double>(ctx); }
-int euler_launch_reduce_max(melt_ctx* ctx, int what, unsigned long long* d) {
-  return ctx->fp32 ? launch_reduce_max<float>(ctx, what, d) : launch_reduce_max<double>(ctx, what, d);
+int euler_launch_reduce_max(melt_ctx* ctx, int what, unsigned long long* d, double dt) {
+  return ctx->fp32 ? launch_reduce_max<float>(ctx, what, d, dt) : launch_reduce_max<double>(ctx, what, d, dt);
 }
 int euler_fold_slots(melt_ctx* ctx, unsigned long long* slots, unsigned long long* out) {
   hipLaunchKernelGGL(k_fold_slots, dim3(1), dim3(REDUCE_SLOTS), 0, ctx->stream, slots, out);
@@ -1476,9 +1479,9 @@ int euler_launch_refine_gate(melt_ctx* ctx, const unsigned long long* dmax, int*
   MELT_HIP(hipGetLastError());
   return 0;
 }
-int euler_reduce_max(melt_ctx* ctx, int what, double* out) {
+int euler_reduce_max(melt_ctx* ctx, int what, double* out, double dt) {
   unsigned long long* d = reinterpret_cast<unsigned long long*>(ctx->d_red);
-  int rc = euler_launch_reduce_max(ctx, what, d);
+  int rc = euler_launch_reduce_max(ctx, what, d, dt);
   if (rc) return rc;
   unsigned long long h = 0;
   MELT_HIP(hipMemcpyAsync(&h, d, sizeof h, hipMemcpyDeviceToHost, ctx->stream));

@@ -1041,7 +1041,9 @@ static int ray_tag_init_t(melt_ctx* ctx) {
     hipLaunchKernelGGL(k_fill<R>, dim3(1024), dim3(256), 0, ctx->stream, fld<R>(ctx, id), 1.0, ctx->nc);
   MELT_HIP(hipGetLastError());
   int js = bx.hi[1] - bx.lo[1] + 1, ks = bx.hi[2] - bx.lo[2] + 1;
-  int buff = ctx->n1;  // BUFF_SIZE = n1m (pencil_rayTag.f90:84)
+  // BUFF_SIZE: n1m as the reference sets it (pencil_rayTag.f90:84) unless the
+  // caller sized it (tuning key pbuf); tags do not depend on it below overflow
+  int buff = ctx->tune[TUNE_PBUF] > 0 ? ctx->tune[TUNE_PBUF] : ctx->n1;
   size_t np = (size_t)js * ks;
   if (np * buff > ctx->plist_cap) {
     if (ctx->d_plist) hipFree(ctx->d_plist);

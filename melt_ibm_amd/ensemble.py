@@ -76,14 +76,16 @@ class Simulation:
     step_ms: list = field(default_factory=list)
 
     @classmethod
-    def build(cls, m: Member, precision: int = 1, lib=None, tuning: dict | None = None, graph: bool = True) -> "Simulation":
+    def build(cls, m: Member, precision: int = 1, lib=None, tuning: dict | None = None, graph: bool = True,
+              device_inputs: bool = False) -> "Simulation":
         """Set the member up.  A mesh finer than the reference's ray tagging
         allows (more than n1m faces in one pencil, pencil_rayTag.f90:126-130)
         is replaced by the next coarser icosahedron level (or a frequency 3/4
-        as fine)."""
+        as fine).  device_inputs: synthesise the initial velocity on the GPU
+        (inputs.solenoidal_velocity_device) instead of on the host."""
         while True:
             try:
-                return cls._build(m, precision, lib, tuning, graph)
+                return cls._build(m, precision, lib, tuning, graph, device_inputs)
             except MeltError as e:
                 f0 = m.freq or 2 ** m.level
                 if e.code != 5 or f0 <= 2:
@@ -97,21 +99,32 @@ class Simulation:
                       file=sys.stderr, flush=True)
 
     @classmethod
-    def _build(cls, m: Member, precision: int, lib, tuning: dict | None, graph: bool) -> "Simulation":
+    def _build(cls, m: Member, precision: int, lib, tuning: dict | None, graph: bool, device_inputs: bool = False) -> "Simulation":
         cfg = m.config(precision)
         ctx = MeltContext(cfg, lib=lib) if lib is not None else MeltContext(cfg)
         try:
             mesh = m.mesh()
             ctx.set_geometry(mesh.xyz, mesh.vert_of_edge, mesh.edge_of_face, m.rad)
             ctx.inqpr()
-            u, v, w = inputs.solenoidal_velocity(cfg, seed=m.seed, kmax=4.0, rms=0.1)
-            for name, a in (("vx", u), ("vy", v), ("vz", w)):
-                ctx.upload(name, a)
-            dt = inputs.cfl_dt(cfg, u, v, w, cfl=0.5)
+            if device_inputs:
+                import torch
+                u, v, w, dt = inputs.solenoidal_velocity_device(cfg, seed=m.seed, kmax=4.0, rms=0.1)
+                for name, a in (("vx", u), ("vy", v), ("vz", w)):
+                    ctx.upload_device(name, a.data_ptr())
+                del u, v, w
+                torch.cuda.empty_cache()
+            else:
+                u, v, w = inputs.solenoidal_velocity(cfg, seed=m.seed, kmax=4.0, rms=0.1)
+                for name, a in (("vx", u), ("vy", v), ("vz", w)):
+                    ctx.upload(name, a)
+                dt = inputs.cfl_dt(cfg, u, v, w, cfl=0.5)
             ctx.set_dt(dt)
             # many contexts share the GPU: each keeps its work on one stream
-            # (their streams then spread over the device's hardware queues)
-            ctx.set_tuning(streams=1, **(tuning or {}))
+            # (their streams then spread over the device's hardware queues);
+            # the ray-tagging pencils hold up to 4 n1m faces (the reference's
+            # BUFF_SIZE, pencil_rayTag.f90:84, is n1m), so that members keep
+            # their level-3/4 meshes on the coarse grids
+            ctx.set_tuning(streams=1, pbuf=4 * m.n, **(tuning or {}))
             ctx.tagCells()
             ctx.sync()
             if graph:

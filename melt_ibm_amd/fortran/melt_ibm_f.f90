@@ -139,6 +139,13 @@ module melt_ibm_f
       type(c_ptr), value :: ctx
       real(c_double), value :: b
     end function
+    integer(c_int) function c_solxk_array(ctx, q, lay, b) bind(C, name="melt_solxk_array")
+      import :: c_int, c_ptr, c_double, melt_layout_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(inout) :: q(*)
+      type(melt_layout_t), intent(in) :: lay
+      real(c_double), value :: b
+    end function
     integer(c_int) function c_solxk(ctx, id, b) bind(C, name="melt_solxk")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
@@ -195,6 +202,13 @@ module melt_ibm_f
   public :: melt_open, melt_close, melt_push, melt_pull, melt_geometry, melt_dt, melt_stage
   public :: tschem, tagCells, hdnl1, hdnl2, hdnl3, hdnlte, invtr1, invtr2, invtr3, invtrte
   public :: solxi, solxj, solxk, divg, phcalc, updvp, prcalc
+
+  ! solxk(q, betadx) as the reference calls it, q an array section such as
+  ! vx(1:n1,1:n2,kstart:kend) (invtr1.f90:68); solxk(id, betadx) names a
+  ! device-resident field instead
+  interface solxk
+    module procedure solxk_array, solxk_field
+  end interface
   public :: findCentroidIndices, mlsWeight, mlsForce, velforce, tempforce, particle
 
 contains
@@ -300,8 +314,23 @@ contains
     real(c_double), intent(in) :: betadx
     call check(c_solxj(handle, betadx), 'solxj')
   end subroutine
-  ! solxk(q, betadx): q is named by its field id (the device holds the array)
-  subroutine solxk(id, betadx)
+  ! solxk.f90:1-60: q(1:n1,1:n2,1:n3m) += z solve of the device rhs
+  subroutine solxk_array(q, betadx)
+    real(c_double), intent(inout) :: q(:, :, :)
+    real(c_double), intent(in) :: betadx
+    real(c_double), allocatable :: tmp(:, :, :)
+    type(melt_layout_t) :: lay
+    lay = melt_layout_t(int(size(q, 1), c_int32_t), int(size(q, 2), c_int32_t), 0, 0)
+    if (is_contiguous(q)) then
+      call check(c_solxk_array(handle, q, lay, betadx), 'solxk')
+    else
+      tmp = q
+      call check(c_solxk_array(handle, tmp, lay, betadx), 'solxk')
+      q = tmp
+    end if
+  end subroutine
+  ! the same with q named by its field id (the device holds the array)
+  subroutine solxk_field(id, betadx)
     integer, intent(in) :: id
     real(c_double), intent(in) :: betadx
     call check(c_solxk(handle, int(id, c_int), betadx), 'solxk')

@@ -84,3 +84,45 @@ def cfl_dt(cfg: MeltConfig, u, v, w, cfl: float = 0.5, dt0: float = 1.0) -> floa
     c = np.max(np.abs(u) * dt0 * dx1 + np.abs(v) * dt0 * dx2 + np.abs(w) * dt0 * dx3)
     c = max(c, 6.0 / cfg.ren * dt0 * dx1 ** 2)
     return dt0 * cfl / c
+
+
+def solenoidal_velocity_device(cfg: MeltConfig, seed: int = 0, kmax: float = 4.0, rms: float = 0.1):
+    """The construction of `solenoidal_velocity` on the GPU (torch, float64):
+    a random-phase vector potential with |k| <= kmax and amplitude 1/|k|^2,
+    its forward-difference curl (divergence-free under divg.f90:17-20),
+    scaled to the target rms.  Phases come from torch's generator seeded with
+    `seed`, so the field differs from the host construction's but is
+    reproducible.  Returns device tensors [k, j, i] and the dt of CFL 0.5
+    (cfl.f90:33-77 with dt0 = 1)."""
+    import torch
+    dev = torch.device("cuda")
+    n3, n2, n1 = cfg.shape
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(seed))
+    k3 = torch.fft.fftfreq(n3, 1.0 / n3, dtype=torch.float64, device=dev)[:, None, None]
+    k2 = torch.fft.fftfreq(n2, 1.0 / n2, dtype=torch.float64, device=dev)[None, :, None]
+    k1 = torch.fft.rfftfreq(n1, 1.0 / n1, dtype=torch.float64, device=dev)[None, None, :]
+    kk = torch.sqrt(k1 ** 2 + k2 ** 2 + k3 ** 2)
+    amp = torch.where((kk <= kmax) & (kk > 0), 1.0 / torch.clamp(kk, min=1.0) ** 2, torch.zeros_like(kk))
+    del kk
+    pot = []
+    for _ in range(3):
+        phase = torch.rand((n3, n2, n1 // 2 + 1), generator=gen, dtype=torch.float64, device=dev) * (2.0 * np.pi)
+        pot.append(torch.fft.irfftn(torch.polar(amp.expand_as(phase).contiguous(), phase), s=(n3, n2, n1), dim=(0, 1, 2)))
+        del phase
+    d1, d2, d3 = cfg.n1m / cfg.xlen, cfg.n2m / cfg.ylen, cfg.n3m / cfg.zlen
+
+    def fwd(a, axis, d):
+        return (torch.roll(a, -1, dims=axis) - a) * d
+
+    ax, ay, az = pot
+    u = fwd(az, 1, d2) - fwd(ay, 0, d3)
+    v = fwd(ax, 0, d3) - fwd(az, 2, d1)
+    w = fwd(ay, 2, d1) - fwd(ax, 1, d2)
+    del pot, ax, ay, az
+    s = rms / torch.sqrt((u * u + v * v + w * w).mean() / 3.0)
+    u, v, w = u * s, v * s, w * s
+    c = torch.max(torch.abs(u) * d1 + torch.abs(v) * d2 + torch.abs(w) * d3).item()
+    c = max(c, 6.0 / cfg.ren * d1 ** 2)
+    torch.cuda.current_stream().synchronize()
+    return u.contiguous(), v.contiguous(), w.contiguous(), 0.5 / c

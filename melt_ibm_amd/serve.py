@@ -1,15 +1,26 @@
 """Serving loop: melting-solid simulation requests arrive over time, are
 admitted against an HBM budget, keep their state resident on the device and
-advance together (continuous batching).
+advance concurrently (continuous batching).
 
-Each scheduler tick admits, in arrival order, every request that has arrived
-and fits in the free budget (a request that does not fit waits; later ones
-wait behind it, so admission is first-come first-served), then advances every
-resident simulation by one RK3 step: each step is a replay of that
-simulation's captured graph on its own stream, so the steps of all resident
-requests overlap on the device.  A request that has run its steps is retired
-and its memory returns to the budget.  Per-request step latency is the time
-from the tick's launch to that request's completion as seen by the host.
+Admission is first come first served: every request that has arrived and
+fits in the free budget has its bytes reserved and goes to a builder thread
+(a request that does not fit waits; later ones wait behind it).  The builder
+sets the simulation up off the scheduler's path (context and mesh, initial
+state synthesised on the GPU, initial ray tagging, and the first RK3 step,
+which captures the step's HIP graph) and hands it to the scheduler.  The
+scheduler keeps one step in flight per resident request: it polls each
+request's stream without blocking, and as soon as a request's step has
+finished (its device error word read) it launches that request's next step,
+a replay of its graph on its own stream.  Requests therefore advance at their
+own pace, a small grid does not wait for a large one, and the steps of all
+resident requests overlap on the device.  A request that has run its steps,
+or hit a terminal condition, is retired and its bytes return to the budget.
+
+Latency per request: the time between consecutive step completions of that
+request (inter-step time, what a client streaming the request's steps sees,
+including any wait behind other requests' work), beside the device latency of
+each step (launch to completion), the queue wait (arrival to admission) and
+the set-up time (admission to hand-over).
 
 The request stream (python -m melt_ibm_amd.serve): Poisson arrivals, mean 4
 requests/s, seed 7; grids from {64, 128, 256, 384}^3; spheres or spheroids of
@@ -20,7 +31,9 @@ from __future__ import annotations
 
 import argparse
 import json
+import queue
 import sys
+import threading
 import time
 from dataclasses import dataclass, field
 
@@ -37,7 +50,8 @@ def request_bytes(m: Member, precision: int) -> int:
     """Device bytes of one simulation (what the library allocates, with a
     margin): 26 fields + 7 solve buffers, the half spectrum, the Lagrangian
     arrays (with the 125 tagging-band distances and the 486 gradient-probe
-    coefficients per face) and the immersed-boundary windows."""
+    coefficients per face) and the immersed-boundary windows, plus the
+    transient initial-state synthesis (about 10 field-sized float64 buffers)."""
     esz = 4 if precision else 8
     n = m.n
     n1hp = ((n // 2 + 1) + 7) & ~7
@@ -46,7 +60,7 @@ def request_bytes(m: Member, precision: int) -> int:
     nv = nf // 2 + 2
     win = min(n, int(np.ceil(2.0 * m.rad * max(1.0, m.aspect) * n / (2 * np.pi))) + 10) ** 3
     b = esz * n ** 3 * 33 + 2 * esz * n1hp * n * n + 8 * (nf * (4 * 27 + 40 + 125 + 486) + nv * 24) + win * (4 * 16 + 12)
-    return int(b * 1.10) + (64 << 20)
+    return int(b * 1.10) + 10 * 8 * n ** 3 + (64 << 20)
 
 
 @dataclass
@@ -56,11 +70,16 @@ class Request:
     member: Member
     sim: Simulation | None = None
     admitted: float = -1.0
+    ready: float = -1.0
     finished: float = -1.0
     done_steps: int = 0
-    step_ms: list = field(default_factory=list)
+    step_ms: list = field(default_factory=list)       # inter-step times (completion to completion)
+    device_ms: list = field(default_factory=list)     # launch to completion of each step
     bytes: int = 0
     status: str = "ok"
+    inflight: bool = False
+    launched: float = 0.0
+    last_done: float = 0.0
 
 
 def request_stream(duration_s: float, rate: float = 4.0, seed: int = 7, steps_scale: float = 1.0, grids=SERVE_GRIDS) -> list[Request]:
@@ -79,30 +98,85 @@ def request_stream(duration_s: float, rate: float = 4.0, seed: int = 7, steps_sc
     return out
 
 
+def _terminal_status(r: Request, e: MeltError) -> str:
+    """The reference's terminal conditions end the request, not the server:
+    code 4 is a volume below V_thresh (particle.f90:218-224), which a tangled
+    surface (melting front faster than a cell per step, no remeshing) also
+    trips, with a negative or non-finite volume."""
+    if e.code != 4:
+        return f"error {e.code}"
+    try:
+        vol = r.sim.ctx.body().volume
+    except MeltError:
+        vol = float("nan")
+    return "melted" if np.isfinite(vol) and vol > 0.0 else "diverged"
+
+
 class Server:
-    def __init__(self, budget_bytes: float = 200e9, precision: int = 1, lib=None, tune: bool = False):
+    def __init__(self, budget_bytes: float = 200e9, precision: int = 1, lib=None, tune: bool = False,
+                 device_inputs: bool = True):
         self.budget = int(budget_bytes)
         self.precision = precision
         self.lib = lib
         self.tune = tune
+        self.device_inputs = device_inputs
         self.used = 0
         self.active: list[Request] = []
         self.done: list[Request] = []
         self.peak = 0
         self.tuning: dict[int, dict] = {}
+        self._t0 = 0.0
 
-    def _admit(self, r: Request, now: float) -> None:
+    def _clock(self) -> float:
+        return time.perf_counter() - self._t0
+
+    # ------------------------------------------------------------ builder
+    def _setup(self, r: Request) -> None:
         tuning = None
         if self.tune:
             from .tune import autotune
             if r.member.n not in self.tuning:
                 self.tuning[r.member.n] = autotune(r.member.n, self.precision, lib=self.lib)
             tuning = self.tuning[r.member.n]
-        r.sim = Simulation.build(r.member, self.precision, self.lib, tuning, graph=True)
-        r.admitted = now
-        self.used += r.bytes
-        self.peak = max(self.peak, self.used)
-        self.active.append(r)
+        r.sim = Simulation.build(r.member, self.precision, self.lib, tuning, graph=True, device_inputs=self.device_inputs)
+        # the first step captures the graph (host work) here, off the scheduler
+        t = self._clock()
+        r.sim.ctx.tschem()
+        try:
+            r.sim.ctx.sync()
+        except MeltError as e:
+            r.status = _terminal_status(r, e)
+        r.done_steps = 1
+        r.last_done = self._clock()
+        r.device_ms.append((r.last_done - t) * 1e3)
+
+    def _builder(self, todo: queue.Queue, built: queue.Queue) -> None:
+        while True:
+            r = todo.get()
+            if r is None:
+                return
+            try:
+                self._setup(r)
+            except MeltError as e:
+                r.status = f"failed {e.code}"
+            except Exception as e:  # noqa: BLE001 - any set-up failure ends only this request
+                r.status = f"failed {type(e).__name__}"
+            built.put(r)
+
+    # ---------------------------------------------------------- scheduler
+    def _retire(self, r: Request, keep_results: bool, results: dict, log) -> None:
+        r.finished = self._clock()
+        if keep_results and r.status == "ok" and r.sim is not None:
+            results[r.rid] = {q: r.sim.ctx.download(q) for q in ("vx", "vy", "vz", "temp", "pr")}
+            results[r.rid]["volume"] = r.sim.ctx.body().volume
+        if r.sim is not None:
+            r.sim.close()
+            r.sim = None
+        self.used -= r.bytes
+        self.done.append(r)
+        if log:
+            print(f"# request {r.rid} ({r.member.n}^3, {r.done_steps}/{r.member.steps} steps, {r.status}) done at "
+                  f"{r.finished:.2f} s", file=log, flush=True)
 
     def run(self, requests: list[Request], keep_results: bool = False, log=None) -> dict:
         pending = sorted(requests, key=lambda r: r.arrival)
@@ -110,64 +184,93 @@ class Server:
             r.bytes = request_bytes(r.member, self.precision)
             if r.bytes > self.budget:
                 raise ValueError(f"request {r.rid} ({r.member.n}^3) exceeds the HBM budget on its own")
-        t0 = time.perf_counter()
+        todo: queue.Queue = queue.Queue()
+        built: queue.Queue = queue.Queue()
+        worker = threading.Thread(target=self._builder, args=(todo, built), daemon=True)
+        self._t0 = time.perf_counter()
+        worker.start()
+        building = 0
         cell_steps = 0
-        results = {}
-        while pending or self.active:
-            now = time.perf_counter() - t0
-            while pending and pending[0].arrival <= now and self.used + pending[0].bytes <= self.budget:
-                self._admit(pending.pop(0), now)
-            if not self.active:
-                time.sleep(max(0.0, min(pending[0].arrival - now, 0.05)))
-                continue
-            ts = time.perf_counter()
-            for r in self.active:
-                r.sim.ctx.tschem()
-            for r in self.active:
-                try:
-                    r.sim.ctx.sync()
-                except MeltError as e:
-                    # the reference's terminal conditions end the request, not the server:
-                    # code 4 is a volume below V_thresh (particle.f90:218-224), which a
-                    # tangled surface (melting front faster than a cell per step, no
-                    # remeshing) also trips, with a negative or non-finite volume
-                    r.status = f"error {e.code}"
-                    if e.code == 4:
+        results: dict = {}
+        try:
+            while pending or self.active or building:
+                now = self._clock()
+                while pending and pending[0].arrival <= now and self.used + pending[0].bytes <= self.budget:
+                    r = pending.pop(0)
+                    r.admitted = now
+                    self.used += r.bytes
+                    self.peak = max(self.peak, self.used)
+                    todo.put(r)
+                    building += 1
+                while True:
+                    try:
+                        r = built.get_nowait()
+                    except queue.Empty:
+                        break
+                    building -= 1
+                    r.ready = self._clock()
+                    if r.status.startswith("failed"):
+                        self._retire(r, keep_results, results, log)
+                        continue
+                    cell_steps += r.member.cells
+                    self.active.append(r)
+                progressed = False
+                still = []
+                for r in self.active:
+                    if r.inflight:
+                        if not r.sim.ctx.poll():
+                            still.append(r)
+                            continue
+                        t = self._clock()
+                        r.inflight = False
+                        progressed = True
                         try:
-                            vol = r.sim.ctx.body().volume
-                        except MeltError:
-                            vol = float("nan")
-                        r.status = "melted" if np.isfinite(vol) and vol > 0.0 else "diverged"
-                r.step_ms.append((time.perf_counter() - ts) * 1e3)
-                r.done_steps += 1
-                cell_steps += r.member.cells
-            still = []
-            for r in self.active:
-                if r.done_steps >= r.member.steps or r.status != "ok":
-                    r.finished = time.perf_counter() - t0
-                    if keep_results and r.status == "ok":
-                        results[r.rid] = {q: r.sim.ctx.download(q) for q in ("vx", "vy", "vz", "temp", "pr")}
-                        results[r.rid]["volume"] = r.sim.ctx.body().volume
-                    r.sim.close()
-                    r.sim = None
-                    self.used -= r.bytes
-                    self.done.append(r)
-                    if log:
-                        print(f"# request {r.rid} ({r.member.n}^3, {r.done_steps}/{r.member.steps} steps, {r.status}) done at "
-                              f"{r.finished:.2f} s", file=log, flush=True)
-                else:
+                            r.sim.ctx.sync()
+                        except MeltError as e:
+                            r.status = _terminal_status(r, e)
+                        r.device_ms.append((t - r.launched) * 1e3)
+                        r.step_ms.append((t - r.last_done) * 1e3)
+                        r.last_done = t
+                        r.done_steps += 1
+                        cell_steps += r.member.cells
+                    if r.status != "ok" or r.done_steps >= r.member.steps:
+                        self._retire(r, keep_results, results, log)
+                        continue
+                    r.launched = self._clock()
+                    r.sim.ctx.tschem()
+                    r.inflight = True
                     still.append(r)
-            self.active = still
-        wall = time.perf_counter() - t0
-        lat = np.concatenate([np.array(r.step_ms) for r in self.done]) if self.done else np.zeros(1)
+                self.active = still
+                if not progressed:
+                    if not self.active and not building and pending:
+                        time.sleep(max(0.0, min(pending[0].arrival - self._clock(), 0.005)))
+                    else:
+                        time.sleep(0.0002)
+        finally:
+            todo.put(None)
+            worker.join()
+        wall = self._clock()
+        inter = np.concatenate([np.array(r.step_ms) for r in self.done if r.step_ms] or [np.zeros(1)])
+        dev = np.concatenate([np.array(r.device_ms) for r in self.done if r.device_ms] or [np.zeros(1)])
         wait = np.array([r.admitted - r.arrival for r in self.done]) if self.done else np.zeros(1)
-        status = {}
+        setup = np.array([r.ready - r.admitted for r in self.done]) if self.done else np.zeros(1)
+        status: dict = {}
         for r in self.done:
             status[r.status] = status.get(r.status, 0) + 1
+        by_grid = {}
+        for n in sorted({r.member.n for r in self.done}):
+            a = np.concatenate([np.array(r.step_ms) for r in self.done if r.member.n == n and r.step_ms] or [np.zeros(1)])
+            by_grid[str(n)] = {"requests": sum(1 for r in self.done if r.member.n == n),
+                               "steps": int(sum(r.done_steps for r in self.done if r.member.n == n)),
+                               "step_ms_p50": float(np.percentile(a, 50)), "step_ms_p99": float(np.percentile(a, 99))}
         rep = {"requests": len(self.done), "status": status, "wall_s": wall, "mcell_steps_per_s": cell_steps / wall / 1e6,
-               "step_ms_p50": float(np.percentile(lat, 50)), "step_ms_p99": float(np.percentile(lat, 99)),
+               "steps": int(sum(r.done_steps for r in self.done)),
+               "step_ms_p50": float(np.percentile(inter, 50)), "step_ms_p99": float(np.percentile(inter, 99)),
+               "device_step_ms_p50": float(np.percentile(dev, 50)), "device_step_ms_p99": float(np.percentile(dev, 99)),
                "queue_wait_s_p50": float(np.percentile(wait, 50)), "queue_wait_s_p99": float(np.percentile(wait, 99)),
+               "setup_s_p50": float(np.percentile(setup, 50)), "setup_s_p99": float(np.percentile(setup, 99)),
                "peak_budget_gb": self.peak / 1e9, "budget_gb": self.budget / 1e9,
+               "by_grid": by_grid,
                "grids": sorted({r.member.n for r in self.done}),
                "faces_range": [min(20 * (r.member.freq or 2 ** r.member.level) ** 2 for r in self.done),
                                max(20 * (r.member.freq or 2 ** r.member.level) ** 2 for r in self.done)] if self.done else []}
@@ -178,7 +281,7 @@ class Server:
 
 def main() -> None:
     ap = argparse.ArgumentParser(description="serving loop over a Poisson request stream")
-    ap.add_argument("--duration", type=float, default=10.0, help="arrival window (s)")
+    ap.add_argument("--duration", type=float, default=4.0, help="arrival window (s)")
     ap.add_argument("--rate", type=float, default=4.0)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--budget-gb", type=float, default=200.0)

@@ -28,6 +28,10 @@
  *    shape functions.  Tests bound it to 1 ulp of libm exp.
  *  - For_* ghost-plane folding, the mean force and all face/vertex sums follow
  *    the reference's serial order exactly.
+ *  - The HIT forcing's Gaussian numbers (hit.f90:39-44, gfortran random_number)
+ *    are Box-Muller pairs from SplitMix64 at a counter, the generator the
+ *    library uses (gfortran's stream is not reproducible elsewhere); the OU
+ *    statistics and the synthesis are pinned by closed forms.
  */
 #ifndef ORC_H
 #define ORC_H
@@ -56,6 +60,14 @@ typedef struct orc_ctx {
   double *r_x_prn_tri, *r_x_tau_n; /* 3*nf each */
   int *pind;                 /* 6*nf */
   melt_body body;
+  /* HIT forcing (hit.f90): nmodes wave numbers per direction, bhat/Fhat as
+   * (re, im) pairs in the reference's (3, nm, nm, nm) order, exp(i k x) tables */
+  int hit_nm;
+  double hit_tl, hit_eps, hit_kf;
+  uint64_t hit_seed, hit_steps;
+  int *hit_wave;
+  double *hit_bhat, *hit_fhat;
+  double *hit_ex[2], *hit_ey[2], *hit_ez[2];  /* [0] faces (xc...), [1] centres (xm...): (N, nm) complex */
   int status;
   char err[256];
 } orc_ctx;
@@ -99,5 +111,6 @@ int orc_velforce(orc_ctx* o);
 int orc_tempforce(orc_ctx* o);
 int orc_particle(orc_ctx* o);
 int orc_geometry_init(orc_ctx* o, double rad_p);
+void orc_hit_free(orc_ctx* o);
 
 #endif

@@ -108,11 +108,13 @@ void orc_ctx_destroy(orc_ctx* o) {
   free(o->vert_of_edge); free(o->edge_of_face); free(o->vert_of_face); free(o->face_of_edge);
   free(o->ghost_v); free(o->ghost_e); free(o->ghost_f);
   free(o->xyz0); free(o->r_x_prn_tri); free(o->r_x_tau_n); free(o->pind);
+  orc_hit_free(o);
   free(o);
 }
 
 const char* orc_last_error(const orc_ctx* o) { return o->err; }
 int orc_sync(orc_ctx* o) { return o->status; }
+int orc_poll(orc_ctx* o, int32_t* done) { (void)o; *done = 1; return 0; }  /* host work is done on return */
 
 int orc_set_dt(orc_ctx* o, double dt) { o->dt = dt; return 0; }
 
@@ -343,6 +345,21 @@ int orc_solxk(orc_ctx* o, int field, double betadx) {  /* solxk.f90:1-60 */
   double* q = o->f[field];
   const double* r = o->f[MELT_RHS];
   for (size_t c = 0; c < o->nc; ++c) q[c] = q[c] + r[c];
+  return 0;
+}
+
+/* solxk.f90:1-60 with q the caller's array section (invtr1.f90:68): rhs keeps
+ * the z solution, q(ic,jc,kc) += rhs over the interior (solxk.f90:44-50). */
+int orc_solxk_array(orc_ctx* o, double* q, const melt_layout* L, double betadx) {
+  solve_dir(o, o->f[MELT_RHS], 2, betadx);
+  const double* r = o->f[MELT_RHS];
+  int ld1 = L ? L->ld1 : o->n1, ld2 = L ? L->ld2 : o->n2, kg = L ? L->kghost : 0;
+  for (int k = 0; k < o->n3; ++k)
+    for (int j = 0; j < o->n2; ++j)
+      for (int i = 0; i < o->n1; ++i) {
+        size_t h = (size_t)i + (size_t)ld1 * ((size_t)j + (size_t)ld2 * (size_t)(k + kg));
+        q[h] = q[h] + r[(size_t)i + (size_t)o->n1 * ((size_t)j + (size_t)o->n2 * k)];
+      }
   return 0;
 }
 
@@ -626,6 +643,80 @@ int orc_inqpr(orc_ctx* o) {  /* inqpr.f90:3-44 */
         F(MELT_TEMP, ic, jc, kc) = o->cfg.tliq - (o->cfg.tliq - o->cfg.tsol) * 0.5 * (1.0 - tanh((rr - o->rad_p) * o->dx1 / 2.0));
       }
   return 0;
+}
+
+/* ------------------------------------------------------------ diagnostics */
+int orc_get_dt(orc_ctx* o, double cflfix, double* dt_out) {  /* cfl.f90:33-77 */
+  double buf = 0.0;
+  for (int k = 1; k <= o->n3; ++k)
+    for (int j = 1; j <= o->n2; ++j)
+      for (int i = 1; i <= o->n1; ++i) {
+        buf = fmax(buf, fabs(F(MELT_VX, i, j, k)) * o->dt * o->dx1 + fabs(F(MELT_VY, i, j, k)) * o->dt * o->dx2 +
+                            fabs(F(MELT_VZ, i, j, k)) * o->dt * o->dx3);
+        buf = fmax(buf, 6.0 / o->cfg.ren * o->dt * (o->dx1 * o->dx1));
+      }
+  *dt_out = buf > 0.0 ? o->dt * cflfix / buf : o->dt;
+  return 0;
+}
+
+int orc_dissipation(orc_ctx* o, double* out) {  /* diss.f90:11-201 */
+  int moving = o->has_geom && o->cfg.imlsfor == 1 && (o->cfg.imlsstr == 1 || o->cfg.imelt == 1);
+  if (moving) {
+    int rc = orc_tagcells(o);
+    if (rc) return rc;
+  }
+  double diss_v = 0.0, chi_v = 0.0, kenerg = 0.0, sumvof = 0.0;
+  double udx1 = o->dx1, udx2 = o->dx2, udx3 = o->dx3;
+  double pec = o->cfg.ren * o->cfg.prandtl;
+  for (int kc = 1; kc <= o->n3; ++kc) {
+    int kp = kc + 1, km = kc - 1;
+    for (int jc = 1; jc <= o->n2; ++jc) {
+      int jm = o->jmv[jc], jp = o->jpv[jc];
+      for (int ic = 1; ic <= o->n1; ++ic) {
+        int im = o->imv[ic], ip = o->ipv[ic];
+        double h11 = (F(MELT_VX, ip, jc, kc) - F(MELT_VX, ic, jc, kc)) * udx1;
+        double h12 = ((F(MELT_VX, ic, jp, kc) + F(MELT_VX, ip, jp, kc)) - (F(MELT_VX, ic, jm, kc) + F(MELT_VX, ip, jm, kc))) * 0.25 * udx2;
+        double h13 = ((F(MELT_VX, ic, jc, kp) + F(MELT_VX, ip, jc, kp)) - (F(MELT_VX, ic, jc, km) + F(MELT_VX, ip, jc, km))) * 0.25 * udx3;
+        double h21 = ((F(MELT_VY, ip, jc, kc) + F(MELT_VY, ip, jp, kc)) - (F(MELT_VY, im, jc, kc) + F(MELT_VY, im, jp, kc))) * 0.25 * udx1;
+        double h22 = (F(MELT_VY, ic, jp, kc) - F(MELT_VY, ic, jc, kc)) * udx2;
+        double h23 = ((F(MELT_VY, ic, jc, kp) + F(MELT_VY, ic, jp, kp)) - (F(MELT_VY, ic, jc, km) + F(MELT_VY, ic, jp, km))) * 0.25 * udx3;
+        double h31 = ((F(MELT_VZ, ip, jc, kc) + F(MELT_VZ, ip, jc, kp)) - (F(MELT_VZ, im, jc, kc) + F(MELT_VZ, im, jc, kp))) * 0.25 * udx1;
+        double h32 = ((F(MELT_VZ, ic, jp, kc) + F(MELT_VZ, ic, jp, kp)) - (F(MELT_VZ, ic, jm, kc) + F(MELT_VZ, ic, jm, kp))) * 0.25 * udx2;
+        double h33 = (F(MELT_VZ, ic, jc, kp) - F(MELT_VZ, ic, jc, kc)) * udx3;
+        double tke = (F(MELT_VX, ic, jc, kc) * F(MELT_VX, ic, jc, kc) + F(MELT_VX, ip, jc, kc) * F(MELT_VX, ip, jc, kc) +
+                      F(MELT_VY, ic, jc, kc) * F(MELT_VY, ic, jc, kc) + F(MELT_VY, ic, jp, kc) * F(MELT_VY, ic, jp, kc) +
+                      F(MELT_VZ, ic, jc, kc) * F(MELT_VZ, ic, jc, kc) + F(MELT_VZ, ic, jc, kp) * F(MELT_VZ, ic, jc, kp)) * 0.5;
+        tke = tke * 0.5;
+        double d = h11 * h11 + h22 * h22 + h33 * h33 + 0.5 * ((h21 + h12) * (h21 + h12)) + 0.5 * ((h31 + h13) * (h31 + h13)) +
+                   0.5 * ((h32 + h23) * (h32 + h23));
+        d = d * 2.0 / o->cfg.ren;
+        double ax = 0.5 * o->dx1 * (F(MELT_TEMP, ip, jc, kc) - F(MELT_TEMP, im, jc, kc));
+        double ay = 0.5 * o->dx2 * (F(MELT_TEMP, ic, jp, kc) - F(MELT_TEMP, ic, jm, kc));
+        double az = 0.5 * o->dx3 * (F(MELT_TEMP, ic, jc, kp) - F(MELT_TEMP, ic, jc, km));
+        double chi = (ax * ax + ay * ay + az * az) / pec;
+        if (F(MELT_VOFP, ic, jc, kc) == 1.0) {
+          sumvof = sumvof + F(MELT_VOFP, ic, jc, kc);
+          kenerg = kenerg + tke;
+          diss_v = diss_v + d;
+          chi_v = chi_v + chi;
+        }
+      }
+    }
+  }
+  kenerg = kenerg / sumvof;
+  double nu = 1.0 / o->cfg.ren;
+  diss_v = diss_v / sumvof;
+  chi_v = chi_v / sumvof;
+  double urms = sqrt(2.0 / 3.0 * kenerg);
+  double eta = pow(nu * nu * nu / diss_v, 0.25);
+  double kmax_eta = o->pi * o->dx1 * eta;
+  double re_l = kenerg * kenerg / (nu * diss_v);
+  double lambda_t = sqrt(15.0 * nu * (urms * urms) / diss_v);
+  double re_lam = urms * lambda_t / nu;
+  double l_int = sqrt(kenerg * kenerg * kenerg) / diss_v;
+  double v[11] = {diss_v, kenerg, urms, eta, lambda_t, re_l, re_lam, l_int, chi_v, kmax_eta, sumvof};
+  for (int q = 0; q < 11; ++q) out[q] = v[q];
+  return o->status;
 }
 
 /* ----------------------------------------------------------------- tschem */

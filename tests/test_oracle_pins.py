@@ -329,3 +329,87 @@ def test_inqpr_profile(lib):
     near = np.abs(rs - 1.0) < 0.1 * h
     slope = np.polyfit(rs[near], ts[near], 1)[0]
     assert abs(slope - (1.0 - 0.2) * (cfg.n1m / cfg.xlen) / 4.0) < 0.01 * slope
+
+
+def test_solxk_array_form_equals_field_form(lib):
+    # solxk(q, betadx) on an array section with the reference's padding
+    # column / row (ld1 = n1m + 1) adds the same z solution as the field form
+    cfg = make_cfg(16, 12)
+    o = oracle.context(cfg)
+    rng = np.random.default_rng(3)
+    rhs, q = rng.normal(size=cfg.shape), rng.normal(size=cfg.shape)
+    o.upload("rhs", rhs)
+    o.upload("vx", q)
+    o.solxk("vx", 0.4)
+    padded = np.full((cfg.n3m, cfg.n2m + 1, cfg.n1m + 1), 5.0)
+    padded[:, :cfg.n2m, :cfg.n1m] = q
+    o.upload("rhs", rhs)
+    o.solxk_array(padded, 0.4)
+    assert np.array_equal(padded[:, :cfg.n2m, :cfg.n1m], o.download("vx"))
+    assert np.all(padded[:, cfg.n2m, :] == 5.0) and np.all(padded[:, :, cfg.n1m] == 5.0)
+
+
+# ------------------------------------------------------ get_dt / dissipation
+def test_get_dt_closed_forms(lib):
+    cfg = make_cfg(16, 12, ren=50.0, xlen=6.0, ylen=5.0, zlen=4.0)
+    d = (cfg.n1m / cfg.xlen, cfg.n2m / cfg.ylen, cfg.n3m / cfg.zlen)
+    o = oracle.context(cfg)
+    z = np.zeros(cfg.shape)
+    dt, cflfix = 0.01, 0.7
+    load_state(o, {"vx": z, "vy": z, "vz": z})
+    o.set_dt(dt)
+    # at rest the viscous restriction 6 dt dx1^2 / Re sets dt (cfl.f90:54-55)
+    assert abs(o.get_dt(cflfix) - cflfix * cfg.ren / (6.0 * d[0] ** 2)) < 1e-15
+    for ax, name in enumerate(("vx", "vy", "vz")):
+        a = z.copy()
+        a[2, 3, 4] = -3.0
+        load_state(o, {"vx": z, "vy": z, "vz": z, name: a})
+        # a single face value s: |s| dt dx_axis (not averaged, cfl.f90:48-50)
+        assert abs(o.get_dt(cflfix) - cflfix / (3.0 * d[ax])) < 1e-15 * cflfix / (3.0 * d[ax]) * 4, name
+
+
+def _diss_case(cfg, st):
+    o = oracle.context(cfg)
+    z = np.zeros(cfg.shape)
+    load_state(o, {"vx": z, "vy": z, "vz": z, "temp": z, "VOFp": np.ones(cfg.shape), **st})
+    return o.dissipation()
+
+
+def test_dissipation_closed_forms(lib):
+    cfg = make_cfg(32, 24, ren=40.0, prandtl=0.7, xlen=6.0, ylen=5.0, zlen=4.0)
+    g = _grid(cfg)
+    z = np.zeros(cfg.shape)
+    # uniform translation: no dissipation, k = |U|^2 / 2 (diss.f90:107-111)
+    U = (0.3, -0.4, 0.2)
+    r = _diss_case(cfg, {"vx": z + U[0], "vy": z + U[1], "vz": z + U[2], "temp": z + 0.7})
+    assert r["diss"] == 0.0 and r["chi"] == 0.0
+    assert abs(r["kenerg"] - 0.5 * sum(x * x for x in U)) < 1e-12 * r["kenerg"] and r["sumvof"] == z.size
+    # shear u = A sin(k y): the central difference of diss.f90:67-69 gives
+    # du/dy = A cos(k y) sin(k h) / h, so diss = (du/dy)^2 / Re averages to
+    # A^2 sin^2(k h) / (2 h^2 Re); T = B sin(k x) gives chi the same form with Pe
+    A, B, m = 0.8, 0.5, 3
+    ky, kx = 2 * np.pi * m / cfg.ylen, 2 * np.pi * m / cfg.xlen
+    hy, hx = cfg.ylen / cfg.n2m, cfg.xlen / cfg.n1m
+    u = A * np.sin(ky * g["ym"])[None, :, None] + z
+    T = B * np.sin(kx * g["xm"])[None, None, :] + z
+    r = _diss_case(cfg, {"vx": u, "temp": T})
+    pec = cfg.ren * cfg.prandtl
+    assert abs(r["diss"] - A * A * np.sin(ky * hy) ** 2 / (2 * hy * hy * cfg.ren)) < 1e-12 * r["diss"]
+    assert abs(r["chi"] - B * B * np.sin(kx * hx) ** 2 / (2 * hx * hx * pec)) < 1e-12 * r["chi"]
+    assert abs(r["kenerg"] - A * A / 4.0) < 1e-12 * r["kenerg"]
+    # the derived scales obey their definitions (Pope 2000, eqs. 6.56-6.65):
+    # Re_lambda = sqrt(20/3 Re_L), eta = (nu^3/eps)^(1/4), L = k^(3/2)/eps
+    nu = 1.0 / cfg.ren
+    assert abs(r["re_lambda"] - np.sqrt(20.0 / 3.0 * r["re_l"])) < 1e-12 * r["re_lambda"]
+    assert abs(r["eta"] - (nu ** 3 / r["diss"]) ** 0.25) < 1e-14 * r["eta"]
+    assert abs(r["l_int"] - r["kenerg"] ** 1.5 / r["diss"]) < 1e-12 * r["l_int"]
+    assert abs(r["urms"] - np.sqrt(2.0 / 3.0 * r["kenerg"])) < 1e-15
+    assert abs(r["kmax_eta"] - np.pi * cfg.n1m / cfg.xlen * r["eta"]) < 1e-14 * r["kmax_eta"]
+    # only fully fluid cells (VOFp == 1) are averaged (diss.f90:144-153)
+    vof = np.ones(cfg.shape)
+    vof[:, :, : cfg.n1m // 2] = 0.999
+    o = oracle.context(cfg)
+    load_state(o, {"vx": u, "vy": z, "vz": z, "temp": T, "VOFp": vof})
+    half = o.dissipation()
+    assert half["sumvof"] == z.size // 2
+    assert abs(half["diss"] - r["diss"]) < 1e-12 * r["diss"]     # u depends on y only

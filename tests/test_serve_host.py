@@ -24,6 +24,9 @@ class _Ctx:
     def tschem(self):
         self.steps += 1
 
+    def poll(self):
+        return True
+
     def sync(self):
         if self.fail_at and self.steps == self.fail_at:
             raise MeltError(4, "device: Geometry V/VE is smaller than threshold")
@@ -44,7 +47,7 @@ class _Sim:
         self.closed = False
 
     @classmethod
-    def build(cls, m, precision, lib, tuning, graph=True):
+    def build(cls, m, precision, lib, tuning, graph=True, device_inputs=False):
         s = cls(m)
         cls.built.append((m.seed, s))
         return s
@@ -109,7 +112,7 @@ def test_pencil_overflow_steps_the_mesh_down(monkeypatch):
     from melt_ibm_amd import ensemble
     tried = []
 
-    def fake_build(cls, m, precision, lib, tuning, graph):
+    def fake_build(cls, m, precision, lib, tuning, graph, device_inputs=False):
         f = m.freq or 2 ** m.level
         tried.append(f)
         if f > 8:
@@ -123,10 +126,80 @@ def test_pencil_overflow_steps_the_mesh_down(monkeypatch):
     assert ensemble.Simulation.build(Member(n=32, rad=1.0, st=1.0, re=500.0, seed=0, freq=20)) == ("built", 8)
     assert tried == [20, 15, 11, 8]
 
-    def other_error(cls, m, precision, lib, tuning, graph):
+    def other_error(cls, m, precision, lib, tuning, graph, device_inputs=False):
         raise MeltError(3, "device: window overflow")
 
     monkeypatch.setattr(ensemble.Simulation, "_build", classmethod(other_error))
     with pytest.raises(MeltError) as ei:
         ensemble.Simulation.build(Member(n=32, rad=1.0, st=1.0, re=500.0, seed=0, level=4))
     assert ei.value.code == 3
+
+
+def test_setup_failure_ends_only_its_request(fake_sims, monkeypatch):
+    # any error while a request is set up (context, mesh, tagging) marks that
+    # request failed and returns its bytes; the others run to completion
+    real_build = fake_sims.build.__func__
+
+    def build(cls, m, precision, lib, tuning, graph=True, device_inputs=False):
+        if m.seed == 1:
+            raise MeltError(2, "hipMalloc: out of memory")
+        if m.seed == 2:
+            raise RuntimeError("mesh file unreadable")
+        return real_build(cls, m, precision, lib, tuning, graph, device_inputs)
+
+    monkeypatch.setattr(fake_sims, "build", classmethod(build))
+    reqs = [serve.Request(i, 0.0, _member(i, 3)) for i in range(4)]
+    srv = serve.Server(budget_bytes=1e12, precision=1)
+    rep = srv.run(reqs)
+    status = {r.rid: r.status for r in srv.done}
+    assert status == {0: "ok", 1: "failed 2", 2: "failed RuntimeError", 3: "ok"}
+    assert {r.rid: r.done_steps for r in srv.done} == {0: 3, 1: 0, 2: 0, 3: 3}
+    assert srv.used == 0 and rep["requests"] == 4
+
+
+class _SlowCtx(_Ctx):
+    """A step that stays in flight for `polls` polls after its launch."""
+
+    def __init__(self, polls, log, rid):
+        super().__init__(0, 0.0)
+        self.polls, self.left, self.log, self.rid = polls, 0, log, rid
+
+    def tschem(self):
+        super().tschem()
+        self.left = self.polls
+
+    def poll(self):
+        if self.left:
+            self.left -= 1
+            return False
+        return True
+
+    def sync(self):
+        self.log.append(self.rid)
+
+
+def test_requests_advance_at_their_own_pace(fake_sims):
+    # one step of request 0 stays in flight for 200 polls, request 1's steps
+    # for none: request 1 runs all its steps while request 0's second step is
+    # still running (no lock-step tick), and each step's inter-step time is
+    # recorded per request
+    log = []
+    reqs = [serve.Request(0, 0.0, _member(0, 2)), serve.Request(1, 0.0, _member(1, 20))]
+    srv = serve.Server(budget_bytes=1e12, precision=1)
+    orig = fake_sims.__init__
+
+    def init(self, m):
+        orig(self, m)
+        self.ctx = _SlowCtx(200 if m.seed == 0 else 0, log, m.seed)
+
+    fake_sims.__init__ = init
+    try:
+        rep = srv.run(reqs)
+    finally:
+        fake_sims.__init__ = orig
+    done = {r.rid: r for r in srv.done}
+    assert done[0].done_steps == 2 and done[1].done_steps == 20
+    # every step of request 1 completed before request 0's second step did
+    assert log.index(0, log.index(0) + 1) > max(i for i, rid in enumerate(log) if rid == 1)
+    assert len(done[1].step_ms) == 19 and len(done[0].step_ms) == 1
+    assert rep["steps"] == 22
