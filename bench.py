@@ -1,13 +1,14 @@
 #!/usr/bin/env python3
 """RK3 substep throughput of libmeltgpu.so on one MI355X.
 
-Workload (default): 256^3 periodic cube of side 2*pi with low-wavenumber
-(|k| <= 2.5) HIT forcing fields, a melting sphere of radius 0.5 meshed as a
-level-5 icosphere (10242 vertices, 20480 faces), Stefan number 2 and
-Newton-Euler coupling on, FP64.  A "step" is one RK3 time step (three
-substeps of tsch.f90) of the whole field and body; inputs are resident in HBM
-before the timed region.  Inputs are synthetic and seeded (no datasets exist
-for this path).
+Workload (default): 256^3 periodic cube of side 2*pi with stochastic HIT
+forcing at |k| <= 2.5 (hit.f90, regenerated on the device every step), a
+melting sphere of radius 0.5 meshed as a level-5 icosphere (10242 vertices,
+20480 faces), Stefan number 2 and Newton-Euler coupling on, FP64.  A "step"
+is one time step of the reference's loop: the HIT forcing and one RK3 step
+(three substeps of tsch.f90) of the whole field and body; inputs are resident
+in HBM before the timed region.  Inputs are synthetic and seeded (no datasets
+exist for this path).
 
 Prints one JSON line: value = Mcell-steps/s (whole job), plus
  - roofline: algorithmic HBM bytes of the dominant kernel over its average
@@ -47,6 +48,8 @@ KERNEL_BYTES = {
     "project": (5 + 4) * 8,            # dph, u,v,w,p -> u,v,w,p
 }
 B_ALG_CELL = 480  # algorithmic bytes per cell per substep of the whole path (FP64)
+# HIT forcing (HITForcing.in): TL, EPSSTAR, KF/KMIN = 2.5 (|k| <= 2.5, 5 wave numbers per direction)
+HIT_PARAMS = (0.5, 0.05, 2.5)
 # precision form: the same passes at 4 B per value (240 B) plus its two FP64
 # residual reductions over u, v, w (provisional and projected field, 12 B each)
 B_ALG_CELL_MIXED = 240 + 24
@@ -72,24 +75,28 @@ def run_gpu(args, precision=0, keep_fields=False):
     ctx.set_geometry(m.xyz, m.vert_of_edge, m.edge_of_face, rad)
     ctx.inqpr()
     u, v, w = inputs.solenoidal_velocity(cfg, seed=0, kmax=4.0, rms=0.1)
-    fx, fy, fz = inputs.low_mode_forcing(cfg, seed=1, kf=2.5, amplitude=0.5)
-    for name, a in (("vx", u), ("vy", v), ("vz", w), ("forcx", fx), ("forcy", fy), ("forcz", fz)):
+    for name, a in (("vx", u), ("vy", v), ("vz", w)):
         ctx.upload(name, a)
     dt = inputs.cfl_dt(cfg, u, v, w, cfl=0.5)
-    del u, v, w, fx, fy, fz
+    del u, v, w
     ctx.set_dt(dt)
+    ctx.hit_init(*HIT_PARAMS, seed=1)
     ctx.tagCells()
     ctx.sync()
     tag = "fp64" if precision == 0 else "mixed"
     print(f"# {tag} setup done: N={args.n} dt={dt:.4e} faces={m.counts[2]}", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
+
+    def step():   # the time loop's order (gcurv.f90:163-172): HIT forcing, then the RK3 step
+        ctx.hit_force()
         ctx.tschem()
+    for _ in range(args.warmup):
+        step()
     ctx.sync()
     ctx.timing(True)
     ctx.timing_reset()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        ctx.tschem()
+        step()
     ctx.sync()
     t1 = time.perf_counter()
     tm = ctx.timing_read()
@@ -132,13 +139,13 @@ def _oracle_substeps(cfg, level, rad, forcing: bool, warm: int, timed: int) -> l
     o.set_geometry(m.xyz, m.vert_of_edge, m.edge_of_face, rad)
     o.inqpr()
     u, v, w = inputs.solenoidal_velocity(cfg, seed=0, kmax=4.0, rms=0.1)
-    fields = [("vx", u), ("vy", v), ("vz", w)]
-    if forcing:
-        fields += list(zip(("forcx", "forcy", "forcz"), inputs.low_mode_forcing(cfg, seed=1, kf=2.5, amplitude=0.5)))
-    for name, a in fields:
+    for name, a in (("vx", u), ("vy", v), ("vz", w)):
         o.upload(name, a)
     o.set_dt(inputs.cfl_dt(cfg, u, v, w, cfl=0.5))
-    del u, v, w, fields
+    del u, v, w
+    if forcing:
+        o.hit_init(*HIT_PARAMS, seed=1)
+        o.hit_force()      # the forcing field of the first step (outside the substep timings)
     o.tagCells()
     times = []
     for i in range(warm + timed):
@@ -357,10 +364,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the precision-form leg")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
-    ap.add_argument("--ens-steps", type=int, default=10, help="timed ensemble steps")
+    ap.add_argument("--ens-steps", type=int, default=100, help="timed ensemble steps")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-window", type=float, default=4.0, help="arrival window of the serving leg (s)")
-    ap.add_argument("--serve-steps-scale", type=float, default=0.1, help="scale of the 50-500 step counts in the serving leg")
+    ap.add_argument("--serve-steps-scale", type=float, default=1.0, help="scale of the 50-500 step counts in the serving leg")
     ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_DB", "WRITE_DB", "COPY_FETCH_DB", "COPY_WRITE_DB"),
                     help="write the HBM-traffic summary under profiles/ from rocprofv3 --pmc databases")
     ap.add_argument("--summarize-trace", metavar="TRACE_DB", help="write the kernel-time summary under profiles/")
@@ -425,7 +432,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded solenoidal velocity |k|<=4 rms 0.1, HIT-like forcing |k|<=2.5, inqpr temperature, level-5 icosphere)",
+        "data": "synthetic (seeded solenoidal velocity |k|<=4 rms 0.1, stochastic HIT forcing |k|<=2.5 regenerated every step, "
+                "inqpr temperature, level-5 icosphere)",
         "config": {"workload": f"{cfg.n1m}^3 periodic 2pi cube, HIT forcing, melting sphere R=0.5 (level-5 icosphere, 20480 faces), "
                                f"St=2, Newton-Euler on, Re=1000, FP64",
                    "grid": [cfg.n1m, cfg.n2m, cfg.n3m], "faces": 20480, "precision": "fp64"},
@@ -481,14 +489,13 @@ def main():
         print(f"# ensemble: p50 {st['step_ms_p50']:.2f} ms, p99 {st['step_ms_p99']:.2f} ms", file=sys.stderr, flush=True)
     if not args.no_serve:
         from melt_ibm_amd.serve import Server, request_stream
-        # the bench leg draws grids up to 256^3 (a 384^3 request spends ~10 s in host-side
-        # initial-field synthesis); python -m melt_ibm_amd.serve runs the full stream
-        reqs = request_stream(args.serve_window, rate=4.0, seed=7, steps_scale=args.serve_steps_scale, grids=(64, 128, 256))
+        reqs = request_stream(args.serve_window, rate=4.0, seed=7, steps_scale=args.serve_steps_scale)
         print(f"# serving: {len(reqs)} requests over {args.serve_window} s", file=sys.stderr, flush=True)
         rep = Server(200e9, precision=1, tune=False).run(reqs, log=sys.stderr)
-        rep["workload"] = (f"Poisson arrivals (4/s, seed 7) over {args.serve_window} s; grids from {{64,128,256}}^3; spheres or "
-                           f"spheroids (aspect 1-3, 2k-60k faces); St in [0.5,4]; 50-500 RK3 steps x {args.serve_steps_scale}; "
-                           f"200 GB budget; FP32 bulk with FP64 refinement")
+        scale = "" if args.serve_steps_scale == 1.0 else f" x {args.serve_steps_scale}"
+        rep["workload"] = (f"Poisson arrivals (4/s, seed 7) over {args.serve_window} s; grids from {{64,128,256,384}}^3; spheres or "
+                           f"spheroids (aspect 1-3, 2k-60k faces); St in [0.5,4]; 50-500 RK3 steps{scale}; 200 GB budget; "
+                           f"FP32 bulk with FP64 refinement; initial states synthesised on the GPU")
         out["serving"] = rep
     if not args.no_cpu:
         try:

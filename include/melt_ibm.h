@@ -234,6 +234,19 @@ int melt_hit_state(melt_ctx* ctx, double* bhat, uint64_t* steps, int32_t set);
 int melt_hit_nmodes(melt_ctx* ctx, int32_t* nmodes, int32_t* nforced);
 int melt_abc_force(melt_ctx* ctx, double c_hit);
 
+/* Snapshot and resume (the continuation files of mpi_write_continua /
+ * mpi_read_continua, mpi_routines.f90:1223-1631, continua_particle /
+ * import_particles, create_geo.f90:135-409, and the bhat file, hit.f90:267-356)
+ * as one versioned host blob: the fields that carry state between steps (at
+ * the context's precision), the body scalars, every Lagrangian array, the
+ * immersed-boundary windows, dt and the HIT forcing state with its generator
+ * position.  melt_snapshot_load needs a context with the same configuration
+ * and mesh (melt_set_geometry with the same connectivity); the resumed context
+ * continues bit for bit.  Call between steps. */
+int melt_snapshot_size(melt_ctx* ctx, uint64_t* bytes);
+int melt_snapshot_save(melt_ctx* ctx, void* buf, uint64_t bytes);
+int melt_snapshot_load(melt_ctx* ctx, const void* buf, uint64_t bytes);
+
 /* Phase timing with HIP events around the four TIMED buckets of tsch.f90:37-84
  * (tagging, Eulerian solve, MLS/particle, pressure) and around each fused-path
  * kernel.  melt_timing_read returns the accumulated milliseconds per kernel slot

@@ -186,6 +186,9 @@ _SIGS = {
     "hit_state": (C.c_int, [_P, _P, C.POINTER(C.c_uint64), C.c_int32]),
     "hit_nmodes": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "abc_force": (C.c_int, [_P, C.c_double]),
+    "snapshot_size": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "snapshot_save": (C.c_int, [_P, _P, C.c_uint64]),
+    "snapshot_load": (C.c_int, [_P, _P, C.c_uint64]),
     "dissipation": (C.c_int, [_P, _P]),
     "timing_enable": (C.c_int, [_P, C.c_int32]), "timing_reset": (C.c_int, [_P]),
     "timing_count": (C.c_int, []), "timing_name": (C.c_char_p, [C.c_int32]),
@@ -200,7 +203,7 @@ _SIGS = {
 }
 OPTIONAL = {"timing_enable", "timing_reset", "timing_count", "timing_name", "timing_read", "device_ok", "set_refinement",
             "get_refinement_stats", "graph_enable", "stream",
-            "set_tuning", "poll", "upload_field_device"}
+            "set_tuning", "poll", "upload_field_device", "snapshot_size", "snapshot_save", "snapshot_load"}
 
 
 def bind(lib: C.CDLL, prefix: str) -> dict:
@@ -427,6 +430,28 @@ class MeltContext:
     def abc_force(self, c_hit: float) -> None:
         """CalcABC_HITForce (hit.f90:234-265)."""
         self._chk(self._f["abc_force"](self._ctx, float(c_hit)))
+
+    # ------------------------------------------------------- snapshot / resume
+    def snapshot(self) -> bytearray:
+        """The context's state between steps as one blob (melt_snapshot_save)."""
+        n = C.c_uint64(0)
+        self._chk(self._f["snapshot_size"](self._ctx, C.byref(n)))
+        buf = bytearray(n.value)
+        self._chk(self._f["snapshot_save"](self._ctx, (C.c_char * len(buf)).from_buffer(buf), n.value))
+        return buf
+
+    def restore(self, blob) -> None:
+        """Resume from a snapshot taken on a context of the same configuration and mesh."""
+        b = bytearray(blob)
+        self._chk(self._f["snapshot_load"](self._ctx, (C.c_char * len(b)).from_buffer(b), len(b)))
+
+    def save(self, path) -> None:
+        with open(path, "wb") as f:
+            f.write(self.snapshot())
+
+    def load(self, path) -> None:
+        with open(path, "rb") as f:
+            self.restore(f.read())
 
     DISS_KEYS = ("diss", "kenerg", "urms", "eta", "lambda_t", "re_l", "re_lambda", "l_int", "chi", "kmax_eta", "sumvof")
 

@@ -43,9 +43,15 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"command failed ({res.returncode}): {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
 
 
+def _source_stamp() -> float:
+    files = [CSRC / f for f in SOURCES] + list(CSRC.glob("*.h")) + [ROOT / "include" / "melt_ibm.h"]
+    return max(f.stat().st_mtime for f in files)
+
+
 def build_library(verbose: bool = False) -> Path:
     hipcc = _hipcc()
     OBJDIR.mkdir(exist_ok=True)
+    stamp = _source_stamp()
     jobs = []
     for src in SOURCES:
         obj = OBJDIR / (Path(src).stem + ".o")
@@ -53,6 +59,9 @@ def build_library(verbose: bool = False) -> Path:
     with cf.ThreadPoolExecutor(max_workers=min(len(jobs), 8)) as ex:
         for fut in [ex.submit(_run, j) for j in jobs]:
             fut.result()
+    # every object must see the same headers (the context struct is shared)
+    if _source_stamp() != stamp:
+        raise RuntimeError("sources changed while the library was compiling; build again")
     tmp = LIB.with_suffix(".so.tmp")
     _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *[str(OBJDIR / (Path(s).stem + ".o")) for s in SOURCES],
           "-o", str(tmp)])

@@ -176,6 +176,11 @@ extern "C" int melt_ctx_create(const melt_config* cfg, melt_ctx** out) {
   int rc = 0;
   do {
     if (hipGetDevice(&ctx->device) != hipSuccess) { rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "no HIP device"); break; }
+    {
+      int lds = 0;
+      if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) == hipSuccess && lds > 0)
+        ctx->lds_max = (size_t)lds;
+    }
     if (const char* tv = getenv("MELT_TUNE")) {
       static const char* keys[TUNE_N] = {"xf_w", "xf_nt", "solve_w", "solve_nt", "xf_z", "yz_jb", "streams", "coef", "pbuf"};
       std::string spec(tv);
@@ -475,7 +480,8 @@ extern "C" int melt_tagcells(melt_ctx* ctx) {
   cancel_ahead(ctx);
   int rc = ibm_tag(ctx);
   if (rc) return rc;
-  return euler_launch_interp_vof(ctx);
+  // sphereTagging sets VOFx/y/z itself; interp_vof belongs to the narrow band (genSDF.f90:80)
+  return ctx->cfg.tagtype == 0 ? 0 : euler_launch_interp_vof(ctx);
 }
 extern "C" int melt_interp_vof(melt_ctx* ctx) { return check_ctx(ctx) ? MELT_ERR_ARG : euler_launch_interp_vof(ctx); }
 
@@ -911,5 +917,146 @@ extern "C" int melt_get_refinement_stats(melt_ctx* ctx, int64_t* checks, int64_t
   *checks = h[0];
   *refined = h[1];
   memcpy(last_ratio, &h[2], sizeof(double));
+  return 0;
+}
+
+// ------------------------------------------------------- snapshot / resume
+// The continuation files of the reference (mpi_write_continua /
+// mpi_read_continua, mpi_routines.f90:1223-1631: the fields; continua_particle
+// / import_particles, create_geo.f90:135-409: the body and its vertices; the
+// bhat file, hit.f90:267-356) as one versioned host blob of the context's
+// state at a step boundary: header, body scalars, immersed-boundary windows,
+// the fields that carry state between steps (stored at the context's
+// precision), every Lagrangian array, the probe indices and the ghost flags,
+// and the HIT forcing state with its generator position.  A context created
+// with the same configuration and mesh resumes from it bit for bit.
+namespace {
+
+constexpr char SNAP_MAGIC[8] = {'M', 'E', 'L', 'T', 'S', 'N', 'P', '1'};
+constexpr uint32_t SNAP_VERSION = 1;
+const int SNAP_FIELDS[] = {MELT_VX, MELT_VY, MELT_VZ, MELT_TEMP, MELT_PR, MELT_RU1, MELT_RU2, MELT_RU3, MELT_RUT,
+                           MELT_FORCX, MELT_FORCY, MELT_FORCZ, MELT_VOFP, MELT_VOFX, MELT_VOFY, MELT_VOFZ, MELT_SDF};
+
+struct SnapHeader {
+  char magic[8];
+  uint32_t version, pad0;
+  melt_config cfg;
+  int32_t has_geom, nv, ne, nf;
+  int32_t hit_nm, pad1;
+  uint64_t bytes, mesh_hash;
+  double dt, rad_p, hit_tl, hit_eps, hit_kf;
+  uint64_t hit_seed, hit_steps;
+};
+
+uint64_t fnv1a(const void* p, size_t n, uint64_t h = 1469598103934665603ull) {
+  const unsigned char* b = static_cast<const unsigned char*>(p);
+  for (size_t i = 0; i < n; ++i) { h ^= b[i]; h *= 1099511628211ull; }
+  return h;
+}
+
+// One entry per stored device array: pointer and byte count.
+std::vector<std::pair<void*, size_t>> snap_arrays(melt_ctx* ctx) {
+  std::vector<std::pair<void*, size_t>> a;
+  a.push_back({ctx->d_body, sizeof(melt_body)});
+  a.push_back({ctx->d_win, sizeof(melt_window) * 2});
+  for (int id : SNAP_FIELDS) a.push_back({ctx->f[id], ctx->esz * ctx->nc});
+  if (ctx->has_geom) {
+    for (int id = 0; id < MELT_L_NLAG; ++id) a.push_back({ctx->d_lag[id], sizeof(double) * lag_count(ctx, id)});
+    a.push_back({ctx->d_pind, sizeof(int) * 6 * (size_t)ctx->nf});
+    a.push_back({ctx->d_gv, (size_t)ctx->nv});
+    a.push_back({ctx->d_ge, (size_t)ctx->ne});
+    a.push_back({ctx->d_gf, (size_t)ctx->nf});
+  }
+  if (ctx->hit_nm) a.push_back({ctx->d_hit_bhat, sizeof(double) * 2 * 3 * (size_t)ctx->hit_nm * ctx->hit_nm * ctx->hit_nm});
+  return a;
+}
+
+int snap_mesh_hash(melt_ctx* ctx, uint64_t* h) {
+  *h = 0;
+  if (!ctx->has_geom) return 0;
+  std::vector<int> vof(3 * (size_t)ctx->nf);
+  MELT_HIP(hipMemcpy(vof.data(), ctx->d_vof, sizeof(int) * vof.size(), hipMemcpyDeviceToHost));
+  *h = fnv1a(vof.data(), sizeof(int) * vof.size());
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int melt_snapshot_size(melt_ctx* ctx, uint64_t* bytes) {
+  if (check_ctx(ctx) || !bytes) return MELT_ERR_ARG;
+  uint64_t n = sizeof(SnapHeader);
+  for (auto& e : snap_arrays(ctx)) n += e.second;
+  *bytes = n;
+  return 0;
+}
+
+extern "C" int melt_snapshot_save(melt_ctx* ctx, void* buf, uint64_t bytes) {
+  if (check_ctx(ctx) || !buf) return MELT_ERR_ARG;
+  uint64_t need = 0;
+  melt_snapshot_size(ctx, &need);
+  if (bytes < need) return melt_fail(ctx, MELT_ERR_ARG, "snapshot_save", "buffer smaller than melt_snapshot_size");
+  MELT_HIP(hipStreamSynchronize(ctx->stream));
+  if (ctx->side) MELT_HIP(hipStreamSynchronize(ctx->side));
+  SnapHeader h;
+  memset(&h, 0, sizeof h);
+  memcpy(h.magic, SNAP_MAGIC, 8);
+  h.version = SNAP_VERSION;
+  h.cfg = ctx->cfg;
+  h.has_geom = ctx->has_geom; h.nv = ctx->nv; h.ne = ctx->ne; h.nf = ctx->nf;
+  h.hit_nm = ctx->hit_nm;
+  h.bytes = need;
+  if (int rc = snap_mesh_hash(ctx, &h.mesh_hash)) return rc;
+  h.dt = ctx->dt; h.rad_p = ctx->rad_p;
+  h.hit_tl = ctx->hit_tl; h.hit_eps = ctx->hit_eps; h.hit_kf = ctx->hit_kf; h.hit_seed = ctx->hit_seed;
+  if (ctx->hit_nm) {
+    unsigned long long s = 0;
+    MELT_HIP(hipMemcpy(&s, ctx->d_hit_count, sizeof s, hipMemcpyDeviceToHost));
+    h.hit_steps = s;
+  }
+  char* p = static_cast<char*>(buf);
+  memcpy(p, &h, sizeof h);
+  p += sizeof h;
+  for (auto& e : snap_arrays(ctx)) {
+    MELT_HIP(hipMemcpy(p, e.first, e.second, hipMemcpyDeviceToHost));
+    p += e.second;
+  }
+  return 0;
+}
+
+extern "C" int melt_snapshot_load(melt_ctx* ctx, const void* buf, uint64_t bytes) {
+  if (check_ctx(ctx) || !buf || bytes < sizeof(SnapHeader)) return MELT_ERR_ARG;
+  SnapHeader h;
+  memcpy(&h, buf, sizeof h);
+  if (memcmp(h.magic, SNAP_MAGIC, 8) || h.version != SNAP_VERSION)
+    return melt_fail(ctx, MELT_ERR_ARG, "snapshot_load", "not a snapshot of this library version");
+  if (memcmp(&h.cfg, &ctx->cfg, sizeof(melt_config)))
+    return melt_fail(ctx, MELT_ERR_ARG, "snapshot_load", "the context's configuration differs from the snapshot's");
+  if (h.has_geom != ctx->has_geom || h.nv != ctx->nv || h.ne != ctx->ne || h.nf != ctx->nf)
+    return melt_fail(ctx, MELT_ERR_ARG, "snapshot_load", "the context's mesh differs from the snapshot's");
+  uint64_t mh = 0;
+  if (int rc = snap_mesh_hash(ctx, &mh)) return rc;
+  if (mh != h.mesh_hash) return melt_fail(ctx, MELT_ERR_ARG, "snapshot_load", "the context's mesh connectivity differs from the snapshot's");
+  if (h.hit_nm && (h.hit_nm != ctx->hit_nm || h.hit_kf != ctx->hit_kf)) {
+    if (int rc = melt_hit_init(ctx, h.hit_tl, h.hit_eps, h.hit_kf, h.hit_seed)) return rc;
+  }
+  if (h.hit_nm) { ctx->hit_tl = h.hit_tl; ctx->hit_eps = h.hit_eps; ctx->hit_seed = h.hit_seed; }
+  uint64_t need = 0;
+  melt_snapshot_size(ctx, &need);
+  if (bytes < need || h.bytes != need) return melt_fail(ctx, MELT_ERR_ARG, "snapshot_load", "snapshot size does not match the context");
+  cancel_ahead(ctx);
+  graph_drop(ctx);
+  MELT_HIP(hipStreamSynchronize(ctx->stream));
+  if (ctx->side) MELT_HIP(hipStreamSynchronize(ctx->side));
+  const char* p = static_cast<const char*>(buf) + sizeof h;
+  for (auto& e : snap_arrays(ctx)) {
+    MELT_HIP(hipMemcpy(e.first, p, e.second, hipMemcpyHostToDevice));
+    p += e.second;
+  }
+  if (h.hit_nm) {
+    unsigned long long s = h.hit_steps;
+    MELT_HIP(hipMemcpy(ctx->d_hit_count, &s, sizeof s, hipMemcpyHostToDevice));
+  }
+  ctx->dt = h.dt;
+  MELT_HIP(hipMemcpy(&ctx->h_body, ctx->d_body, sizeof(melt_body), hipMemcpyDeviceToHost));
   return 0;
 }

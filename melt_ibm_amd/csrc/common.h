@@ -75,6 +75,7 @@ struct melt_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int device = 0;
+  size_t lds_max = 64 * 1024;   // dynamic LDS a workgroup may use (hipDeviceAttributeMaxSharedMemoryPerBlock)
 
   // Eulerian fields: double, or float in the precision form (fp32 = 1); fld<R>()
   int fp32 = 0;
@@ -134,7 +135,7 @@ struct melt_ctx {
   // HIT forcing (hit.f90): bhat and Fhat (3 x nm^3 complex), exp(i k x) of the
   // staggered coordinates, the forced modes, the generator's step counter
   int hit_nm = 0, hit_nact = 0;
-  double hit_tl = 0, hit_eps = 0;
+  double hit_tl = 0, hit_eps = 0, hit_kf = 0;
   unsigned long long hit_seed = 0;
   double *d_hit_bhat = nullptr, *d_hit_fhat = nullptr, *d_hit_tab = nullptr;
   int* d_hit_act = nullptr;

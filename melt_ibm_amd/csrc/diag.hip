@@ -126,7 +126,7 @@ extern "C" int melt_dissipation(melt_ctx* ctx, double* out) {
   const melt_config& c = ctx->cfg;
   const bool moving = c.imlsfor == 1 && (c.imlsstr == 1 || c.imelt == 1) && ctx->has_geom;
   int rc;
-  if (moving && ((rc = ibm_tag(ctx)) || (rc = euler_launch_interp_vof(ctx)))) return rc;
+  if (moving && ((rc = ibm_tag(ctx)) || (c.tagtype != 0 && (rc = euler_launch_interp_vof(ctx))))) return rc;
   double s[4];
   if ((rc = diag_dissipation(ctx, s))) return rc;
   const double sumvof = s[3];

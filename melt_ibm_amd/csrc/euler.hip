@@ -94,6 +94,12 @@ struct EulerArgs {
   const R* __restrict__ forcz;
   const melt_body* __restrict__ body;
   const double* __restrict__ coord;  // coordinate tables (coord_off)
+  // face volume fractions as fields (sphereTagging sets them geometrically);
+  // null: 1/2 (VOFp of the two cells), their value after every narrow-band
+  // tagging (tagCells.f90:105-107)
+  const R* __restrict__ vofx;
+  const R* __restrict__ vofy;
+  const R* __restrict__ vofz;
   Grid g;
   double rlen[3];                    // 1/xlen, 1/ylen, 1/zlen
   double betagz, tliq;
@@ -260,6 +266,10 @@ EulerArgs<R> make_args(melt_ctx* ctx) {
   A.forcx = fld<R>(ctx, MELT_FORCX); A.forcy = fld<R>(ctx, MELT_FORCY); A.forcz = fld<R>(ctx, MELT_FORCZ);
   A.body = ctx->d_body;
   A.coord = ctx->d_coord;
+  const bool faces = ctx->cfg.tagtype == 0;
+  A.vofx = faces ? fld<R>(ctx, MELT_VOFX) : nullptr;
+  A.vofy = faces ? fld<R>(ctx, MELT_VOFY) : nullptr;
+  A.vofz = faces ? fld<R>(ctx, MELT_VOFZ) : nullptr;
   A.g = make_grid(ctx);
   A.rlen[0] = 1.0 / A.g.xlen; A.rlen[1] = 1.0 / A.g.ylen; A.rlen[2] = 1.0 / A.g.zlen;
   A.betagz = ctx->cfg.betagz; A.tliq = ctx->cfg.tliq;
@@ -287,9 +297,9 @@ __global__ __launch_bounds__(256) void k_hdnl(EulerArgs<R> A, R* __restrict__ ou
   const auto s = ptr_stencil(A, make_cell(A.g, c));
   R p = SV(FVOF, 0, 0, 0);
   R r;
-  if (which == 0) r = h1_val(A, s, R(0.5) * (SV(FVOF, -1, 0, 0) + p));
-  else if (which == 1) r = h2_val(A, s, R(0.5) * (SV(FVOF, 0, -1, 0) + p));
-  else if (which == 2) r = h3_val(A, s, R(0.5) * (SV(FVOF, 0, 0, -1) + p));
+  if (which == 0) r = h1_val(A, s, A.vofx ? A.vofx[c] : R(0.5) * (SV(FVOF, -1, 0, 0) + p));
+  else if (which == 1) r = h2_val(A, s, A.vofy ? A.vofy[c] : R(0.5) * (SV(FVOF, 0, -1, 0) + p));
+  else if (which == 2) r = h3_val(A, s, A.vofz ? A.vofz[c] : R(0.5) * (SV(FVOF, 0, 0, -1) + p));
   else r = ht_val(A, s, p);
   out[c] = r;
 }
@@ -515,9 +525,9 @@ __global__ __launch_bounds__(256) void k_explicit_xsolve(EulerArgs<R> A, RhsCoef
     auto cell = [&](const auto& cl) {
       const auto s = ptr_stencil(A, cl);
       R p = SV(FVOF, 0, 0, 0);
-      R hu = h1_val(A, s, R(0.5) * (SV(FVOF, -1, 0, 0) + p));
-      R hv = h2_val(A, s, R(0.5) * (SV(FVOF, 0, -1, 0) + p));
-      R hw = h3_val(A, s, R(0.5) * (SV(FVOF, 0, 0, -1) + p));
+      R hu = h1_val(A, s, A.vofx ? cl.ld(A.vofx, 0, 0, 0) : R(0.5) * (SV(FVOF, -1, 0, 0) + p));
+      R hv = h2_val(A, s, A.vofy ? cl.ld(A.vofy, 0, 0, 0) : R(0.5) * (SV(FVOF, 0, -1, 0) + p));
+      R hw = h3_val(A, s, A.vofz ? cl.ld(A.vofz, 0, 0, 0) : R(0.5) * (SV(FVOF, 0, 0, -1) + p));
       R ht = ht_val(A, s, p);
       R o1 = cl.ref(ru1), o2 = cl.ref(ru2), o3 = cl.ref(ru3), ot = cl.ref(rut);
       const double d[4] = {(double)rhs_u<FU>(cf, s, hu, o1) * sc_u, (double)rhs_v<FV>(cf, s, hv, o2) * sc_u,
@@ -1173,7 +1183,9 @@ int launch_explicit_xsolve(melt_ctx* ctx, double bxu, double bxt) {
               static_cast<R*>(ctx->rhs4[RHS_T])};
   // z-march (one x-line per block, D planes) unless line-tile shapes are asked for
   const size_t zshm = sizeof(double) * 4 * (ctx->n1 + 2 * FIR_KMAX) + sizeof(R) * 16 * (size_t)ctx->n1;
-  if (fir && ctx->tune[TUNE_XF_W] == 0 && ctx->tune[TUNE_XF_NT] == 0 && ctx->n1 % 64 == 0 && ctx->n1 <= 512 && zshm <= 64 * 1024) {
+  // (with geometric face fractions, tagType 0, the line tiles read them)
+  if (fir && ctx->tune[TUNE_XF_W] == 0 && ctx->tune[TUNE_XF_NT] == 0 && ctx->n1 % 64 == 0 && ctx->n1 <= 512 && zshm <= ctx->lds_max &&
+      ctx->cfg.tagtype != 0) {
     int D = ctx->tune[TUNE_XF_Z];
     if (D <= 0 || ctx->n3 % D) {
       D = 1;
@@ -1206,7 +1218,7 @@ int launch_explicit_xsolve(melt_ctx* ctx, double bxu, double bxt) {
   unsigned blocks = (unsigned)((size_t)(ctx->n2 / W) * ctx->n3);
   const bool flat = ctx->nc * sizeof(R) < (size_t(1) << 32);   // 32-bit byte offsets reach every element
   const size_t shm = fir ? sizeof(double) * 4 * (size_t)W * (ctx->n1 + 2 * FIR_KMAX) : shm0;
-  if (shm > 64 * 1024) return melt_fail(ctx, MELT_ERR_ARG, "explicit_xsolve", "tile exceeds LDS");
+  if (shm > ctx->lds_max) return melt_fail(ctx, MELT_ERR_ARG, "explicit_xsolve", "tile exceeds LDS");
 #define XF_LAUNCH(EE, FIRV, FLATV)                                                                                      \
   hipLaunchKernelGGL((k_explicit_xsolve<EE, R, FIRV, FLATV>), dim3(blocks), dim3(nt), shm, ctx->stream, A, cf,           \
                      fld<R>(ctx, MELT_RU1), fld<R>(ctx, MELT_RU2), fld<R>(ctx, MELT_RU3), fld<R>(ctx, MELT_RUT), r4[0], r4[1], \
@@ -1251,7 +1263,7 @@ int launch_solve(melt_ctx* ctx, int dir, void* const* fields, int nfield, const 
   }
   size_t tiles = dir == 0 ? (size_t)(ctx->n2 / W) * ctx->n3 : dir == 1 ? (size_t)(ctx->n1 / W) * ctx->n3 : (size_t)(ctx->n1 / W) * ctx->n2;
   size_t shm = dir == 0 ? sizeof(double) * (size_t)W * (M + 1) : sizeof(double) * (size_t)W * M;
-  if (shm > 64 * 1024) return melt_fail(ctx, MELT_ERR_ARG, "line solve", "tile exceeds LDS");
+  if (shm > ctx->lds_max) return melt_fail(ctx, MELT_ERR_ARG, "line solve", "tile exceeds LDS");
   dim3 grid((unsigned)tiles, nfield);
 #define PCR_CASE(D, EE) \
   case EE: hipLaunchKernelGGL((k_pcr<D, EE, R>), grid, dim3(nt), shm, ctx->stream, B, M, W, ctx->n1, ctx->n2, ctx->n3); break;
@@ -1492,3 +1504,4 @@ int euler_reduce_max(melt_ctx* ctx, int what, double* out, double dt) {
   return 0;
 }
 int euler_launch_inqpr(melt_ctx* ctx) { return ctx->fp32 ? launch_inqpr<float>(ctx) : launch_inqpr<double>(ctx); }
+

@@ -220,6 +220,7 @@ extern "C" int melt_hit_init(melt_ctx* ctx, double tl, double epsstar, double kf
   ctx->hit_nact = (int)act.size();
   ctx->hit_tl = tl;
   ctx->hit_eps = epsstar;
+  ctx->hit_kf = kf_on_kmin;
   ctx->hit_seed = seed;
   return 0;
 }

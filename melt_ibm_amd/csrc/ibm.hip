@@ -1024,6 +1024,91 @@ MeshArgs mesh_args(melt_ctx* ctx) {
 }  // namespace
 
 // ----------------------------------------------------------------- host
+// sphereTagging (tagType 0, sphereTagging.f90:5-549): the level set of
+// Kempe & Froehlich (2012) for a rigid sphere of radius rad_p.  A thread owns
+// one (i, j, kk) of the bounding box of get_bbox_inds (tagCells.f90:31-62,
+// lim from k_vbox, all three axes scaled by dx1 as there), unwrapped in x and
+// y; kk is folded into the domain with the centre shifted by zlen
+// (get_periodic_indices, tagCells.f90:68-84).  Each of VOFp, VOFx, VOFy, VOFz
+// is 1 - alpha over the 8 corners of its control volume (level_setc2, 12, 22,
+// 32), alpha = sum (-phi) H(-phi) / sum |phi|, phi the distance to the centre
+// minus rad_p (geom.f90:56-80), corners in the reference's i, j, k order.
+__device__ inline double sphere_phi(double x, double y, double z, const double c[3], double rad) {
+  const double dx = x - c[0], dy = y - c[1], dz = z - c[2];
+  return sqrt(dx * dx + dy * dy + dz * dz) - rad;
+}
+
+template <int XS, int YS, int ZS>   // per axis: 0 corners at the faces (c, c+1), 1 at the centres (c-1, c)
+__device__ inline double sphere_alpha(const Grid& g, int ic, int jc, int kc, const double c[3], double rad) {
+  double alpha = 0.0, tot = 0.0;
+  for (int a = 0; a < 2; ++a) {
+    const int i = XS ? ic - 1 + a : ic + a;
+    const double x = XS ? gxm(g, i) : gxc(g, i);
+    for (int b = 0; b < 2; ++b) {
+      const int j = YS ? jc - 1 + b : jc + b;
+      const double y = YS ? gym(g, j) : gyc(g, j);
+      for (int d = 0; d < 2; ++d) {
+        const int k = ZS ? kc - 1 + d : kc + d;
+        const double z = ZS ? gzm(g, k) : gzc(g, k);
+        const double phi = sphere_phi(x, y, z, c, rad);
+        alpha = alpha + (-phi) * (-phi >= 0.0 ? 1.0 : 0.0);
+        tot = tot + fabs(phi);
+      }
+    }
+  }
+  return alpha / tot;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_sphere_tag(Grid g, const double* __restrict__ lim, const melt_body* __restrict__ body, double rad,
+                                                    R* __restrict__ vofp, R* __restrict__ vofx, R* __restrict__ vofy, R* __restrict__ vofz,
+                                                    int* err) {
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = (int)floor(lim[d] * g.dx1) + 1 - 2;
+    hi[d] = (int)floor(lim[3 + d] * g.dx1) + 1 + 2;
+  }
+  const int ext[3] = {hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1};
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0 && (ext[0] > g.n1 || ext[1] > g.n2 || ext[2] > g.n3)) raise_error(err, MELT_ERR_WINDOW);  // a cell would be visited twice
+  if (t >= (size_t)g.n1 * g.n2 * g.n3) return;
+  const int di = (int)(t % g.n1), dj = (int)((t / g.n1) % g.n2), dk = (int)(t / ((size_t)g.n1 * g.n2));
+  if (di >= ext[0] || dj >= ext[1] || dk >= ext[2]) return;
+  const int i = lo[0] + di, j = lo[1] + dj;
+  int k = lo[2] + dk;
+  double c[3] = {body->pos_cm[0], body->pos_cm[1], body->pos_cm[2]};
+  if (k >= g.n3 + 1) { k = k - g.n3; c[2] = c[2] - g.zlen; }
+  if (k < 1) { k = k + g.n3; c[2] = c[2] + g.zlen; }
+  if (k < 1 || k > g.n3) return;
+  const size_t o = (size_t)pmod(i - 1, g.n1) + (size_t)g.n1 * ((size_t)pmod(j - 1, g.n2) + (size_t)g.n2 * (k - 1));
+  vofp[o] = (R)(1.0 - sphere_alpha<0, 0, 0>(g, i, j, k, c, rad));
+  vofx[o] = (R)(1.0 - sphere_alpha<1, 0, 0>(g, i, j, k, c, rad));
+  vofy[o] = (R)(1.0 - sphere_alpha<0, 1, 0>(g, i, j, k, c, rad));
+  vofz[o] = (R)(1.0 - sphere_alpha<0, 0, 1>(g, i, j, k, c, rad));
+}
+
+template <typename R>
+static int sphere_tag_t(melt_ctx* ctx) {
+  const Grid g = make_grid(ctx);
+  Win3 ef{{ctx->fwin[0], ctx->fwin[1], ctx->fwin[2]}}, et{{ctx->twin[0], ctx->twin[1], ctx->twin[2]}};
+  hipLaunchKernelGGL(k_window_box, dim3(nb(ctx->nf, 256)), dim3(256), 0, ctx->stream, g, ctx->d_lag[MELT_L_TRI_BAR], ctx->d_gf, ctx->nf,
+                     ctx->d_win);
+  hipLaunchKernelGGL(k_window_final, dim3(1), dim3(1), 0, ctx->stream, ctx->d_win, ef, et, ctx->force_window_from_tag ? 3 : 2,
+                     ctx->d_err);
+  hipLaunchKernelGGL(k_vbox, dim3(1), dim3(256), 0, ctx->stream, ctx->d_lag[MELT_L_XYZV], ctx->d_gv, ctx->nv, ctx->d_red);
+  for (int id = MELT_VOFP; id <= MELT_VOFZ; ++id)
+    hipLaunchKernelGGL(k_fill<R>, dim3(1024), dim3(256), 0, ctx->stream, fld<R>(ctx, id), 1.0, ctx->nc);
+  hipLaunchKernelGGL(k_sphere_tag<R>, dim3(nb(ctx->nc, 256)), dim3(256), 0, ctx->stream, g, ctx->d_red, ctx->d_body, ctx->rad_p,
+                     fld<R>(ctx, MELT_VOFP), fld<R>(ctx, MELT_VOFX), fld<R>(ctx, MELT_VOFY), fld<R>(ctx, MELT_VOFZ), ctx->d_err);
+  MELT_HIP(hipGetLastError());
+  if (ctx->h_body.initial_tag) {   // no ray tagging in this mode: the flag only gates graph capture
+    ctx->h_body.initial_tag = 0;
+    MELT_HIP(hipMemcpyAsync(&ctx->d_body->initial_tag, &ctx->h_body.initial_tag, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    MELT_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return 0;
+}
+
 template <typename R>
 static int ray_tag_init_t(melt_ctx* ctx) {
   Grid g = make_grid(ctx);
@@ -1069,6 +1154,7 @@ static int ray_tag_init_t(melt_ctx* ctx) {
 template <typename R>
 static int ibm_tag_t(melt_ctx* ctx) {  // tagCells.f90:4-27 (tagType 1) + interp_vof when not fused
   if (!ctx->has_geom) return melt_fail(ctx, MELT_ERR_NO_GEOMETRY, "tagCells", "no geometry");
+  if (ctx->cfg.tagtype == 0) return sphere_tag_t<R>(ctx);
   Grid g = make_grid(ctx);
   if (ctx->h_body.initial_tag) {
     int rc = ray_tag_init_t<R>(ctx);

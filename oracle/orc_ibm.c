@@ -428,8 +428,67 @@ int orc_interp_vof(orc_ctx* o) {  /* tagCells.f90:85-111 */
   return 0;
 }
 
-int orc_tagcells(orc_ctx* o) {  /* tagCells.f90:4-27 (tagType 1) */
+/* ---------------------------------------------- sphereTagging (tagType 0) */
+static double sphere_dist(const double x0[3], const double xc[3], double rad) {  /* geom.f90:56-80 */
+  return sqrt((x0[0] - xc[0]) * (x0[0] - xc[0]) + (x0[1] - xc[1]) * (x0[1] - xc[1]) + (x0[2] - xc[2]) * (x0[2] - xc[2])) - rad;
+}
+
+/* level_setc2 / level_set12 / 22 / 32 (sphereTagging.f90:91-549): corners of
+ * the control volume of the quantity; stag[d] = 1 takes the centres c-1, c of
+ * axis d, 0 the faces c, c+1. */
+static double level_set(orc_ctx* o, int ic, int jc, int kc, const double xgc[3], const int stag[3]) {
+  double phi_tot = 0.0, alpha = 0.0;
+  for (int i = (stag[0] ? ic - 1 : ic); i <= (stag[0] ? ic : ic + 1); ++i)
+    for (int j = (stag[1] ? jc - 1 : jc); j <= (stag[1] ? jc : jc + 1); ++j)
+      for (int k = (stag[2] ? kc - 1 : kc); k <= (stag[2] ? kc : kc + 1); ++k) {
+        double x[3] = {stag[0] ? orc_xm(o, i) : orc_xc(o, i), stag[1] ? orc_ym(o, j) : orc_yc(o, j),
+                       stag[2] ? orc_zm(o, k) : orc_zc(o, k)};
+        double phi = sphere_dist(x, xgc, o->rad_p);
+        alpha = alpha + (-phi) * (-phi >= 0.0 ? 1.0 : 0.0);   /* heaviside, geom.f90:129-138 */
+        phi_tot = phi_tot + fabs(phi);
+      }
+  return alpha / phi_tot;
+}
+
+static int sphereTagging(orc_ctx* o) {  /* sphereTagging.f90:5-34, tagCells.f90:11-16 */
+  double lim[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};   /* get_bbox_inds, tagCells.f90:31-62 */
+  for (int v = 0; v < o->nv; ++v) {
+    if (o->ghost_v[v]) continue;
+    for (int d = 0; d < 3; ++d) {
+      lim[d] = fmin(lim[d], LAG(MELT_L_XYZV)[3 * v + d]);
+      lim[3 + d] = fmax(lim[3 + d], LAG(MELT_L_XYZV)[3 * v + d]);
+    }
+  }
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = (int)floor(lim[d] * o->dx1) + 1 - 2;
+    hi[d] = (int)floor(lim[3 + d] * o->dx1) + 1 + 2;
+  }
+  for (int id = MELT_VOFP; id <= MELT_VOFZ; ++id) orc_fill_field(o, id, 1.0);
+  static const int ids[4] = {MELT_VOFP, MELT_VOFX, MELT_VOFY, MELT_VOFZ};   /* qc2, q12, q22, q32 */
+  static const int stags[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  for (int q = 0; q < 4; ++q)
+    for (int i = lo[0]; i <= hi[0]; ++i)
+      for (int j = lo[1]; j <= hi[1]; ++j)
+        for (int kk = lo[2]; kk <= hi[2]; ++kk) {
+          double xgc[3] = {o->body.pos_cm[0], o->body.pos_cm[1], o->body.pos_cm[2]};
+          int k = kk;
+          if (k >= o->n3 + 1) { k = k - o->n3; xgc[2] = xgc[2] - o->cfg.zlen; }   /* get_periodic_indices */
+          if (k < 1) { k = k + o->n3; xgc[2] = xgc[2] + o->cfg.zlen; }
+          if (k < 1 || k > o->n3) continue;
+          double alpha = level_set(o, i, j, k, xgc, stags[q]);
+          int ii = orc_mod(i - 1, o->n1) + 1, jj = orc_mod(j - 1, o->n2) + 1;
+          if (!(F(ids[q], ii, jj, k) < 1.0)) F(ids[q], ii, jj, k) = 1.0 - alpha;
+        }
+  return 0;
+}
+
+int orc_tagcells(orc_ctx* o) {  /* tagCells.f90:4-27 */
   if (!o->has_geom) return MELT_ERR_NO_GEOMETRY;
+  if (o->cfg.tagtype == 0) {
+    o->body.initial_tag = 0;
+    return sphereTagging(o);
+  }
   if (o->body.initial_tag) {
     int rc = pencilTag(o);
     if (rc) return rc;

@@ -25,8 +25,8 @@ def small_members():
             Member(n=48, rad=0.8, st=3.5, re=250.0, seed=3, level=3)]
 
 
-def alone(m, steps, precision, lib, tuning=None, graph=False):
-    s = Simulation.build(m, precision, lib, tuning, graph=graph)
+def alone(m, steps, precision, lib, tuning=None, graph=False, device_inputs=False):
+    s = Simulation.build(m, precision, lib, tuning, graph=graph, device_inputs=device_inputs)
     for _ in range(steps):
         s.ctx.tschem()
     s.ctx.sync()
@@ -77,7 +77,7 @@ def test_serving_results_match_alone_and_budget_holds(gpu_lib):
     assert rep["peak_budget_gb"] * 1e9 <= budget
     assert rep["step_ms_p99"] >= rep["step_ms_p50"] > 0
     for i, m in enumerate(members):
-        ref = alone(m, steps[i], 1, gpu_lib)
+        ref = alone(m, steps[i], 1, gpu_lib, device_inputs=True)    # the server synthesises inputs on the GPU
         for q in FIELDS:
             assert np.array_equal(rep["results"][i][q], ref[q]), (i, q)
         assert rep["results"][i]["volume"] == ref["volume"]
@@ -93,7 +93,7 @@ def test_diverged_request_ends_cheaply(gpu_lib):
     4.2 ms, against 272 ms when each thread queued a compare-and-swap on it)."""
     m = Member(n=256, rad=0.5558868667770105, st=3.0961983157665, re=273.19648405043654, seed=6, aspect=2.6943004927317387,
                freq=35, steps=45)
-    srv = Server(200e9, precision=1, lib=gpu_lib)
+    srv = Server(200e9, precision=1, lib=gpu_lib, device_inputs=False)     # the seeded host state that tangles
     rep = srv.run([Request(0, 0.0, m)])
     r = srv.done[0]
     assert rep["status"] == {"diverged": 1}

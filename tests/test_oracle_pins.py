@@ -413,3 +413,46 @@ def test_dissipation_closed_forms(lib):
     half = o.dissipation()
     assert half["sumvof"] == z.size // 2
     assert abs(half["diss"] - r["diss"]) < 1e-12 * r["diss"]     # u depends on y only
+
+
+# ------------------------------------------------------ sphereTagging (tagType 0)
+def _sphere_case(n, centre=(0.03, -0.05, 0.02), rad=1.0):
+    cfg = make_cfg(n, tagtype=0, imelt=0)
+    o = oracle.context(cfg)
+    with_body(o, 3, rad)
+    b = o.body()
+    for d in range(3):
+        b.pos_cm[d] += centre[d]
+    o.set_body(b)
+    o.set_lag("xyzv", o.lag("xyzv") + np.array(centre))
+    o.tagCells()
+    return cfg, o, np.array(o.body().pos_cm[:])
+
+
+def test_sphere_tagging_volume_and_staggering(lib):
+    """The Kempe-Froehlich fractions of sphereTagging.f90 add up to the
+    sphere's volume at second order, and the solid of each staggered field,
+    weighted at that field's own points (VOFx at the u faces, ...), is centred
+    on the sphere (a half-cell slip of the corner sets moves it by h/2)."""
+    errs = []
+    for n in (24, 48):
+        cfg, o, pc = _sphere_case(n)
+        h = cfg.xlen / n
+        (xm, ym, zm), (xc, yc, zc) = inputs.grid_coords(cfg)
+        vol = 4.0 / 3.0 * np.pi
+        e = []
+        for name, pts in (("VOFp", (xm, ym, zm)), ("VOFx", (xc, ym, zm)), ("VOFy", (xm, yc, zm)), ("VOFz", (xm, ym, zc))):
+            s = 1.0 - o.download(name)
+            assert s.min() >= -1e-15 and s.max() <= 1.0 + 1e-15
+            e.append(abs(s.sum() * h ** 3 - vol) / vol)
+            X = np.broadcast_to(pts[0][None, None, :], s.shape)
+            Y = np.broadcast_to(pts[1][None, :, None], s.shape)
+            Z = np.broadcast_to(pts[2][:, None, None], s.shape)
+            cen = np.array([(s * X).sum(), (s * Y).sum(), (s * Z).sum()]) / s.sum()
+            assert np.max(np.abs(cen - pc)) < 0.02 * h, (name, cen - pc)
+        errs.append(max(e))
+        # far outside untouched, deep inside solid
+        r = np.sqrt((xm[None, None, :] - pc[0]) ** 2 + (ym[None, :, None] - pc[1]) ** 2 + (zm[:, None, None] - pc[2]) ** 2)
+        vofp = o.download("VOFp")
+        assert np.all(vofp[r > 1.0 + 2 * h] == 1.0) and np.all(vofp[r < 1.0 - 2 * h] == 0.0)
+    assert errs[1] < errs[0] / 3.0 and errs[1] < 1e-2, errs
