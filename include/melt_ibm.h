@@ -205,6 +205,17 @@ int melt_particle(melt_ctx* ctx);     /* particle.f90:1-229 (IREMESH=0)         
 /* Diagnostics the reference's time loop (gcurv.f90) calls around tschem. */
 int melt_cfl(melt_ctx* ctx, double* cflm);     /* cfl.f90:1-30 (max over cells)     */
 int melt_divgck(melt_ctx* ctx, double* qmax);  /* divg.f90:26-56                    */
+/* No counterpart in the reference.  melt_divgck for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops): no context and
+ * no hipMalloc; vx, vy, vz are device arrays of n1*n2*n3 values (double for
+ * elem_bytes 8, float for 4), i fastest, periodic; max |div u| goes to the
+ * device double *qmax, enqueued on stream (a hipStream_t, null for the null
+ * stream), kernels only.  Its scratch is a fixed 8 KiB device array per GPU
+ * held until the process ends: calls that overlap on one GPU race.  Limits:
+ * n1, n2, n3 >= 2, n1 a multiple of 16 / elem_bytes, n1*n2*n3 < 2^31
+ * (MELT_ERR_ARG otherwise). */
+int melt_divgck_noalloc(const void* vx, const void* vy, const void* vz, int32_t n1, int32_t n2, int32_t n3, double xlen,
+                        double ylen, double zlen, int32_t elem_bytes, void* stream, double* qmax);
 int melt_inqpr(melt_ctx* ctx);                 /* inqpr.f90:3-44                    */
 /* get_dt (cfl.f90:33-77): *dt_out = dt * cflfix / max(max over cells of
  * |u| dt dx1 + |v| dt dx2 + |w| dt dx3, 6 dt dx1^2 / Re) at the context's dt

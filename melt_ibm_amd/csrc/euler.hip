@@ -1505,3 +1505,70 @@ int euler_reduce_max(melt_ctx* ctx, int what, double* out, double dt) {
 }
 int euler_launch_inqpr(melt_ctx* ctx) { return ctx->fp32 ? launch_inqpr<float>(ctx) : launch_inqpr<double>(ctx); }
 
+// ------------------------------------------- allocation-free divergence check
+// Scratch of melt_divgck_noalloc: the REDUCE_SLOTS partial maxima of the
+// reduction (8 KiB).  Every GPU the library runs on holds its own copy until
+// the process ends; host code reaches it through its device address, looked
+// up once per device, never through this name.
+static __device__ __align__(256) unsigned long long scratch[REDUCE_SLOTS];
+
+#include <mutex>
+
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+unsigned long long* noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+std::mutex noalloc_mu;
+
+// Device address of `scratch` on the current device (hipGetSymbolAddress on
+// the first call per device, with that device current).
+int noalloc_scratch(unsigned long long** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES) return MELT_ERR_HIP;
+  std::lock_guard<std::mutex> lock(noalloc_mu);
+  if (!noalloc_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return MELT_ERR_HIP;
+    noalloc_ptr[dev] = static_cast<unsigned long long*>(p);
+  }
+  *out = noalloc_ptr[dev];
+  return 0;
+}
+
+__global__ __launch_bounds__(REDUCE_SLOTS) void k_clear_slots(unsigned long long* __restrict__ slots) { slots[threadIdx.x] = 0ull; }
+
+template <typename R>
+int divgck_noalloc(const void* vx, const void* vy, const void* vz, const Grid& g, hipStream_t s, double* qmax, unsigned long long* slots) {
+  constexpr int V = 16 / sizeof(R);
+  const size_t nc = (size_t)g.n1 * g.n2 * g.n3;
+  hipLaunchKernelGGL(k_clear_slots, dim3(1), dim3(REDUCE_SLOTS), 0, s, slots);
+  hipLaunchKernelGGL(k_reduce_max<R>, dim3(nblocks(nc / V, 256)), dim3(256), 0, s, g, static_cast<const R*>(vx), static_cast<const R*>(vy),
+                     static_cast<const R*>(vz), 1, 0.0, (int)nc, slots);
+  hipLaunchKernelGGL(k_fold_slots, dim3(1), dim3(REDUCE_SLOTS), 0, s, slots, reinterpret_cast<unsigned long long*>(qmax));
+  return hipGetLastError() == hipSuccess ? 0 : MELT_ERR_HIP;
+}
+
+}  // namespace
+
+// melt_divgck (divg.f90:26-56) without a context and without allocating: the
+// three velocity components are device arrays of n1*n2*n3 values (double for
+// elem_bytes 8, float for 4), i fastest, periodic in all directions; the
+// maximum |div u| is written to the device double *qmax on `stream` (kernels
+// only, so the call may be captured into a graph).  Limits: n1, n2, n3 >= 2,
+// n1 a multiple of 16 / elem_bytes, n1*n2*n3 < 2^31.  Calls that overlap on
+// one GPU race on the shared scratch.
+extern "C" int melt_divgck_noalloc(const void* vx, const void* vy, const void* vz, int32_t n1, int32_t n2, int32_t n3, double xlen,
+                                   double ylen, double zlen, int32_t elem_bytes, void* stream, double* qmax) {
+  if (!vx || !vy || !vz || !qmax || (elem_bytes != 4 && elem_bytes != 8)) return MELT_ERR_ARG;
+  if (n1 < 2 || n2 < 2 || n3 < 2 || !(xlen > 0.0) || !(ylen > 0.0) || !(zlen > 0.0)) return MELT_ERR_ARG;
+  if (n1 % (16 / elem_bytes)) return MELT_ERR_ARG;
+  if ((size_t)n1 * (size_t)n2 * (size_t)n3 >= ((size_t)1 << 31)) return MELT_ERR_ARG;
+  unsigned long long* slots = nullptr;
+  if (int rc = noalloc_scratch(&slots)) return rc;
+  Grid g;
+  g.n1 = n1; g.n2 = n2; g.n3 = n3;
+  g.xlen = xlen; g.ylen = ylen; g.zlen = zlen;
+  g.dx1 = (double)n1 / xlen; g.dx2 = (double)n2 / ylen; g.dx3 = (double)n3 / zlen;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return elem_bytes == 8 ? divgck_noalloc<double>(vx, vy, vz, g, s, qmax, slots) : divgck_noalloc<float>(vx, vy, vz, g, s, qmax, slots);
+}
