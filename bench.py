@@ -358,7 +358,7 @@ def precision_study(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256, help="cells per side")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -461,7 +461,7 @@ def main():
         n_flip = int(flip.sum())
         del f32, f64, flip, near
         out["mixed"] = {
-            "value": round(cells * args.steps / el32 / 1e6, 3), "unit": "Mcell-steps/s", "dtype": "f32 storage, f64 refinement",
+            "value": round(cells * args.steps / el32 / 1e6, 3), "unit": "Mcell-steps/s", "dtype": "f32 storage, f64 line reductions / sums / Lagrangian",
             "ms_per_step": round(el32 / args.steps * 1e3, 4), "speedup_vs_fp64": round(elapsed / el32, 3),
             "substep_ms": round(sub32 * 1e3, 4), "substep_frac": round(B_ALG_CELL_MIXED * cells / sub32 / HBM_PEAK, 4),
             "bytes_per_cell_substep": B_ALG_CELL_MIXED,
@@ -483,7 +483,7 @@ def main():
         ens.close()
         del ens
         st["workload"] = ("48 independent melting spheres (R 0.3-0.7, level-3/4 icosahedra), grids uniform from {48,64,96,128}^3, "
-                          "St in [0.5,4], Re in [200,1000], seeded; FP32 bulk with FP64 refinement; one graph replay per simulation "
+                          "St in [0.5,4], Re in [200,1000], seeded; FP32 bulk, FP64 Lagrangian; one graph replay per simulation "
                           "and step on its own stream")
         out["ensemble"] = st
         print(f"# ensemble: p50 {st['step_ms_p50']:.2f} ms, p99 {st['step_ms_p99']:.2f} ms", file=sys.stderr, flush=True)
@@ -495,7 +495,7 @@ def main():
         scale = "" if args.serve_steps_scale == 1.0 else f" x {args.serve_steps_scale}"
         rep["workload"] = (f"Poisson arrivals (4/s, seed 7) over {args.serve_window} s; grids from {{64,128,256,384}}^3; spheres or "
                            f"spheroids (aspect 1-3, 2k-60k faces); St in [0.5,4]; 50-500 RK3 steps{scale}; 200 GB budget; "
-                           f"FP32 bulk with FP64 refinement; initial states synthesised on the GPU")
+                           f"FP32 bulk, FP64 Lagrangian; initial states synthesised on the GPU")
         out["serving"] = rep
     if not args.no_cpu:
         try:

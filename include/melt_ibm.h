@@ -38,7 +38,9 @@ enum melt_status {
   MELT_ERR_SINGULAR_MLS = 3,    /* dgetrf info /= 0: mlsWeight.f90:121-124      */
   MELT_ERR_SMALL_VOLUME = 4,    /* Volume < V_thresh: particle.f90:218-224      */
   MELT_ERR_PENCIL_OVERFLOW = 5, /* > BUFF_SIZE faces in a pencil: pencil_rayTag.f90:126-130 */
-  MELT_ERR_REMESH = 6,          /* remeshing requested (IREMESH=1): not on this path */
+  MELT_ERR_REMESH = 6,          /* remeshing failed: topology not preserved, geometry reduced to a
+                                   tetrahedron, singular QEM system or volume residual > 1e-14
+                                   (remesh_coarsen.f90:69-253, particle.f90:199-211) */
   MELT_ERR_RAY_RECURSION = 7,   /* parallel ray recursion did not terminate: rayAux.f90:37-44 */
   MELT_ERR_NO_GEOMETRY = 8,     /* immersed-boundary call before melt_set_geometry */
   MELT_ERR_WINDOW = 9           /* body outgrew the device window sized at melt_set_geometry */
@@ -161,6 +163,39 @@ int melt_get_lag(melt_ctx* ctx, int id, double* host);
 int melt_set_lag(melt_ctx* ctx, int id, const double* host);
 int melt_get_pind(melt_ctx* ctx, int32_t* host);           /* pind(6,nf)             */
 int melt_get_vert_of_face(melt_ctx* ctx, int32_t* host);   /* 3*nf, 1-based          */
+/* The connectivity as remeshing leaves it (allotri.f90:13-28: vert_of_face,
+ * edge_of_face 3*nf, vert_of_edge, face_of_edge 2*ne, 1-based) and the ghost
+ * flags isGhostVert/Edge/Face (0/1); any pointer may be null. */
+int melt_get_topology(melt_ctx* ctx, int32_t* vert_of_face, int32_t* edge_of_face, int32_t* vert_of_edge, int32_t* face_of_edge,
+                      int32_t* is_ghost_vert, int32_t* is_ghost_edge, int32_t* is_ghost_face);
+/* Remeshing counters (the figures writeRemeshStats reports, particle.f90:196):
+ * out[8] = {events so far, edge collapses, vertex deletions, edge relaxations,
+ * largest vertex drift, vol_melt - vol_smooth, volume refreshes (all of the
+ * last event), collapses so far}. */
+int melt_get_remesh_stats(melt_ctx* ctx, double* out);
+
+/* ------------------------------------------------------------ remeshing */
+/* remesh_coarsen (remesh_coarsen.f90:2-258) on host arrays with the
+ * reference's layout: 1-based connectivity, logical arrays as int32 0/1,
+ * tri_nor(3,nf) and eLengths(ne) as particle.f90:127-133 leave them, rm_flag
+ * in/out.  vmelt (3*nv, may be null) is mls_param's vmelt(:,:,1), averaged
+ * onto the kept vertex of each collapse.  Counts of edge collapses and
+ * valence-3 vertex deletions go to *ecol_cnt, *vdel_cnt.  Runs on the host.
+ * MELT_ERR_REMESH where the reference aborts (topology not preserved,
+ * geometry reduced to a tetrahedron, QEM solve) or would overrun its 30-entry
+ * ring buffers. */
+int melt_remesh_coarsen(int32_t nf, int32_t ne, int32_t nv, double* xyz, double* tri_nor, double* e_lengths, double e_thresh,
+                        int32_t* vert_of_face, int32_t* edge_of_face, int32_t* vert_of_edge, int32_t* face_of_edge, int32_t* is_ghost_face,
+                        int32_t* is_ghost_edge, int32_t* is_ghost_vert, int32_t* rm_flag, int32_t* anchor_vert, int32_t* flagged_edge,
+                        double* vmelt, int32_t* ecol_cnt, int32_t* vdel_cnt);
+/* remesh_smooth (remesh_smooth.f90:3-347): volume-restoring edge relaxation
+ * of the flagged edges toward vol_old with the volume change target_dv; the
+ * relaxation count, largest vertex drift and volume refreshes come back as
+ * n_erel, drift, cnt_refresh (the reference's outputs).  Runs on the host. */
+int melt_remesh_smooth(double vol_old, double target_dv, int32_t nv, int32_t ne, int32_t nf, double* xyz, const int32_t* is_ghost_vert,
+                       const int32_t* is_ghost_edge, const int32_t* is_ghost_face, int32_t* flagged_edge, const int32_t* vert_of_edge,
+                       const int32_t* vert_of_face, const int32_t* face_of_edge, const int32_t* edge_of_face, int32_t* n_erel, double* drift,
+                       int32_t* cnt_refresh);
 
 /* --------------------------------------------------------- time control */
 int melt_set_dt(melt_ctx* ctx, double dt);       /* param dt                      */
@@ -200,7 +235,7 @@ int melt_mlsweight(melt_ctx* ctx);    /* mlsWeight.f90:1-45 (wght1/2/3/temp)    
 int melt_mlsforce(melt_ctx* ctx);     /* mlsForce.f90:1-36 (forc1/2/3/temp), zeroes for_* first */
 int melt_velforce(melt_ctx* ctx);     /* velforce.f90:1-59                          */
 int melt_tempforce(melt_ctx* ctx);    /* tempforce.f90:1-46                         */
-int melt_particle(melt_ctx* ctx);     /* particle.f90:1-229 (IREMESH=0)             */
+int melt_particle(melt_ctx* ctx);     /* particle.f90:1-229                         */
 
 /* Diagnostics the reference's time loop (gcurv.f90) calls around tschem. */
 int melt_cfl(melt_ctx* ctx, double* cflm);     /* cfl.f90:1-30 (max over cells)     */
@@ -275,7 +310,9 @@ int melt_timing_read(melt_ctx* ctx, double* ms, int64_t* samples);
  * Poisson residual run in FP64.  After each projection the FP64 divergence of
  * the projected velocity is compared with that of the provisional one; while
  * it exceeds rtol times it (at most max_iter times) a correction Poisson solve
- * and projection are applied.  Defaults: max_iter 1, rtol 1e-4.  Host arrays
+ * and projection are applied.  Off by default (max_iter 0: the measured ratio
+ * is about 1e-6, far below rtol); melt_set_refinement(ctx, 1, 1e-4) turns it
+ * on.  Host arrays
  * of upload/download stay double in both forms. */
 int melt_set_refinement(melt_ctx* ctx, int32_t max_iter, double rtol);
 

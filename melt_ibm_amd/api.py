@@ -44,7 +44,7 @@ _LAG_SHAPE = {  # (per-entity width, entity) entity 0 vertex, 1 face, 2 edge
     "eLengths": (1, 2)}
 
 STATUS = {0: "ok", 1: "bad argument", 2: "HIP error", 3: "singular MLS moment matrix", 4: "body volume below threshold",
-          5: "pencil buffer overflow", 6: "remeshing not on this path", 7: "ray recursion", 8: "no geometry",
+          5: "pencil buffer overflow", 6: "remeshing failed", 7: "ray recursion", 8: "no geometry",
           9: "window overflow"}
 
 
@@ -128,6 +128,8 @@ class MeltConfig:
         return (self.n3m, self.n2m, self.n1m)
 
 
+REMESH_KEYS = ("events", "collapses", "deletions", "relaxations", "drift", "residual", "refreshes", "total_collapses")
+
 _lib_cache: dict[str, C.CDLL] = {}
 
 
@@ -200,6 +202,8 @@ _SIGS = {
     "stream": (C.c_void_p, [_P]),
     "poll": (C.c_int, [_P, C.POINTER(C.c_int32)]),
     "upload_field_device": (C.c_int, [_P, C.c_int, C.c_void_p]),
+    "get_topology": (C.c_int, [_P] * 8),
+    "get_remesh_stats": (C.c_int, [_P, _P]),
 }
 OPTIONAL = {"timing_enable", "timing_reset", "timing_count", "timing_name", "timing_read", "device_ok", "set_refinement",
             "get_refinement_stats", "graph_enable", "stream",
@@ -328,6 +332,22 @@ class MeltContext:
         a = np.empty((self.nf, 6), dtype=np.int32)
         self._chk(self._f["get_pind"](self._ctx, a.ctypes.data))
         return a
+
+    def topology(self) -> dict:
+        """Connectivity (1-based) and ghost flags; remeshing edits them in place."""
+        nv, ne, nf = self.nv, self.ne, self.nf
+        out = {"vert_of_face": np.empty((nf, 3), np.int32), "edge_of_face": np.empty((nf, 3), np.int32),
+               "vert_of_edge": np.empty((ne, 2), np.int32), "face_of_edge": np.empty((ne, 2), np.int32),
+               "ghost_v": np.empty(nv, np.int32), "ghost_e": np.empty(ne, np.int32), "ghost_f": np.empty(nf, np.int32)}
+        self._chk(self._f["get_topology"](self._ctx, *(a.ctypes.data for a in out.values())))
+        return out
+
+    def remesh_stats(self) -> dict:
+        """Remeshing counters: events so far, and of the last event the edge collapses, vertex
+        deletions, edge relaxations, largest vertex drift, volume residual and refreshes."""
+        a = np.zeros(8)
+        self._chk(self._f["get_remesh_stats"](self._ctx, a.ctypes.data))
+        return dict(zip(REMESH_KEYS, a.tolist()))
 
     def vert_of_face(self) -> np.ndarray:
         a = np.empty((self.nf, 3), dtype=np.int32)

@@ -19,7 +19,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libmeltgpu.so"
 OBJDIR = PKG / "_build"
-SOURCES = ["api.hip", "euler.hip", "poisson.hip", "ibm.hip", "lagr.hip", "diag.hip", "hit.hip"]
+SOURCES = ["api.hip", "euler.hip", "poisson.hip", "ibm.hip", "lagr.hip", "diag.hip", "hit.hip", "remesh.hip"]
 ARCH = os.environ.get("MELT_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -140,11 +140,14 @@ extern "C" void melt_ctx_destroy(melt_ctx* ctx) {
   for (int i = 0; i < 4; ++i) if (ctx->tmp4[i]) hipFree(ctx->tmp4[i]);
   void* ps[] = {ctx->spec, ctx->ak1, ctx->ak2, ctx->ak3, ctx->tw1, ctx->tw2, ctx->tw3, ctx->d_body, ctx->d_vof, ctx->d_eof, ctx->d_voe,
                 ctx->d_v2f_ptr, ctx->d_v2f, ctx->d_gv, ctx->d_ge, ctx->d_gf, ctx->d_pind, ctx->d_pind2, ctx->d_probe, ctx->d_rxp, ctx->d_rxt, ctx->d_red,
-                ctx->d_vbox, ctx->d_slots, ctx->d_coord, ctx->d_acc, ctx->d_fmean, ctx->d_tkey, ctx->d_tface, ctx->d_tdist, ctx->d_pcoef, ctx->d_win, ctx->d_pcount, ctx->d_plist, ctx->d_err, ctx->d_gate, ctx->d_refine_stats};
+                ctx->d_vbox, ctx->d_slots, ctx->d_coord, ctx->d_acc, ctx->d_fmean, ctx->d_tkey, ctx->d_tface, ctx->d_tdist, ctx->d_pcoef, ctx->d_win, ctx->d_pcount, ctx->d_plist, ctx->d_err, ctx->d_gate, ctx->d_refine_stats,
+                ctx->d_foe, ctx->d_xyz_melt};
   for (void* p : ps) if (p) hipFree(p);
   for (int i = 0; i < MELT_L_NLAG; ++i) if (ctx->d_lag[i]) hipFree(ctx->d_lag[i]);
   hit_free(ctx);
   if (ctx->h_err) hipHostFree(ctx->h_err);
+  if (ctx->h_rm) hipHostFree(ctx->h_rm);
+  if (ctx->ev_rm) hipEventDestroy(ctx->ev_rm);
   for (auto e : ctx->ev_pool) hipEventDestroy(e);
   if (ctx->side) hipStreamSynchronize(ctx->side);
   for (hipEvent_t e : {ctx->ev_fork, ctx->ev_join}) if (e) hipEventDestroy(e);
@@ -163,6 +166,7 @@ extern "C" int melt_ctx_create(const melt_config* cfg, melt_ctx** out) {
   ctx->cfg = *cfg;
   ctx->fp32 = cfg->precision == 1;
   ctx->esz = ctx->fp32 ? sizeof(float) : sizeof(double);
+  ctx->spec64 = !ctx->fp32;
   ctx->n1 = cfg->n1m; ctx->n2 = cfg->n2m; ctx->n3 = cfg->n3m;
   ctx->nc = (size_t)ctx->n1 * ctx->n2 * ctx->n3;
   ctx->n1h = ctx->n1 / 2 + 1;
@@ -652,6 +656,18 @@ static int side_run(melt_ctx* ctx, F&& work) {
   MELT_HIP(hipEventRecord(ctx->ev_join, ctx->side));
   return 0;
 }
+// More work on the side stream after what side_run queued there (no new fork).
+template <class F>
+static int side_continue(melt_ctx* ctx, F&& work) {
+  if (one_stream(ctx)) return work();
+  hipStream_t main_stream = ctx->stream;
+  ctx->stream = ctx->side;
+  int rc = work();
+  ctx->stream = main_stream;
+  if (rc) return rc;
+  MELT_HIP(hipEventRecord(ctx->ev_join, ctx->side));
+  return 0;
+}
 static int side_join(melt_ctx* ctx) {
   if (one_stream(ctx)) return 0;
   MELT_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
@@ -726,6 +742,25 @@ static int substep_impl(melt_ctx* ctx, int32_t ns, bool tag_next, bool coef_next
   timing_end(ctx, SLOT_PH_EULER, e_ph);
 
   timing_begin(ctx, SLOT_PH_MLS, &e_ph);
+  const bool remesh = ibm && c.iremesh == 1;
+  // what the side stream queues after the Lagrangian update: the next
+  // substep's tagging (tag_next) and probe geometry stage (coef_next)
+  auto queue_next = [&]() -> int {
+    int r = 0;
+    if (tag_next && moving) {
+      timing_begin(ctx, SLOT_TAG, &e);
+      ctx->force_window_from_tag = 1;
+      r = ibm_tag(ctx);
+      ctx->force_window_from_tag = 0;
+      timing_end(ctx, SLOT_TAG, e);
+      if (!r) ctx->tag_ahead = 1;
+    }
+    if (!r && coef_next && probes) {
+      r = ibm_probe_coefs(ctx);
+      if (!r) ctx->coef_ahead = 1;
+    }
+    return r;
+  };
   if (ibm) {
     timing_begin(ctx, SLOT_IBM, &e);
     if ((rc = ibm_find_centroid_indices(ctx)) || (rc = ibm_weights(ctx))) return rc;
@@ -742,23 +777,13 @@ static int substep_impl(melt_ctx* ctx, int32_t ns, bool tag_next, bool coef_next
     // results and the body arrays, which the pressure phase does not touch:
     // it runs on the side stream, concurrently with the pressure solve, and
     // rejoins before the substep ends (the next tagging needs the geometry).
+    // With remeshing the update stops after measuring the edges; the host
+    // reads that flag only after the pressure solve is queued (below).
     if ((rc = side_run(ctx, [&] {
           timing_begin(ctx, SLOT_LAGR, &e);
-          int r = lagr_after_force(ctx, true);
+          int r = lagr_after_force(ctx, true, remesh);
           timing_end(ctx, SLOT_LAGR, e);
-          if (!r && tag_next && moving) {
-            timing_begin(ctx, SLOT_TAG, &e);
-            ctx->force_window_from_tag = 1;
-            r = ibm_tag(ctx);
-            ctx->force_window_from_tag = 0;
-            timing_end(ctx, SLOT_TAG, e);
-            if (!r) ctx->tag_ahead = 1;
-          }
-          if (!r && coef_next && probes) {
-            r = ibm_probe_coefs(ctx);
-            if (!r) ctx->coef_ahead = 1;
-          }
-          return r;
+          return r || remesh ? r : queue_next();
         })))
       return rc;
   }
@@ -788,6 +813,10 @@ static int substep_impl(melt_ctx* ctx, int32_t ns, bool tag_next, bool coef_next
     timing_end(ctx, SLOT_REFINE, e);
   }
   timing_end(ctx, SLOT_PH_PRESSURE, e_ph);
+  // remeshing: wait for the edge flag (the pressure solve runs meanwhile),
+  // remesh on the host if needed, then refresh the geometry and queue the
+  // next substep's work on the side stream
+  if (remesh && (rc = side_continue(ctx, [&] { int r = lagr_finish(ctx); return r ? r : queue_next(); }))) return rc;
   if (ibm && (rc = side_join(ctx))) return rc;
   timing_end(ctx, SLOT_SUBSTEP, e_sub);
   if (ctx->timing && ctx->ev_pending.size() > 4096) timing_flush(ctx);
@@ -824,7 +853,8 @@ extern "C" int melt_tschem(melt_ctx* ctx) {
   if (check_ctx(ctx)) return MELT_ERR_ARG;
   if (!(ctx->dt > 0.0)) return melt_fail(ctx, MELT_ERR_ARG, "tschem", "dt not set");
   // the first tagging (ray casting, host round trip) happens outside graphs
-  bool graph = ctx->use_graph && !ctx->timing && !(ctx->has_geom && ctx->h_body.initial_tag);
+  // the first tagging (ray casting) and remeshing (a host decision every substep) are not captured
+  bool graph = ctx->use_graph && !ctx->timing && !(ctx->has_geom && ctx->h_body.initial_tag) && ctx->cfg.iremesh != 1;
   if (!graph) return tschem_launch(ctx);
   if (!ctx->graph_exec) {
     if (int rc = ensure_side(ctx)) return rc;   // not created inside the capture
@@ -927,13 +957,14 @@ extern "C" int melt_get_refinement_stats(melt_ctx* ctx, int64_t* checks, int64_t
 // bhat file, hit.f90:267-356) as one versioned host blob of the context's
 // state at a step boundary: header, body scalars, immersed-boundary windows,
 // the fields that carry state between steps (stored at the context's
-// precision), every Lagrangian array, the probe indices and the ghost flags,
-// and the HIT forcing state with its generator position.  A context created
+// precision), every Lagrangian array, the probe indices, the ghost flags and
+// the connectivity (which remeshing edits), and the HIT forcing state with its
+// generator position.  A context created
 // with the same configuration and mesh resumes from it bit for bit.
 namespace {
 
 constexpr char SNAP_MAGIC[8] = {'M', 'E', 'L', 'T', 'S', 'N', 'P', '1'};
-constexpr uint32_t SNAP_VERSION = 1;
+constexpr uint32_t SNAP_VERSION = 2;
 const int SNAP_FIELDS[] = {MELT_VX, MELT_VY, MELT_VZ, MELT_TEMP, MELT_PR, MELT_RU1, MELT_RU2, MELT_RU3, MELT_RUT,
                            MELT_FORCX, MELT_FORCY, MELT_FORCZ, MELT_VOFP, MELT_VOFX, MELT_VOFY, MELT_VOFZ, MELT_SDF};
 
@@ -948,12 +979,6 @@ struct SnapHeader {
   uint64_t hit_seed, hit_steps;
 };
 
-uint64_t fnv1a(const void* p, size_t n, uint64_t h = 1469598103934665603ull) {
-  const unsigned char* b = static_cast<const unsigned char*>(p);
-  for (size_t i = 0; i < n; ++i) { h ^= b[i]; h *= 1099511628211ull; }
-  return h;
-}
-
 // One entry per stored device array: pointer and byte count.
 std::vector<std::pair<void*, size_t>> snap_arrays(melt_ctx* ctx) {
   std::vector<std::pair<void*, size_t>> a;
@@ -966,17 +991,21 @@ std::vector<std::pair<void*, size_t>> snap_arrays(melt_ctx* ctx) {
     a.push_back({ctx->d_gv, (size_t)ctx->nv});
     a.push_back({ctx->d_ge, (size_t)ctx->ne});
     a.push_back({ctx->d_gf, (size_t)ctx->nf});
+    // the connectivity as remeshing left it
+    a.push_back({ctx->d_vof, sizeof(int) * 3 * (size_t)ctx->nf});
+    a.push_back({ctx->d_eof, sizeof(int) * 3 * (size_t)ctx->nf});
+    a.push_back({ctx->d_voe, sizeof(int) * 2 * (size_t)ctx->ne});
+    a.push_back({ctx->d_foe, sizeof(int) * 2 * (size_t)ctx->ne});
+    a.push_back({ctx->d_v2f_ptr, sizeof(int) * ((size_t)ctx->nv + 1)});
+    a.push_back({ctx->d_v2f, sizeof(int) * 3 * (size_t)ctx->nf});
   }
   if (ctx->hit_nm) a.push_back({ctx->d_hit_bhat, sizeof(double) * 2 * 3 * (size_t)ctx->hit_nm * ctx->hit_nm * ctx->hit_nm});
   return a;
 }
 
+// the mesh the context imported (remeshing later edits it; the blob carries that)
 int snap_mesh_hash(melt_ctx* ctx, uint64_t* h) {
-  *h = 0;
-  if (!ctx->has_geom) return 0;
-  std::vector<int> vof(3 * (size_t)ctx->nf);
-  MELT_HIP(hipMemcpy(vof.data(), ctx->d_vof, sizeof(int) * vof.size(), hipMemcpyDeviceToHost));
-  *h = fnv1a(vof.data(), sizeof(int) * vof.size());
+  *h = ctx->has_geom ? ctx->mesh_hash0 : 0;
   return 0;
 }
 
@@ -1058,5 +1087,5 @@ extern "C" int melt_snapshot_load(melt_ctx* ctx, const void* buf, uint64_t bytes
   }
   ctx->dt = h.dt;
   MELT_HIP(hipMemcpy(&ctx->h_body, ctx->d_body, sizeof(melt_body), hipMemcpyDeviceToHost));
-  return 0;
+  return remesh_pull_mirrors(ctx);
 }

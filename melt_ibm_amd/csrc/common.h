@@ -80,16 +80,20 @@ struct melt_ctx {
   // Eulerian fields: double, or float in the precision form (fp32 = 1); fld<R>()
   int fp32 = 0;
   size_t esz = sizeof(double);
+  // Poisson spectrum, twiddles and FFT arithmetic in FP64 (the FP64 form); the
+  // precision form keeps them in FP32 (an FP64 spectrum there was measured to
+  // leave its pressure difference to FP64 unchanged, at 0.28 ms per step)
+  int spec64 = 1;
   void* f[MELT_NFIELDS] = {};
   void* rhs4[4] = {};
   void* tmp4[4] = {};      // output of the y-direction Green filters (the z pass reads it)
   int no_green = 0;        // MELT_TUNE green=0: cyclic reduction for every line solve
-  void* spec = nullptr;                                    // complex, element type as the fields
+  void* spec = nullptr;                                    // complex, double if spec64 else float
   double *ak1 = nullptr, *ak2 = nullptr, *ak3 = nullptr;   // 0-based wavenumber index
   void *tw1 = nullptr, *tw2 = nullptr, *tw3 = nullptr;     // exp(-2 pi i t / N), complex of the field type
   // precision-form pressure refinement: up to refine_max correction solves while
   // the FP64 divergence after projection exceeds refine_rtol times the one before
-  int refine_max = 1;
+  int refine_max = 0;                    // opt-in (melt_set_refinement)
   double refine_rtol = 1e-4;
   int* d_gate = nullptr;                 // device flag read by the gated refinement kernels
   long long* d_refine_stats = nullptr;   // [checks, refinements, bits of the last ratio]
@@ -103,7 +107,19 @@ struct melt_ctx {
   int *d_vof = nullptr, *d_eof = nullptr, *d_voe = nullptr;  // 0-based connectivity
   int *d_v2f_ptr = nullptr, *d_v2f = nullptr;                 // vertex -> faces (ascending)
   unsigned char *d_gv = nullptr, *d_ge = nullptr, *d_gf = nullptr;
+  int* d_foe = nullptr;          // faces of each edge (0-based), kept for snapshots of a remeshed body
   double* d_lag[MELT_L_NLAG] = {};
+  // remeshing (IREMESH = 1, particle.f90:121-160): host mirror of the 0-based
+  // connectivity and ghost flags (authoritative; the device copies follow each
+  // edit), anchorVert / flagged_edge, the positions after the Stefan update
+  // (the volume the smoothing restores), the device's short-edge flag read
+  // through pinned memory, and the counters of melt_get_remesh_stats
+  std::vector<int32_t> h_vof, h_eof, h_voe, h_foe, h_gv, h_ge, h_gf, h_anchor, h_flagged;
+  double* d_xyz_melt = nullptr;
+  int32_t* h_rm = nullptr;
+  hipEvent_t ev_rm = nullptr;
+  double remesh_stats[8] = {};
+  uint64_t mesh_hash0 = 0;       // FNV-1a of the imported vert_of_face (snapshot compatibility)
   int* d_pind = nullptr;
   int* d_pind2 = nullptr;        // probe indices of the fused substep, copied into d_pind after the spreading
   double* d_probe = nullptr;     // PRB_N * nf
@@ -138,6 +154,7 @@ struct melt_ctx {
   double hit_tl = 0, hit_eps = 0, hit_kf = 0;
   unsigned long long hit_seed = 0;
   double *d_hit_bhat = nullptr, *d_hit_fhat = nullptr, *d_hit_tab = nullptr;
+  double* d_hit_ax = nullptr;   // the x sums of the synthesis, 3 nm^2 n1 complex
   int* d_hit_act = nullptr;
   unsigned long long* d_hit_count = nullptr;
 
@@ -484,8 +501,17 @@ int ibm_probes(melt_ctx* ctx, int* pind_out = nullptr, bool coefs_ready = false)
 // geometry part of the gradient probes into d_pcoef (then ibm_probes(..., true))
 int ibm_probe_coefs(melt_ctx* ctx);
 int ibm_window_reset(melt_ctx* ctx);
-// with commit_pind (the fused substep) it also commits the staged heat-flux probe indices
-int lagr_after_force(melt_ctx* ctx, bool commit_pind = false);
+// with commit_pind (the fused substep) it also commits the staged heat-flux probe indices.
+// With IREMESH = 1 the update stops after measuring the edges; lagr_finish
+// then remeshes on the host if an edge is short and refreshes the geometry
+// (called at once unless defer_finish, when the fused substep calls it after
+// queueing the pressure solve, so the host's wait overlaps it).
+int lagr_after_force(melt_ctx* ctx, bool commit_pind = false, bool defer_finish = false);
+int lagr_finish(melt_ctx* ctx);
+// host remeshing of the context's body (remesh.hip): *did = 1 when it ran
+int remesh_context(melt_ctx* ctx, int* did);
+// host mirrors from the device connectivity (after a snapshot load)
+int remesh_pull_mirrors(melt_ctx* ctx);
 int lagr_alloc(melt_ctx* ctx, const double* xyz0, const int* voe, const int* eof, const int* vof, double rad_p);
 void timing_begin(melt_ctx* ctx, int slot, hipEvent_t* ev);
 void timing_end(melt_ctx* ctx, int slot, hipEvent_t ev);

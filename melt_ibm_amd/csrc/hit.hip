@@ -83,41 +83,61 @@ __global__ __launch_bounds__(1024) void k_hit_ou(HitArgs a, const int* __restric
   if (threadIdx.x == 0) *count = step + 1ull;
 }
 
-// Synthesis: tab holds exp(i k x) for the 6 staggered coordinate sets, each
-// (nm, N) complex: [0] xc, [1] xm, [2] yc, [3] ym, [4] zc, [5] zm.  Component
-// c sits at (xc, ym, zm), (xm, yc, zm), (xm, ym, zc).
+// Synthesis F_c(x, y, z) = Re sum_{l,m,n} Fhat(c,l,m,n) e^{i k_l x} e^{i k_m y}
+// e^{i k_n z} at each component's staggered points, in three separable
+// sums.  tab holds exp(i k x) for the 6 staggered coordinate sets, each
+// (nm, N) complex: [0] xc, [1] xm, [2] yc, [3] ym, [4] zc, [5] zm; component c
+// sits at (xc, ym, zm), (xm, yc, zm), (xm, ym, zc).
+//
+// The x sum depends only on i: k_hit_xsum forms ax(c, n, m, i) =
+// sum_l Fhat(c,l,m,n) e^{i k_l x_i} once per grid line instead of once per
+// (i, j) column.  k_hit_synth then takes, per (i, j) column and 32-plane
+// segment, the y sum of those (75 complex products for nm = 5) and per plane
+// the z sum (15 real parts): about 40 FP64 FMAs per cell, so the pass is
+// bound by its field stores.
+template <int NM>
+__global__ __launch_bounds__(256) void k_hit_xsum(int n1, const double* __restrict__ fhat, const double* __restrict__ tab,
+                                                  double* __restrict__ ax) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= 3 * NM * NM * n1) return;
+  const int i = t % n1, r = t / n1, m = r % NM, n = (r / NM) % NM, c = r / (NM * NM);
+  const double* tx = c == 0 ? tab : tab + 2 * NM * n1;   // xc for u, xm for v and w
+  double gr = 0.0, gi = 0.0;
+#pragma unroll
+  for (int l = 0; l < NM; ++l) {
+    const double* F = &fhat[2 * (c + 3 * (l + NM * (m + NM * n)))];
+    const double* E = &tx[2 * (l * n1 + i)];
+    gr = fma(F[0], E[0], fma(-F[1], E[1], gr));
+    gi = fma(F[0], E[1], fma(F[1], E[0], gi));
+  }
+  ax[2 * t] = gr;
+  ax[2 * t + 1] = gi;
+}
+
 template <typename R, int NM>
-__global__ __launch_bounds__(256) void k_hit_synth(Grid g, const double* __restrict__ fhat, const double* __restrict__ tab,
+__global__ __launch_bounds__(256) void k_hit_synth(Grid g, const double* __restrict__ ax, const double* __restrict__ tab,
                                                    R* __restrict__ fx, R* __restrict__ fy, R* __restrict__ fz) {
-  constexpr int nm = NM;
   const int cols = g.n1 * g.n2, nseg = (g.n3 + HIT_ZSEG - 1) / HIT_ZSEG;
   const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (t >= cols * nseg) return;
   const int col = t % cols, seg = t / cols;
   const int i = col % g.n1, j = col / g.n1;
   const int n1 = g.n1, n2 = g.n2, n3 = g.n3;
-  const double* tx[3] = {tab, tab + 2 * nm * n1, tab + 2 * nm * n1};                       // xc, xm, xm
-  const double* ty[3] = {tab + 2 * nm * (2 * n1) + 2 * nm * n2, tab + 2 * nm * (2 * n1), tab + 2 * nm * (2 * n1) + 2 * nm * n2};  // ym, yc, ym
-  const double* zb = tab + 2 * nm * (2 * n1 + 2 * n2);
-  const double* tz[3] = {zb + 2 * nm * n3, zb + 2 * nm * n3, zb};                            // zm, zm, zc
+  const double* ty[3] = {tab + 2 * NM * (2 * n1) + 2 * NM * n2, tab + 2 * NM * (2 * n1), tab + 2 * NM * (2 * n1) + 2 * NM * n2};  // ym, yc, ym
+  const double* zb = tab + 2 * NM * (2 * n1 + 2 * n2);
+  const double* tz[3] = {zb + 2 * NM * n3, zb + 2 * NM * n3, zb};                            // zm, zm, zc
   double hr[3][NM], hi[3][NM];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
 #pragma unroll
-    for (int n = 0; n < nm; ++n) {
+    for (int n = 0; n < NM; ++n) {
       double sr = 0.0, si = 0.0;
-      for (int m = 0; m < nm; ++m) {
-        double gr = 0.0, gi = 0.0;
 #pragma unroll
-        for (int l = 0; l < nm; ++l) {
-          const double* F = &fhat[2 * (c + 3 * (l + nm * (m + nm * n)))];
-          const double* E = &tx[c][2 * (l * n1 + i)];
-          gr += F[0] * E[0] - F[1] * E[1];
-          gi += F[0] * E[1] + F[1] * E[0];
-        }
+      for (int m = 0; m < NM; ++m) {
+        const double* A = &ax[2 * (((c * NM + n) * NM + m) * n1 + i)];
         const double* E = &ty[c][2 * (m * n2 + j)];
-        sr += gr * E[0] - gi * E[1];
-        si += gr * E[1] + gi * E[0];
+        sr = fma(A[0], E[0], fma(-A[1], E[1], sr));
+        si = fma(A[0], E[1], fma(A[1], E[0], si));
       }
       hr[c][n] = sr;
       hi[c][n] = si;
@@ -131,9 +151,9 @@ __global__ __launch_bounds__(256) void k_hit_synth(Grid g, const double* __restr
     for (int c = 0; c < 3; ++c) {
       double v = 0.0;
 #pragma unroll
-      for (int n = 0; n < nm; ++n) {
+      for (int n = 0; n < NM; ++n) {
         const double* E = &tz[c][2 * (n * n3 + k)];
-        v += hr[c][n] * E[0] - hi[c][n] * E[1];
+        v = fma(hr[c][n], E[0], fma(-hi[c][n], E[1], v));
       }
       out[c][o] = (R)v;
     }
@@ -160,9 +180,10 @@ inline unsigned nblk(size_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 }  // namespace
 
 int hit_free(melt_ctx* ctx) {
-  for (void* p : {(void*)ctx->d_hit_bhat, (void*)ctx->d_hit_fhat, (void*)ctx->d_hit_tab, (void*)ctx->d_hit_act, (void*)ctx->d_hit_count})
+  for (void* p : {(void*)ctx->d_hit_bhat, (void*)ctx->d_hit_fhat, (void*)ctx->d_hit_tab, (void*)ctx->d_hit_act, (void*)ctx->d_hit_count,
+                  (void*)ctx->d_hit_ax})
     if (p) hipFree(p);
-  ctx->d_hit_bhat = ctx->d_hit_fhat = ctx->d_hit_tab = nullptr;
+  ctx->d_hit_bhat = ctx->d_hit_fhat = ctx->d_hit_tab = ctx->d_hit_ax = nullptr;
   ctx->d_hit_act = nullptr;
   ctx->d_hit_count = nullptr;
   ctx->hit_nm = 0;
@@ -209,6 +230,7 @@ extern "C" int melt_hit_init(melt_ctx* ctx, double tl, double epsstar, double kf
   MELT_HIP(hipMalloc(&ctx->d_hit_bhat, nb));
   MELT_HIP(hipMalloc(&ctx->d_hit_fhat, nb));
   MELT_HIP(hipMalloc(&ctx->d_hit_tab, sizeof(double) * tab.size()));
+  MELT_HIP(hipMalloc(&ctx->d_hit_ax, sizeof(double) * 2 * 3 * (size_t)nm * nm * ctx->n1));
   MELT_HIP(hipMalloc(&ctx->d_hit_act, sizeof(int) * (act.empty() ? 1 : act.size())));
   MELT_HIP(hipMalloc(&ctx->d_hit_count, sizeof(unsigned long long)));
   MELT_HIP(hipMemset(ctx->d_hit_bhat, 0, nb));
@@ -229,7 +251,9 @@ template <typename R, int NM>
 static int launch_hit_synth_nm(melt_ctx* ctx) {
   const int nseg = (ctx->n3 + HIT_ZSEG - 1) / HIT_ZSEG;
   const size_t nt = (size_t)ctx->n1 * ctx->n2 * nseg;
-  hipLaunchKernelGGL((k_hit_synth<R, NM>), dim3(nblk(nt, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), ctx->d_hit_fhat,
+  hipLaunchKernelGGL((k_hit_xsum<NM>), dim3(nblk((size_t)3 * NM * NM * ctx->n1, 256)), dim3(256), 0, ctx->stream, ctx->n1, ctx->d_hit_fhat,
+                     ctx->d_hit_tab, ctx->d_hit_ax);
+  hipLaunchKernelGGL((k_hit_synth<R, NM>), dim3(nblk(nt, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), ctx->d_hit_ax,
                      ctx->d_hit_tab, fld<R>(ctx, MELT_FORCX), fld<R>(ctx, MELT_FORCY), fld<R>(ctx, MELT_FORCZ));
   MELT_HIP(hipGetLastError());
   return 0;

@@ -568,9 +568,41 @@ int refresh_geometry(melt_ctx* ctx, LagArgs& A) {
 
 }  // namespace
 
+// update_tri_velocity (RigidAuxRoutines.f90:902-928): after remeshing, the
+// face velocities from the refreshed centroids and centre of mass
+__global__ __launch_bounds__(256) void k_tri_velocity(LagArgs A, const melt_body* __restrict__ b) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= A.nf || A.gf[f]) return;
+  double rad[3], om[3];
+  for (int d = 0; d < 3; ++d) rad[d] = A.L[MELT_L_TRI_BAR][3 * f + d] - b->pos_cm[d];
+  om[0] = b->omega_c[1] * rad[2] - rad[1] * b->omega_c[2];
+  om[1] = b->omega_c[2] * rad[0] - rad[2] * b->omega_c[0];
+  om[2] = b->omega_c[0] * rad[1] - rad[0] * b->omega_c[1];
+  for (int d = 0; d < 3; ++d) A.L[MELT_L_VEL_TRI][3 * f + d] = b->vel_cm[d] + om[d];
+}
+
+// the rest of particle.f90 after the edges were measured (IREMESH = 1):
+// remesh on the host when an edge is short, then the geometry refresh
+int lagr_finish(melt_ctx* ctx) {
+  if (ctx->cfg.iremesh != 1) return 0;
+  LagArgs A = lag_args(ctx);
+  MELT_HIP(hipEventSynchronize(ctx->ev_rm));
+  int did = 0;
+  if (*ctx->h_rm) {
+    if (int rc = remesh_context(ctx, &did)) return rc;
+    A = lag_args(ctx);
+  }
+  if (int rc = refresh_geometry(ctx, A)) return rc;
+  if (did) {
+    hipLaunchKernelGGL(k_tri_velocity, dim3(nb(ctx->nf, 256)), dim3(256), 0, ctx->stream, A, ctx->d_body);
+    MELT_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
 // particle.f90:62-193 after the MLS interpolation: heat flux -> vertices,
-// loads, Stefan update, Newton-Euler motion, geometry refresh.
-int lagr_after_force(melt_ctx* ctx, bool commit_pind) {
+// loads, Stefan update, Newton-Euler motion, (remeshing,) geometry refresh.
+int lagr_after_force(melt_ctx* ctx, bool commit_pind, bool defer_finish) {
   LagArgs A = lag_args(ctx);
   hipStream_t s = ctx->stream;
   int heat = ctx->cfg.imelt == 1, loads = ctx->cfg.imlsstr == 1;
@@ -584,6 +616,9 @@ int lagr_after_force(melt_ctx* ctx, bool commit_pind) {
     hipLaunchKernelGGL(k_vert_heat_stefan, dim3(nb(ctx->nv, 256)), dim3(256), 0, s, A, ctx->d_body, ctx->cfg.cpliquid / ctx->cfg.latheat, ctx->dt,
                        ctx->ga, ctx->ro, 1);
     face_sums(ctx, A, ctx->d_lag[MELT_L_XYZV], (int)SUM_VOLUME, ctx->d_red, FIN_VOLUME);
+    // the volume after melting is the one the remeshing restores (particle.f90:86-90)
+    if (ctx->cfg.iremesh == 1)
+      MELT_HIP(hipMemcpyAsync(ctx->d_xyz_melt, ctx->d_lag[MELT_L_XYZV], sizeof(double) * 3 * ctx->nv, hipMemcpyDeviceToDevice, s));
   }
   if (loads) {
     const double* x = ctx->d_lag[MELT_L_XYZV];
@@ -596,8 +631,14 @@ int lagr_after_force(melt_ctx* ctx, bool commit_pind) {
     hipLaunchKernelGGL(k_face_vel, dim3(nb(ctx->nf, 256)), dim3(256), 0, s, A, ctx->d_body);
   }
   MELT_HIP(hipGetLastError());
-  if (ctx->cfg.iremesh == 1) return melt_fail(ctx, MELT_ERR_REMESH, "particle", "remeshing (IREMESH=1) is not on this path");
-  return refresh_geometry(ctx, A);
+  if (ctx->cfg.iremesh != 1) return refresh_geometry(ctx, A);
+  // calculate_eLengths (particle.f90:128): the short-edge flag goes to the host
+  hipLaunchKernelGGL(k_clear_rm, dim3(1), dim3(1), 0, s, ctx->d_body);
+  hipLaunchKernelGGL(k_edge_len, dim3(nb(ctx->ne, 256)), dim3(256), 0, s, A, ctx->d_lag[MELT_L_XYZV], ctx->d_body, ctx->h_body.e_thresh);
+  MELT_HIP(hipGetLastError());
+  MELT_HIP(hipMemcpyAsync(ctx->h_rm, &ctx->d_body->rm_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  MELT_HIP(hipEventRecord(ctx->ev_rm, s));
+  return defer_finish ? 0 : lagr_finish(ctx);
 }
 
 // tri_geo.f90:15-30 + setup_particles + init_geomCoords + init_mlsForce on the
@@ -716,7 +757,28 @@ int lagr_alloc(melt_ctx* ctx, const double* xyz0, const int* voe, const int* eof
   rc |= dalloc((void**)&ctx->d_tface, sizeof(int) * ctx->twin_cells);
   rc |= dalloc((void**)&ctx->d_tdist, sizeof(double) * 125 * nf);
   rc |= dalloc((void**)&ctx->d_pcoef, sizeof(double) * 27 * 3 * 6 * nf);
+  rc |= dalloc((void**)&ctx->d_foe, sizeof(int) * 2 * ne);
+  if (ctx->cfg.iremesh == 1) rc |= dalloc((void**)&ctx->d_xyz_melt, sizeof(double) * 3 * nv);
   if (rc) return MELT_ERR_HIP;
+  if (ctx->cfg.iremesh == 1) {
+    if (!ctx->h_rm) MELT_HIP(hipHostMalloc((void**)&ctx->h_rm, sizeof(int32_t)));
+    if (!ctx->ev_rm) MELT_HIP(hipEventCreateWithFlags(&ctx->ev_rm, hipEventDisableTiming));
+  }
+  // host mirror of the connectivity (remeshing edits it there first)
+  ctx->h_vof.assign(vof.begin(), vof.end());
+  ctx->h_eof.assign(e0.begin(), e0.end());
+  ctx->h_voe.assign(v0.begin(), v0.end());
+  ctx->h_foe.resize(2 * (size_t)ne);
+  for (size_t i = 0; i < ctx->h_foe.size(); ++i) ctx->h_foe[i] = foe[i] - 1;
+  ctx->h_gv.assign(nv, 0); ctx->h_ge.assign(ne, 0); ctx->h_gf.assign(nf, 0);
+  ctx->h_anchor.assign(nv, 1); ctx->h_flagged.assign(ne, 0);
+  for (double& v : ctx->remesh_stats) v = 0.0;
+  {
+    uint64_t h = 1469598103934665603ull;
+    const unsigned char* b = reinterpret_cast<const unsigned char*>(vof.data());
+    for (size_t i = 0; i < sizeof(int) * vof.size(); ++i) { h ^= b[i]; h *= 1099511628211ull; }
+    ctx->mesh_hash0 = h;
+  }
   MELT_HIP(hipDeviceSynchronize());  // the zero fills above are not ordered with the context's stream
   if (int rc2 = ibm_window_reset(ctx)) return rc2;
   MELT_HIP(hipMemcpy(ctx->d_vof, vof.data(), sizeof(int) * 3 * nf, hipMemcpyHostToDevice));
@@ -724,6 +786,7 @@ int lagr_alloc(melt_ctx* ctx, const double* xyz0, const int* voe, const int* eof
   MELT_HIP(hipMemcpy(ctx->d_voe, v0.data(), sizeof(int) * 2 * ne, hipMemcpyHostToDevice));
   MELT_HIP(hipMemcpy(ctx->d_v2f_ptr, ptr.data(), sizeof(int) * (nv + 1), hipMemcpyHostToDevice));
   MELT_HIP(hipMemcpy(ctx->d_v2f, lst.data(), sizeof(int) * 3 * nf, hipMemcpyHostToDevice));
+  MELT_HIP(hipMemcpy(ctx->d_foe, ctx->h_foe.data(), sizeof(int) * 2 * ne, hipMemcpyHostToDevice));
   double* d_xyz0 = nullptr;
   MELT_HIP(hipMalloc(&d_xyz0, sizeof(double) * 3 * nv));
   MELT_HIP(hipMemcpy(d_xyz0, xs.data(), sizeof(double) * 3 * nv, hipMemcpyHostToDevice));

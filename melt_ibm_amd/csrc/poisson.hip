@@ -659,7 +659,7 @@ int poisson_init(melt_ctx* ctx) {
   for (int d = 0; d < 3; ++d) {
     FftPlan P;
     if (!make_plan(n[d], P)) return melt_fail(ctx, MELT_ERR_ARG, "poisson_init", "grid size must be 2^a 3^b");
-    int rc = ctx->fp32 ? upload_twiddles<float>(ctx, n[d], tw[d]) : upload_twiddles<double>(ctx, n[d], tw[d]);
+    int rc = ctx->spec64 ? upload_twiddles<double>(ctx, n[d], tw[d]) : upload_twiddles<float>(ctx, n[d], tw[d]);
     if (rc) return rc;
     // phini.f90:35-55, 0-based index m -> reference index m+1
     std::vector<double> a(n[d]);
@@ -672,7 +672,7 @@ int poisson_init(melt_ctx* ctx) {
     MELT_HIP(hipMalloc(ak[d], sizeof(double) * n[d]));
     MELT_HIP(hipMemcpy(*ak[d], a.data(), sizeof(double) * n[d], hipMemcpyHostToDevice));
   }
-  size_t sz = 2 * ctx->esz * (size_t)ctx->n1hp * ctx->n2 * ctx->n3;
+  size_t sz = 2 * (ctx->spec64 ? sizeof(double) : sizeof(float)) * (size_t)ctx->n1hp * ctx->n2 * ctx->n3;
   MELT_HIP(hipMalloc(&ctx->spec, sz));
   MELT_HIP(hipMemset(ctx->spec, 0, sz));
   return 0;
@@ -681,8 +681,9 @@ int poisson_init(melt_ctx* ctx) {
 // fused_divg = 1: dph := divergence / (al dt) is formed on the fly from the
 // velocities (divg.f90) and never stored; 0: transforms MELT_DPH as phcalc.
 // gate != null: every kernel returns at once unless *gate != 0 (the
-// refinement solve of the precision form).
-template <typename T>
+// refinement solve of the precision form).  Rf is the field type, T the
+// spectrum's (and the transforms' arithmetic).
+template <typename Rf, typename T>
 int solve_t(melt_ctx* ctx, int fused_divg, const int* gate, unsigned long long* div_slots) {
   SpecArgs<T> S;
   S.g = make_grid(ctx);
@@ -701,18 +702,18 @@ int solve_t(melt_ctx* ctx, int fused_divg, const int* gate, unsigned long long* 
   // x lines of 64 or 256 points: register FFTs (RF_XT threads, RF_XT / P line pairs per block)
   const int px = reg_fft_p(ctx->n1);
   const unsigned grx = px ? (unsigned)((ctx->n2 / (2 * RF_XT / px)) * ctx->n3) : 0;
-  const T* src = fused_divg ? nullptr : fld<T>(ctx, MELT_DPH);
+  const Rf* src = fused_divg ? nullptr : fld<Rf>(ctx, MELT_DPH);
   unsigned long long* dsl = fused_divg ? div_slots : nullptr;
   if (!gate) timing_begin(ctx, SLOT_DIVG_FFTX, &ev);
   if (px == 16)
-    hipLaunchKernelGGL((k_fft_x_fwd_reg<T, T, 16>), dim3(grx), dim3(RF_XT), 0, ctx->stream, S, twx, fld<T>(ctx, MELT_VX), fld<T>(ctx, MELT_VY),
-                       fld<T>(ctx, MELT_VZ), src, usdtal, gate, dsl);
+    hipLaunchKernelGGL((k_fft_x_fwd_reg<Rf, T, 16>), dim3(grx), dim3(RF_XT), 0, ctx->stream, S, twx, fld<Rf>(ctx, MELT_VX), fld<Rf>(ctx, MELT_VY),
+                       fld<Rf>(ctx, MELT_VZ), src, usdtal, gate, dsl);
   else if (px == 8)
-    hipLaunchKernelGGL((k_fft_x_fwd_reg<T, T, 8>), dim3(grx), dim3(RF_XT), 0, ctx->stream, S, twx, fld<T>(ctx, MELT_VX), fld<T>(ctx, MELT_VY),
-                       fld<T>(ctx, MELT_VZ), src, usdtal, gate, dsl);
+    hipLaunchKernelGGL((k_fft_x_fwd_reg<Rf, T, 8>), dim3(grx), dim3(RF_XT), 0, ctx->stream, S, twx, fld<Rf>(ctx, MELT_VX), fld<Rf>(ctx, MELT_VY),
+                       fld<Rf>(ctx, MELT_VZ), src, usdtal, gate, dsl);
   else
-    hipLaunchKernelGGL((k_fft_x_fwd<T, T>), dim3(gx), dim3(FFT_NT), shx, ctx->stream, S, Px, twx, nlx, fld<T>(ctx, MELT_VX), fld<T>(ctx, MELT_VY),
-                       fld<T>(ctx, MELT_VZ), src, usdtal, gate, dsl);
+    hipLaunchKernelGGL((k_fft_x_fwd<Rf, T>), dim3(gx), dim3(FFT_NT), shx, ctx->stream, S, Px, twx, nlx, fld<Rf>(ctx, MELT_VX), fld<Rf>(ctx, MELT_VY),
+                       fld<Rf>(ctx, MELT_VZ), src, usdtal, gate, dsl);
   MELT_HIP(hipGetLastError());
   if (!gate) timing_end(ctx, SLOT_DIVG_FFTX, ev);
 
@@ -753,16 +754,17 @@ int solve_t(melt_ctx* ctx, int fused_divg, const int* gate, unsigned long long* 
 
   if (!gate) timing_begin(ctx, SLOT_IFFTX, &ev);
   if (px == 16)
-    hipLaunchKernelGGL((k_fft_x_bwd_reg<T, T, 16>), dim3(grx), dim3(RF_XT), 0, ctx->stream, S, twx, fld<T>(ctx, MELT_DPH), gate);
+    hipLaunchKernelGGL((k_fft_x_bwd_reg<Rf, T, 16>), dim3(grx), dim3(RF_XT), 0, ctx->stream, S, twx, fld<Rf>(ctx, MELT_DPH), gate);
   else if (px == 8)
-    hipLaunchKernelGGL((k_fft_x_bwd_reg<T, T, 8>), dim3(grx), dim3(RF_XT), 0, ctx->stream, S, twx, fld<T>(ctx, MELT_DPH), gate);
+    hipLaunchKernelGGL((k_fft_x_bwd_reg<Rf, T, 8>), dim3(grx), dim3(RF_XT), 0, ctx->stream, S, twx, fld<Rf>(ctx, MELT_DPH), gate);
   else
-    hipLaunchKernelGGL((k_fft_x_bwd<T, T>), dim3(gx), dim3(FFT_NT), shx, ctx->stream, S, Px, twx, nlx, fld<T>(ctx, MELT_DPH), gate);
+    hipLaunchKernelGGL((k_fft_x_bwd<Rf, T>), dim3(gx), dim3(FFT_NT), shx, ctx->stream, S, Px, twx, nlx, fld<Rf>(ctx, MELT_DPH), gate);
   MELT_HIP(hipGetLastError());
   if (!gate) timing_end(ctx, SLOT_IFFTX, ev);
   return 0;
 }
 
 int poisson_solve(melt_ctx* ctx, int fused_divg, const int* gate, unsigned long long* div_slots) {
-  return ctx->fp32 ? solve_t<float>(ctx, fused_divg, gate, div_slots) : solve_t<double>(ctx, fused_divg, gate, div_slots);
+  if (!ctx->fp32) return solve_t<double, double>(ctx, fused_divg, gate, div_slots);
+  return ctx->spec64 ? solve_t<float, double>(ctx, fused_divg, gate, div_slots) : solve_t<float, float>(ctx, fused_divg, gate, div_slots);
 }

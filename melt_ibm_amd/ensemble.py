@@ -12,7 +12,7 @@ The ensemble workload (48 simulations): grids drawn uniformly from
 to 64) or level-4 icosahedron, St uniform in [0.5, 4], Re uniform in
 [200, 1000], seeded random-phase initial velocity (|k| <= 4, rms 0.1) from the
 member's own seed, T = 1 in the fluid and 0 in the solid (inqpr), no HIT
-forcing, dt from CFL 0.5, FP32 bulk with FP64 refinement.
+forcing, dt from CFL 0.5, FP32 bulk with FP64 Lagrangian updates.
 """
 from __future__ import annotations
 

@@ -3,8 +3,9 @@
 ! Provides the reference's hot-path subroutines by their own names (tschem,
 ! tagCells, hdnl1, hdnl2, hdnl3, hdnlte, invtr1, invtr2, invtr3, invtrte, solxi,
 ! solxj, solxk, divg, phcalc, updvp, prcalc, mlsWeight, mlsForce, velforce,
-! tempforce, particle), each a thin wrapper over the C ABI acting on the
-! device-resident context that replaces the module globals.  Field transfers
+! tempforce, particle, remesh_coarsen, remesh_smooth), each a thin wrapper over
+! the C ABI acting on the device-resident context that replaces the module
+! globals (the two remeshing routines work on the caller's host arrays).  Field transfers
 ! take the reference's own array shape (1:n1,1:n2,kstart-1:kend+1), ghosts
 ! included, so module arrays can be passed unchanged.
 !
@@ -192,6 +193,24 @@ module melt_ibm_f
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
     end function
+    integer(c_int) function c_remesh_coarsen(nf, ne, nv, xyz, tri_nor, el, thr, vof, eof, voe, foe, gf, ge, gv, rm, anchor, flagged, &
+                                             vmelt, ecol, vdel) bind(C, name="melt_remesh_coarsen")
+      import :: c_int, c_int32_t, c_double, c_ptr
+      integer(c_int32_t), value :: nf, ne, nv
+      real(c_double) :: xyz(*), tri_nor(*), el(*)
+      real(c_double), value :: thr
+      integer(c_int32_t) :: vof(*), eof(*), voe(*), foe(*), gf(*), ge(*), gv(*), rm, anchor(*), flagged(*)
+      type(c_ptr), value :: vmelt
+      integer(c_int32_t) :: ecol, vdel
+    end function
+    integer(c_int) function c_remesh_smooth(vol_old, target_dv, nv, ne, nf, xyz, gv, ge, gf, flagged, voe, vof, foe, eof, &
+                                            n_erel, drift, cnt_refresh) bind(C, name="melt_remesh_smooth")
+      import :: c_int, c_int32_t, c_double
+      real(c_double), value :: vol_old, target_dv
+      integer(c_int32_t), value :: nv, ne, nf
+      real(c_double) :: xyz(*), drift
+      integer(c_int32_t) :: gv(*), ge(*), gf(*), flagged(*), voe(*), vof(*), foe(*), eof(*), n_erel, cnt_refresh
+    end function
     function c_strlen(s) bind(C, name="strlen") result(n)
       import :: c_ptr, c_size_t
       type(c_ptr), value :: s
@@ -210,6 +229,7 @@ module melt_ibm_f
     module procedure solxk_array, solxk_field
   end interface
   public :: findCentroidIndices, mlsWeight, mlsForce, velforce, tempforce, particle
+  public :: remesh_coarsen, remesh_smooth
 
 contains
 
@@ -365,6 +385,63 @@ contains
   subroutine particle()
     call check(c_particle(handle), 'particle')
     call check(c_sync(handle), 'particle')
+  end subroutine
+
+  ! remesh_coarsen.f90:2-258 with the reference's arguments (one body's
+  ! slices); the reference reads vmelt(:,:,inp) from mls_param, passed here
+  ! as an optional last argument.  Runs on the host, no context needed.
+  subroutine remesh_coarsen(ecol_cnt, eLengths, nf, ne, nv, xyz, tri_nor, E_thresh, vert_of_face, edge_of_face, vert_of_edge, &
+                            face_of_edge, isGhostFace, isGhostEdge, isGhostVert, rm_flag, anchorVert, flagged_edge, vmelt)
+    integer, intent(inout) :: ecol_cnt
+    integer, intent(in) :: nf, ne, nv
+    real(c_double), intent(inout) :: eLengths(ne), xyz(3, nv), tri_nor(3, nf)
+    real(c_double), intent(in) :: E_thresh
+    integer, intent(inout) :: vert_of_face(3, nf), edge_of_face(3, nf), vert_of_edge(2, ne), face_of_edge(2, ne)
+    logical, intent(inout) :: isGhostFace(nf), isGhostEdge(ne), isGhostVert(nv), rm_flag, anchorVert(nv), flagged_edge(ne)
+    real(c_double), intent(inout), optional, target :: vmelt(3, nv)
+    integer(c_int32_t) :: vof(3, nf), eof(3, nf), voe(2, ne), foe(2, ne), gf(nf), ge(ne), gv(nv), rm, an(nv), fl(ne), ec, vd
+    integer(c_int) :: rc
+    type(c_ptr) :: pv
+    vof = vert_of_face; eof = edge_of_face; voe = vert_of_edge; foe = face_of_edge
+    gf = merge(1, 0, isGhostFace); ge = merge(1, 0, isGhostEdge); gv = merge(1, 0, isGhostVert)
+    rm = merge(1, 0, rm_flag); an = merge(1, 0, anchorVert); fl = merge(1, 0, flagged_edge)
+    pv = c_null_ptr
+    if (present(vmelt)) pv = c_loc(vmelt)
+    rc = c_remesh_coarsen(int(nf, c_int32_t), int(ne, c_int32_t), int(nv, c_int32_t), xyz, tri_nor, eLengths, E_thresh, vof, eof, &
+                          voe, foe, gf, ge, gv, rm, an, fl, pv, ec, vd)
+    vert_of_face = vof; edge_of_face = eof; vert_of_edge = voe; face_of_edge = foe
+    isGhostFace = gf /= 0; isGhostEdge = ge /= 0; isGhostVert = gv /= 0
+    rm_flag = rm /= 0; anchorVert = an /= 0; flagged_edge = fl /= 0
+    ecol_cnt = int(ec)
+    if (rc /= 0) then
+      write(*, '(a,i0)') 'melt_ibm: remesh_coarsen failed (topology not preserved or QEM solve), status ', rc
+      error stop 1
+    end if
+  end subroutine
+
+  ! remesh_smooth.f90:3-347 with the reference's arguments
+  subroutine remesh_smooth(vol_old, target_DV, n_erel, drift, cnt_refresh, nv, ne, nf, xyz, isGhostVert, isGhostEdge, isGhostFace, &
+                           flagged_edge, vert_of_edge, vert_of_face, face_of_edge, edge_of_face)
+    real(c_double), intent(in) :: vol_old, target_DV
+    integer, intent(out) :: n_erel, cnt_refresh
+    real(c_double), intent(out) :: drift
+    integer, intent(in) :: nv, ne, nf
+    real(c_double), intent(inout) :: xyz(3, nv)
+    logical, intent(in) :: isGhostVert(nv), isGhostEdge(ne), isGhostFace(nf)
+    logical, intent(inout) :: flagged_edge(ne)
+    integer, intent(in) :: vert_of_edge(2, ne), vert_of_face(3, nf), face_of_edge(2, ne), edge_of_face(3, nf)
+    integer(c_int32_t) :: gv(nv), ge(ne), gf(nf), fl(ne), voe(2, ne), vof(3, nf), foe(2, ne), eof(3, nf), ner, cr
+    integer(c_int) :: rc
+    gv = merge(1, 0, isGhostVert); ge = merge(1, 0, isGhostEdge); gf = merge(1, 0, isGhostFace); fl = merge(1, 0, flagged_edge)
+    voe = vert_of_edge; vof = vert_of_face; foe = face_of_edge; eof = edge_of_face
+    rc = c_remesh_smooth(vol_old, target_DV, int(nv, c_int32_t), int(ne, c_int32_t), int(nf, c_int32_t), xyz, gv, ge, gf, fl, voe, &
+                         vof, foe, eof, ner, drift, cr)
+    flagged_edge = fl /= 0
+    n_erel = int(ner); cnt_refresh = int(cr)
+    if (rc /= 0) then
+      write(*, '(a,i0)') 'melt_ibm: remesh_smooth failed, status ', rc
+      error stop 1
+    end if
   end subroutine
 
 end module melt_ibm_f

@@ -54,7 +54,9 @@ typedef struct orc_ctx {
   int nv, ne, nf, has_geom;
   double rad_p;
   int *vert_of_edge, *edge_of_face, *vert_of_face, *face_of_edge;
-  unsigned char *ghost_v, *ghost_e, *ghost_f;
+  int32_t *ghost_v, *ghost_e, *ghost_f;   /* isGhostVert/Edge/Face as 0/1 */
+  int32_t *anchor_v, *flagged_e;          /* anchorVert, flagged_edge (allotri.f90:27-28) */
+  double remesh_stats[8];                 /* see orc_get_remesh_stats */
   double *lag[MELT_L_NLAG];
   double *xyz0;
   double *r_x_prn_tri, *r_x_tau_n; /* 3*nf each */
@@ -111,6 +113,8 @@ int orc_velforce(orc_ctx* o);
 int orc_tempforce(orc_ctx* o);
 int orc_particle(orc_ctx* o);
 int orc_geometry_init(orc_ctx* o, double rad_p);
+/* remeshing unit (orc_remesh.c) */
+int orc_remesh_step(orc_ctx* o, double vol_melt, int* did);
 void orc_hit_free(orc_ctx* o);
 
 #endif

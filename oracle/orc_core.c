@@ -106,7 +106,7 @@ void orc_ctx_destroy(orc_ctx* o) {
   free(o->ak1); free(o->ak2); free(o->ak3);
   free(o->imv); free(o->ipv); free(o->jmv); free(o->jpv);
   free(o->vert_of_edge); free(o->edge_of_face); free(o->vert_of_face); free(o->face_of_edge);
-  free(o->ghost_v); free(o->ghost_e); free(o->ghost_f);
+  free(o->ghost_v); free(o->ghost_e); free(o->ghost_f); free(o->anchor_v); free(o->flagged_e);
   free(o->xyz0); free(o->r_x_prn_tri); free(o->r_x_tau_n); free(o->pind);
   orc_hit_free(o);
   free(o);

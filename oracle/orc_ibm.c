@@ -1126,10 +1126,11 @@ static void stefan(orc_ctx* o) {  /* stefanCondition.f90:1-61 */
   }
 }
 
-int orc_particle(orc_ctx* o) {  /* particle.f90:1-229 with IREMESH = 0 */
+int orc_particle(orc_ctx* o) {  /* particle.f90:1-229 */
   if (!o->has_geom) return MELT_ERR_NO_GEOMETRY;
   if (o->cfg.imlsfor != 1) return 0;
   melt_body* b = &o->body;
+  double vol_melt = 0.0;   /* particle.f90:32-35: zero unless the body melts */
   orc_find_centroid_indices(o);
   if (orc_mlsweight(o)) return o->status;
   orc_mlsforce(o);
@@ -1140,6 +1141,7 @@ int orc_particle(orc_ctx* o) {  /* particle.f90:1-229 with IREMESH = 0 */
   if (o->cfg.imelt == 1) {
     stefan(o);
     b->volume = calc_volume(o);
+    vol_melt = b->volume;
   }
   if (o->cfg.imlsstr == 1) {
     centroids(o);
@@ -1148,7 +1150,12 @@ int orc_particle(orc_ctx* o) {  /* particle.f90:1-229 with IREMESH = 0 */
     update_part_pos(o);
     if (o->status) return o->status;
   }
-  if (o->cfg.iremesh == 1) { orc_set_error(o, MELT_ERR_REMESH, "remeshing is not on this path"); return o->status; }
+  /* remeshing particle.f90:121-160 */
+  int did_remesh = 0;
+  if (o->cfg.iremesh == 1) {
+    int rc = orc_remesh_step(o, vol_melt, &did_remesh);
+    if (rc) { orc_set_error(o, rc, "remeshing failed (topology, QEM solve or ring walk)"); return o->status; }
+  }
   /* primitive geometry info particle.f90:163-183 */
   calc_area(o, LAG(MELT_L_XYZV));
   edge_lengths(o, LAG(MELT_L_XYZV));
@@ -1159,8 +1166,21 @@ int orc_particle(orc_ctx* o) {  /* particle.f90:1-229 with IREMESH = 0 */
   b->volume = calc_volume(o);
   vert_normal(o);
   rigid_params(o, LAG(MELT_L_XYZV), b->pos_cm, &b->volume, b->inert_tensor);
+  if (did_remesh) {  /* update_tri_velocity RigidAuxRoutines.f90:902-928 */
+    for (int f = 0; f < o->nf; ++f) {
+      if (o->ghost_f[f]) continue;
+      double rad[3], om[3];
+      for (int d = 0; d < 3; ++d) rad[d] = LAG(MELT_L_TRI_BAR)[3 * f + d] - b->pos_cm[d];
+      orc_cross(om, b->omega_c, rad);
+      for (int d = 0; d < 3; ++d) LAG(MELT_L_VEL_TRI)[3 * f + d] = b->vel_cm[d] + om[d];
+    }
+  }
   if (o->cfg.imelt == 1)
     for (int f = 0; f < o->nf; ++f) LAG(MELT_L_CFAC)[f] = (LAG(MELT_L_SUR)[f] * b->h_eulerian) / b->celvol;
+  if (did_remesh && fabs(o->remesh_stats[5]) > 1.e-14) {  /* particle.f90:199-211 */
+    orc_set_error(o, MELT_ERR_REMESH, "Non-zero remeshing residual detected");
+    return o->status;
+  }
   if (b->volume < b->v_thresh) { orc_set_error(o, MELT_ERR_SMALL_VOLUME, "Geometry V/VE smaller than threshold"); return o->status; }
   return 0;
 }
@@ -1187,7 +1207,11 @@ int orc_set_geometry(orc_ctx* o, int32_t nv, int32_t ne, int32_t nf, const doubl
   o->edge_of_face = (int*)malloc(sizeof(int) * 3 * nf);
   o->vert_of_face = (int*)malloc(sizeof(int) * 3 * nf);
   o->face_of_edge = (int*)calloc(2 * (size_t)ne, sizeof(int));
-  o->ghost_v = (unsigned char*)calloc(nv, 1); o->ghost_e = (unsigned char*)calloc(ne, 1); o->ghost_f = (unsigned char*)calloc(nf, 1);
+  o->ghost_v = (int32_t*)calloc(nv, sizeof(int32_t)); o->ghost_e = (int32_t*)calloc(ne, sizeof(int32_t)); o->ghost_f = (int32_t*)calloc(nf, sizeof(int32_t));
+  free(o->anchor_v); free(o->flagged_e);
+  o->anchor_v = (int32_t*)malloc(sizeof(int32_t) * nv); o->flagged_e = (int32_t*)calloc(ne, sizeof(int32_t));
+  for (int v = 0; v < nv; ++v) o->anchor_v[v] = 1;
+  memset(o->remesh_stats, 0, sizeof o->remesh_stats);
   o->xyz0 = (double*)malloc(sizeof(double) * 3 * nv);
   o->r_x_prn_tri = (double*)calloc(3 * (size_t)nf, sizeof(double));
   o->r_x_tau_n = (double*)calloc(3 * (size_t)nf, sizeof(double));

@@ -66,14 +66,16 @@ def test_benchmark_workload_matches_oracle(gpu_lib):
 
 
 def test_benchmark_workload_precision_form_within_bar(gpu_lib):
-    """The precision form (FP32 fields, FP64 reductions, probes and Lagrangian
-    updates) against the FP64 path on the same 256^3 workload from the same
-    state, over 4 RK3 steps: every field within 1e-5 of its scale (max norm;
-    measured u, v, w, T <= 4.2e-7, p 2.0e-6), no narrow-band tag flipped and
-    the melted volume within 1e-6.  The bar holds through step 9 on this
-    workload (p, the largest, reaches 9.5e-6 at step 7: accumulated FP32 solve
-    error of about 2e-7 of max |p| per substep); after that, interface tags
-    start to differ."""
+    """The precision form (FP32 fields; FP64 line reductions, probes, sums and
+    Lagrangian updates) against the FP64 path on the benchmark's 256^3 workload
+    (HIT forcing synthesised every step) from the same state, over 30 RK3
+    steps: every field within 1e-5 of its scale (max norm; measured p 7.5e-6,
+    u, v, w <= 6e-7, T 8.7e-7 at step 30), no narrow-band tag flipped and the
+    melted volume within 1e-6.  The pressure difference is a smooth field (93%
+    of its energy at |k| < 4); rounding every state field of an FP64 run to
+    FP32 after each substep moves p by only 6e-7, and an FP64 spectrum in the
+    precision form leaves it unchanged, so it comes from FP32 arithmetic in the
+    stencils, not from storage or the FFT."""
     import bench
     ctxs = []
     for precision in (0, 1):
@@ -84,25 +86,26 @@ def test_benchmark_workload_precision_form_within_bar(gpu_lib):
         c.set_geometry(m.xyz, m.vert_of_edge, m.edge_of_face, rad)
         c.inqpr()
         u, v, w = inputs.solenoidal_velocity(cfg, seed=0, kmax=4.0, rms=0.1)
-        fx, fy, fz = inputs.low_mode_forcing(cfg, seed=1, kf=2.5, amplitude=0.5)
-        for name, a in (("vx", u), ("vy", v), ("vz", w), ("forcx", fx), ("forcy", fy), ("forcz", fz)):
+        for name, a in (("vx", u), ("vy", v), ("vz", w)):
             c.upload(name, a)
         c.set_dt(inputs.cfl_dt(cfg, u, v, w, cfl=0.5))
+        c.hit_init(*bench.HIT_PARAMS, seed=1)
         c.tagCells()
-        del u, v, w, fx, fy, fz
+        del u, v, w
         ctxs.append(c)
     d, s = ctxs
-    for _ in range(4):
-        d.tschem()
-        s.tschem()
+    for step in range(30):
+        for c in ctxs:
+            c.hit_force()
+            c.tschem()
     d.sync()
     s.sync()
     errs = {q: rel_err(s.download(q), d.download(q)) for q in ("vx", "vy", "vz", "temp", "pr")}
-    print("precision form vs FP64 after 4 steps:", {q: f"{e:.2e}" for q, e in errs.items()}, flush=True)
+    print("precision form vs FP64 after 30 steps:", {q: f"{e:.2e}" for q, e in errs.items()}, flush=True)
     for q, e in errs.items():
         assert e < 1e-5, f"{q}: rel err {e}"
     # VOFp is a continuous function of the signed distance, stored in FP32 in
-    # the precision form: it differs by rounding (measured 5.7e-8), while a cell whose tag flips
+    # the precision form: it differs by rounding, while a cell whose tag flips
     # (nearest face or sign resolved differently) differs by 1e-4 or more
     dv = np.max(np.abs(s.download("VOFp") - d.download("VOFp")))
     print(f"VOFp max difference {dv:.2e}", flush=True)
