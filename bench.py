@@ -64,6 +64,22 @@ def workload(n: int):
     return cfg, 5, 0.5
 
 
+def c3_workload(n: int = 512, perc_ethresh: float = 0.38):
+    """The precision-form configuration: the melting workload at n^3 with a
+    prolate spheroid (semi-axes 0.3, 0.3, 0.6, icosphere frequency 55: 60500
+    faces) and remeshing on.  Its edges span 0.40 h to 1.17 h at 512^3, so
+    E_thresh = 0.38 h starts them above the threshold, as the reference's deck
+    does (level-5 sphere of radius 0.2 on a 128^3 unit cube: edges near 0.9 h
+    against 0.7 h); coarsening then follows the melting.  Returns (cfg, mesh,
+    volume-equivalent radius)."""
+    from melt_ibm_amd import geometry
+    cfg, _, _ = workload(n)
+    cfg.iremesh = 1
+    cfg.perc_ethresh = perc_ethresh
+    r_eq = (0.3 * 0.3 * 0.6) ** (1.0 / 3.0)   # volume-equivalent radius: the mesh is scaled by it, inqpr uses it
+    return cfg, geometry.spheroid(0.3 / r_eq, 0.6 / r_eq, 55), r_eq
+
+
 def run_gpu(args, precision=0, keep_fields=False):
     from melt_ibm_amd.api import MeltContext
     cfg, level, rad = workload(args.n)
@@ -272,41 +288,39 @@ def measured_traffic(slot: str, n: int):
 
 
 def precision_study(args):
-    """The precision-form case of the benchmark configurations: a prolate
-    spheroid (semi-axes 0.3, 0.3, 0.6, icosphere frequency 55: 60500 faces) in
-    a 512^3 periodic cube, melting (St 2) with Newton-Euler coupling, seeded
-    solenoidal velocity and the low-wavenumber forcing field of the FP64 leg
-    (HIT synthesis and remeshing are not part of this path).  FP64 and the
-    precision form advance side by side from the same state; at every
+    """The precision-form case of the benchmark configurations (c3_workload):
+    a prolate spheroid (semi-axes 0.3, 0.3, 0.6, 60500 faces) in a 512^3
+    periodic cube, melting (St 2) with Newton-Euler coupling and remeshing,
+    seeded solenoidal velocity synthesised on the GPU and HIT forcing
+    regenerated every step.  FP64 and the precision form advance side by side
+    from the same state; at every
     checkpoint the max-norm relative error of each field is reported over the
     whole field and away from interface cells whose tag differs (see the
     mixed leg), with the melted-volume error and the time per substep."""
     from scipy.ndimage import binary_dilation
 
-    from melt_ibm_amd import geometry, inputs
+    from melt_ibm_amd import inputs
     from melt_ibm_amd.api import MeltContext
     n = args.n if args.n != 256 else 512
-    cfg, _, _ = workload(n)
-    r_eq = (0.3 * 0.3 * 0.6) ** (1.0 / 3.0)   # volume-equivalent radius: the mesh is scaled by it, inqpr uses it
-    mesh = geometry.spheroid(0.3 / r_eq, 0.6 / r_eq, 55)
+    cfg, mesh, r_eq = c3_workload(n, args.study_ethresh)
     print(f"# building the {n}^3 initial state", file=sys.stderr, flush=True)
-    init = dict(zip(("vx", "vy", "vz"), inputs.solenoidal_velocity(cfg, seed=0, kmax=4.0, rms=0.1)))
-    init.update(zip(("forcx", "forcy", "forcz"), inputs.low_mode_forcing(cfg, seed=1, kf=2.5, amplitude=args.study_forcing)))
-    dt = inputs.cfl_dt(cfg, init["vx"], init["vy"], init["vz"], cfl=args.study_cfl)
+    u, v, w, dt = inputs.solenoidal_velocity_device(cfg, seed=0, kmax=4.0, rms=0.1)
+    dt = dt * args.study_cfl / 0.5
     ctxs = []
     for prec in (0, 1):
         cfg.precision = prec
         c = MeltContext(cfg)
         c.set_geometry(mesh.xyz, mesh.vert_of_edge, mesh.edge_of_face, r_eq)
         c.inqpr()
-        for name, a in init.items():
-            c.upload(name, a)
+        for name, a in (("vx", u), ("vy", v), ("vz", w)):
+            c.upload_device(name, a.data_ptr())
         c.set_dt(dt)
+        c.hit_init(*HIT_PARAMS, seed=3)
         c.tagCells()
         c.sync()
         ctxs.append(c)
         print(f"# precision {prec} set up: N={n} faces={mesh.edge_of_face.shape[0]} dt={dt:.4e}", file=sys.stderr, flush=True)
-    del init
+    del u, v, w
     qs = ("vx", "vy", "vz", "temp", "pr")
     t_run = [0.0, 0.0]
     checkpoints = []
@@ -320,6 +334,7 @@ def precision_study(args):
                 c.sync()
                 t0 = time.perf_counter()
                 for _ in range(k):
+                    c.hit_force()
                     c.tschem()
                 c.sync()
                 t_run[p_] += time.perf_counter() - t0
@@ -340,6 +355,8 @@ def precision_study(args):
         cp["volume_rel_err"] = float(f"{abs(v1 - v0) / v0:.3e}")
         cp["fp64_umax"] = float(f"{max(float(np.max(np.abs(f[0][q]))) for q in ('vx', 'vy', 'vz')):.4f}")
         cp["fp64_cfl"] = float(f"{ctxs[0].cfl() * dt:.3f}")
+        cp["remesh"] = {("fp64", "mixed")[p_]: {k_: c.remesh_stats()[k_] for k_ in ("events", "total_collapses")}
+                        for p_, c in enumerate(ctxs)}
         checkpoints.append(cp)
         del f, flip, near
         print(f"# step {done}: {cp}", file=sys.stderr, flush=True)
@@ -348,8 +365,9 @@ def precision_study(args):
         c.close()
     sub = [t / (3 * max(done, 1)) * 1e3 for t in t_run]
     print(json.dumps({"study": "precision form vs FP64", "grid": [n, n, n], "faces": int(mesh.edge_of_face.shape[0]),
-                      "forcing_amplitude": args.study_forcing, "cfl": args.study_cfl, "dt": dt,
-                      "solid": "prolate spheroid, semi-axes 0.3/0.3/0.6, icosphere frequency 55, melting St 2, Newton-Euler on, no remeshing",
+                      "forcing": f"HIT (TL, eps*, kf/kmin) = {HIT_PARAMS}, regenerated every step", "cfl": args.study_cfl, "dt": dt,
+                      "solid": f"prolate spheroid, semi-axes 0.3/0.3/0.6, icosphere frequency 55, melting St 2, Newton-Euler on, "
+                               f"remeshing with E_thresh = {args.study_ethresh} h",
                       "steps": done, "substep_ms": {"fp64": round(sub[0], 4), "mixed": round(sub[1], 4)},
                       "speedup": round(sub[0] / sub[1], 3), "refinement": {"checks": stats[0], "refined": stats[1], "last_ratio": stats[2]},
                       "checkpoints": checkpoints, "stopped": stopped}))
@@ -379,7 +397,7 @@ def main():
                     help="precision form vs FP64 side by side on the 512^3 spheroid case (errors per field over --study-steps)")
     ap.add_argument("--study-steps", type=int, default=100)
     ap.add_argument("--study-every", type=int, default=20, help="steps between error checkpoints")
-    ap.add_argument("--study-forcing", type=float, default=0.5, help="amplitude of the low-wavenumber forcing field")
+    ap.add_argument("--study-ethresh", type=float, default=0.38, help="PERCT_ETHRESH of the remeshing (E_thresh / h)")
     ap.add_argument("--study-cfl", type=float, default=0.5, help="CFL number of the fixed dt")
     args = ap.parse_args()
     if args.precision_study:

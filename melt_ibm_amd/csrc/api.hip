@@ -656,7 +656,17 @@ static int side_run(melt_ctx* ctx, F&& work) {
   MELT_HIP(hipEventRecord(ctx->ev_join, ctx->side));
   return 0;
 }
-// More work on the side stream after what side_run queued there (no new fork).
+// The fork point of later side-stream work: the side stream waits for
+// everything queued on the main stream so far, and work queued on it later
+// (side_continue) starts from here, whatever the main stream gets meanwhile.
+static int side_fork(melt_ctx* ctx) {
+  if (one_stream(ctx)) return 0;
+  if (int rc = ensure_side(ctx)) return rc;
+  MELT_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+  MELT_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+  return 0;
+}
+// More work on the side stream after its last fork (no new fork).
 template <class F>
 static int side_continue(melt_ctx* ctx, F&& work) {
   if (one_stream(ctx)) return work();
@@ -742,7 +752,6 @@ static int substep_impl(melt_ctx* ctx, int32_t ns, bool tag_next, bool coef_next
   timing_end(ctx, SLOT_PH_EULER, e_ph);
 
   timing_begin(ctx, SLOT_PH_MLS, &e_ph);
-  const bool remesh = ibm && c.iremesh == 1;
   // what the side stream queues after the Lagrangian update: the next
   // substep's tagging (tag_next) and probe geometry stage (coef_next)
   auto queue_next = [&]() -> int {
@@ -773,19 +782,15 @@ static int substep_impl(melt_ctx* ctx, int32_t ns, bool tag_next, bool coef_next
     if (rc || (rc = side_join(ctx)) || (rc = ibm_force_drain(ctx, 1))) return rc;
     timing_end(ctx, SLOT_IBM, e);
     // The Lagrangian update after the forcing (loads, heat flux, Stefan
-    // melting, Newton-Euler motion, geometry refresh) reads only the probe
-    // results and the body arrays, which the pressure phase does not touch:
-    // it runs on the side stream, concurrently with the pressure solve, and
-    // rejoins before the substep ends (the next tagging needs the geometry).
-    // With remeshing the update stops after measuring the edges; the host
-    // reads that flag only after the pressure solve is queued (below).
-    if ((rc = side_run(ctx, [&] {
-          timing_begin(ctx, SLOT_LAGR, &e);
-          int r = lagr_after_force(ctx, true, remesh);
-          timing_end(ctx, SLOT_LAGR, e);
-          return r || remesh ? r : queue_next();
-        })))
-      return rc;
+    // melting, Newton-Euler motion, remeshing, geometry refresh) reads only
+    // the probe results and the body arrays, which the pressure phase does
+    // not touch: it runs on the side stream from here, concurrently with the
+    // pressure solve, and rejoins before the substep ends (the next tagging
+    // needs the geometry).  Its 20-30 short launches are queued after the
+    // pressure phase's, so that the host has queued the pressure solve before
+    // the device needs it (queued first, they left the main stream idle for
+    // 0.1-0.23 ms per substep while the host submitted them).
+    if ((rc = side_fork(ctx))) return rc;
   }
   timing_end(ctx, SLOT_PH_MLS, e_ph);
 
@@ -813,10 +818,16 @@ static int substep_impl(melt_ctx* ctx, int32_t ns, bool tag_next, bool coef_next
     timing_end(ctx, SLOT_REFINE, e);
   }
   timing_end(ctx, SLOT_PH_PRESSURE, e_ph);
-  // remeshing: wait for the edge flag (the pressure solve runs meanwhile),
-  // remesh on the host if needed, then refresh the geometry and queue the
-  // next substep's work on the side stream
-  if (remesh && (rc = side_continue(ctx, [&] { int r = lagr_finish(ctx); return r ? r : queue_next(); }))) return rc;
+  // the Lagrangian update from the fork point above (with remeshing the host
+  // waits for the edge flag here, while the pressure solve runs), then the
+  // next substep's tagging and probe geometry stage
+  if (ibm && (rc = side_continue(ctx, [&] {
+                timing_begin(ctx, SLOT_LAGR, &e);
+                int r = lagr_after_force(ctx, true);
+                timing_end(ctx, SLOT_LAGR, e);
+                return r ? r : queue_next();
+              })))
+    return rc;
   if (ibm && (rc = side_join(ctx))) return rc;
   timing_end(ctx, SLOT_SUBSTEP, e_sub);
   if (ctx->timing && ctx->ev_pending.size() > 4096) timing_flush(ctx);

@@ -502,12 +502,10 @@ int ibm_probes(melt_ctx* ctx, int* pind_out = nullptr, bool coefs_ready = false)
 int ibm_probe_coefs(melt_ctx* ctx);
 int ibm_window_reset(melt_ctx* ctx);
 // with commit_pind (the fused substep) it also commits the staged heat-flux probe indices.
-// With IREMESH = 1 the update stops after measuring the edges; lagr_finish
-// then remeshes on the host if an edge is short and refreshes the geometry
-// (called at once unless defer_finish, when the fused substep calls it after
-// queueing the pressure solve, so the host's wait overlaps it).
-int lagr_after_force(melt_ctx* ctx, bool commit_pind = false, bool defer_finish = false);
-int lagr_finish(melt_ctx* ctx);
+// With IREMESH = 1 the host reads the device's short-edge flag after the
+// Newton-Euler update (waiting for the queue up to there) and remeshes on the
+// host before the geometry refresh.
+int lagr_after_force(melt_ctx* ctx, bool commit_pind = false);
 // host remeshing of the context's body (remesh.hip): *did = 1 when it ran
 int remesh_context(melt_ctx* ctx, int* did);
 // host mirrors from the device connectivity (after a snapshot load)

@@ -583,7 +583,7 @@ __global__ __launch_bounds__(256) void k_tri_velocity(LagArgs A, const melt_body
 
 // the rest of particle.f90 after the edges were measured (IREMESH = 1):
 // remesh on the host when an edge is short, then the geometry refresh
-int lagr_finish(melt_ctx* ctx) {
+static int lagr_finish(melt_ctx* ctx) {
   if (ctx->cfg.iremesh != 1) return 0;
   LagArgs A = lag_args(ctx);
   MELT_HIP(hipEventSynchronize(ctx->ev_rm));
@@ -602,7 +602,7 @@ int lagr_finish(melt_ctx* ctx) {
 
 // particle.f90:62-193 after the MLS interpolation: heat flux -> vertices,
 // loads, Stefan update, Newton-Euler motion, (remeshing,) geometry refresh.
-int lagr_after_force(melt_ctx* ctx, bool commit_pind, bool defer_finish) {
+int lagr_after_force(melt_ctx* ctx, bool commit_pind) {
   LagArgs A = lag_args(ctx);
   hipStream_t s = ctx->stream;
   int heat = ctx->cfg.imelt == 1, loads = ctx->cfg.imlsstr == 1;
@@ -638,7 +638,7 @@ int lagr_after_force(melt_ctx* ctx, bool commit_pind, bool defer_finish) {
   MELT_HIP(hipGetLastError());
   MELT_HIP(hipMemcpyAsync(ctx->h_rm, &ctx->d_body->rm_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   MELT_HIP(hipEventRecord(ctx->ev_rm, s));
-  return defer_finish ? 0 : lagr_finish(ctx);
+  return lagr_finish(ctx);
 }
 
 // tri_geo.f90:15-30 + setup_particles + init_geomCoords + init_mlsForce on the

@@ -146,32 +146,50 @@ class Ensemble:
             for n in sorted({m.n for m in members}):
                 self.tuning[n] = autotune(n, precision, lib=lib)
         self.sims = [Simulation.build(m, precision, lib, self.tuning.get(m.n), graph) for m in members]
+        self.active = list(range(len(self.sims)))
+        self.melted: dict[int, int] = {}        # simulation index -> ensemble step in which its body melted
+        self.steps_done = 0
         self.step_ms: list[float] = []
+        self.step_cells: list[int] = []
 
     @property
     def cells(self) -> int:
         return sum(s.member.cells for s in self.sims)
 
     def step(self) -> float:
+        """One RK3 step of every simulation still running.  A body that melts
+        below V_thresh ends its simulation (the reference's terminal condition,
+        particle.f90:218-224); the others continue."""
+        from .api import MeltError
+        run = list(self.active)
         t0 = time.perf_counter()
-        for s in self.sims:
-            s.ctx.tschem()
-        for s in self.sims:
-            s.ctx.sync()
+        for i in run:
+            self.sims[i].ctx.tschem()
+        for i in run:
+            try:
+                self.sims[i].ctx.sync()
+            except MeltError as e:
+                if e.code != 4:       # MELT_ERR_SMALL_VOLUME
+                    raise
+                self.melted[i] = self.steps_done
+                self.active.remove(i)
         ms = (time.perf_counter() - t0) * 1e3
+        self.steps_done += 1
         self.step_ms.append(ms)
+        self.step_cells.append(sum(self.sims[i].member.cells for i in run))
         return ms
 
     def run(self, steps: int, warmup: int = 1) -> dict:
         for _ in range(warmup):
             self.step()
-        self.step_ms = []
+        self.step_ms, self.step_cells = [], []
         for _ in range(steps):
             self.step()
         a = np.array(self.step_ms)
         return {"sims": len(self.sims), "cells": self.cells, "steps": steps, "warmup": warmup,
                 "step_ms_p50": float(np.percentile(a, 50)), "step_ms_p99": float(np.percentile(a, 99)),
-                "step_ms_mean": float(a.mean()), "mcell_steps_per_s": self.cells * steps / (a.sum() / 1e3) / 1e6,
+                "step_ms_mean": float(a.mean()), "mcell_steps_per_s": sum(self.step_cells) / (a.sum() / 1e3) / 1e6,
+                "melted": {str(i): k for i, k in sorted(self.melted.items())},
                 "faces": sorted({20 * (s.member.freq or 2 ** s.member.level) ** 2 for s in self.sims}),
                 "tuning": {str(k): v for k, v in self.tuning.items()}}
 

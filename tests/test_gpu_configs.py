@@ -1,10 +1,10 @@
 """The benchmark's larger configurations on the GPU.
 
  - 512^3 precision form (prolate spheroid of semi-axes 0.3 / 0.3 / 0.6 meshed
-   with 60500 faces, melting, Newton-Euler on, HIT forcing): the precision form
-   against the FP64 path from the same state over 3 RK3 steps, every field
-   within 1e-5 of its scale and the melted volume within 1e-6.  Remeshing,
-   which the reference's deck turns on for this case, is not part of this path.
+   with 60500 faces, melting, Newton-Euler on, HIT forcing, remeshing on with
+   E_thresh = 0.38 h): the precision form against the FP64 path from the same
+   state over 3 RK3 steps, every field within 1e-5 of its scale, the melted
+   volume within 1e-6 and the same remeshing events.
  - The 48-simulation ensemble (grids from {48, 64, 96, 128}^3, level-3/4
    icosahedra kept: the ray-tagging pencils are sized for them), one graph
    replay per simulation and step: sampled members equal the same simulation
@@ -16,7 +16,7 @@
 import numpy as np
 import pytest
 
-from melt_ibm_amd import geometry, inputs
+from melt_ibm_amd import inputs
 from melt_ibm_amd.api import MeltContext
 from melt_ibm_amd.ensemble import Ensemble, Simulation, ensemble_members
 from melt_ibm_amd.serve import Server, request_stream
@@ -29,10 +29,8 @@ FIELDS = ("vx", "vy", "vz", "temp", "pr")
 
 def test_precision_form_512_spheroid(gpu_lib):
     import bench
-    cfg, _, _ = bench.workload(512)
-    r_eq = (0.3 * 0.3 * 0.6) ** (1.0 / 3.0)
-    mesh = geometry.spheroid(0.3 / r_eq, 0.6 / r_eq, 55)
-    assert mesh.edge_of_face.shape[0] == 60500
+    cfg, mesh, r_eq = bench.c3_workload(512)
+    assert mesh.edge_of_face.shape[0] == 60500 and cfg.iremesh == 1
     print("# 512^3 initial state on the device", flush=True)
     u, v, w, dt = inputs.solenoidal_velocity_device(cfg, seed=0, kmax=4.0, rms=0.1)
     ctxs = []
@@ -62,6 +60,7 @@ def test_precision_form_512_spheroid(gpu_lib):
     for q, e in errs.items():
         assert e < 1e-5, (q, e)
     assert abs(s.body().volume - d.body().volume) < 1e-6 * d.body().volume
+    assert s.remesh_stats()["events"] == d.remesh_stats()["events"]
     for c in ctxs:
         c.close()
 

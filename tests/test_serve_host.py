@@ -43,6 +43,7 @@ class _Sim:
     built = []
 
     def __init__(self, m):
+        self.member = m
         self.ctx = _Ctx(*self.behaviour.get(m.seed, (0, 0.0)))
         self.closed = False
 
@@ -203,3 +204,27 @@ def test_requests_advance_at_their_own_pace(fake_sims):
     assert log.index(0, log.index(0) + 1) > max(i for i, rid in enumerate(log) if rid == 1)
     assert len(done[1].step_ms) == 19 and len(done[0].step_ms) == 1
     assert rep["steps"] == 22
+
+
+def test_ensemble_retires_melted_members(monkeypatch):
+    """A member whose body melts below V_thresh ends there (particle.f90:218-224)
+    while the others keep stepping; any other device error ends the ensemble."""
+    from melt_ibm_amd import ensemble
+    monkeypatch.setattr(ensemble, "Simulation", _Sim)
+    _Sim.behaviour, _Sim.built = {1: (3, 1e-7), 2: (5, 1e-7)}, []
+    ens = ensemble.Ensemble([_member(i, 0) for i in range(4)], precision=1, tune=False)
+    st = ens.run(steps=6, warmup=0)
+    assert st["melted"] == {"1": 2, "2": 4}
+    assert [s.ctx.steps for s in ens.sims] == [6, 3, 5, 6]
+    assert sum(ens.step_cells) == _member(0, 0).cells * (6 + 3 + 5 + 6)
+    _Sim.behaviour, _Sim.built = {0: (2, None)}, []
+
+    class _Broken(_Ctx):
+        def sync(self):
+            if self.steps == 2:
+                raise MeltError(2, "hipStreamSynchronize: unspecified launch failure")
+
+    ens = ensemble.Ensemble([_member(0, 0)], precision=1, tune=False)
+    ens.sims[0].ctx = _Broken(0, 0.0)
+    with pytest.raises(MeltError):
+        ens.run(steps=3, warmup=0)
