@@ -15,6 +15,7 @@
 // indexing.
 #include <array>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -648,8 +649,12 @@ int remesh_context(melt_ctx* ctx, int* did) {
   if (int rc = push_flags(ctx, ctx->h_gv, ctx->d_gv, s, tmp)) return rc;
   if (int rc = push_flags(ctx, ctx->h_ge, ctx->d_ge, s, tmp)) return rc;
   if (int rc = push_flags(ctx, ctx->h_gf, ctx->d_gf, s, tmp)) return rc;
-  if (std::fabs(vol_melt - vol_smooth) > 1.e-14)   // particle.f90:199-211
-    return melt_fail(ctx, MELT_ERR_REMESH, "remesh_smooth", "non-zero remeshing residual");
+  if (std::fabs(vol_melt - vol_smooth) > 1.e-14) {   // particle.f90:199-211
+    char msg[192];
+    std::snprintf(msg, sizeof msg, "non-zero remeshing residual %.3e (volume %.6e, %d collapses, %d deletions, %d relaxations, %d refreshes, drift %.3e)",
+                  vol_melt - vol_smooth, vol_melt, ncol, ndel, nrel, nref, drift);
+    return melt_fail(ctx, MELT_ERR_REMESH, "remesh_smooth", msg);
+  }
   return 0;
 }
 

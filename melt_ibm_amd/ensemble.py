@@ -12,7 +12,8 @@ The ensemble workload (48 simulations): grids drawn uniformly from
 to 64) or level-4 icosahedron, St uniform in [0.5, 4], Re uniform in
 [200, 1000], seeded random-phase initial velocity (|k| <= 4, rms 0.1) from the
 member's own seed, T = 1 in the fluid and 0 in the solid (inqpr), no HIT
-forcing, dt from CFL 0.5, FP32 bulk with FP64 Lagrangian updates.
+forcing, dt from CFL 0.5, FP32 bulk with FP64 Lagrangian updates, remeshing
+on (E_thresh at 0.95 of the shortest initial edge).
 """
 from __future__ import annotations
 
@@ -27,6 +28,31 @@ from .api import MeltConfig, MeltContext, MeltError
 
 GRIDS = (48, 64, 96, 128)
 
+# The reference's fatal conditions that end one simulation (it stops or calls
+# MPI_Abort): a body volume below V_thresh (particle.f90:218-224), remeshing
+# that breaks the topology or leaves a volume residual above 1e-14
+# (remesh_coarsen.f90:77-253, particle.f90:205-210), and, here, a surface that
+# outgrew the device window sized at set-up.  In a batch they end only the
+# simulation that raised them.
+TERMINAL_CODES = (4, 6, 9)   # MELT_ERR_SMALL_VOLUME, MELT_ERR_REMESH, MELT_ERR_WINDOW
+
+
+def terminal_status(ctx: MeltContext, e: MeltError) -> str:
+    """"melted" for a positive volume below V_thresh; "diverged" for a tangled
+    surface (a negative or non-finite volume, or one that left its window);
+    "remesh failed" for the remeshing aborts; "error N" for anything else."""
+    if e.code == 6:
+        return "remesh failed"
+    if e.code == 9:
+        return "diverged"
+    if e.code != 4:
+        return f"error {e.code}"
+    try:
+        vol = ctx.body().volume
+    except MeltError:
+        vol = float("nan")
+    return "melted" if np.isfinite(vol) and vol > 0.0 else "diverged"
+
 
 @dataclass
 class Member:
@@ -39,16 +65,25 @@ class Member:
     aspect: float = 1.0     # spheroid c/a (1 = sphere), volume-equivalent radius rad
     freq: int = 0           # icosphere frequency (faces = 20 freq^2); 0: 2**level
     steps: int = 0          # RK3 steps to run (serving)
+    remesh: bool = True     # IREMESH = 1, E_thresh just below the shortest initial edge
 
     @property
     def cells(self) -> int:
         return self.n ** 3
 
     def config(self, precision: int) -> MeltConfig:
-        # latHeat = cp (Tliq - Tmelt) / St with cp = 1, Tliq = 1, Tmelt = 0
+        # latHeat = cp (Tliq - Tmelt) / St with cp = 1, Tliq = 1, Tmelt = 0.  With
+        # remeshing, PERCT_ETHRESH puts E_thresh at 0.95 of the shortest initial
+        # edge: the reference's deck likewise starts its edges above E_thresh, and
+        # edges that the melting shrinks are collapsed from there on.
+        perc = 0.7
+        if self.remesh:
+            m = self.mesh()
+            d = m.xyz[m.vert_of_edge[:, 0] - 1] - m.xyz[m.vert_of_edge[:, 1] - 1]
+            perc = 0.95 * float(np.sqrt((d * d).sum(axis=1)).min()) * self.rad / (2.0 * np.pi / self.n)
         return MeltConfig(n1m=self.n, n2m=self.n, n3m=self.n, ren=self.re, prandtl=1.0, betagz=0.0, tmelt=0.0, tliq=1.0, tsol=0.0,
-                          cpliquid=1.0, latheat=1.0 / self.st, dens_ratio=1.0, imlsfor=1, imlsstr=1, imelt=1, iremesh=0, forcing=0,
-                          precision=precision)
+                          cpliquid=1.0, latheat=1.0 / self.st, dens_ratio=1.0, imlsfor=1, imlsstr=1, imelt=1,
+                          iremesh=1 if self.remesh else 0, forcing=0, precision=precision, perc_ethresh=perc)
 
     def mesh(self) -> geometry.Mesh:
         f = self.freq or 2 ** self.level
@@ -139,6 +174,12 @@ class Simulation:
 
 
 class Ensemble:
+    """With remeshing the host reads each member's short-edge flag every
+    substep inside tschem (and edits the mesh when it is set), so the members
+    are stepped from a pool of host threads: while one waits for its flag the
+    others keep submitting work.  Without remeshing each step is one captured
+    graph per member."""
+
     def __init__(self, members: list[Member], precision: int = 1, lib=None, tune: bool = True, graph: bool = True):
         self.tuning = {}
         if tune:
@@ -147,31 +188,37 @@ class Ensemble:
                 self.tuning[n] = autotune(n, precision, lib=lib)
         self.sims = [Simulation.build(m, precision, lib, self.tuning.get(m.n), graph) for m in members]
         self.active = list(range(len(self.sims)))
-        self.melted: dict[int, int] = {}        # simulation index -> ensemble step in which its body melted
+        self.ended: dict[int, tuple[int, str]] = {}   # simulation index -> (ensemble step, terminal_status)
+        self.ended_detail: dict[int, str] = {}        # simulation index -> the library's message
         self.steps_done = 0
         self.step_ms: list[float] = []
         self.step_cells: list[int] = []
+        from concurrent.futures import ThreadPoolExecutor
+        self.pool = ThreadPoolExecutor(max_workers=min(16, max(1, len(self.sims))))
 
     @property
     def cells(self) -> int:
         return sum(s.member.cells for s in self.sims)
 
     def step(self) -> float:
-        """One RK3 step of every simulation still running.  A body that melts
-        below V_thresh ends its simulation (the reference's terminal condition,
-        particle.f90:218-224); the others continue."""
-        from .api import MeltError
+        """One RK3 step of every simulation still running.  The reference's
+        terminal conditions (TERMINAL_CODES) end only their simulation, which
+        is recorded in `ended` with terminal_status; the others continue.  Any
+        other device error ends the ensemble.  Remeshing errors are raised by
+        tschem itself (the host edits the mesh inside it), the others by the
+        sync that reads the device's error word."""
         run = list(self.active)
         t0 = time.perf_counter()
-        for i in run:
-            self.sims[i].ctx.tschem()
-        for i in run:
+        futs = [self.pool.submit(self.sims[i].ctx.tschem) for i in run]
+        for i, f in zip(run, futs):
             try:
+                f.result()
                 self.sims[i].ctx.sync()
             except MeltError as e:
-                if e.code != 4:       # MELT_ERR_SMALL_VOLUME
+                if e.code not in TERMINAL_CODES:
                     raise
-                self.melted[i] = self.steps_done
+                self.ended[i] = (self.steps_done, terminal_status(self.sims[i].ctx, e))
+                self.ended_detail[i] = str(e)
                 self.active.remove(i)
         ms = (time.perf_counter() - t0) * 1e3
         self.steps_done += 1
@@ -189,11 +236,13 @@ class Ensemble:
         return {"sims": len(self.sims), "cells": self.cells, "steps": steps, "warmup": warmup,
                 "step_ms_p50": float(np.percentile(a, 50)), "step_ms_p99": float(np.percentile(a, 99)),
                 "step_ms_mean": float(a.mean()), "mcell_steps_per_s": sum(self.step_cells) / (a.sum() / 1e3) / 1e6,
-                "melted": {str(i): k for i, k in sorted(self.melted.items())},
+                "ended": {str(i): {"step": k, "status": st} for i, (k, st) in sorted(self.ended.items())},
+                "ended_detail": {str(i): d for i, d in sorted(self.ended_detail.items())},
                 "faces": sorted({20 * (s.member.freq or 2 ** s.member.level) ** 2 for s in self.sims}),
                 "tuning": {str(k): v for k, v in self.tuning.items()}}
 
     def close(self) -> None:
+        self.pool.shutdown()
         for s in self.sims:
             s.close()
         self.sims = []

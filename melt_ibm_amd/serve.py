@@ -6,13 +6,16 @@ Admission is first come first served: every request that has arrived and
 fits in the free budget has its bytes reserved and goes to a builder thread
 (a request that does not fit waits; later ones wait behind it).  The builder
 sets the simulation up off the scheduler's path (context and mesh, initial
-state synthesised on the GPU, initial ray tagging, and the first RK3 step,
-which captures the step's HIP graph) and hands it to the scheduler.  The
-scheduler keeps one step in flight per resident request: it polls each
-request's stream without blocking, and as soon as a request's step has
-finished (its device error word read) it launches that request's next step,
-a replay of its graph on its own stream.  Requests therefore advance at their
-own pace, a small grid does not wait for a large one, and the steps of all
+state synthesised on the GPU, initial ray tagging, and the first RK3 step)
+and hands it to the scheduler.  The scheduler keeps one step in flight per
+resident request: it polls each request's stream without blocking, and as
+soon as a request's step has finished (its device error word read) it hands
+that request's next step to a pool of host threads, which queue it on the
+request's own stream (with remeshing, the default, tschem reads the device's
+short-edge flag every substep and edits the mesh on the host when it is set,
+so it returns only after the last substep is queued; without it, a step is
+the replay of one captured graph).  Requests therefore advance at their own
+pace, a small grid does not wait for a large one, and the steps of all
 resident requests overlap on the device.  A request that has run its steps,
 or hit a terminal condition, is retired and its bytes return to the budget.
 
@@ -35,12 +38,13 @@ import queue
 import sys
 import threading
 import time
+from concurrent.futures import Future, ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
 
 from .api import MeltError
-from .ensemble import Member, Simulation
+from .ensemble import Member, Simulation, terminal_status
 
 SERVE_GRIDS = (64, 128, 256, 384)
 FREQ_FOR_GRID = {64: 10, 128: 20, 256: 35, 384: 55}   # 2000, 8000, 24500, 60500 faces
@@ -78,6 +82,7 @@ class Request:
     bytes: int = 0
     status: str = "ok"
     inflight: bool = False
+    future: Future | None = None                      # the host call that queued the current step
     launched: float = 0.0
     last_done: float = 0.0
 
@@ -96,20 +101,6 @@ def request_stream(duration_s: float, rate: float = 4.0, seed: int = 7, steps_sc
                    steps=max(1, int(round(rng.integers(50, 501) * steps_scale))))
         out.append(Request(len(out), t, m))
     return out
-
-
-def _terminal_status(r: Request, e: MeltError) -> str:
-    """The reference's terminal conditions end the request, not the server:
-    code 4 is a volume below V_thresh (particle.f90:218-224), which a tangled
-    surface (melting front faster than a cell per step, no remeshing) also
-    trips, with a negative or non-finite volume."""
-    if e.code != 4:
-        return f"error {e.code}"
-    try:
-        vol = r.sim.ctx.body().volume
-    except MeltError:
-        vol = float("nan")
-    return "melted" if np.isfinite(vol) and vol > 0.0 else "diverged"
 
 
 class Server:
@@ -141,11 +132,11 @@ class Server:
         r.sim = Simulation.build(r.member, self.precision, self.lib, tuning, graph=True, device_inputs=self.device_inputs)
         # the first step captures the graph (host work) here, off the scheduler
         t = self._clock()
-        r.sim.ctx.tschem()
         try:
+            r.sim.ctx.tschem()
             r.sim.ctx.sync()
         except MeltError as e:
-            r.status = _terminal_status(r, e)
+            r.status = terminal_status(r.sim.ctx, e)
         r.done_steps = 1
         r.last_done = self._clock()
         r.device_ms.append((r.last_done - t) * 1e3)
@@ -187,6 +178,9 @@ class Server:
         todo: queue.Queue = queue.Queue()
         built: queue.Queue = queue.Queue()
         worker = threading.Thread(target=self._builder, args=(todo, built), daemon=True)
+        # steps are submitted from a pool of host threads: with remeshing,
+        # tschem waits inside for each substep's short-edge flag
+        pool = ThreadPoolExecutor(max_workers=16)
         self._t0 = time.perf_counter()
         worker.start()
         building = 0
@@ -218,16 +212,17 @@ class Server:
                 still = []
                 for r in self.active:
                     if r.inflight:
-                        if not r.sim.ctx.poll():
+                        if not (r.future.done() and r.sim.ctx.poll()):
                             still.append(r)
                             continue
                         t = self._clock()
                         r.inflight = False
                         progressed = True
                         try:
+                            r.future.result()
                             r.sim.ctx.sync()
                         except MeltError as e:
-                            r.status = _terminal_status(r, e)
+                            r.status = terminal_status(r.sim.ctx, e)
                         r.device_ms.append((t - r.launched) * 1e3)
                         r.step_ms.append((t - r.last_done) * 1e3)
                         r.last_done = t
@@ -237,7 +232,7 @@ class Server:
                         self._retire(r, keep_results, results, log)
                         continue
                     r.launched = self._clock()
-                    r.sim.ctx.tschem()
+                    r.future = pool.submit(r.sim.ctx.tschem)
                     r.inflight = True
                     still.append(r)
                 self.active = still
@@ -249,6 +244,7 @@ class Server:
         finally:
             todo.put(None)
             worker.join()
+            pool.shutdown()
         wall = self._clock()
         inter = np.concatenate([np.array(r.step_ms) for r in self.done if r.step_ms] or [np.zeros(1)])
         dev = np.concatenate([np.array(r.device_ms) for r in self.done if r.device_ms] or [np.zeros(1)])

@@ -84,15 +84,15 @@ def test_serving_results_match_alone_and_budget_holds(gpu_lib):
 
 
 def test_diverged_request_ends_cheaply(gpu_lib):
-    """A 256^3 spheroid (aspect 2.7, 24500 faces) at St 3.1 and Re 273 without
-    remeshing: its melting front moves about a cell per step, the surface
+    """A 256^3 spheroid (aspect 2.7, 24500 faces) at St 3.1 and Re 273 with
+    remeshing off: its melting front moves about a cell per step, the surface
     tangles at step 29 and the volume check (particle.f90:218-224) ends the
     request, with a negative volume, so the server reports it as diverged.
     The blown-up coordinates put every thread of the tagging and forcing
     passes on the error word; that step costs as much as any other (measured
     4.2 ms, against 272 ms when each thread queued a compare-and-swap on it)."""
     m = Member(n=256, rad=0.5558868667770105, st=3.0961983157665, re=273.19648405043654, seed=6, aspect=2.6943004927317387,
-               freq=35, steps=45)
+               freq=35, steps=45, remesh=False)
     srv = Server(200e9, precision=1, lib=gpu_lib, device_inputs=False)     # the seeded host state that tangles
     rep = srv.run([Request(0, 0.0, m)])
     r = srv.done[0]

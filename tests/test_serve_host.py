@@ -2,7 +2,9 @@
 admission under the byte budget, and how a terminal device error ends one
 request without ending the server.  The simulations are stand-ins that count
 steps and raise the device's small-volume error (particle.f90:218-224) at a
-chosen step, with the volume the body then has."""
+chosen step, with the volume the body then has, or the window error, or the
+remeshing residual error (particle.f90:205-210), which tschem raises itself
+because the host edits the mesh inside it."""
 import math
 
 import pytest
@@ -23,12 +25,16 @@ class _Ctx:
 
     def tschem(self):
         self.steps += 1
+        if self.fail_at and self.steps == self.fail_at and self.volume_after == "remesh":
+            raise MeltError(6, "remesh_smooth: non-zero remeshing residual")
 
     def poll(self):
         return True
 
     def sync(self):
         if self.fail_at and self.steps == self.fail_at:
+            if self.volume_after == "window":
+                raise MeltError(9, "device: immersed body outgrew the device window")
             raise MeltError(4, "device: Geometry V/VE is smaller than threshold")
 
     def body(self):
@@ -70,15 +76,15 @@ def _member(seed, steps, n=64):
 
 def test_terminal_errors_end_only_their_request(fake_sims):
     # 4: the body cannot even be read back after the error
-    fake_sims.behaviour = {1: (2, 1e-7), 2: (3, -7.4e33), 3: (2, math.nan), 4: (1, None)}
-    reqs = [serve.Request(i, 0.0, _member(i, 5)) for i in range(5)]
+    fake_sims.behaviour = {1: (2, 1e-7), 2: (3, -7.4e33), 3: (2, math.nan), 4: (1, None), 5: (3, "remesh"), 6: (1, "remesh")}
+    reqs = [serve.Request(i, 0.0, _member(i, 5)) for i in range(7)]
     srv = serve.Server(budget_bytes=1e12, precision=1)
     rep = srv.run(reqs)
     status = {r.rid: r.status for r in srv.done}
-    assert status == {0: "ok", 1: "melted", 2: "diverged", 3: "diverged", 4: "diverged"}
+    assert status == {0: "ok", 1: "melted", 2: "diverged", 3: "diverged", 4: "diverged", 5: "remesh failed", 6: "remesh failed"}
     steps = {r.rid: r.done_steps for r in srv.done}
-    assert steps == {0: 5, 1: 2, 2: 3, 3: 2, 4: 1}
-    assert rep["status"] == {"ok": 1, "melted": 1, "diverged": 3}
+    assert steps == {0: 5, 1: 2, 2: 3, 3: 2, 4: 1, 5: 3, 6: 1}
+    assert rep["status"] == {"ok": 1, "melted": 1, "diverged": 3, "remesh failed": 2}
     assert srv.used == 0 and not srv.active
     assert all(s.closed for _, s in fake_sims.built)
 
@@ -207,16 +213,19 @@ def test_requests_advance_at_their_own_pace(fake_sims):
 
 
 def test_ensemble_retires_melted_members(monkeypatch):
-    """A member whose body melts below V_thresh ends there (particle.f90:218-224)
-    while the others keep stepping; any other device error ends the ensemble."""
+    """A member whose body melts below V_thresh ends there (particle.f90:218-224),
+    as "diverged" when its volume is negative, and one whose remeshing leaves a
+    residual ends there too (particle.f90:205-210), while the others keep
+    stepping; any other device error ends the ensemble."""
     from melt_ibm_amd import ensemble
     monkeypatch.setattr(ensemble, "Simulation", _Sim)
-    _Sim.behaviour, _Sim.built = {1: (3, 1e-7), 2: (5, 1e-7)}, []
-    ens = ensemble.Ensemble([_member(i, 0) for i in range(4)], precision=1, tune=False)
+    _Sim.behaviour, _Sim.built = {1: (3, 1e-7), 2: (5, -3.0), 3: (4, "window"), 4: (2, "remesh")}, []
+    ens = ensemble.Ensemble([_member(i, 0) for i in range(5)], precision=1, tune=False)
     st = ens.run(steps=6, warmup=0)
-    assert st["melted"] == {"1": 2, "2": 4}
-    assert [s.ctx.steps for s in ens.sims] == [6, 3, 5, 6]
-    assert sum(ens.step_cells) == _member(0, 0).cells * (6 + 3 + 5 + 6)
+    assert st["ended"] == {"1": {"step": 2, "status": "melted"}, "2": {"step": 4, "status": "diverged"},
+                           "3": {"step": 3, "status": "diverged"}, "4": {"step": 1, "status": "remesh failed"}}
+    assert [s.ctx.steps for s in ens.sims] == [6, 3, 5, 4, 2]
+    assert sum(ens.step_cells) == _member(0, 0).cells * (6 + 3 + 5 + 4 + 2)
     _Sim.behaviour, _Sim.built = {0: (2, None)}, []
 
     class _Broken(_Ctx):
