@@ -298,6 +298,11 @@ int melt_snapshot_load(melt_ctx* ctx, const void* buf, uint64_t bytes);
  * kernel.  melt_timing_read returns the accumulated milliseconds per kernel slot
  * and the number of samples; names from melt_timing_name. */
 int melt_timing_enable(melt_ctx* ctx, int32_t on);
+/* Time only the slots in slot_mask (bit s = slot s, s < melt_timing_count());
+ * 0 turns timing off.  Each timed slot adds two event records to the stream,
+ * so a caller that wants one kernel's duration over a timed loop selects that
+ * slot alone.  MELT_ERR_ARG for bits beyond the last slot. */
+int melt_timing_select(melt_ctx* ctx, uint32_t slot_mask);
 int melt_timing_reset(melt_ctx* ctx);
 int melt_timing_count(void);
 const char* melt_timing_name(int32_t slot);

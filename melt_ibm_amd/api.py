@@ -192,7 +192,7 @@ _SIGS = {
     "snapshot_save": (C.c_int, [_P, _P, C.c_uint64]),
     "snapshot_load": (C.c_int, [_P, _P, C.c_uint64]),
     "dissipation": (C.c_int, [_P, _P]),
-    "timing_enable": (C.c_int, [_P, C.c_int32]), "timing_reset": (C.c_int, [_P]),
+    "timing_enable": (C.c_int, [_P, C.c_int32]), "timing_select": (C.c_int, [_P, C.c_uint32]), "timing_reset": (C.c_int, [_P]),
     "timing_count": (C.c_int, []), "timing_name": (C.c_char_p, [C.c_int32]),
     "timing_read": (C.c_int, [_P, _P, _P]),
     "set_refinement": (C.c_int, [_P, C.c_int32, C.c_double]),
@@ -205,7 +205,7 @@ _SIGS = {
     "get_topology": (C.c_int, [_P] * 8),
     "get_remesh_stats": (C.c_int, [_P, _P]),
 }
-OPTIONAL = {"timing_enable", "timing_reset", "timing_count", "timing_name", "timing_read", "device_ok", "set_refinement",
+OPTIONAL = {"timing_enable", "timing_select", "timing_reset", "timing_count", "timing_name", "timing_read", "device_ok", "set_refinement",
             "get_refinement_stats", "graph_enable", "stream",
             "set_tuning", "poll", "upload_field_device", "snapshot_size", "snapshot_save", "snapshot_load"}
 
@@ -487,8 +487,17 @@ class MeltContext:
         return v.value
 
     # ------------------------------------------------------------ timing
-    def timing(self, on: bool = True) -> None:
-        self._chk(self._f["timing_enable"](self._ctx, 1 if on else 0))
+    def timing(self, on: bool = True, slots: tuple[str, ...] | None = None) -> None:
+        """HIP-event timing of every kernel slot, or of the named slots only
+        (each timed slot puts two event records on the stream)."""
+        if not on or slots is None:
+            self._chk(self._f["timing_enable"](self._ctx, 1 if on else 0))
+            return
+        names = [self._f["timing_name"](i).decode() for i in range(self._f["timing_count"]())]
+        mask = 0
+        for s in slots:
+            mask |= 1 << names.index(s)
+        self._chk(self._f["timing_select"](self._ctx, mask))
 
     def timing_reset(self) -> None:
         self._chk(self._f["timing_reset"](self._ctx))

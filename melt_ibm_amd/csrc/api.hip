@@ -86,9 +86,8 @@ bool tag_field(int field) {
 
 // --------------------------------------------------------------- timing
 void timing_begin(melt_ctx* ctx, int slot, hipEvent_t* ev) {
-  (void)slot;
   *ev = nullptr;
-  if (!ctx->timing) return;
+  if (!ctx->timing || !((ctx->timing_mask >> slot) & 1u)) return;
   if (ctx->ev_next >= ctx->ev_pool.size()) {
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return;
@@ -134,6 +133,8 @@ extern "C" int melt_device_ok(void) {
 extern "C" void melt_ctx_destroy(melt_ctx* ctx) {
   if (!ctx) return;
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  if (ctx->side) hipStreamSynchronize(ctx->side);   // work left there after an error
+  if (ctx->side2) hipStreamSynchronize(ctx->side2);
   graph_drop(ctx);
   for (int i = 0; i < MELT_NFIELDS; ++i) if (ctx->f[i]) hipFree(ctx->f[i]);
   for (int i = 1; i < 4; ++i) if (ctx->rhs4[i]) hipFree(ctx->rhs4[i]);
@@ -150,8 +151,10 @@ extern "C" void melt_ctx_destroy(melt_ctx* ctx) {
   if (ctx->ev_rm) hipEventDestroy(ctx->ev_rm);
   for (auto e : ctx->ev_pool) hipEventDestroy(e);
   if (ctx->side) hipStreamSynchronize(ctx->side);
-  for (hipEvent_t e : {ctx->ev_fork, ctx->ev_join}) if (e) hipEventDestroy(e);
+  if (ctx->side2) hipStreamSynchronize(ctx->side2);
+  for (hipEvent_t e : {ctx->ev_fork, ctx->ev_join, ctx->ev_fork2, ctx->ev_join2, ctx->ev_forc, ctx->ev_hit}) if (e) hipEventDestroy(e);
   if (ctx->side) hipStreamDestroy(ctx->side);
+  if (ctx->side2) hipStreamDestroy(ctx->side2);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -208,7 +211,11 @@ extern "C" int melt_ctx_create(const melt_config* cfg, melt_ctx** out) {
     // over all queues only if no side streams are interleaved among them
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_fork2, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_join2, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_forc, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_hit, hipEventDisableTiming) != hipSuccess) {
       rc = MELT_ERR_HIP;
       break;
     }
@@ -317,6 +324,7 @@ static int copy_from_device(melt_ctx* ctx, int field, double* dst) {
 }
 
 extern "C" int melt_upload_field(melt_ctx* ctx, int field, const double* host, const melt_layout* lay) {
+  if (ctx) ctx->forc_free = 0;   // it may touch the forcing on the main stream
   if (check_ctx(ctx) || field < 0 || field >= MELT_NFIELDS || !host) return MELT_ERR_ARG;
   if (tag_field(field)) cancel_ahead(ctx);
   melt_layout L = lay ? *lay : melt_layout{ctx->n1, ctx->n2, 0, 0};
@@ -332,6 +340,7 @@ extern "C" int melt_upload_field(melt_ctx* ctx, int field, const double* host, c
 }
 
 extern "C" int melt_download_field(melt_ctx* ctx, int field, double* host, const melt_layout* lay) {
+  if (ctx) ctx->forc_free = 0;   // it may touch the forcing on the main stream
   if (check_ctx(ctx) || field < 0 || field >= MELT_NFIELDS || !host) return MELT_ERR_ARG;
   melt_layout L = lay ? *lay : melt_layout{ctx->n1, ctx->n2, 0, 0};
   if (L.ld1 < ctx->n1 || L.ld2 < ctx->n2) return melt_fail(ctx, MELT_ERR_ARG, "download", "leading dimension too small");
@@ -359,6 +368,7 @@ __global__ void k_from_double(R* __restrict__ dst, const double* __restrict__ sr
 // (i fastest) that is complete before the call; the precision form stores
 // them rounded to float.  Returns after the copy has finished.
 extern "C" int melt_upload_field_device(melt_ctx* ctx, int field, const double* src) {
+  if (ctx) ctx->forc_free = 0;   // it may touch the forcing on the main stream
   if (check_ctx(ctx) || field < 0 || field >= MELT_NFIELDS || !src) return MELT_ERR_ARG;
   if (tag_field(field)) cancel_ahead(ctx);
   if (ctx->fp32) hipLaunchKernelGGL(k_from_double<float>, dim3(2048), dim3(256), 0, ctx->stream, fld<float>(ctx, field), src, ctx->nc);
@@ -638,8 +648,13 @@ static bool one_stream(const melt_ctx* ctx) { return ctx->tune[TUNE_STREAMS] == 
 // (beside the pressure solve), 2 = on the side stream at the start of the
 // substep that uses it (beside the explicit pass).
 static int coef_place(const melt_ctx* ctx) { return ctx->tune[TUNE_COEF] ? ctx->tune[TUNE_COEF] : 1; }
+bool capturing(melt_ctx* ctx) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(ctx->stream, &st) != hipSuccess || st != hipStreamCaptureStatusNone;
+}
 static int ensure_side(melt_ctx* ctx) {
   if (!ctx->side && !one_stream(ctx)) MELT_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+  if (!ctx->side2 && !one_stream(ctx)) MELT_HIP(hipStreamCreateWithFlags(&ctx->side2, hipStreamNonBlocking));
   return 0;
 }
 template <class F>
@@ -681,20 +696,57 @@ static int side_continue(melt_ctx* ctx, F&& work) {
 static int side_join(melt_ctx* ctx) {
   if (one_stream(ctx)) return 0;
   MELT_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+  if (ctx->join2_pending) {
+    MELT_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join2, 0));
+    ctx->join2_pending = 0;
+  }
+  return 0;
+}
+// Queue `work`, which writes only the forcing and its own state, on the third
+// stream from the last explicit pass's read of the forcing (ev_forc) when
+// nothing on the main stream has touched the forcing since; the main stream
+// waits for it.  Otherwise on the main stream.
+int run_after_forcing_read(melt_ctx* ctx, const std::function<int()>& work) {
+  if (!ctx->forc_free || one_stream(ctx) || !ctx->side2 || capturing(ctx)) return work();
+  ctx->forc_free = 0;
+  MELT_HIP(hipStreamWaitEvent(ctx->side2, ctx->ev_forc, 0));
+  hipStream_t cur = ctx->stream;
+  ctx->stream = ctx->side2;
+  int rc = work();
+  ctx->stream = cur;
+  MELT_HIP(hipEventRecord(ctx->ev_hit, ctx->side2));
+  MELT_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_hit, 0));
+  return rc;
+}
+// Queue `work` on the third stream, forked from the stream now current (the
+// side stream, inside side_continue); side_join waits for it.
+template <class F>
+static int side2_run(melt_ctx* ctx, F&& work) {
+  MELT_HIP(hipEventRecord(ctx->ev_fork2, ctx->stream));
+  MELT_HIP(hipStreamWaitEvent(ctx->side2, ctx->ev_fork2, 0));
+  hipStream_t cur = ctx->stream;
+  ctx->stream = ctx->side2;
+  int rc = work();
+  ctx->stream = cur;
+  if (rc) return rc;
+  MELT_HIP(hipEventRecord(ctx->ev_join2, ctx->side2));
+  ctx->join2_pending = 1;
   return 0;
 }
 
 // One fused substep.  With tag_next (tschem's first two substeps, where no
 // caller code runs before the next one) the tagging of the next substep is
-// queued on the side stream right after this substep's Lagrangian update: it
-// reads only the geometry that update writes and writes only VOFp, sdf, its
-// own scratch and the window box, none of which the pressure phase touches,
-// so it overlaps the pressure solve; so does, with coef_next (every substep
-// of tschem, also for the next call's first substep: a caller's change of the
-// faces or body in between cancels it), the geometry part of the next
-// substep's gradient probes (it writes only its coefficient buffer).  The
-// tagging itself is not queued across calls: VOFp and sdf after tschem hold
-// the last substep's tagging, as in the reference.
+// queued during this substep's Lagrangian update, on a third stream forked
+// as soon as the geometry refresh has written the face centroids and normals:
+// it reads only the vertices and faces and writes only VOFp, sdf, its own
+// scratch and the window box, none of which the rest of the refresh, the
+// probe stage or the pressure phase touches, so it overlaps them all; so
+// does, with coef_next (every substep of tschem, also for the next call's
+// first substep: a caller's change of the faces or body in between cancels
+// it), the geometry part of the next substep's gradient probes (side stream,
+// after the refresh; it writes only its coefficient buffer).  The tagging
+// itself is not queued across calls: VOFp and sdf after tschem hold the last
+// substep's tagging, as in the reference.
 static int substep_impl(melt_ctx* ctx, int32_t ns, bool tag_next, bool coef_next) {
   if (check_ctx(ctx)) return MELT_ERR_ARG;
   int rc = melt_set_stage(ctx, ns);
@@ -728,6 +780,10 @@ static int substep_impl(melt_ctx* ctx, int32_t ns, bool tag_next, bool coef_next
   timing_begin(ctx, SLOT_EXPLICIT, &e);
   if ((rc = euler_launch_explicit_xsolve(ctx, betadx_q(ctx, 0, 0), betadx_q(ctx, 3, 0)))) return rc;
   timing_end(ctx, SLOT_EXPLICIT, e);
+  if (ctx->hit_nm && !one_stream(ctx) && !capturing(ctx)) {
+    MELT_HIP(hipEventRecord(ctx->ev_forc, ctx->stream));
+    ctx->forc_free = 1;
+  }
   void* r[4] = {ctx->rhs4[RHS_U], ctx->rhs4[RHS_V], ctx->rhs4[RHS_W], ctx->rhs4[RHS_T]};
   void* q[4] = {ctx->f[MELT_VX], ctx->f[MELT_VY], ctx->f[MELT_VZ], ctx->f[MELT_TEMP]};
   double b[4], bz[4];
@@ -753,17 +809,24 @@ static int substep_impl(melt_ctx* ctx, int32_t ns, bool tag_next, bool coef_next
 
   timing_begin(ctx, SLOT_PH_MLS, &e_ph);
   // what the side stream queues after the Lagrangian update: the next
-  // substep's tagging (tag_next) and probe geometry stage (coef_next)
+  // substep's tagging (tag_next) and probe geometry stage (coef_next).  The
+  // tagging needs only the refreshed faces (positions, centroids, normals):
+  // with a second side stream it forks there as soon as the refresh has
+  // written them, beside the rest of the refresh and the probe stage
+  auto tag_now = [&]() -> int {
+    hipEvent_t et = nullptr;   // its own event: it may run inside the Lagrangian slot
+    timing_begin(ctx, SLOT_TAG, &et);
+    ctx->force_window_from_tag = 1;
+    int r = ibm_tag(ctx);
+    ctx->force_window_from_tag = 0;
+    timing_end(ctx, SLOT_TAG, et);
+    if (!r) ctx->tag_ahead = 1;
+    return r;
+  };
+  const std::function<int()> tag_fork = [&]() -> int { return side2_run(ctx, tag_now); };
   auto queue_next = [&]() -> int {
     int r = 0;
-    if (tag_next && moving) {
-      timing_begin(ctx, SLOT_TAG, &e);
-      ctx->force_window_from_tag = 1;
-      r = ibm_tag(ctx);
-      ctx->force_window_from_tag = 0;
-      timing_end(ctx, SLOT_TAG, e);
-      if (!r) ctx->tag_ahead = 1;
-    }
+    if (tag_next && moving && !ctx->tag_ahead) r = tag_now();
     if (!r && coef_next && probes) {
       r = ibm_probe_coefs(ctx);
       if (!r) ctx->coef_ahead = 1;
@@ -823,7 +886,9 @@ static int substep_impl(melt_ctx* ctx, int32_t ns, bool tag_next, bool coef_next
   // next substep's tagging and probe geometry stage
   if (ibm && (rc = side_continue(ctx, [&] {
                 timing_begin(ctx, SLOT_LAGR, &e);
+                if (tag_next && moving && !one_stream(ctx)) ctx->after_faces = &tag_fork;
                 int r = lagr_after_force(ctx, true);
+                ctx->after_faces = nullptr;
                 timing_end(ctx, SLOT_LAGR, e);
                 return r ? r : queue_next();
               })))
@@ -921,9 +986,13 @@ extern "C" int melt_inqpr(melt_ctx* ctx) {
 
 // ------------------------------------------------------------ timing API
 extern "C" int melt_timing_enable(melt_ctx* ctx, int32_t on) {
-  if (check_ctx(ctx)) return MELT_ERR_ARG;
-  if (!on) timing_flush(ctx);
-  ctx->timing = on ? 1 : 0;
+  return melt_timing_select(ctx, on ? (1u << SLOT_COUNT) - 1u : 0u);
+}
+extern "C" int melt_timing_select(melt_ctx* ctx, uint32_t slot_mask) {
+  if (check_ctx(ctx) || (slot_mask >> SLOT_COUNT)) return MELT_ERR_ARG;
+  if (!slot_mask) timing_flush(ctx);
+  ctx->timing = slot_mask ? 1 : 0;
+  ctx->timing_mask = slot_mask;
   return 0;
 }
 extern "C" int melt_timing_reset(melt_ctx* ctx) {
@@ -1031,6 +1100,7 @@ extern "C" int melt_snapshot_size(melt_ctx* ctx, uint64_t* bytes) {
 }
 
 extern "C" int melt_snapshot_save(melt_ctx* ctx, void* buf, uint64_t bytes) {
+  if (ctx) ctx->forc_free = 0;   // it may touch the forcing on the main stream
   if (check_ctx(ctx) || !buf) return MELT_ERR_ARG;
   uint64_t need = 0;
   melt_snapshot_size(ctx, &need);
@@ -1064,6 +1134,7 @@ extern "C" int melt_snapshot_save(melt_ctx* ctx, void* buf, uint64_t bytes) {
 }
 
 extern "C" int melt_snapshot_load(melt_ctx* ctx, const void* buf, uint64_t bytes) {
+  if (ctx) ctx->forc_free = 0;   // it may touch the forcing on the main stream
   if (check_ctx(ctx) || !buf || bytes < sizeof(SnapHeader)) return MELT_ERR_ARG;
   SnapHeader h;
   memcpy(&h, buf, sizeof h);

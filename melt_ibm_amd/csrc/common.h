@@ -8,6 +8,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -69,6 +70,22 @@ struct melt_ctx {
   // concurrently with the pressure phase (fork / join events)
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // third stream: the next substep's tagging, forked from the side stream as
+  // soon as the faces are refreshed, so that it runs beside the rest of the
+  // geometry refresh and the probe geometry stage; joined with the side stream
+  hipStream_t side2 = nullptr;
+  hipEvent_t ev_fork2 = nullptr, ev_join2 = nullptr;
+  int join2_pending = 0;
+  // called once by the next geometry refresh right after the face centroids
+  // and normals are written (set by the fused substep, consumed by the call)
+  const std::function<int()>* after_faces = nullptr;
+  // HIT forcing synthesis beside the end of the step: ev_forc is recorded on
+  // the main stream after the last explicit pass (the last reader of
+  // forcx/y/z); while forc_free is set, nothing queued on the main stream
+  // after it touches the forcing, so melt_hit_force may run on side2 from
+  // ev_forc on (the main stream then waits for ev_hit)
+  hipEvent_t ev_forc = nullptr, ev_hit = nullptr;
+  int forc_free = 0;
   // graph replay of one RK3 step (melt_graph_enable): captured on first use,
   // dropped whenever a host-side launch parameter (dt, refinement) changes
   int use_graph = 0;
@@ -168,6 +185,7 @@ struct melt_ctx {
 
   // timing
   int timing = 0;
+  uint32_t timing_mask = 0;   // slots timed while timing is on (bit s = slot s)
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
   size_t ev_next = 0;
@@ -506,6 +524,8 @@ int ibm_window_reset(melt_ctx* ctx);
 // Newton-Euler update (waiting for the queue up to there) and remeshes on the
 // host before the geometry refresh.
 int lagr_after_force(melt_ctx* ctx, bool commit_pind = false);
+bool capturing(melt_ctx* ctx);
+int run_after_forcing_read(melt_ctx* ctx, const std::function<int()>& work);
 // host remeshing of the context's body (remesh.hip): *did = 1 when it ran
 int remesh_context(melt_ctx* ctx, int* did);
 // host mirrors from the device connectivity (after a snapshot load)

@@ -264,6 +264,7 @@ EulerArgs<R> make_args(melt_ctx* ctx) {
   A.vx = fld<R>(ctx, MELT_VX); A.vy = fld<R>(ctx, MELT_VY); A.vz = fld<R>(ctx, MELT_VZ);
   A.temp = fld<R>(ctx, MELT_TEMP); A.pr = fld<R>(ctx, MELT_PR); A.vofp = fld<R>(ctx, MELT_VOFP);
   A.forcx = fld<R>(ctx, MELT_FORCX); A.forcy = fld<R>(ctx, MELT_FORCY); A.forcz = fld<R>(ctx, MELT_FORCZ);
+  ctx->forc_free = 0;   // a pass that reads the forcing is being queued
   A.body = ctx->d_body;
   A.coord = ctx->d_coord;
   const bool faces = ctx->cfg.tagtype == 0;

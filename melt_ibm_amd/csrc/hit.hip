@@ -192,6 +192,7 @@ int hit_free(melt_ctx* ctx) {
 
 // InitRandomForce (hit.f90:128-232) with the forcing parameters of HITForcing.in.
 extern "C" int melt_hit_init(melt_ctx* ctx, double tl, double epsstar, double kf_on_kmin, uint64_t seed) {
+  if (ctx) ctx->forc_free = 0;
   if (!ctx || !(tl > 0.0) || !(epsstar >= 0.0) || !(kf_on_kmin > 0.0)) return MELT_ERR_ARG;
   const int nbuf = (int)floor(kf_on_kmin) + 1;
   const int nm = (nbuf - 1) * 2 + 1;
@@ -286,9 +287,14 @@ extern "C" int melt_hit_force(melt_ctx* ctx) {
   a.twopi_y = 2.0 * ctx->pi / ctx->cfg.ylen;
   a.twopi_z = 2.0 * ctx->pi / ctx->cfg.zlen;
   a.seed = ctx->hit_seed;
-  hipLaunchKernelGGL(k_hit_ou, dim3(1), dim3(256), 0, ctx->stream, a, ctx->d_hit_act, ctx->d_hit_bhat, ctx->d_hit_fhat, ctx->d_hit_count);
-  MELT_HIP(hipGetLastError());
-  return ctx->fp32 ? launch_hit_synth<float>(ctx) : launch_hit_synth<double>(ctx);
+  // the synthesis writes only forcx/y/z and the generator's state: after a
+  // tschem it starts when the last explicit pass has read the forcing, beside
+  // the rest of that step
+  return run_after_forcing_read(ctx, [&]() -> int {
+    hipLaunchKernelGGL(k_hit_ou, dim3(1), dim3(256), 0, ctx->stream, a, ctx->d_hit_act, ctx->d_hit_bhat, ctx->d_hit_fhat, ctx->d_hit_count);
+    MELT_HIP(hipGetLastError());
+    return ctx->fp32 ? launch_hit_synth<float>(ctx) : launch_hit_synth<double>(ctx);
+  });
 }
 
 // bhat(3, nmodes, nmodes, nmodes) as (re, im) pairs and the number of OU

@@ -559,6 +559,12 @@ int refresh_geometry(melt_ctx* ctx, LagArgs& A) {
   const double* x = ctx->d_lag[MELT_L_XYZV];
   hipStream_t s = ctx->stream;
   hipLaunchKernelGGL(k_face_geom, dim3(nb(ctx->nf, 256)), dim3(256), 0, s, A, x, ctx->d_body);
+  // work that needs only the refreshed faces (the fused substep's next
+  // tagging) forks here; the rest of the refresh does not touch what it reads
+  if (const std::function<int()>* hook = ctx->after_faces) {
+    ctx->after_faces = nullptr;
+    if (int rc = (*hook)()) return rc;
+  }
   hipLaunchKernelGGL(k_edge_len, dim3(nb(ctx->ne, 256)), dim3(256), 0, s, A, x, ctx->d_body, ctx->h_body.e_thresh);
   hipLaunchKernelGGL(k_face_vert_post, dim3(nb((size_t)ctx->nf + ctx->nv, 256)), dim3(256), 0, s, A, ctx->d_body, ctx->cfg.imelt == 1 ? 1 : 0);
   face_sums(ctx, A, x, (int)SUM_AREA_VOL_RIGID, ctx->d_red, FIN_FINAL);

@@ -13,7 +13,8 @@ to 64) or level-4 icosahedron, St uniform in [0.5, 4], Re uniform in
 [200, 1000], seeded random-phase initial velocity (|k| <= 4, rms 0.1) from the
 member's own seed, T = 1 in the fluid and 0 in the solid (inqpr), no HIT
 forcing, dt from CFL 0.5, FP32 bulk with FP64 Lagrangian updates, remeshing
-on (E_thresh at 0.95 of the shortest initial edge).
+on (E_thresh = 0.7 h as in the reference's deck, or 0.95 of the shortest
+initial edge where that is shorter).
 """
 from __future__ import annotations
 
@@ -65,7 +66,7 @@ class Member:
     aspect: float = 1.0     # spheroid c/a (1 = sphere), volume-equivalent radius rad
     freq: int = 0           # icosphere frequency (faces = 20 freq^2); 0: 2**level
     steps: int = 0          # RK3 steps to run (serving)
-    remesh: bool = True     # IREMESH = 1, E_thresh just below the shortest initial edge
+    remesh: bool = True     # IREMESH = 1, E_thresh = min(0.7 h, 0.95 x shortest initial edge)
 
     @property
     def cells(self) -> int:
@@ -73,14 +74,15 @@ class Member:
 
     def config(self, precision: int) -> MeltConfig:
         # latHeat = cp (Tliq - Tmelt) / St with cp = 1, Tliq = 1, Tmelt = 0.  With
-        # remeshing, PERCT_ETHRESH puts E_thresh at 0.95 of the shortest initial
-        # edge: the reference's deck likewise starts its edges above E_thresh, and
-        # edges that the melting shrinks are collapsed from there on.
+        # remeshing, E_thresh is the reference deck's 0.7 h (PERCT_ETHRESH,
+        # part.in), lowered to 0.95 of the shortest initial edge on a finer mesh:
+        # the deck's edges start above E_thresh, and edges that the melting
+        # shrinks are collapsed from there on.
         perc = 0.7
         if self.remesh:
             m = self.mesh()
             d = m.xyz[m.vert_of_edge[:, 0] - 1] - m.xyz[m.vert_of_edge[:, 1] - 1]
-            perc = 0.95 * float(np.sqrt((d * d).sum(axis=1)).min()) * self.rad / (2.0 * np.pi / self.n)
+            perc = min(0.7, 0.95 * float(np.sqrt((d * d).sum(axis=1)).min()) * self.rad / (2.0 * np.pi / self.n))
         return MeltConfig(n1m=self.n, n2m=self.n, n3m=self.n, ren=self.re, prandtl=1.0, betagz=0.0, tmelt=0.0, tliq=1.0, tsol=0.0,
                           cpliquid=1.0, latheat=1.0 / self.st, dens_ratio=1.0, imlsfor=1, imlsstr=1, imelt=1,
                           iremesh=1 if self.remesh else 0, forcing=0, precision=precision, perc_ethresh=perc)

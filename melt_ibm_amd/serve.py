@@ -27,8 +27,9 @@ the set-up time (admission to hand-over).
 
 The request stream (python -m melt_ibm_amd.serve): Poisson arrivals, mean 4
 requests/s, seed 7; grids from {64, 128, 256, 384}^3; spheres or spheroids of
-aspect 1-3 whose triangle count grows with the grid (2k to 60k); St in
-[0.5, 4]; 50-500 RK3 steps; 200 GB budget.
+aspect 1-3 meshed for a mean edge of about 0.9 h (2k to 60k triangles, as the
+reference's deck relates its mesh to its grid); St in [0.5, 4]; 50-500 RK3
+steps; 200 GB budget.
 """
 from __future__ import annotations
 
@@ -43,11 +44,26 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+from . import geometry
 from .api import MeltError
 from .ensemble import Member, Simulation, terminal_status
 
 SERVE_GRIDS = (64, 128, 256, 384)
-FREQ_FOR_GRID = {64: 10, 128: 20, 256: 35, 384: 55}   # 2000, 8000, 24500, 60500 faces
+FREQ_RANGE = (10, 55)   # icosphere frequencies of the stream's meshes: 2000 to 60500 faces
+EDGE_ON_H = 0.9         # mean edge / grid spacing the mesh is sized for
+
+
+def freq_for(n: int, rad: float, aspect: float) -> int:
+    """Icosphere frequency whose mesh has a mean edge of about 0.9 h on an n^3
+    grid of the 2 pi cube (the reference's deck, part.in: a level-5 sphere of
+    radius 0.2 on a 128^3 unit cube, edges near 0.9 h with E_thresh = 0.7 h),
+    within the stream's 2000-60500 faces.  Edges scale as 1/frequency."""
+    a = aspect ** (-1.0 / 3.0)
+    m = geometry.icosphere(10) if aspect == 1.0 else geometry.spheroid(a, aspect * a, 10)
+    d = m.xyz[m.vert_of_edge[:, 0] - 1] - m.xyz[m.vert_of_edge[:, 1] - 1]
+    mean10 = float(np.sqrt((d * d).sum(axis=1)).mean()) * rad
+    f = int(round(10.0 * mean10 / (EDGE_ON_H * 2.0 * np.pi / n)))
+    return min(max(f, FREQ_RANGE[0]), FREQ_RANGE[1])
 
 
 def request_bytes(m: Member, precision: int) -> int:
@@ -96,8 +112,9 @@ def request_stream(duration_s: float, rate: float = 4.0, seed: int = 7, steps_sc
             break
         n = int(rng.choice(grids))
         aspect = 1.0 if rng.random() < 0.5 else float(rng.uniform(1.0, 3.0))
-        m = Member(n=n, rad=float(rng.uniform(0.3, 0.7)), st=float(rng.uniform(0.5, 4.0)), re=float(rng.uniform(200.0, 1000.0)),
-                   seed=len(out), aspect=aspect, freq=FREQ_FOR_GRID.get(n, 10),
+        rad = float(rng.uniform(0.3, 0.7))
+        m = Member(n=n, rad=rad, st=float(rng.uniform(0.5, 4.0)), re=float(rng.uniform(200.0, 1000.0)),
+                   seed=len(out), aspect=aspect, freq=freq_for(n, rad, aspect),
                    steps=max(1, int(round(rng.integers(50, 501) * steps_scale))))
         out.append(Request(len(out), t, m))
     return out

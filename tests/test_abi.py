@@ -30,7 +30,7 @@ def test_oracle_mirrors_the_abi():
     lib = oracle.load()
     # device-side services, and the precision form's controls (the oracle is the
     # FP64 reference path, it has no FP32 form to refine)
-    skip = {"melt_device_ok", "melt_timing_enable", "melt_timing_reset", "melt_timing_count", "melt_timing_name", "melt_timing_read",
+    skip = {"melt_device_ok", "melt_timing_enable", "melt_timing_select", "melt_timing_reset", "melt_timing_count", "melt_timing_name", "melt_timing_read",
             "melt_set_refinement", "melt_get_refinement_stats", "melt_graph_enable", "melt_stream",
             "melt_set_tuning", "melt_upload_field_device", "melt_divgck_noalloc",
             "melt_snapshot_size", "melt_snapshot_save", "melt_snapshot_load"}

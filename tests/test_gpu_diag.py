@@ -53,3 +53,38 @@ def test_dissipation_precision_form(gpu_lib):
     for k in out[0]:
         if k != "sumvof":
             assert abs(out[1][k] - out[0][k]) <= 1e-6 * abs(out[0][k]), k
+
+
+def test_timing_of_selected_slots(gpu_lib):
+    """melt_timing_select times only the slots in its mask (the bench times
+    the dominant kernel alone over its timed loop); melt_timing_enable times
+    every slot; timing changes no result."""
+    from tests.helpers import melt_case
+    cfg = make_cfg(32)
+    st, dt = melt_case(cfg, 2, 1.2, seed=3)
+    a, b = MeltContext(cfg, lib=gpu_lib), MeltContext(cfg, lib=gpu_lib)
+    for c in (a, b):
+        load_state(c, st)
+        with_body(c, 2, 1.2)
+        c.set_dt(dt)
+        c.tagCells()
+    a.timing(True, slots=("explicit_xsolve",))
+    a.timing_reset()
+    a.tschem()
+    b.tschem()
+    tm = a.timing_read()
+    assert tm["explicit_xsolve"][1] == 3 and tm["explicit_xsolve"][0] > 0.0
+    assert all(n == 0 for k, (_, n) in tm.items() if k != "explicit_xsolve"), tm
+    a.timing(True)
+    a.timing_reset()
+    a.tschem()
+    b.tschem()
+    tm = a.timing_read()
+    a.timing(False)
+    assert tm["substep"][1] == 3 and tm["project"][1] == 3 and tm["explicit_xsolve"][1] == 3
+    a.sync(); b.sync()
+    for q in ("vx", "vy", "vz", "temp", "pr"):
+        assert np.array_equal(a.download(q), b.download(q)), q
+    with pytest.raises(Exception):
+        a._chk(a._f["timing_select"](a._ctx, 1 << 31))
+    a.close(); b.close()
