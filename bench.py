@@ -399,6 +399,7 @@ def main():
     ap.add_argument("--ens-steps", type=int, default=100, help="timed ensemble steps")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-window", type=float, default=4.0, help="arrival window of the serving leg (s)")
+    ap.add_argument("--variant-remesh", choices=("on", "off"), default="on", help="remeshing in the ensemble and serving legs")
     ap.add_argument("--serve-steps-scale", type=float, default=1.0, help="scale of the 50-500 step counts in the serving leg")
     ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_DB", "WRITE_DB", "COPY_FETCH_DB", "COPY_WRITE_DB"),
                     help="write the HBM-traffic summary under profiles/ from rocprofv3 --pmc databases")
@@ -511,7 +512,7 @@ def main():
         # batched form: 48 melting simulations of uneven grids, graph-replayed
         from melt_ibm_amd.ensemble import Ensemble, ensemble_members
         print("# ensemble: building 48 simulations", file=sys.stderr, flush=True)
-        ens = Ensemble(ensemble_members(48, seed=0), precision=1, tune=True)
+        ens = Ensemble(ensemble_members(48, seed=0, remesh=args.variant_remesh == "on"), precision=1, tune=True)
         st = ens.run(steps=args.ens_steps, warmup=2)
         ens.close()
         del ens
@@ -522,7 +523,8 @@ def main():
         print(f"# ensemble: p50 {st['step_ms_p50']:.2f} ms, p99 {st['step_ms_p99']:.2f} ms", file=sys.stderr, flush=True)
     if not args.no_serve:
         from melt_ibm_amd.serve import Server, request_stream
-        reqs = request_stream(args.serve_window, rate=4.0, seed=7, steps_scale=args.serve_steps_scale)
+        reqs = request_stream(args.serve_window, rate=4.0, seed=7, steps_scale=args.serve_steps_scale,
+                              remesh=args.variant_remesh == "on")
         print(f"# serving: {len(reqs)} requests over {args.serve_window} s", file=sys.stderr, flush=True)
         rep = Server(200e9, precision=1, tune=False).run(reqs, log=sys.stderr)
         scale = "" if args.serve_steps_scale == 1.0 else f" x {args.serve_steps_scale}"

@@ -140,7 +140,7 @@ extern "C" void melt_ctx_destroy(melt_ctx* ctx) {
   for (int i = 1; i < 4; ++i) if (ctx->rhs4[i]) hipFree(ctx->rhs4[i]);
   for (int i = 0; i < 4; ++i) if (ctx->tmp4[i]) hipFree(ctx->tmp4[i]);
   void* ps[] = {ctx->spec, ctx->ak1, ctx->ak2, ctx->ak3, ctx->tw1, ctx->tw2, ctx->tw3, ctx->d_body, ctx->d_vof, ctx->d_eof, ctx->d_voe,
-                ctx->d_v2f_ptr, ctx->d_v2f, ctx->d_gv, ctx->d_ge, ctx->d_gf, ctx->d_pind, ctx->d_pind2, ctx->d_probe, ctx->d_rxp, ctx->d_rxt, ctx->d_red,
+                ctx->d_v2f_ptr, ctx->d_v2f, ctx->d_gv, ctx->d_ge, ctx->d_gf, ctx->d_pind, ctx->d_pind2, ctx->d_probe, ctx->d_rxp, ctx->d_rxt, ctx->d_red, ctx->d_sbox,
                 ctx->d_vbox, ctx->d_slots, ctx->d_coord, ctx->d_acc, ctx->d_fmean, ctx->d_tkey, ctx->d_tface, ctx->d_tdist, ctx->d_pcoef, ctx->d_win, ctx->d_pcount, ctx->d_plist, ctx->d_err, ctx->d_gate, ctx->d_refine_stats,
                 ctx->d_foe, ctx->d_xyz_melt};
   for (void* p : ps) if (p) hipFree(p);
@@ -231,7 +231,7 @@ extern "C" int melt_ctx_create(const melt_config* cfg, melt_ctx** out) {
       if (hipMalloc(&ctx->tmp4[i], ctx->esz * ctx->nc) != hipSuccess) rc = melt_fail(ctx, MELT_ERR_HIP, "ctx_create", "rhs allocation");
     if (rc) break;
     if (hipMalloc(&ctx->d_body, sizeof(melt_body)) != hipSuccess || hipMemset(ctx->d_body, 0, sizeof(melt_body)) != hipSuccess ||
-        hipMalloc(&ctx->d_red, sizeof(double) * 16384) != hipSuccess || hipMalloc(&ctx->d_fmean, sizeof(double) * 4) != hipSuccess ||
+        hipMalloc(&ctx->d_red, sizeof(double) * 16384) != hipSuccess || hipMalloc(&ctx->d_sbox, sizeof(double) * 8) != hipSuccess || hipMalloc(&ctx->d_fmean, sizeof(double) * 4) != hipSuccess ||
         hipMemset(ctx->d_fmean, 0, sizeof(double) * 4) != hipSuccess || hipMalloc(&ctx->d_err, sizeof(int)) != hipSuccess ||
         hipMemset(ctx->d_err, 0, sizeof(int)) != hipSuccess || hipHostMalloc(&ctx->h_err, sizeof(int)) != hipSuccess ||
         hipMalloc(&ctx->d_win, sizeof(melt_window) * 2) != hipSuccess || hipMemset(ctx->d_win, 0, sizeof(melt_window) * 2) != hipSuccess ||

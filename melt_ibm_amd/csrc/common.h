@@ -142,6 +142,7 @@ struct melt_ctx {
   double* d_probe = nullptr;     // PRB_N * nf
   double *d_rxp = nullptr, *d_rxt = nullptr;  // r_x_prn_tri, r_x_tau_n (3*nf)
   double* d_red = nullptr;       // scratch for deterministic reductions
+  double* d_sbox = nullptr;      // vertex bounding box of the tagging (its own: the tagging may run beside the reductions)
   int* d_vbox = nullptr;         // vertex bounding box (6 ints) for ray tagging
   unsigned long long* d_slots = nullptr;  // REDUCE_SLOTS partial maxima (zero between uses)
   double* d_coord = nullptr;     // xc, xm, yc, ym, zc, zm tables (coord_off)

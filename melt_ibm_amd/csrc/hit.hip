@@ -114,13 +114,16 @@ __global__ __launch_bounds__(256) void k_hit_xsum(int n1, const double* __restri
   ax[2 * t + 1] = gi;
 }
 
+//
+// A block covers 256 columns of one segment (bps blocks per segment), so the
+// plane index is the same for the whole block and the z table is read with
+// scalar loads.
 template <typename R, int NM>
 __global__ __launch_bounds__(256) void k_hit_synth(Grid g, const double* __restrict__ ax, const double* __restrict__ tab,
-                                                   R* __restrict__ fx, R* __restrict__ fy, R* __restrict__ fz) {
-  const int cols = g.n1 * g.n2, nseg = (g.n3 + HIT_ZSEG - 1) / HIT_ZSEG;
-  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (t >= cols * nseg) return;
-  const int col = t % cols, seg = t / cols;
+                                                   R* __restrict__ fx, R* __restrict__ fy, R* __restrict__ fz, int bps) {
+  const int cols = g.n1 * g.n2;
+  const int seg = (int)blockIdx.x / bps, col = ((int)blockIdx.x % bps) * 256 + (int)threadIdx.x;
+  if (col >= cols) return;
   const int i = col % g.n1, j = col / g.n1;
   const int n1 = g.n1, n2 = g.n2, n3 = g.n3;
   const double* ty[3] = {tab + 2 * NM * (2 * n1) + 2 * NM * n2, tab + 2 * NM * (2 * n1), tab + 2 * NM * (2 * n1) + 2 * NM * n2};  // ym, yc, ym
@@ -251,11 +254,11 @@ extern "C" int melt_hit_init(melt_ctx* ctx, double tl, double epsstar, double kf
 template <typename R, int NM>
 static int launch_hit_synth_nm(melt_ctx* ctx) {
   const int nseg = (ctx->n3 + HIT_ZSEG - 1) / HIT_ZSEG;
-  const size_t nt = (size_t)ctx->n1 * ctx->n2 * nseg;
+  const int bps = (int)nblk((size_t)ctx->n1 * ctx->n2, 256);
   hipLaunchKernelGGL((k_hit_xsum<NM>), dim3(nblk((size_t)3 * NM * NM * ctx->n1, 256)), dim3(256), 0, ctx->stream, ctx->n1, ctx->d_hit_fhat,
                      ctx->d_hit_tab, ctx->d_hit_ax);
-  hipLaunchKernelGGL((k_hit_synth<R, NM>), dim3(nblk(nt, 256)), dim3(256), 0, ctx->stream, make_grid(ctx), ctx->d_hit_ax,
-                     ctx->d_hit_tab, fld<R>(ctx, MELT_FORCX), fld<R>(ctx, MELT_FORCY), fld<R>(ctx, MELT_FORCZ));
+  hipLaunchKernelGGL((k_hit_synth<R, NM>), dim3((unsigned)(bps * nseg)), dim3(256), 0, ctx->stream, make_grid(ctx), ctx->d_hit_ax,
+                     ctx->d_hit_tab, fld<R>(ctx, MELT_FORCX), fld<R>(ctx, MELT_FORCY), fld<R>(ctx, MELT_FORCZ), bps);
   MELT_HIP(hipGetLastError());
   return 0;
 }

@@ -1095,10 +1095,10 @@ static int sphere_tag_t(melt_ctx* ctx) {
                      ctx->d_win);
   hipLaunchKernelGGL(k_window_final, dim3(1), dim3(1), 0, ctx->stream, ctx->d_win, ef, et, ctx->force_window_from_tag ? 3 : 2,
                      ctx->d_err);
-  hipLaunchKernelGGL(k_vbox, dim3(1), dim3(256), 0, ctx->stream, ctx->d_lag[MELT_L_XYZV], ctx->d_gv, ctx->nv, ctx->d_red);
+  hipLaunchKernelGGL(k_vbox, dim3(1), dim3(256), 0, ctx->stream, ctx->d_lag[MELT_L_XYZV], ctx->d_gv, ctx->nv, ctx->d_sbox);
   for (int id = MELT_VOFP; id <= MELT_VOFZ; ++id)
     hipLaunchKernelGGL(k_fill<R>, dim3(1024), dim3(256), 0, ctx->stream, fld<R>(ctx, id), 1.0, ctx->nc);
-  hipLaunchKernelGGL(k_sphere_tag<R>, dim3(nb(ctx->nc, 256)), dim3(256), 0, ctx->stream, g, ctx->d_red, ctx->d_body, ctx->rad_p,
+  hipLaunchKernelGGL(k_sphere_tag<R>, dim3(nb(ctx->nc, 256)), dim3(256), 0, ctx->stream, g, ctx->d_sbox, ctx->d_body, ctx->rad_p,
                      fld<R>(ctx, MELT_VOFP), fld<R>(ctx, MELT_VOFX), fld<R>(ctx, MELT_VOFY), fld<R>(ctx, MELT_VOFZ), ctx->d_err);
   MELT_HIP(hipGetLastError());
   if (ctx->h_body.initial_tag) {   // no ray tagging in this mode: the flag only gates graph capture
@@ -1112,10 +1112,10 @@ static int sphere_tag_t(melt_ctx* ctx) {
 template <typename R>
 static int ray_tag_init_t(melt_ctx* ctx) {
   Grid g = make_grid(ctx);
-  hipLaunchKernelGGL(k_vbox, dim3(1), dim3(256), 0, ctx->stream, ctx->d_lag[MELT_L_XYZV], ctx->d_gv, ctx->nv, ctx->d_red);
+  hipLaunchKernelGGL(k_vbox, dim3(1), dim3(256), 0, ctx->stream, ctx->d_lag[MELT_L_XYZV], ctx->d_gv, ctx->nv, ctx->d_sbox);
   MELT_HIP(hipGetLastError());
   double lim[6];
-  MELT_HIP(hipMemcpyAsync(lim, ctx->d_red, sizeof lim, hipMemcpyDeviceToHost, ctx->stream));
+  MELT_HIP(hipMemcpyAsync(lim, ctx->d_sbox, sizeof lim, hipMemcpyDeviceToHost, ctx->stream));
   MELT_HIP(hipStreamSynchronize(ctx->stream));
   Box bx;  // get_bbox_inds tagCells.f90:31-62 (the reference scales all three axes by dx1)
   for (int d = 0; d < 3; ++d) {

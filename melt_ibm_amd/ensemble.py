@@ -95,13 +95,13 @@ class Member:
         return geometry.spheroid(a, self.aspect * a, f)
 
 
-def ensemble_members(count: int = 48, seed: int = 0) -> list[Member]:
+def ensemble_members(count: int = 48, seed: int = 0, remesh: bool = True) -> list[Member]:
     rng = np.random.default_rng(seed)
     out = []
     for i in range(count):
         n = int(rng.choice(GRIDS))
         out.append(Member(n=n, rad=float(rng.uniform(0.3, 0.7)), st=float(rng.uniform(0.5, 4.0)), re=float(rng.uniform(200.0, 1000.0)),
-                          seed=i, level=3 if n <= 64 else 4))
+                          seed=i, level=3 if n <= 64 else 4, remesh=remesh))
     return out
 
 

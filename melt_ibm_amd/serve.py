@@ -103,7 +103,8 @@ class Request:
     last_done: float = 0.0
 
 
-def request_stream(duration_s: float, rate: float = 4.0, seed: int = 7, steps_scale: float = 1.0, grids=SERVE_GRIDS) -> list[Request]:
+def request_stream(duration_s: float, rate: float = 4.0, seed: int = 7, steps_scale: float = 1.0, grids=SERVE_GRIDS,
+                   remesh: bool = True) -> list[Request]:
     rng = np.random.default_rng(seed)
     t, out = 0.0, []
     while True:
@@ -115,7 +116,7 @@ def request_stream(duration_s: float, rate: float = 4.0, seed: int = 7, steps_sc
         rad = float(rng.uniform(0.3, 0.7))
         m = Member(n=n, rad=rad, st=float(rng.uniform(0.5, 4.0)), re=float(rng.uniform(200.0, 1000.0)),
                    seed=len(out), aspect=aspect, freq=freq_for(n, rad, aspect),
-                   steps=max(1, int(round(rng.integers(50, 501) * steps_scale))))
+                   steps=max(1, int(round(rng.integers(50, 501) * steps_scale))), remesh=remesh)
         out.append(Request(len(out), t, m))
     return out
 

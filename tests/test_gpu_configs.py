@@ -10,8 +10,15 @@
    replay per simulation and step: sampled members equal the same simulation
    run alone, bit for bit.
  - A window of the serving stream (Poisson arrivals, seed 7) that holds a
-   256^3, a 64^3, a 128^3 and a 384^3 request (60500 faces), step counts
+   256^3, a 64^3, a 128^3 and a 384^3 request (13520 faces), step counts
    scaled down: each result equals the same request run alone, bit for bit.
+ - The 256^3 workload (HIT forcing, level-5 sphere of radius 0.5, St 2) for
+   its 200 RK3 steps, with remeshing, under the reference's CFL control
+   (IDTV = 1, gcurv.f90:151-157: get_dt brings the CFL number to CFLFIX
+   before every step): it completes, the body melts steadily and stays
+   divergence-free.  At the fixed dt of the first step the CFL number climbs
+   from 0.5 to 2.9 by step 130 as the forcing adds energy, and that run
+   blows up at step 136, on the GPU and in the oracle alike.
 """
 import numpy as np
 import pytest
@@ -105,3 +112,39 @@ def test_serving_window_with_384_request_matches_alone(gpu_lib):
             assert np.array_equal(rep["results"][r.rid][q], s.ctx.download(q)), (r.member.n, q)
         assert rep["results"][r.rid]["volume"] == s.ctx.body().volume
         s.close()
+
+
+def test_hit_workload_200_steps_with_cfl_control(gpu_lib):
+    import bench
+    from melt_ibm_amd import geometry
+    cfg, level, rad = bench.workload(256)
+    m = geometry.sphere_level(level)
+    d = m.xyz[m.vert_of_edge[:, 0] - 1] - m.xyz[m.vert_of_edge[:, 1] - 1]
+    h = 2.0 * np.pi / 256
+    cfg.iremesh = 1
+    cfg.perc_ethresh = min(0.7, 0.95 * float(np.sqrt((d * d).sum(axis=1)).min()) * rad / h)
+    c = MeltContext(cfg, lib=gpu_lib)
+    c.set_geometry(m.xyz, m.vert_of_edge, m.edge_of_face, rad)
+    c.inqpr()
+    u, v, w, dt = inputs.solenoidal_velocity_device(cfg, seed=0, kmax=4.0, rms=0.1)
+    for name, a in (("vx", u), ("vy", v), ("vz", w)):
+        c.upload_device(name, a.data_ptr())
+    del u, v, w
+    c.set_dt(dt)
+    c.hit_init(*bench.HIT_PARAMS, seed=1)
+    c.tagCells()
+    vols = [c.body().volume]
+    for step in range(200):
+        c.set_dt(c.get_dt(0.5))
+        c.hit_force()
+        c.tschem()
+        if step % 25 == 24:
+            c.sync()
+            vols.append(c.body().volume)
+            print(f"# step {step + 1}: dt {c.dt:.3e} volume {vols[-1]:.6e}", flush=True)
+    c.sync()
+    assert abs(c.cfl() * c.dt - 0.5) < 0.1        # the last step ran near CFLFIX
+    assert all(b < a for a, b in zip(vols, vols[1:])) and vols[-1] > 0.5 * vols[0], vols
+    assert c.remesh_stats()["events"] > 0
+    assert c.divgck() < 1e-10
+    c.close()
