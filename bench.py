@@ -316,6 +316,8 @@ def precision_study(args):
     from melt_ibm_amd.api import MeltContext
     n = args.n if args.n != 256 else 512
     cfg, mesh, r_eq = c3_workload(n, args.study_ethresh)
+    if args.study_no_remesh:
+        cfg.iremesh = 0
     print(f"# building the {n}^3 initial state", file=sys.stderr, flush=True)
     u, v, w, dt = inputs.solenoidal_velocity_device(cfg, seed=0, kmax=4.0, rms=0.1)
     dt = dt * args.study_cfl / 0.5
@@ -336,6 +338,7 @@ def precision_study(args):
     del u, v, w
     qs = ("vx", "vy", "vz", "temp", "pr")
     t_run = [0.0, 0.0]
+    dts = []
     checkpoints = []
     done = 0
     from melt_ibm_amd.api import MeltError
@@ -345,12 +348,20 @@ def precision_study(args):
         try:
             for p_, c in enumerate(ctxs):
                 c.sync()
-                t0 = time.perf_counter()
-                for _ in range(k):
+                for i in range(k):
+                    if args.study_cfl_control:
+                        # the reference's CFL mode (IDTV = 1, gcurv.f90:151-157): dt
+                        # from the FP64 run's CFL number before every step, the
+                        # same sequence replayed in the precision form
+                        if p_ == 0:
+                            dts.append(c.get_dt(args.study_cfl))
+                        c.set_dt(dts[done + i])
                     c.hit_force()
+                    c.sync()
+                    t0 = time.perf_counter()
                     c.tschem()
-                c.sync()
-                t_run[p_] += time.perf_counter() - t0
+                    c.sync()
+                    t_run[p_] += time.perf_counter() - t0
         except MeltError as e:   # a terminal condition of the reference (e.g. the body left its window)
             stopped = {"after_step": done, "precision": ("fp64", "mixed")[p_], "error": str(e)}
             break
@@ -367,7 +378,8 @@ def precision_study(args):
         v0, v1 = ctxs[0].body().volume, ctxs[1].body().volume
         cp["volume_rel_err"] = float(f"{abs(v1 - v0) / v0:.3e}")
         cp["fp64_umax"] = float(f"{max(float(np.max(np.abs(f[0][q]))) for q in ('vx', 'vy', 'vz')):.4f}")
-        cp["fp64_cfl"] = float(f"{ctxs[0].cfl() * dt:.3f}")
+        cp["fp64_cfl"] = float(f"{ctxs[0].cfl() * ctxs[0].dt:.3f}")
+        cp["dt"] = float(f"{ctxs[0].dt:.4e}")
         cp["remesh"] = {("fp64", "mixed")[p_]: {k_: c.remesh_stats()[k_] for k_ in ("events", "total_collapses")}
                         for p_, c in enumerate(ctxs)}
         checkpoints.append(cp)
@@ -378,9 +390,11 @@ def precision_study(args):
         c.close()
     sub = [t / (3 * max(done, 1)) * 1e3 for t in t_run]
     print(json.dumps({"study": "precision form vs FP64", "grid": [n, n, n], "faces": int(mesh.edge_of_face.shape[0]),
-                      "forcing": f"HIT (TL, eps*, kf/kmin) = {HIT_PARAMS}, regenerated every step", "cfl": args.study_cfl, "dt": dt,
+                      "forcing": f"HIT (TL, eps*, kf/kmin) = {HIT_PARAMS}, regenerated every step", "cfl": args.study_cfl,
+                      "dt": ("from get_dt before every step (CFL mode), the FP64 run's sequence replayed in the precision form"
+                             if args.study_cfl_control else dt),
                       "solid": f"prolate spheroid, semi-axes 0.3/0.3/0.6, icosphere frequency 55, melting St 2, Newton-Euler on, "
-                               f"remeshing with E_thresh = {args.study_ethresh} h",
+                               + (f"remeshing with E_thresh = {args.study_ethresh} h" if not args.study_no_remesh else "no remeshing"),
                       "steps": done, "substep_ms": {"fp64": round(sub[0], 4), "mixed": round(sub[1], 4)},
                       "speedup": round(sub[0] / sub[1], 3), "refinement": {"checks": stats[0], "refined": stats[1], "last_ratio": stats[2]},
                       "checkpoints": checkpoints, "stopped": stopped}))
@@ -413,7 +427,10 @@ def main():
     ap.add_argument("--study-steps", type=int, default=100)
     ap.add_argument("--study-every", type=int, default=20, help="steps between error checkpoints")
     ap.add_argument("--study-ethresh", type=float, default=0.38, help="PERCT_ETHRESH of the remeshing (E_thresh / h)")
-    ap.add_argument("--study-cfl", type=float, default=0.5, help="CFL number of the fixed dt")
+    ap.add_argument("--study-cfl", type=float, default=0.5, help="CFL number (of every step, or of the fixed dt)")
+    ap.add_argument("--study-no-remesh", action="store_true", help="the precision study without remeshing (IREMESH = 0)")
+    ap.add_argument("--study-fixed-dt", dest="study_cfl_control", action="store_false",
+                    help="keep the first step's dt instead of the reference's CFL control")
     args = ap.parse_args()
     if args.precision_study:
         precision_study(args)
