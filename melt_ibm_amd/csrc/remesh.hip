@@ -40,6 +40,40 @@ struct Surface {
 
 struct Failure {};
 
+// Which containers (faces or edges) of an index array hold each value,
+// complete at every moment: built by a counting sort of the array, then kept
+// up as values are rewritten (a rewrite of every `from` into `to` moves the
+// whole set of `from` over to `to`; a single write adds its container).  Sets
+// may hold containers that no longer reference the value, or twice: every
+// use re-checks the array.
+struct Holders {
+  std::vector<int> ptr, idx;                // the counting sort at build time
+  std::vector<std::vector<int>> extra;      // added since
+  std::vector<unsigned char> moved;         // the build-time set went to another value
+  void build(const int32_t* a, int width, int n, int nvals) {
+    ptr.assign(nvals + 1, 0);
+    for (int i = 0; i < width * n; ++i) ++ptr[a[i] + 1];
+    for (int v = 0; v < nvals; ++v) ptr[v + 1] += ptr[v];
+    idx.resize(ptr[nvals]);
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (int i = 0; i < width * n; ++i) idx[fill[a[i]]++] = i / width;
+    extra.assign(nvals, {});
+    moved.assign(nvals, 0);
+  }
+  template <class F>
+  void each(int v, F&& f) const {
+    if (!moved[v])
+      for (int p = ptr[v]; p < ptr[v + 1]; ++p) f(idx[p]);
+    for (int c : extra[v]) f(c);
+  }
+  void add(int v, int c) { extra[v].push_back(c); }
+  void move(int from, int to) {
+    each(from, [&](int c) { extra[to].push_back(c); });
+    moved[from] = 1;
+    extra[from].clear();
+  }
+};
+
 class Editor {
  public:
   explicit Editor(Surface& s) : m_(s) { flagged_faces_.fill(-1); }
@@ -81,6 +115,8 @@ class Editor {
   // reference aborts on (or would run out of bounds on)
   bool coarsen(double* el, double thresh, int32_t& rm_flag, int32_t* anchor, int32_t* flagged_edge, double* vmelt, int& ncol, int& ndel) {
     ncol = ndel = 0;
+    start_counts(el, thresh);
+    bool measured = false;
     try {
       for (int sweep = 0; rm_flag && sweep < 5; ++sweep) {
         for (int e = 0; e < m_.ne; ++e) {
@@ -95,8 +131,21 @@ class Editor {
             normals_of_flagged();
             if (!euler_two()) return false;
           } else {
+            hold();
             collapse(e, v1, v2, anchor, flagged_edge, vmelt);
-            rm_flag = measure(el, thresh) ? 1 : 0;
+            // calculate_eLengths after the collapse (remesh_coarsen.f90:170):
+            // all live edges the first time, then only those around v1, the
+            // only ones whose length changed (the same arithmetic, so the same
+            // values), with the short ones counted as live edges go and come
+            if (!measured) {
+              measure(el, thresh);
+              nshort_ = 0;
+              for (int q = 0; q < m_.ne; ++q) nshort_ += !m_.ge[q] && el[q] < thresh;
+              measured = true;
+            } else {
+              remeasure_at(v1);
+            }
+            rm_flag = nshort_ > 0 ? 1 : 0;
             normals_of_flagged();
             if (!euler_two() || live_faces() <= 4) return false;
             ++ncol;
@@ -299,17 +348,65 @@ class Editor {
     }
   }
 
-  bool euler_two() const {
-    long V = 0, E = 0, F = 0;
-    for (int i = 0; i < m_.nv; ++i) V += !m_.gv[i];
-    for (int i = 0; i < m_.ne; ++i) E += !m_.ge[i];
-    for (int i = 0; i < m_.nf; ++i) F += !m_.gf[i];
-    return V - E + F == 2;
+  // Live counts (Euler's V - E + F and the live faces, remesh_coarsen.f90:
+  // 93-99, 175-190), the short live edges and the ghost entities of the
+  // coarsening, kept as the ghost flags change (every ghosting goes through
+  // ghost_v/e/f) instead of counted over all entities after every edit.
+  long V_ = 0, E_ = 0, F_ = 0, nshort_ = 0;
+  double* el_ = nullptr;
+  double thresh_ = 0.0;
+  bool held_ = false;             // the holders below are built (at the first collapse)
+  Holders vf_, ef_, ve_;          // vertex -> faces (vof), edge -> faces (eof), vertex -> edges (voe)
+  void start_counts(double* el, double thresh) {
+    el_ = el;
+    thresh_ = thresh;
+    V_ = E_ = F_ = nshort_ = 0;
+    held_ = false;
+    for (int i = 0; i < m_.nv; ++i) V_ += !m_.gv[i];
+    for (int i = 0; i < m_.ne; ++i) E_ += !m_.ge[i];
+    for (int i = 0; i < m_.nf; ++i) F_ += !m_.gf[i];
   }
-  long live_faces() const {
-    long F = 0;
-    for (int i = 0; i < m_.nf; ++i) F += !m_.gf[i];
-    return F;
+  void hold() {
+    if (held_) return;
+    vf_.build(m_.vof, 3, m_.nf, m_.nv);
+    ef_.build(m_.eof, 3, m_.nf, m_.ne);
+    ve_.build(m_.voe, 2, m_.ne, m_.nv);
+    held_ = true;
+  }
+  // every `from` in the width-wide containers held for it becomes `to`
+  static void rewrite(Holders& h, int32_t* a, int width, int from, int to) {
+    h.each(from, [&](int c) { for (int k = 0; k < width; ++k) if (a[width * c + k] == from) a[width * c + k] = to; });
+    h.move(from, to);
+  }
+  void ghost_v(int v) { if (!m_.gv[v]) { m_.gv[v] = 1; --V_; } }
+  void ghost_e(int e) {
+    if (m_.ge[e]) return;
+    m_.ge[e] = 1;
+    --E_;
+    if (el_[e] < thresh_) --nshort_;
+  }
+  void ghost_f(int f) {
+    if (m_.gf[f]) return;
+    m_.gf[f] = 1;
+    --F_;
+  }
+  bool euler_two() const { return V_ - E_ + F_ == 2; }
+  long live_faces() const { return F_; }
+
+  double edge_length(int e) const {
+    const int a = m_.voe[2 * e], b = m_.voe[2 * e + 1];
+    const double dx = m_.xv(a, 0) - m_.xv(b, 0), dy = m_.xv(a, 1) - m_.xv(b, 1), dz = m_.xv(a, 2) - m_.xv(b, 2);
+    return std::sqrt(dx * dx + dy * dy + dz * dz);
+  }
+  // the live edges that hold v (after a collapse onto v: every edge whose
+  // length changed): new lengths, short count
+  void remeasure_at(int v) {
+    ve_.each(v, [&](int e) {
+      if (m_.ge[e] || (m_.voe[2 * e] != v && m_.voe[2 * e + 1] != v)) return;
+      const bool was = el_[e] < thresh_;
+      el_[e] = edge_length(e);
+      nshort_ += (long)(el_[e] < thresh_) - (long)was;
+    });
   }
 
   // the two edges of face f other than e: (ghosted, kept)
@@ -348,14 +445,16 @@ class Editor {
     int e1, re1, e2, re2;
     others(F1, e, e1, re1);
     others(F2, e, e2, re2);
-    m_.gf[F1] = 1; m_.gf[F2] = 1;
-    m_.gv[v2] = 1;
-    m_.ge[e] = 1; m_.ge[e1] = 1; m_.ge[e2] = 1;
-    // update_face_connectivity :368-381, update_edge_connectivity :383-421
-    for (int i = 0; i < 3 * m_.nf; ++i) if (m_.vof[i] == v2) m_.vof[i] = v1;
-    for (int i = 0; i < 3 * m_.nf; ++i) if (m_.eof[i] == e1) m_.eof[i] = re1;
-    for (int i = 0; i < 3 * m_.nf; ++i) if (m_.eof[i] == e2) m_.eof[i] = re2;
-    for (int i = 0; i < 2 * m_.ne; ++i) if (m_.voe[i] == v2) m_.voe[i] = v1;
+    ghost_f(F1); ghost_f(F2);
+    ghost_v(v2);
+    ghost_e(e); ghost_e(e1); ghost_e(e2);
+    // update_face_connectivity :368-381, update_edge_connectivity :383-421:
+    // every v2 -> v1, e1 -> re1, e2 -> re2 in vof / eof / voe, in the
+    // containers that hold them (the reference loops over all of them)
+    rewrite(vf_, m_.vof, 3, v2, v1);
+    rewrite(ef_, m_.eof, 3, e1, re1);
+    rewrite(ef_, m_.eof, 3, e2, re2);
+    rewrite(ve_, m_.voe, 2, v2, v1);
     merge(e1, re1, F1);
     merge(e2, re2, F2);
     // flag_neighbours_1ring :454-513 around v1 from re1
@@ -414,6 +513,7 @@ class Editor {
     for (int i = 0; i < 3; ++i) {
       if (m_.vof[3 * fL + i] == k) continue;
       if (n == 2) fail();
+      if (held_) vf_.add(lv[n], fL);
       m_.vof[3 * fL + i] = lv[n++];
     }
     const int repl[2] = {e4, e5};
@@ -421,11 +521,12 @@ class Editor {
     for (int i = 0; i < 3; ++i) {
       if (m_.eof[3 * fL + i] == e3) continue;
       if (n == 2) fail();
+      if (held_) ef_.add(repl[n], fL);
       m_.eof[3 * fL + i] = repl[n++];
     }
-    m_.gv[v] = 1;
-    m_.ge[e] = 1; m_.ge[e1] = 1; m_.ge[e2] = 1;
-    m_.gf[fR] = 1; m_.gf[f3] = 1;
+    ghost_v(v);
+    ghost_e(e); ghost_e(e1); ghost_e(e2);
+    ghost_f(fR); ghost_f(f3);
     flagged_faces_[0] = fL;
   }
 
