@@ -341,7 +341,9 @@ int melt_graph_enable(melt_ctx* ctx, int32_t on);
  * update (beside the pressure solve), 2 at the start of its own substep
  * (beside the explicit pass), 8 = faces a ray-tagging pencil may hold
  * (BUFF_SIZE of pencil_rayTag.f90:84, n1m there; more faces raise
- * MELT_ERR_PENCIL_OVERFLOW as the reference aborts); value 0 = built-in choice.
+ * MELT_ERR_PENCIL_OVERFLOW as the reference aborts), 9 = 1 keeps one cell per
+ * thread in the z-marching explicit pass of the precision form (default: two
+ * x-neighbour cells per thread there); value 0 = built-in choice.
  * Results do not depend on it (key 8: as long as no pencil overflows). */
 int melt_set_tuning(melt_ctx* ctx, int32_t key, int32_t value);
 void* melt_stream(melt_ctx* ctx);

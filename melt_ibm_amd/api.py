@@ -517,7 +517,7 @@ class MeltContext:
         """Replay tschem as one captured HIP graph."""
         self._chk(self._f["graph_enable"](self._ctx, 1 if on else 0))
 
-    TUNE_KEYS = {"xf_w": 0, "xf_nt": 1, "solve_w": 2, "solve_nt": 3, "xf_z": 4, "yz_jb": 5, "streams": 6, "coef": 7, "pbuf": 8}
+    TUNE_KEYS = {"xf_w": 0, "xf_nt": 1, "solve_w": 2, "solve_nt": 3, "xf_z": 4, "yz_jb": 5, "streams": 6, "coef": 7, "pbuf": 8, "xf_one": 9}
 
     def set_tuning(self, **kw) -> None:
         """Launch-shape overrides (xf_w, xf_nt, solve_w, solve_nt, xf_z, yz_jb, streams, coef) and the ray-tagging

@@ -189,7 +189,7 @@ extern "C" int melt_ctx_create(const melt_config* cfg, melt_ctx** out) {
         ctx->lds_max = (size_t)lds;
     }
     if (const char* tv = getenv("MELT_TUNE")) {
-      static const char* keys[TUNE_N] = {"xf_w", "xf_nt", "solve_w", "solve_nt", "xf_z", "yz_jb", "streams", "coef", "pbuf"};
+      static const char* keys[TUNE_N] = {"xf_w", "xf_nt", "solve_w", "solve_nt", "xf_z", "yz_jb", "streams", "coef", "pbuf", "xf_one"};
       std::string spec(tv);
       size_t pos = 0;
       while (pos < spec.size()) {

@@ -53,7 +53,7 @@ enum {
 
 // Launch-shape overrides (0 = built-in choice), set from MELT_TUNE at
 // context creation ("xf_w=4,solve_w=16,...") for tile sweeps.
-enum melt_tune { TUNE_XF_W = 0, TUNE_XF_NT, TUNE_SOLVE_W, TUNE_SOLVE_NT, TUNE_XF_Z, TUNE_YZ_JB, TUNE_STREAMS, TUNE_COEF, TUNE_PBUF, TUNE_N };
+enum melt_tune { TUNE_XF_W = 0, TUNE_XF_NT, TUNE_SOLVE_W, TUNE_SOLVE_NT, TUNE_XF_Z, TUNE_YZ_JB, TUNE_STREAMS, TUNE_COEF, TUNE_PBUF, TUNE_XF_ONE, TUNE_N };
 
 struct melt_ctx {
   melt_config cfg;

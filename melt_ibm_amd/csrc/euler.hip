@@ -7,6 +7,7 @@
 // coalesced unit-stride access and to read each input field once per pass.
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include "common.h"
 
@@ -642,6 +643,23 @@ struct ZStencil {
   }
 };
 
+// One cell of pass A: the explicit terms H of u, v, w, T (hdnl1-3, hdnlte)
+// and the four right-hand sides scaled for the x-line solve.  o: ru at the
+// cell; h receives the new ru values, d the scaled rhs.
+template <typename R, typename S>
+__device__ __forceinline__ void zcell(const EulerArgs<R>& A, const RhsCoef<R>& cf, const S& s, R o1, R o2, R o3, R ot, double sc_u,
+                                      double sc_t, R h[4], double d[4]) {
+  R p = SV(FVOF, 0, 0, 0);
+  h[0] = h1_val(A, s, R(0.5) * (SV(FVOF, -1, 0, 0) + p));
+  h[1] = h2_val(A, s, R(0.5) * (SV(FVOF, 0, -1, 0) + p));
+  h[2] = h3_val(A, s, R(0.5) * (SV(FVOF, 0, 0, -1) + p));
+  h[3] = ht_val(A, s, p);
+  d[0] = (double)rhs_u<FU>(cf, s, h[0], o1) * sc_u;
+  d[1] = (double)rhs_v<FV>(cf, s, h[1], o2) * sc_u;
+  d[2] = (double)rhs_w<FW>(cf, s, h[2], o3) * sc_u;
+  d[3] = (double)rhs_t<FT>(cf, s, h[3], ot) * sc_t;
+}
+
 // Pass A as a z-march: a block owns one x-line (one thread per i) of row j and
 // walks D consecutive planes.  Per plane a cell loads the centre column at k+1
 // (u, v, w, T) and at k (p, VOFp), the rows j-1 / j+1 it needs (u, v, w, T at
@@ -671,6 +689,11 @@ __global__ __launch_bounds__(512) void k_explicit_zmarch(EulerArgs<R> A, RhsCoef
   auto off = [&](int jj, int kk) { return ((Off)i + (Off)M * ((Off)jj + (Off)g.n2 * (Off)kk)) * (Off)sizeof(R); };
   auto ld = [](const R* p, Off o) { return *reinterpret_cast<const R*>(reinterpret_cast<const char*>(p) + o); };
   auto st = [](R* p, Off o, R v) { *reinterpret_cast<R*>(reinterpret_cast<char*>(p) + o) = v; };
+  // values read once (forcing, ru) and every store stream past the caches
+  // (nontemporal), which leaves the L2 to the rows the neighbouring blocks
+  // read again one plane later
+  auto ldn = [](const R* p, Off o) { return __builtin_nontemporal_load(reinterpret_cast<const R*>(reinterpret_cast<const char*>(p) + o)); };
+  auto stn = [](R* p, Off o, R v) { __builtin_nontemporal_store(v, reinterpret_cast<R*>(reinterpret_cast<char*>(p) + o)); };
   ZStencil<R> s;
   s.i = i; s.j = j;
   {
@@ -699,8 +722,8 @@ __global__ __launch_bounds__(512) void k_explicit_zmarch(EulerArgs<R> A, RhsCoef
     s.wjp = ld(A.vz, ojp); s.wjm_kp = ld(A.vz, off(jm, kp));
     s.tjm = ld(A.temp, ojm); s.tjp = ld(A.temp, ojp);
     s.pjm = ld(A.pr, ojm); s.ojm = ld(A.vofp, ojm);
-    s.fx = ld(A.forcx, o0); s.fy = ld(A.forcy, o0); s.fz = ld(A.forcz, o0);
-    const R o1 = ld(ru1, o0), o2 = ld(ru2, o0), o3 = ld(ru3, o0), ot = ld(rut, o0);
+    s.fx = ldn(A.forcx, o0); s.fy = ldn(A.forcy, o0); s.fz = ldn(A.forcz, o0);
+    const R o1 = ldn(ru1, o0), o2 = ldn(ru2, o0), o3 = ldn(ru3, o0), ot = ldn(rut, o0);
     const int cur = t & 1, prv = cur ^ 1;
     row(cur, XU)[i] = s.cu[1]; row(cur, XUJM)[i] = s.ujm; row(cur, XV)[i] = s.cv[1]; row(cur, XVJP)[i] = s.vjp;
     row(cur, XWP)[i] = s.cw[2]; row(cur, XT)[i] = s.ct[1]; row(cur, XP)[i] = s.cp[1]; row(cur, XO)[i] = s.co[1];
@@ -710,13 +733,9 @@ __global__ __launch_bounds__(512) void k_explicit_zmarch(EulerArgs<R> A, RhsCoef
     s.w_im = row(prv, XWP)[im]; s.w_ip = row(prv, XWP)[ip]; s.wkp_im = row(cur, XWP)[im];
     s.t_im = row(cur, XT)[im]; s.t_ip = row(cur, XT)[ip];
     s.p_im = row(cur, XP)[im]; s.o_im = row(cur, XO)[im];
-    R p = SV(FVOF, 0, 0, 0);
-    R hu = h1_val(A, s, R(0.5) * (SV(FVOF, -1, 0, 0) + p));
-    R hv = h2_val(A, s, R(0.5) * (SV(FVOF, 0, -1, 0) + p));
-    R hw = h3_val(A, s, R(0.5) * (SV(FVOF, 0, 0, -1) + p));
-    R ht = ht_val(A, s, p);
-    const double d[4] = {(double)rhs_u<FU>(cf, s, hu, o1) * sc_u, (double)rhs_v<FV>(cf, s, hv, o2) * sc_u,
-                         (double)rhs_w<FW>(cf, s, hw, o3) * sc_u, (double)rhs_t<FT>(cf, s, ht, ot) * sc_t};
+    R hh[4];
+    double d[4];
+    zcell(A, cf, s, o1, o2, o3, ot, sc_u, sc_t, hh, d);
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
       double* L = lds + f * ldx + H;
@@ -724,16 +743,129 @@ __global__ __launch_bounds__(512) void k_explicit_zmarch(EulerArgs<R> A, RhsCoef
       if (i < H) L[i + M] = d[f];
       if (i >= M - H) L[i - M] = d[f];
     }
-    st(ru1, o0, hu); st(ru2, o0, hv); st(ru3, o0, hw); st(rut, o0, ht);
+    stn(ru1, o0, hh[0]); stn(ru2, o0, hh[1]); stn(ru3, o0, hh[2]); stn(rut, o0, hh[3]);
     __syncthreads();
 #pragma unroll
-    for (int f = 0; f < 4; ++f) st(outs[f], o0, (R)green_lds(f == 3 ? gt : gu, lds + f * ldx + H, i));
+    for (int f = 0; f < 4; ++f) stn(outs[f], o0, (R)green_lds(f == 3 ? gt : gu, lds + f * ldx + H, i));
     s.cu[0] = s.cu[1]; s.cu[1] = s.cu[2]; s.cv[0] = s.cv[1]; s.cv[1] = s.cv[2];
     s.cw[0] = s.cw[1]; s.cw[1] = s.cw[2]; s.ct[0] = s.ct[1]; s.ct[1] = s.ct[2];
     s.cp[0] = s.cp[1]; s.co[0] = s.co[1];
     s.vjp_km = s.vjp; s.wjm = s.wjm_kp;
   }
 }
+
+// The z-march with two x-neighbour cells per thread (cells 2t and 2t+1), for
+// 4-byte values: every field access is one 8-byte load or store, half the
+// vector memory instructions of the one-cell form, which is what bounds this
+// pass.  A thread's second cell gives its first cell the +x values and the
+// first gives the second the -x values; the rest come from the neighbouring
+// threads' pairs published in LDS as in the one-cell form.  Same values and
+// expressions per cell, so the results are identical to it.  Block: n1 / 2
+// threads.
+template <typename Off>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_explicit_zmarch_pair(EulerArgs<float> A, RhsCoef<float> cf, float* __restrict__ ru1,
+                                                              float* __restrict__ ru2, float* __restrict__ ru3, float* __restrict__ rut,
+                                                              float* __restrict__ r_u, float* __restrict__ r_v, float* __restrict__ r_w,
+                                                              float* __restrict__ r_t, double sc_u, double sc_t, int D, Green gu, Green gt) {
+  using R = float;
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  extern __shared__ double lds[];
+  constexpr int H = FIR_KMAX;
+  enum { XU, XUJM, XV, XVJP, XWP, XT, XP, XO, NX };   // exchange rows of one set, one pair per thread
+  const Grid& g = A.g;
+  const int M = g.n1, T = M >> 1, t = threadIdx.x, ldx = M + 2 * H;
+  f2* xr = reinterpret_cast<f2*>(lds + 4 * ldx);
+  auto row = [&](int set, int r) { return xr + (set * NX + r) * T; };
+  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  const int j = (int)(b % (unsigned)g.n2), k0 = (int)(b / (unsigned)g.n2) * D;
+  const int jm = j == 0 ? g.n2 - 1 : j - 1, jp = j == g.n2 - 1 ? 0 : j + 1;
+  const int tm = t == 0 ? T - 1 : t - 1, tp = t == T - 1 ? 0 : t + 1;
+  const int i0 = 2 * t;
+  auto off = [&](int jj, int kk) { return ((Off)i0 + (Off)M * ((Off)jj + (Off)g.n2 * (Off)kk)) * (Off)sizeof(R); };
+  auto ld = [](const R* p, Off o) { return *reinterpret_cast<const f2*>(reinterpret_cast<const char*>(p) + o); };
+  auto st = [](R* p, Off o, f2 v) { __builtin_nontemporal_store(v, reinterpret_cast<f2*>(reinterpret_cast<char*>(p) + o)); };
+  auto ldn = [](const R* p, Off o) { return __builtin_nontemporal_load(reinterpret_cast<const f2*>(reinterpret_cast<const char*>(p) + o)); };
+  ZStencil<R> a, c;   // cells 2t and 2t+1
+  a.i = i0; c.i = i0 + 1; a.j = c.j = j;
+#define PAIR_LD(field, expr) { const f2 v_ = (expr); a.field = v_.x; c.field = v_.y; }
+  {
+    const int km = k0 == 0 ? g.n3 - 1 : k0 - 1;
+    const Off om = off(j, km), o0 = off(j, k0);
+    PAIR_LD(cu[0], ld(A.vx, om)); PAIR_LD(cu[1], ld(A.vx, o0));
+    PAIR_LD(cv[0], ld(A.vy, om)); PAIR_LD(cv[1], ld(A.vy, o0));
+    PAIR_LD(cw[0], ld(A.vz, om)); PAIR_LD(cw[1], ld(A.vz, o0));
+    PAIR_LD(ct[0], ld(A.temp, om)); PAIR_LD(ct[1], ld(A.temp, o0));
+    PAIR_LD(cp[0], ld(A.pr, om)); PAIR_LD(co[0], ld(A.vofp, om));
+    PAIR_LD(vjp_km, ld(A.vy, off(jp, km)));
+    PAIR_LD(wjm, ld(A.vz, off(jm, k0)));
+    row(1, XU)[t] = f2{a.cu[0], c.cu[0]};
+    row(1, XWP)[t] = f2{a.cw[1], c.cw[1]};
+  }
+  R* outs[4] = {r_u, r_v, r_w, r_t};
+#pragma unroll 1
+  for (int q = 0; q < D; ++q) {
+    const int k = k0 + q, kp = k == g.n3 - 1 ? 0 : k + 1;
+    a.k = c.k = k;
+    const Off o0 = off(j, k), op = off(j, kp), ojm = off(jm, k), ojp = off(jp, k);
+    PAIR_LD(cu[2], ld(A.vx, op)); PAIR_LD(cv[2], ld(A.vy, op)); PAIR_LD(cw[2], ld(A.vz, op)); PAIR_LD(ct[2], ld(A.temp, op));
+    PAIR_LD(cp[1], ld(A.pr, o0)); PAIR_LD(co[1], ld(A.vofp, o0));
+    PAIR_LD(ujm, ld(A.vx, ojm)); PAIR_LD(ujp, ld(A.vx, ojp));
+    PAIR_LD(vjm, ld(A.vy, ojm)); PAIR_LD(vjp, ld(A.vy, ojp));
+    PAIR_LD(wjp, ld(A.vz, ojp)); PAIR_LD(wjm_kp, ld(A.vz, off(jm, kp)));
+    PAIR_LD(tjm, ld(A.temp, ojm)); PAIR_LD(tjp, ld(A.temp, ojp));
+    PAIR_LD(pjm, ld(A.pr, ojm)); PAIR_LD(ojm, ld(A.vofp, ojm));
+    PAIR_LD(fx, ldn(A.forcx, o0)); PAIR_LD(fy, ldn(A.forcy, o0)); PAIR_LD(fz, ldn(A.forcz, o0));
+    const f2 o1 = ldn(ru1, o0), o2 = ldn(ru2, o0), o3 = ldn(ru3, o0), ot = ldn(rut, o0);
+    const int cur = q & 1, prv = cur ^ 1;
+    row(cur, XU)[t] = f2{a.cu[1], c.cu[1]}; row(cur, XUJM)[t] = f2{a.ujm, c.ujm};
+    row(cur, XV)[t] = f2{a.cv[1], c.cv[1]}; row(cur, XVJP)[t] = f2{a.vjp, c.vjp};
+    row(cur, XWP)[t] = f2{a.cw[2], c.cw[2]}; row(cur, XT)[t] = f2{a.ct[1], c.ct[1]};
+    row(cur, XP)[t] = f2{a.cp[1], c.cp[1]}; row(cur, XO)[t] = f2{a.co[1], c.co[1]};
+    __syncthreads();
+    // cell 2t: -x from the pair of thread t-1, +x from cell 2t+1
+    a.u_im = row(cur, XU)[tm].y; a.u_ip = c.cu[1]; a.ujm_ip = c.ujm; a.ukm_ip = c.cu[0];
+    a.v_im = row(cur, XV)[tm].y; a.v_ip = c.cv[1]; a.vjp_im = row(cur, XVJP)[tm].y;
+    a.w_im = row(prv, XWP)[tm].y; a.w_ip = c.cw[1]; a.wkp_im = row(cur, XWP)[tm].y;
+    a.t_im = row(cur, XT)[tm].y; a.t_ip = c.ct[1];
+    a.p_im = row(cur, XP)[tm].y; a.o_im = row(cur, XO)[tm].y;
+    // cell 2t+1: -x from cell 2t, +x from the pair of thread t+1
+    c.u_im = a.cu[1]; c.u_ip = row(cur, XU)[tp].x; c.ujm_ip = row(cur, XUJM)[tp].x; c.ukm_ip = row(prv, XU)[tp].x;
+    c.v_im = a.cv[1]; c.v_ip = row(cur, XV)[tp].x; c.vjp_im = a.vjp;
+    c.w_im = a.cw[1]; c.w_ip = row(prv, XWP)[tp].x; c.wkp_im = a.cw[2];
+    c.t_im = a.ct[1]; c.t_ip = row(cur, XT)[tp].x;
+    c.p_im = a.cp[1]; c.o_im = a.co[1];
+    R ha[4], hc[4];
+    double da[4], dc[4];
+    zcell(A, cf, a, o1.x, o2.x, o3.x, ot.x, sc_u, sc_t, ha, da);
+    zcell(A, cf, c, o1.y, o2.y, o3.y, ot.y, sc_u, sc_t, hc, dc);
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      double* L = lds + f * ldx + H;
+      *reinterpret_cast<d2*>(L + i0) = d2{da[f], dc[f]};
+      if (i0 < H) { L[i0 + M] = da[f]; L[i0 + 1 + M] = dc[f]; }   // H even: both or neither
+      if (i0 >= M - H) { L[i0 - M] = da[f]; L[i0 + 1 - M] = dc[f]; }
+    }
+    st(ru1, o0, f2{ha[0], hc[0]}); st(ru2, o0, f2{ha[1], hc[1]}); st(ru3, o0, f2{ha[2], hc[2]}); st(rut, o0, f2{ha[3], hc[3]});
+    __syncthreads();
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      const Green& G = f == 3 ? gt : gu;
+      const double* L = lds + f * ldx + H;
+      st(outs[f], o0, f2{(R)green_lds(G, L, i0), (R)green_lds(G, L, i0 + 1)});
+    }
+    a.cu[0] = a.cu[1]; a.cu[1] = a.cu[2]; a.cv[0] = a.cv[1]; a.cv[1] = a.cv[2];
+    a.cw[0] = a.cw[1]; a.cw[1] = a.cw[2]; a.ct[0] = a.ct[1]; a.ct[1] = a.ct[2];
+    a.cp[0] = a.cp[1]; a.co[0] = a.co[1];
+    a.vjp_km = a.vjp; a.wjm = a.wjm_kp;
+    c.cu[0] = c.cu[1]; c.cu[1] = c.cu[2]; c.cv[0] = c.cv[1]; c.cv[1] = c.cv[2];
+    c.cw[0] = c.cw[1]; c.cw[1] = c.cw[2]; c.ct[0] = c.ct[1]; c.ct[1] = c.ct[2];
+    c.cp[0] = c.cp[1]; c.co[0] = c.co[1];
+    c.vjp_km = c.vjp; c.wjm = c.wjm_kp;
+  }
+#undef PAIR_LD
+}
+static_assert(FIR_KMAX % 2 == 0, "the pair z-march writes the periodic images of a pair together");
 
 // Elements per thread the templates are instantiated for.
 constexpr int kEs[] = {1, 2, 3, 4, 6, 8};
@@ -845,6 +977,28 @@ __device__ __forceinline__ void st_c(R* p, const R (&o)[C]) {
     *p = o[0];
   }
 }
+// The same for values a kernel reads or writes once: nontemporal accesses,
+// which stream past the caches and leave them to the values that neighbouring
+// blocks read again.
+template <typename R, int C>
+__device__ __forceinline__ void ld_c_nt(const R* p, R (&o)[C]) {
+  if constexpr (C == 2) {
+    typedef R v2 __attribute__((ext_vector_type(2)));
+    const v2 t = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+    o[0] = t.x; o[1] = t.y;
+  } else {
+    o[0] = __builtin_nontemporal_load(p);
+  }
+}
+template <typename R, int C>
+__device__ __forceinline__ void st_c_nt(R* p, const R (&o)[C]) {
+  if constexpr (C == 2) {
+    typedef R v2 __attribute__((ext_vector_type(2)));
+    __builtin_nontemporal_store(v2{o[0], o[1]}, reinterpret_cast<v2*>(p));
+  } else {
+    __builtin_nontemporal_store(o[0], p);
+  }
+}
 
 // A block is 64 C consecutive i x JB rows j, C (i, j) columns per thread (C =
 // 2 for 4-byte values, so a wave still moves 512-byte rows), and marches
@@ -938,7 +1092,7 @@ __global__ __launch_bounds__(64 * JB) void k_green_yz(GreenYZ<R> B, int n1, int 
       advance(u);
       const size_t c = (size_t)(p + u) * plane + col;
       R qv[C];
-      ld_c<R, C>(q + c, qv);
+      ld_c_nt<R, C>(q + c, qv);
 #pragma unroll
       for (int cc = 0; cc < C; ++cc) {
         double x = gz[0] * w[cc][KT];
@@ -946,7 +1100,7 @@ __global__ __launch_bounds__(64 * JB) void k_green_yz(GreenYZ<R> B, int n1, int 
         for (int m = 1; m <= KT; ++m) x += gz[m] * (w[cc][KT - m] + w[cc][KT + m]);
         qv[cc] = qv[cc] + (R)x;
       }
-      st_c<R, C>(q + c, qv);
+      st_c_nt<R, C>(q + c, qv);
     }
   }
 }
@@ -1042,7 +1196,7 @@ __global__ __launch_bounds__(256) void k_project2(Grid g, R* __restrict__ vx, R*
   const R pim[2] = {dph[cim], l[0]};
   if (do_u) {
     R u[2], v[2], w[2];
-    ld_c<R, 2>(vx + c, u); ld_c<R, 2>(vy + c, v); ld_c<R, 2>(vz + c, w);
+    ld_c_nt<R, 2>(vx + c, u); ld_c_nt<R, 2>(vy + c, v); ld_c_nt<R, 2>(vz + c, w);
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
       if (mean) { u[e] = u[e] - R(mean[0]); v[e] = v[e] - R(mean[1]); w[e] = w[e] - R(mean[2]); }
@@ -1050,16 +1204,16 @@ __global__ __launch_bounds__(256) void k_project2(Grid g, R* __restrict__ vx, R*
       v[e] = v[e] - (l[e] - pjm[e]) * udx2;
       w[e] = w[e] - (l[e] - pkm[e]) * udx3;
     }
-    st_c<R, 2>(vx + c, u); st_c<R, 2>(vy + c, v); st_c<R, 2>(vz + c, w);
+    st_c_nt<R, 2>(vx + c, u); st_c_nt<R, 2>(vy + c, v); st_c_nt<R, 2>(vz + c, w);
   }
   if (do_p) {
     R pjp[2], pkp[2], p[2];
-    ld_c<R, 2>(dph + cjp, pjp); ld_c<R, 2>(dph + ckp, pkp); ld_c<R, 2>(pr + c, p);
+    ld_c<R, 2>(dph + cjp, pjp); ld_c<R, 2>(dph + ckp, pkp); ld_c_nt<R, 2>(pr + c, p);
     const R pip[2] = {l[1], dph[cip2]};
 #pragma unroll
     for (int e = 0; e < 2; ++e)
       p[e] = p[e] + l[e] - be * ((pip[e] - R(2) * l[e] + pim[e]) * dx1q + (pjp[e] - R(2) * l[e] + pjm[e]) * dx2q + (pkp[e] - R(2) * l[e] + pkm[e]) * dx3q);
-    st_c<R, 2>(pr + c, p);
+    st_c_nt<R, 2>(pr + c, p);
   }
 }
 
@@ -1194,6 +1348,20 @@ int launch_explicit_xsolve(melt_ctx* ctx, double bxu, double bxt) {
         if (ctx->n3 % d == 0) { D = d; break; }
     }
     const unsigned blocks = (unsigned)((size_t)ctx->n2 * (ctx->n3 / D));
+    if constexpr (std::is_same_v<R, float>) {
+      if (ctx->tune[TUNE_XF_ONE] == 0) {   // two cells per thread, 8-byte accesses
+        if (ctx->nc * sizeof(R) < (size_t(1) << 32))
+          hipLaunchKernelGGL((k_explicit_zmarch_pair<unsigned>), dim3(blocks), dim3(ctx->n1 / 2), zshm, ctx->stream, A, cf,
+                             fld<R>(ctx, MELT_RU1), fld<R>(ctx, MELT_RU2), fld<R>(ctx, MELT_RU3), fld<R>(ctx, MELT_RUT), r4[0], r4[1],
+                             r4[2], r4[3], sc_u, sc_t, D, gu, gt);
+        else
+          hipLaunchKernelGGL((k_explicit_zmarch_pair<size_t>), dim3(blocks), dim3(ctx->n1 / 2), zshm, ctx->stream, A, cf,
+                             fld<R>(ctx, MELT_RU1), fld<R>(ctx, MELT_RU2), fld<R>(ctx, MELT_RU3), fld<R>(ctx, MELT_RUT), r4[0], r4[1],
+                             r4[2], r4[3], sc_u, sc_t, D, gu, gt);
+        MELT_HIP(hipGetLastError());
+        return 0;
+      }
+    }
     if (ctx->nc * sizeof(R) < (size_t(1) << 32))
       hipLaunchKernelGGL((k_explicit_zmarch<R, unsigned>), dim3(blocks), dim3(ctx->n1), zshm, ctx->stream, A, cf, fld<R>(ctx, MELT_RU1),
                          fld<R>(ctx, MELT_RU2), fld<R>(ctx, MELT_RU3), fld<R>(ctx, MELT_RUT), r4[0], r4[1], r4[2], r4[3], sc_u, sc_t, D, gu, gt);

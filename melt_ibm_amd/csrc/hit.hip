@@ -158,7 +158,7 @@ __global__ __launch_bounds__(256) void k_hit_synth(Grid g, const double* __restr
         const double* E = &tz[c][2 * (n * n3 + k)];
         v = fma(hr[c][n], E[0], fma(-hi[c][n], E[1], v));
       }
-      out[c][o] = (R)v;
+      __builtin_nontemporal_store((R)v, &out[c][o]);   // read once, by the next explicit pass
     }
   }
 }

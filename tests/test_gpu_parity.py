@@ -306,12 +306,13 @@ def test_green_line_solves(gpu_lib, monkeypatch, n, n3):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-@pytest.mark.parametrize("n3,tunes", [(64, ("xf_w=1", "xf_z=1", "xf_z=4", "yz_jb=1", "yz_jb=16", "streams=1", "coef=2")),
+@pytest.mark.parametrize("n3,tunes", [(64, ("xf_w=1", "xf_z=1", "xf_z=4", "xf_one=1", "yz_jb=1", "yz_jb=16", "streams=1", "coef=2")),
                                       (24, ("xf_w=1", "xf_z=3", "yz_jb=1"))])
 def test_zmarch_matches_line_kernel(gpu_lib, monkeypatch, precision, n3, tunes):
     """The explicit pass as a z-march (one x-line per block walking its planes,
-    default where n1 % 64 == 0) and as one x-line per block (xf_w=1) compute
-    the same values with the same expressions, and so do the y and z solves as
+    default where n1 % 64 == 0; in the precision form two x-neighbour cells
+    per thread, or one with xf_one=1) and as one x-line per block (xf_w=1)
+    compute the same values with the same expressions, and so do the y and z solves as
     one pass (default where n3 % 16 == 0) and as two (yz_jb=1), with the
     Lagrangian update, probes and next tagging on a second stream (default) or
     all on one (streams=1), and with the probes' geometry stage queued after
