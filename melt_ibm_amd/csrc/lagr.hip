@@ -30,7 +30,7 @@ struct LagArgs {
 };
 
 // ---------------------------------------------------------- face sums
-enum { SUM_VOLUME = 0, SUM_RIGID = 1, SUM_LOADS = 2, SUM_AREA_VOL_RIGID = 3 };
+enum { SUM_RIGID = 1, SUM_AREA_VOL_RIGID = 3 };
 
 __device__ inline double face_volume(const double* x, const int* vof, int f) {  // triaux.f90:37-45
   int v1 = vof[3 * f], v2 = vof[3 * f + 1], v3 = vof[3 * f + 2];
@@ -63,30 +63,20 @@ __device__ inline void face_rigid(const double* xyz, const int* vof, int f, doub
 // partial to part[q * gridDim + b]; k_sum_partials adds the partials in a
 // fixed tree (a second launch: finishing in the last block would need a
 // device-scope fence per block, which writes back the L2 of every XCD and
-// measured slower).  Output layout per mode:
-//  SUM_VOLUME: [vol]; SUM_RIGID: [10 rigid terms]; SUM_LOADS: [prn 3, rxp 3, tau 3, rxt 3];
-//  SUM_AREA_VOL_RIGID: [area, vol, 10 rigid terms].
-__global__ __launch_bounds__(RB) void k_face_sums(LagArgs A, const double* __restrict__ xyz, const double* __restrict__ rxp,
-                                                  const double* __restrict__ rxt, int mode, double* __restrict__ part) {
+// measured slower).  Output layout per mode: SUM_RIGID: [10 rigid terms];
+// SUM_AREA_VOL_RIGID: [area, vol, 10 rigid terms].  (The motion update's
+// sums have their own pass, k_face_sums_motion.)
+__global__ __launch_bounds__(RB) void k_face_sums(LagArgs A, const double* __restrict__ xyz, int mode, double* __restrict__ part) {
   __shared__ double sm[12][RB];
-  const int ns = mode == SUM_VOLUME ? 1 : mode == SUM_RIGID ? 10 : 12;
+  const int ns = mode == SUM_RIGID ? 10 : 12;
   double acc[12];
   for (int q = 0; q < 12; ++q) acc[q] = 0.0;
   for (int f = blockIdx.x * RB + threadIdx.x; f < A.nf; f += gridDim.x * RB) {
     if (A.gf[f]) continue;
-    if (mode == SUM_VOLUME) {
-      acc[0] = acc[0] + face_volume(xyz, A.vof, f);
-    } else if (mode == SUM_RIGID) {
+    if (mode == SUM_RIGID) {
       double o[10];
       face_rigid(xyz, A.vof, f, o);
       for (int q = 0; q < 10; ++q) acc[q] = acc[q] + o[q];
-    } else if (mode == SUM_LOADS) {
-      for (int d = 0; d < 3; ++d) {
-        acc[d] = acc[d] + A.L[MELT_L_PRESS_N_TRI][3 * f + d];
-        acc[3 + d] = acc[3 + d] + rxp[3 * f + d];
-        acc[6 + d] = acc[6 + d] + A.L[MELT_L_TAU_N][3 * f + d];
-        acc[9 + d] = acc[9 + d] + rxt[3 * f + d];
-      }
     } else {
       acc[0] = acc[0] + A.L[MELT_L_SUR][f];
       acc[1] = acc[1] + face_volume(xyz, A.vof, f);
@@ -108,14 +98,12 @@ __global__ __launch_bounds__(RB) void k_face_sums(LagArgs A, const double* __res
 __device__ void rigid_finish(const double* s, double dens_ratio, double com[3], double* vol, double I[9]);
 
 // Body updates that consume a face sum (run by one thread after stage 2).
-enum { FIN_NONE = 0, FIN_LOADS, FIN_VOLUME, FIN_RIGID, FIN_FINAL, FIN_SURFACE };
+enum { FIN_NONE = 0, FIN_FINAL = 4 };
 
-// Stage 2: out[q] = fixed-pattern sum of the nb <= 64 partials of quantity q
-// (one wave per quantity, butterfly over the lanes), then the body update
-// `fin` on out[].
-__global__ __launch_bounds__(RB) void k_sum_partials(const double* __restrict__ part, int nb, int ns, double* __restrict__ out, int fin,
-                                                     melt_body* body, double dens_ratio, int* err) {
-  __shared__ double sm[16];
+// The fixed-pattern sum of the nb <= 64 partials of quantity q into sm[q]
+// (one wave per quantity, butterfly over the lanes), for q < ns; ends with a
+// block barrier.
+__device__ void reduce_partials(const double* __restrict__ part, int nb, int ns, double* sm) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = RB / 64;
   for (int q = wv; q < ns; q += nw) {
     double v = lane < nb ? part[q * nb + lane] : 0.0;
@@ -124,38 +112,41 @@ __global__ __launch_bounds__(RB) void k_sum_partials(const double* __restrict__ 
     if (lane == 0) sm[q] = v;
   }
   __syncthreads();
-  if ((int)threadIdx.x < ns) out[threadIdx.x] = sm[threadIdx.x];
-  if (threadIdx.x != 0 || fin == FIN_NONE) return;
-  melt_body* b = body;
-  if (fin == FIN_LOADS) {  // mls_structLoads.f90:22-30, 224-256
-    for (int d = 0; d < 3; ++d) {
-      b->int_prn_dA_m1[d] = b->int_prn_dA[d]; b->int_tau_dA_m1[d] = b->int_tau_dA[d];
-      b->int_r_x_prn_dA_m1[d] = b->int_r_x_prn_dA[d]; b->int_r_x_tau_dA_m1[d] = b->int_r_x_tau_dA[d];
-      b->int_prn_dA[d] = sm[d]; b->int_r_x_prn_dA[d] = sm[3 + d];
-      b->int_tau_dA[d] = sm[6 + d]; b->int_r_x_tau_dA[d] = sm[9 + d];
-      b->fp[d] = b->fp[d] + b->int_prn_dA[d]; b->ftau[d] = b->ftau[d] + b->int_tau_dA[d];
-      b->torq_p[d] = b->torq_p[d] + b->int_r_x_prn_dA[d]; b->torq_tau[d] = b->torq_tau[d] + b->int_r_x_tau_dA[d];
-    }
-  } else if (fin == FIN_VOLUME) {
-    b->volume = sm[0] / 6.;
-  } else if (fin == FIN_RIGID) {
-    rigid_finish(sm, dens_ratio, b->pos_cm, &b->volume, b->inert_tensor);
-  } else if (fin == FIN_FINAL) {
-    b->surface = sm[0];
-    b->volume = sm[1] / 6.;
-    rigid_finish(&sm[2], dens_ratio, b->pos_cm, &b->volume, b->inert_tensor);
-    if (b->volume < b->v_thresh) raise_error(err, MELT_ERR_SMALL_VOLUME);
-  } else if (fin == FIN_SURFACE) {
-    b->surface = sm[0];
+}
+
+// mls_structLoads.f90:22-30, 224-256: the integrated loads of this substep
+// from the four face sums s[0..11], the previous ones kept
+__device__ void loads_finish(melt_body* b, const double* s) {
+  for (int d = 0; d < 3; ++d) {
+    b->int_prn_dA_m1[d] = b->int_prn_dA[d]; b->int_tau_dA_m1[d] = b->int_tau_dA[d];
+    b->int_r_x_prn_dA_m1[d] = b->int_r_x_prn_dA[d]; b->int_r_x_tau_dA_m1[d] = b->int_r_x_tau_dA[d];
+    b->int_prn_dA[d] = s[d]; b->int_r_x_prn_dA[d] = s[3 + d];
+    b->int_tau_dA[d] = s[6 + d]; b->int_r_x_tau_dA[d] = s[9 + d];
+    b->fp[d] = b->fp[d] + b->int_prn_dA[d]; b->ftau[d] = b->ftau[d] + b->int_tau_dA[d];
+    b->torq_p[d] = b->torq_p[d] + b->int_r_x_prn_dA[d]; b->torq_tau[d] = b->torq_tau[d] + b->int_r_x_tau_dA[d];
   }
+}
+
+// Stage 2: out[q] = the sum of quantity q, then the body update `fin`.
+__global__ __launch_bounds__(RB) void k_sum_partials(const double* __restrict__ part, int nb, int ns, double* __restrict__ out, int fin,
+                                                     melt_body* body, double dens_ratio, int* err) {
+  __shared__ double sm[16];
+  reduce_partials(part, nb, ns, sm);
+  if ((int)threadIdx.x < ns) out[threadIdx.x] = sm[threadIdx.x];
+  if (threadIdx.x != 0 || fin != FIN_FINAL) return;
+  melt_body* b = body;
+  b->surface = sm[0];
+  b->volume = sm[1] / 6.;
+  rigid_finish(&sm[2], dens_ratio, b->pos_cm, &b->volume, b->inert_tensor);
+  if (b->volume < b->v_thresh) raise_error(err, MELT_ERR_SMALL_VOLUME);
 }
 
 // Two-stage face sum of `mode` into out[] (d_red scratch holds the partials).
 void face_sums(melt_ctx* ctx, const LagArgs& A, const double* xyz, int mode, double* out, int fin = FIN_NONE) {
   int G = std::max(1, std::min(64, (A.nf + RB - 1) / RB));
-  int ns = mode == SUM_VOLUME ? 1 : mode == SUM_RIGID ? 10 : 12;
+  int ns = mode == SUM_RIGID ? 10 : 12;
   double* part = ctx->d_red + 8192;
-  hipLaunchKernelGGL(k_face_sums, dim3(G), dim3(RB), 0, ctx->stream, A, xyz, ctx->d_rxp, ctx->d_rxt, mode, part);
+  hipLaunchKernelGGL(k_face_sums, dim3(G), dim3(RB), 0, ctx->stream, A, xyz, mode, part);
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(RB), 0, ctx->stream, part, G, ns, out, fin, ctx->d_body, ctx->cfg.dens_ratio, ctx->d_err);
 }
 
@@ -254,23 +245,29 @@ __global__ __launch_bounds__(256) void k_vert_heat_stefan(LagArgs A, const melt_
 }
 
 // ---------------------------------------------------------- geometry
+// calc_centroids_from_vert (triaux.f90:247-262) of face f from the vertices
+__device__ inline void face_centroid(const LagArgs& A, int f) {
+  int v1 = A.vof[3 * f], v2 = A.vof[3 * f + 1], v3 = A.vof[3 * f + 2];
+  for (int d = 0; d < 3; ++d) A.L[MELT_L_TRI_BAR][3 * f + d] = (A.L[MELT_L_XYZV][3 * v1 + d] + A.L[MELT_L_XYZV][3 * v2 + d] + A.L[MELT_L_XYZV][3 * v3 + d]) / 3.0;
+}
+// update_tri_normal / calculate_normal of face f from the points xyz
+__device__ inline void face_normal(const LagArgs& A, const double* __restrict__ xyz, int f) {
+  int v1 = A.vof[3 * f], v2 = A.vof[3 * f + 1], v3 = A.vof[3 * f + 2];
+  double e1[3], e2[3], n[3];
+  for (int d = 0; d < 3; ++d) { e1[d] = xyz[3 * v2 + d] - xyz[3 * v1 + d]; e2[d] = xyz[3 * v3 + d] - xyz[3 * v1 + d]; }
+  n[0] = e1[1] * e2[2] - e1[2] * e2[1];
+  n[1] = e1[2] * e2[0] - e1[0] * e2[2];
+  n[2] = e1[0] * e2[1] - e1[1] * e2[0];
+  double s = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  for (int d = 0; d < 3; ++d) A.L[MELT_L_TRI_NOR][3 * f + d] = n[d] / s;
+}
+
 __global__ __launch_bounds__(256) void k_face_centroid_normal(LagArgs A, const double* __restrict__ xyz, int do_bar, int do_nor, int ghost_check) {
   int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= A.nf) return;
   bool ghost = A.gf[f] != 0;
-  int v1 = A.vof[3 * f], v2 = A.vof[3 * f + 1], v3 = A.vof[3 * f + 2];
-  if (do_bar && !ghost) {  // calc_centroids_from_vert triaux.f90:247-262
-    for (int d = 0; d < 3; ++d) A.L[MELT_L_TRI_BAR][3 * f + d] = (A.L[MELT_L_XYZV][3 * v1 + d] + A.L[MELT_L_XYZV][3 * v2 + d] + A.L[MELT_L_XYZV][3 * v3 + d]) / 3.0;
-  }
-  if (do_nor && !(ghost_check && ghost)) {  // update_tri_normal / calculate_normal
-    double e1[3], e2[3], n[3];
-    for (int d = 0; d < 3; ++d) { e1[d] = xyz[3 * v2 + d] - xyz[3 * v1 + d]; e2[d] = xyz[3 * v3 + d] - xyz[3 * v1 + d]; }
-    n[0] = e1[1] * e2[2] - e1[2] * e2[1];
-    n[1] = e1[2] * e2[0] - e1[0] * e2[2];
-    n[2] = e1[0] * e2[1] - e1[1] * e2[0];
-    double s = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-    for (int d = 0; d < 3; ++d) A.L[MELT_L_TRI_NOR][3 * f + d] = n[d] / s;
-  }
+  if (do_bar && !ghost) face_centroid(A, f);
+  if (do_nor && !(ghost_check && ghost)) face_normal(A, xyz, f);
 }
 
 __global__ __launch_bounds__(256) void k_face_area(LagArgs A, const double* __restrict__ x) {  // calculate_area triaux.f90:53-91
@@ -484,31 +481,102 @@ __device__ void newton_update(melt_body* b, double ga, double ro, double al, dou
   for (int i = 0; i < 9; ++i) aa_out[i] = AA[i];
 }
 
-// Relative vertex and face positions (RigidAuxRoutines.f90:20-35).
-__global__ __launch_bounds__(256) void k_rel(LagArgs A, const melt_body* __restrict__ b) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < A.nv) {
-    if (!A.gv[t])
-      for (int d = 0; d < 3; ++d) A.L[MELT_L_DXYZV_S][3 * t + d] = A.L[MELT_L_XYZV][3 * t + d] - b->pos_cm[d];
-  } else if (t < A.nv + A.nf) {
-    int f = t - A.nv;
-    if (!A.gf[f])
-      for (int d = 0; d < 3; ++d) A.L[MELT_L_DXYZ_S][3 * f + d] = A.L[MELT_L_TRI_BAR][3 * f + d] - b->pos_cm[d];
+// The face sums of the motion update in one pass over the faces: the loads
+// (mls_structLoads.f90:224-256; L), the volume after melting
+// (particle.f90:86-90; V) and the rigid-body terms of calc_rigidBody_params
+// (RigidAuxRoutines.f90:685-712; with L, as the motion needs them), and with
+// L the centroids and normals of the live faces from the melted vertices
+// (calc_centroids_from_vert, update_tri_normal).  The loads read only the
+// probe results, which the Stefan update before this pass does not touch.
+// Every quantity is accumulated over the same faces per thread and reduced
+// by the same tree as in its own k_face_sums pass, so each sum is
+// bit-identical to it: one launch where there were three, and one finishing
+// launch (k_sum_motion) instead of three.  Partials: [loads 12][volume 1]
+// [rigid 10], those present, quantity-major as in k_face_sums.
+template <bool L, bool V>
+__global__ __launch_bounds__(RB) void k_face_sums_motion(LagArgs A, const double* __restrict__ xyz, const double* __restrict__ rxp,
+                                                         const double* __restrict__ rxt, double* __restrict__ part) {
+  constexpr int QV = L ? 12 : 0, QG = QV + (V ? 1 : 0), NS = QG + (L ? 10 : 0);
+  __shared__ double sm[NS][RB];
+  double acc[NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) acc[q] = 0.0;
+  for (int f = blockIdx.x * RB + threadIdx.x; f < A.nf; f += gridDim.x * RB) {
+    if (A.gf[f]) continue;
+    if constexpr (L) {
+      for (int d = 0; d < 3; ++d) {
+        acc[d] = acc[d] + A.L[MELT_L_PRESS_N_TRI][3 * f + d];
+        acc[3 + d] = acc[3 + d] + rxp[3 * f + d];
+        acc[6 + d] = acc[6 + d] + A.L[MELT_L_TAU_N][3 * f + d];
+        acc[9 + d] = acc[9 + d] + rxt[3 * f + d];
+      }
+    }
+    if constexpr (V) acc[QV] = acc[QV] + face_volume(xyz, A.vof, f);
+    if constexpr (L) {
+      double o[10];
+      face_rigid(xyz, A.vof, f, o);
+#pragma unroll
+      for (int q = 0; q < 10; ++q) acc[QG + q] = acc[QG + q] + o[q];
+      face_centroid(A, f);
+      face_normal(A, xyz, f);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NS; ++q) sm[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int w = RB / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int q = 0; q < NS; ++q) sm[q][threadIdx.x] = sm[q][threadIdx.x] + sm[q][threadIdx.x + w];
+    __syncthreads();
+  }
+  if ((int)threadIdx.x < NS) part[threadIdx.x * gridDim.x + blockIdx.x] = sm[threadIdx.x][0];
+}
+
+// Stage 2 of k_face_sums_motion and the body updates that consume it, in the
+// reference's order on one thread: the loads, the volume after melting, the
+// rigid-body terms (centre of mass, inertia) and, with L, the Newton-Euler
+// step (its rotation into aa_out).  com_out keeps the centre of mass the step
+// started from, which the relative positions are taken against.
+template <bool L, bool V>
+__global__ __launch_bounds__(RB) void k_sum_motion(const double* __restrict__ part, int nb, double* __restrict__ out, melt_body* body,
+                                                   double dens_ratio, double ga, double ro, double al, double dt, double xlen, double ylen,
+                                                   double zlen, double* __restrict__ aa_out, double* __restrict__ com_out, int* err) {
+  constexpr int QV = L ? 12 : 0, QG = QV + (V ? 1 : 0), NS = QG + (L ? 10 : 0);
+  __shared__ double sm[NS];
+  reduce_partials(part, nb, NS, sm);
+  if ((int)threadIdx.x < NS) out[threadIdx.x] = sm[threadIdx.x];
+  if (threadIdx.x != 0) return;
+  melt_body* b = body;
+  if constexpr (L) loads_finish(b, sm);
+  if constexpr (V) b->volume = sm[QV] / 6.;
+  if constexpr (L) {
+    rigid_finish(&sm[QG], dens_ratio, b->pos_cm, &b->volume, b->inert_tensor);
+    for (int d = 0; d < 3; ++d) com_out[d] = b->pos_cm[d];
+    newton_update(b, ga, ro, al, dt, dens_ratio, xlen, ylen, zlen, aa_out, err);
   }
 }
 
-__global__ void k_newton(melt_body* b, double ga, double ro, double al, double dt, double dens_ratio, double xlen, double ylen, double zlen,
-                         double* __restrict__ aa_out, int* err) {
-  newton_update(b, ga, ro, al, dt, dens_ratio, xlen, ylen, zlen, aa_out, err);
-}
-
-__global__ __launch_bounds__(256) void k_update_xyz(LagArgs A, const melt_body* __restrict__ b, const double* __restrict__ AA) {
-  int v = blockIdx.x * blockDim.x + threadIdx.x;  // update_xyz RigidAuxRoutines.f90:858-862
-  if (v >= A.nv || A.gv[v]) return;
-  double AT[9], t[3];
-  for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) AT[i + 3 * j] = AA[j + 3 * i];
-  mv3(AT, &A.L[MELT_L_DXYZV_S][3 * v], t);
-  for (int d = 0; d < 3; ++d) A.L[MELT_L_XYZV][3 * v + d] = t[d] + b->pos_cm[d];
+// Relative vertex and face positions against the centre of mass com before
+// the Newton-Euler step (RigidAuxRoutines.f90:20-35), and the vertices moved
+// by its rotation AA and new centre (update_xyz, :858-862), in one pass.
+__global__ __launch_bounds__(256) void k_rel_update(LagArgs A, const melt_body* __restrict__ b, const double* __restrict__ com,
+                                                    const double* __restrict__ AA) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < A.nv) {
+    if (A.gv[t]) return;
+    double* r = &A.L[MELT_L_DXYZV_S][3 * t];
+    double* x = &A.L[MELT_L_XYZV][3 * t];
+    double rel[3], AT[9], u[3];
+    for (int d = 0; d < 3; ++d) rel[d] = x[d] - com[d];
+    for (int d = 0; d < 3; ++d) r[d] = rel[d];
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) AT[i + 3 * j] = AA[j + 3 * i];
+    mv3(AT, rel, u);
+    for (int d = 0; d < 3; ++d) x[d] = u[d] + b->pos_cm[d];
+  } else if (t < A.nv + A.nf) {
+    int f = t - A.nv;
+    if (!A.gf[f])
+      for (int d = 0; d < 3; ++d) A.L[MELT_L_DXYZ_S][3 * f + d] = A.L[MELT_L_TRI_BAR][3 * f + d] - com[d];
+  }
 }
 
 __global__ __launch_bounds__(256) void k_face_vel(LagArgs A, const melt_body* __restrict__ b) {  // update_xyz :864-872
@@ -606,6 +674,19 @@ static int lagr_finish(melt_ctx* ctx) {
   return 0;
 }
 
+// k_face_sums_motion and k_sum_motion: the sums into d_red[0..22], the
+// Newton-Euler rotation into d_red[32..40], the centre of mass it started
+// from into d_red[48..50].
+template <bool L, bool V>
+static void motion_sums(melt_ctx* ctx, const LagArgs& A) {
+  const int G = std::max(1, std::min(64, (A.nf + RB - 1) / RB));
+  double* part = ctx->d_red + 8192;
+  hipLaunchKernelGGL((k_face_sums_motion<L, V>), dim3(G), dim3(RB), 0, ctx->stream, A, ctx->d_lag[MELT_L_XYZV], ctx->d_rxp, ctx->d_rxt,
+                     part);
+  hipLaunchKernelGGL((k_sum_motion<L, V>), dim3(1), dim3(RB), 0, ctx->stream, part, G, ctx->d_red, ctx->d_body, ctx->cfg.dens_ratio, ctx->ga,
+                     ctx->ro, ctx->al, ctx->dt, ctx->cfg.xlen, ctx->cfg.ylen, ctx->cfg.zlen, ctx->d_red + 32, ctx->d_red + 48, ctx->d_err);
+}
+
 // particle.f90:62-193 after the MLS interpolation: heat flux -> vertices,
 // loads, Stefan update, Newton-Euler motion, (remeshing,) geometry refresh.
 int lagr_after_force(melt_ctx* ctx, bool commit_pind) {
@@ -615,25 +696,21 @@ int lagr_after_force(melt_ctx* ctx, bool commit_pind) {
   if (heat || loads)
     hipLaunchKernelGGL(k_face_post, dim3(nb(ctx->nf, 256)), dim3(256), 0, s, A, ctx->d_probe, ctx->d_body, ctx->d_rxp, ctx->d_rxt, ctx->cfg.ren,
                        heat, loads, ctx->d_pind, commit_pind && heat ? ctx->d_pind2 : nullptr);
-  if (loads) {
-    face_sums(ctx, A, ctx->d_lag[MELT_L_XYZV], (int)SUM_LOADS, ctx->d_red, FIN_LOADS);
-  }
   if (heat) {
     hipLaunchKernelGGL(k_vert_heat_stefan, dim3(nb(ctx->nv, 256)), dim3(256), 0, s, A, ctx->d_body, ctx->cfg.cpliquid / ctx->cfg.latheat, ctx->dt,
                        ctx->ga, ctx->ro, 1);
-    face_sums(ctx, A, ctx->d_lag[MELT_L_XYZV], (int)SUM_VOLUME, ctx->d_red, FIN_VOLUME);
     // the volume after melting is the one the remeshing restores (particle.f90:86-90)
     if (ctx->cfg.iremesh == 1)
       MELT_HIP(hipMemcpyAsync(ctx->d_xyz_melt, ctx->d_lag[MELT_L_XYZV], sizeof(double) * 3 * ctx->nv, hipMemcpyDeviceToDevice, s));
   }
+  // the loads (the reference sums them before the Stefan update, which does
+  // not touch what they read), the volume after melting, the rigid-body terms
+  // and the Newton-Euler step: one face pass and one finishing launch
+  if (loads && heat) motion_sums<true, true>(ctx, A);
+  else if (loads) motion_sums<true, false>(ctx, A);
+  else if (heat) motion_sums<false, true>(ctx, A);
   if (loads) {
-    const double* x = ctx->d_lag[MELT_L_XYZV];
-    hipLaunchKernelGGL(k_face_centroid_normal, dim3(nb(ctx->nf, 256)), dim3(256), 0, s, A, x, 1, 1, 1);
-    face_sums(ctx, A, x, (int)SUM_RIGID, ctx->d_red, FIN_RIGID);
-    hipLaunchKernelGGL(k_rel, dim3(nb((size_t)ctx->nv + ctx->nf, 256)), dim3(256), 0, s, A, ctx->d_body);
-    hipLaunchKernelGGL(k_newton, dim3(1), dim3(1), 0, s, ctx->d_body, ctx->ga, ctx->ro, ctx->al, ctx->dt, ctx->cfg.dens_ratio, ctx->cfg.xlen,
-                       ctx->cfg.ylen, ctx->cfg.zlen, ctx->d_red + 32, ctx->d_err);
-    hipLaunchKernelGGL(k_update_xyz, dim3(nb(ctx->nv, 256)), dim3(256), 0, s, A, ctx->d_body, ctx->d_red + 32);
+    hipLaunchKernelGGL(k_rel_update, dim3(nb((size_t)ctx->nv + ctx->nf, 256)), dim3(256), 0, s, A, ctx->d_body, ctx->d_red + 48, ctx->d_red + 32);
     hipLaunchKernelGGL(k_face_vel, dim3(nb(ctx->nf, 256)), dim3(256), 0, s, A, ctx->d_body);
   }
   MELT_HIP(hipGetLastError());
