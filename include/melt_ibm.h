@@ -151,7 +151,7 @@ int melt_upload_field_device(melt_ctx* ctx, int field, const double* src);
 
 /* Mesh import: set_connectivity + setup_particles + init_geomCoords + the
  * geometry set-up of tri_geo + init_mlsForce (partaux.f90:160-267,
- * create_geo.f90:1-70, tri_geo.f90:1-50, initmls.f90:1-27).  xyz0 is the
+ * create_geo.f90:1-132, tri_geo.f90:1-50, initmls.f90:1-27).  xyz0 is the
  * unit mesh as read from the GTS file (3*nv, xyz interleaved), scaled by rad_p and
  * centred in the box; vert_of_edge (2*ne) and edge_of_face (3*nf) are 1-based. */
 int melt_set_geometry(melt_ctx* ctx, int32_t nv, int32_t ne, int32_t nf,

@@ -824,13 +824,19 @@ static int substep_impl(melt_ctx* ctx, int32_t ns, bool tag_next, bool coef_next
     return r;
   };
   const std::function<int()> tag_fork = [&]() -> int { return side2_run(ctx, tag_now); };
+  auto queue_coef = [&]() -> int {
+    int r = ibm_probe_coefs(ctx);
+    if (!r) ctx->coef_ahead = 1;
+    return r;
+  };
+  // within a tschem call (tag_next) the main stream does not wait for the
+  // probe geometry stage: only the next substep's probes read its output, and
+  // they run on the side stream after it; the call's last substep joins it
+  const bool coef_after_join = tag_next && coef_next && probes && !one_stream(ctx);
   auto queue_next = [&]() -> int {
     int r = 0;
     if (tag_next && moving && !ctx->tag_ahead) r = tag_now();
-    if (!r && coef_next && probes) {
-      r = ibm_probe_coefs(ctx);
-      if (!r) ctx->coef_ahead = 1;
-    }
+    if (!r && coef_next && probes && !coef_after_join) r = queue_coef();
     return r;
   };
   if (ibm) {
@@ -893,6 +899,13 @@ static int substep_impl(melt_ctx* ctx, int32_t ns, bool tag_next, bool coef_next
                 return r ? r : queue_next();
               })))
     return rc;
+  if (ibm && coef_after_join) {   // on the side stream after its join point
+    hipStream_t main_stream = ctx->stream;
+    ctx->stream = ctx->side;
+    rc = queue_coef();
+    ctx->stream = main_stream;
+    if (rc) return rc;
+  }
   if (ibm && (rc = side_join(ctx))) return rc;
   timing_end(ctx, SLOT_SUBSTEP, e_sub);
   if (ctx->timing && ctx->ev_pending.size() > 4096) timing_flush(ctx);

@@ -269,12 +269,12 @@ struct Win3 { int e[3]; };
 // supports (base-1..base+2) and the tagging band (base-2..base+2) touch.
 // Min / max of the face base indices over all live faces: one block per 256
 // faces, combined with integer atomics (order-independent).
-__global__ __launch_bounds__(256) void k_window_box(Grid g, const double* __restrict__ bar, const unsigned char* __restrict__ gf, int nf,
-                                                    melt_window* win) {
+__device__ void window_box_block(const Grid& g, const double* __restrict__ bar, const unsigned char* __restrict__ gf, int nf, melt_window* win,
+                                 int blk) {
   __shared__ int mn[3], mx[3];
   if (threadIdx.x < 3) { mn[threadIdx.x] = INT_MAX; mx[threadIdx.x] = INT_MIN; }
   __syncthreads();
-  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  int f = blk * blockDim.x + threadIdx.x;
   if (f < nf && !gf[f]) {
     const double dx[3] = {g.dx1, g.dx2, g.dx3};
     for (int d = 0; d < 3; ++d) {
@@ -288,6 +288,10 @@ __global__ __launch_bounds__(256) void k_window_box(Grid g, const double* __rest
     atomicMin(&win->bmin[threadIdx.x], mn[threadIdx.x]);
     atomicMax(&win->bmax[threadIdx.x], mx[threadIdx.x]);
   }
+}
+__global__ __launch_bounds__(256) void k_window_box(Grid g, const double* __restrict__ bar, const unsigned char* __restrict__ gf, int nf,
+                                                    melt_window* win) {
+  window_box_block(g, bar, gf, nf, win, blockIdx.x);
 }
 
 // Initial state of the windows: no box placed yet, empty extents.
@@ -382,18 +386,33 @@ __device__ double dist_point_tri(const double P[3], const double* V0, const doub
   return sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
 }
 
+// The tagging's set-up in one launch: blocks [0, nbox) reduce the face
+// extent (as k_window_box), the other 256 set sdf to 999 over the window the
+// previous tagging wrote (that tagging's cells are re-tagged from scratch)
+// and clear the per-cell keys.  This runs before k_window_final moves the
+// tagging window, so it takes the previous window from where
+// k_window_final's update leaves it (prev_org := org, prev_valid := pad,
+// while prev_valid >= 0).
 template <typename R>
-__global__ __launch_bounds__(256) void k_tag_reset(Grid g, R* __restrict__ sdf, const melt_window* __restrict__ win, Win3 et) {
-  if (win[1].prev_valid <= 0) return;
-  size_t n = (size_t)et.e[0] * et.e[1] * et.e[2];
-  for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += (size_t)gridDim.x * blockDim.x) {
-    int a = (int)(s % et.e[0]), b = (int)((s / et.e[0]) % et.e[1]), c = (int)(s / ((size_t)et.e[0] * et.e[1]));
-    sdf[widx(g, win[1].prev_org[0] + a, win[1].prev_org[1] + b, win[1].prev_org[2] + c)] = R(999.0);
+__global__ __launch_bounds__(256) void k_tag_prepare(Grid g, const double* __restrict__ bar, const unsigned char* __restrict__ gf, int nf,
+                                                     int nbox, melt_window* win, R* __restrict__ sdf, Win3 et,
+                                                     unsigned long long* __restrict__ key, int* __restrict__ face, size_t ncell) {
+  if ((int)blockIdx.x < nbox) {
+    window_box_block(g, bar, gf, nf, win, blockIdx.x);
+    return;
   }
-}
-
-__global__ __launch_bounds__(256) void k_tag_init(unsigned long long* __restrict__ key, int* __restrict__ face, size_t n) {
-  for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += (size_t)gridDim.x * blockDim.x) {
+  const size_t t0 = (size_t)(blockIdx.x - nbox) * blockDim.x + threadIdx.x, step = (size_t)(gridDim.x - nbox) * blockDim.x;
+  const melt_window& w = win[1];
+  const bool moved = w.prev_valid >= 0;
+  if ((moved ? w.pad : w.prev_valid) > 0) {
+    const int o0 = moved ? w.org[0] : w.prev_org[0], o1 = moved ? w.org[1] : w.prev_org[1], o2 = moved ? w.org[2] : w.prev_org[2];
+    const size_t n = (size_t)et.e[0] * et.e[1] * et.e[2];
+    for (size_t s = t0; s < n; s += step) {
+      int a = (int)(s % et.e[0]), b = (int)((s / et.e[0]) % et.e[1]), c = (int)(s / ((size_t)et.e[0] * et.e[1]));
+      sdf[widx(g, o0 + a, o1 + b, o2 + c)] = R(999.0);
+    }
+  }
+  for (size_t s = t0; s < ncell; s += step) {
     key[s] = ~0ull;
     face[s] = INT_MAX;
   }
@@ -1163,14 +1182,14 @@ static int ibm_tag_t(melt_ctx* ctx) {  // tagCells.f90:4-27 (tagType 1) + interp
     MELT_HIP(hipMemcpyAsync(&ctx->d_body->initial_tag, &ctx->h_body.initial_tag, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
   }
   Win3 ef{{ctx->fwin[0], ctx->fwin[1], ctx->fwin[2]}}, et{{ctx->twin[0], ctx->twin[1], ctx->twin[2]}};
-  hipLaunchKernelGGL(k_window_box, dim3(nb(ctx->nf, 256)), dim3(256), 0, ctx->stream, g, ctx->d_lag[MELT_L_TRI_BAR], ctx->d_gf, ctx->nf,
-                     ctx->d_win);
+  // face extent, reset of the previous window, cleared keys: one launch
+  const int nbox = (int)nb(ctx->nf, 256);
+  hipLaunchKernelGGL(k_tag_prepare<R>, dim3(nbox + 256), dim3(256), 0, ctx->stream, g, ctx->d_lag[MELT_L_TRI_BAR], ctx->d_gf, ctx->nf, nbox,
+                     ctx->d_win, fld<R>(ctx, MELT_SDF), et, ctx->d_tkey, ctx->d_tface, ctx->twin_cells);
   // the fused substep places the forcing window from the same extent (the
   // faces do not move between tagging and forcing)
   hipLaunchKernelGGL(k_window_final, dim3(1), dim3(1), 0, ctx->stream, ctx->d_win, ef, et, ctx->force_window_from_tag ? 3 : 2,
                      ctx->d_err);
-  hipLaunchKernelGGL(k_tag_reset<R>, dim3(256), dim3(256), 0, ctx->stream, g, fld<R>(ctx, MELT_SDF), ctx->d_win, et);
-  hipLaunchKernelGGL(k_tag_init, dim3(256), dim3(256), 0, ctx->stream, ctx->d_tkey, ctx->d_tface, ctx->twin_cells);
   MeshArgs M = mesh_args(ctx);
   unsigned blocks = nb((size_t)ctx->nf * 125, 256);
   for (int mode = 0; mode < 3; ++mode)

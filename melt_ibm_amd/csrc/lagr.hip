@@ -32,7 +32,7 @@ struct LagArgs {
 // ---------------------------------------------------------- face sums
 enum { SUM_RIGID = 1, SUM_AREA_VOL_RIGID = 3 };
 
-__device__ inline double face_volume(const double* x, const int* vof, int f) {  // triaux.f90:37-45
+__device__ inline double face_volume(const double* x, const int* vof, int f) {  // calculate_volume triaux.f90:53-80
   int v1 = vof[3 * f], v2 = vof[3 * f + 1], v3 = vof[3 * f + 2];
   return x[3 * v1] * (x[3 * v2 + 1] * x[3 * v3 + 2] - x[3 * v2 + 2] * x[3 * v3 + 1]) +
          x[3 * v2] * (x[3 * v3 + 1] * x[3 * v1 + 2] - x[3 * v3 + 2] * x[3 * v1 + 1]) +
@@ -66,12 +66,13 @@ __device__ inline void face_rigid(const double* xyz, const int* vof, int f, doub
 // measured slower).  Output layout per mode: SUM_RIGID: [10 rigid terms];
 // SUM_AREA_VOL_RIGID: [area, vol, 10 rigid terms].  (The motion update's
 // sums have their own pass, k_face_sums_motion.)
-__global__ __launch_bounds__(RB) void k_face_sums(LagArgs A, const double* __restrict__ xyz, int mode, double* __restrict__ part) {
+// One block (blk of nblk) of stage 1.
+__device__ void face_sums_block(const LagArgs& A, const double* __restrict__ xyz, int mode, double* __restrict__ part, int blk, int nblk) {
   __shared__ double sm[12][RB];
   const int ns = mode == SUM_RIGID ? 10 : 12;
   double acc[12];
   for (int q = 0; q < 12; ++q) acc[q] = 0.0;
-  for (int f = blockIdx.x * RB + threadIdx.x; f < A.nf; f += gridDim.x * RB) {
+  for (int f = blk * RB + threadIdx.x; f < A.nf; f += nblk * RB) {
     if (A.gf[f]) continue;
     if (mode == SUM_RIGID) {
       double o[10];
@@ -92,7 +93,10 @@ __global__ __launch_bounds__(RB) void k_face_sums(LagArgs A, const double* __res
       for (int q = 0; q < ns; ++q) sm[q][threadIdx.x] = sm[q][threadIdx.x] + sm[q][threadIdx.x + w];
     __syncthreads();
   }
-  if ((int)threadIdx.x < ns) part[threadIdx.x * gridDim.x + blockIdx.x] = sm[threadIdx.x][0];
+  if ((int)threadIdx.x < ns) part[threadIdx.x * nblk + blk] = sm[threadIdx.x][0];
+}
+__global__ __launch_bounds__(RB) void k_face_sums(LagArgs A, const double* __restrict__ xyz, int mode, double* __restrict__ part) {
+  face_sums_block(A, xyz, mode, part, blockIdx.x, gridDim.x);
 }
 
 __device__ void rigid_finish(const double* s, double dens_ratio, double com[3], double* vol, double I[9]);
@@ -245,7 +249,7 @@ __global__ __launch_bounds__(256) void k_vert_heat_stefan(LagArgs A, const melt_
 }
 
 // ---------------------------------------------------------- geometry
-// calc_centroids_from_vert (triaux.f90:247-262) of face f from the vertices
+// calc_centroids_from_vert (triaux.f90:308-323) of face f from the vertices
 __device__ inline void face_centroid(const LagArgs& A, int f) {
   int v1 = A.vof[3 * f], v2 = A.vof[3 * f + 1], v3 = A.vof[3 * f + 2];
   for (int d = 0; d < 3; ++d) A.L[MELT_L_TRI_BAR][3 * f + d] = (A.L[MELT_L_XYZV][3 * v1 + d] + A.L[MELT_L_XYZV][3 * v2 + d] + A.L[MELT_L_XYZV][3 * v3 + d]) / 3.0;
@@ -270,7 +274,7 @@ __global__ __launch_bounds__(256) void k_face_centroid_normal(LagArgs A, const d
   if (do_nor && !(ghost_check && ghost)) face_normal(A, xyz, f);
 }
 
-__global__ __launch_bounds__(256) void k_face_area(LagArgs A, const double* __restrict__ x) {  // calculate_area triaux.f90:53-91
+__global__ __launch_bounds__(256) void k_face_area(LagArgs A, const double* __restrict__ x) {  // calculate_area triaux.f90:82-119
   int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= A.nf || A.gf[f]) return;
   int v[3] = {A.vof[3 * f], A.vof[3 * f + 1], A.vof[3 * f + 2]};
@@ -284,13 +288,17 @@ __global__ __launch_bounds__(256) void k_face_area(LagArgs A, const double* __re
   A.L[MELT_L_SUR][f] = sqrt(sp * (sp - dd[0]) * (sp - dd[1]) * (sp - dd[2]));
 }
 
-__global__ __launch_bounds__(256) void k_edge_len(LagArgs A, const double* __restrict__ x, melt_body* b, double e_thresh) {  // triaux.f90:217-245
+__device__ inline double edge_length(const LagArgs& A, const double* __restrict__ x, int e) {  // calculate_eLengths triaux.f90:277-305
+  int v1 = A.voe[2 * e], v2 = A.voe[2 * e + 1];
+  double a = x[3 * v1] - x[3 * v2], c = x[3 * v1 + 1] - x[3 * v2 + 1], d = x[3 * v1 + 2] - x[3 * v2 + 2];
+  return sqrt(a * a + c * c + d * d);
+}
+
+__global__ __launch_bounds__(256) void k_edge_len(LagArgs A, const double* __restrict__ x, melt_body* b, double e_thresh) {  // calculate_eLengths triaux.f90:277-305
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   bool short_edge = false;
   if (e < A.ne && !A.ge[e]) {
-    int v1 = A.voe[2 * e], v2 = A.voe[2 * e + 1];
-    double a = x[3 * v1] - x[3 * v2], c = x[3 * v1 + 1] - x[3 * v2 + 1], d = x[3 * v1 + 2] - x[3 * v2 + 2];
-    double l = sqrt(a * a + c * c + d * d);
+    double l = edge_length(A, x, e);
     A.L[MELT_L_ELENGTHS][e] = l;
     short_edge = l < e_thresh;
   }
@@ -300,7 +308,7 @@ __global__ __launch_bounds__(256) void k_edge_len(LagArgs A, const double* __res
 
 __global__ void k_clear_rm(melt_body* b) { b->rm_flag = 0; }
 
-// calculate_area (triaux.f90:53-91), calc_centroids_from_vert (:247-262) and
+// calculate_area (triaux.f90:82-119), calc_centroids_from_vert (:308-323) and
 // update_tri_normal of every live face in one pass (the same expressions as
 // k_face_area and k_face_centroid_normal); thread 0 also clears the remesh
 // flag that k_edge_len then sets.
@@ -328,7 +336,7 @@ __global__ __launch_bounds__(256) void k_face_geom(LagArgs A, const double* __re
   for (int d = 0; d < 3; ++d) A.L[MELT_L_TRI_NOR][3 * f + d] = n[d] / s;
 }
 
-__global__ __launch_bounds__(256) void k_skewness(LagArgs A) {  // triaux.f90:264-290
+__global__ __launch_bounds__(256) void k_skewness(LagArgs A) {  // calculate_skewness triaux.f90:325-358
   int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= A.nf || A.gf[f]) return;
   const double* el = A.L[MELT_L_ELENGTHS];
@@ -338,7 +346,7 @@ __global__ __launch_bounds__(256) void k_skewness(LagArgs A) {  // triaux.f90:26
   A.L[MELT_L_SKEWNESS][f] = (sopt - A.L[MELT_L_SUR][f]) / sopt;
 }
 
-// vertex areas (triaux.f90:93-116) and area-weighted normals (:183-215)
+// vertex areas (triaux.f90:121-150) and area-weighted normals (:237-274)
 __global__ __launch_bounds__(256) void k_vert_geom(LagArgs A) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= A.nv) return;
@@ -363,23 +371,15 @@ __global__ __launch_bounds__(256) void k_cfac(LagArgs A, const melt_body* __rest
   A.L[MELT_L_CFAC][f] = (A.L[MELT_L_SUR][f] * b->h_eulerian) / b->celvol;
 }
 
-// k_skewness and (do_cfac) k_cfac over the faces and k_vert_geom over the
-// vertices in one launch of nf + nv threads (the same expressions).
-__global__ __launch_bounds__(256) void k_face_vert_post(LagArgs A, const melt_body* __restrict__ b, int do_cfac) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < A.nf) {
-    const int f = t;
-    if (do_cfac) A.L[MELT_L_CFAC][f] = (A.L[MELT_L_SUR][f] * b->h_eulerian) / b->celvol;
-    if (A.gf[f]) return;
-    const double* el = A.L[MELT_L_ELENGTHS];
-    double perim = el[A.eof[3 * f]] + el[A.eof[3 * f + 1]] + el[A.eof[3 * f + 2]];
-    double q = perim / 3.0;
-    double sopt = sqrt(3.0) / 4.0 * (q * q);
-    A.L[MELT_L_SKEWNESS][f] = (sopt - A.L[MELT_L_SUR][f]) / sopt;
-    return;
-  }
-  const int v = t - A.nf;
-  if (v >= A.nv) return;
+// skewness of face f from its edge lengths el[0..2]
+__device__ inline void face_skewness(const LagArgs& A, int f, const double el[3]) {
+  double perim = el[0] + el[1] + el[2];
+  double q = perim / 3.0;
+  double sopt = sqrt(3.0) / 4.0 * (q * q);
+  A.L[MELT_L_SKEWNESS][f] = (sopt - A.L[MELT_L_SUR][f]) / sopt;
+}
+// area and normal of vertex v from its faces
+__device__ void vert_post_geom(const LagArgs& A, int v) {
   double av = 0.0, n[3] = {0.0, 0.0, 0.0};
   for (int p = A.v2f_ptr[v]; p < A.v2f_ptr[v + 1]; ++p) {
     int f = A.v2f[p];
@@ -393,6 +393,47 @@ __global__ __launch_bounds__(256) void k_face_vert_post(LagArgs A, const melt_bo
     for (int d = 0; d < 3; ++d) n[d] = n[d] / m;
   }
   for (int d = 0; d < 3; ++d) A.L[MELT_L_VERT_NOR][3 * v + d] = n[d];
+}
+
+// The geometry refresh after k_face_geom in one launch: blocks [0, G) are
+// stage 1 of the refresh's face sums (surface, volume, rigid-body terms;
+// the same partition as k_face_sums with G blocks, so the same partials);
+// the others take one edge, face or vertex per thread: the edge lengths and
+// the remesh flag as k_edge_len, cfac and skewness of the faces
+// (calculate_skewness triaux.f90:325-358), the vertex areas and normals.  A face measures its own edges with k_edge_len's
+// expression (the same values) instead of reading them back, as the edges
+// are measured in the same launch.
+__global__ __launch_bounds__(RB) void k_refresh_tail(LagArgs A, const double* __restrict__ x, melt_body* b, double e_thresh, int do_cfac, int G,
+                                                     double* __restrict__ part) {
+  if ((int)blockIdx.x < G) {
+    face_sums_block(A, x, SUM_AREA_VOL_RIGID, part, blockIdx.x, G);
+    return;
+  }
+  int t = ((int)blockIdx.x - G) * RB + threadIdx.x;
+  bool short_edge = false;
+  if (t < A.ne && !A.ge[t]) {
+    double l = edge_length(A, x, t);
+    A.L[MELT_L_ELENGTHS][t] = l;
+    short_edge = l < e_thresh;
+  }
+  // the remesh flag is a logical OR: one atomic per wave that saw a short edge
+  if (__ballot(short_edge) && (threadIdx.x & 63) == 0) atomicExch(&b->rm_flag, 1);
+  t -= A.ne;
+  if (t < 0) return;
+  if (t < A.nf) {
+    const int f = t;
+    if (do_cfac) A.L[MELT_L_CFAC][f] = (A.L[MELT_L_SUR][f] * b->h_eulerian) / b->celvol;
+    if (A.gf[f]) return;
+    double e3[3];
+    for (int k = 0; k < 3; ++k) {
+      const int e = A.eof[3 * f + k];
+      e3[k] = A.ge[e] ? A.L[MELT_L_ELENGTHS][e] : edge_length(A, x, e);
+    }
+    face_skewness(A, f, e3);
+    return;
+  }
+  const int v = t - A.nf;
+  if (v < A.nv) vert_post_geom(A, v);
 }
 
 // ---------------------------------------------------------- rigid body
@@ -589,7 +630,7 @@ __global__ __launch_bounds__(256) void k_face_vel(LagArgs A, const melt_body* __
   for (int d = 0; d < 3; ++d) A.L[MELT_L_VEL_TRI][3 * f + d] = b->vel_cm[d] + om[d];
 }
 
-// setup_particles (create_geo.f90:18-46) after calc_rigidBody_params, and the
+// setup_particles (create_geo.f90:46-104) after calc_rigidBody_params, and the
 // init_mlsForce constants (initmls.f90:15-24, evaluated on the host)
 __global__ void k_geom_init_body(melt_body* b, double xlen, double ylen, double zlen, double h_eul, double celvol, double e_thresh,
                                  double v_thresh) {
@@ -633,9 +674,14 @@ int refresh_geometry(melt_ctx* ctx, LagArgs& A) {
     ctx->after_faces = nullptr;
     if (int rc = (*hook)()) return rc;
   }
-  hipLaunchKernelGGL(k_edge_len, dim3(nb(ctx->ne, 256)), dim3(256), 0, s, A, x, ctx->d_body, ctx->h_body.e_thresh);
-  hipLaunchKernelGGL(k_face_vert_post, dim3(nb((size_t)ctx->nf + ctx->nv, 256)), dim3(256), 0, s, A, ctx->d_body, ctx->cfg.imelt == 1 ? 1 : 0);
-  face_sums(ctx, A, x, (int)SUM_AREA_VOL_RIGID, ctx->d_red, FIN_FINAL);
+  // edge lengths, face and vertex post-processing and the face sums: one
+  // launch, then the sums' finish (surface, volume, COM, inertia, abort)
+  const int G = std::max(1, std::min(64, (A.nf + RB - 1) / RB));
+  double* part = ctx->d_red + 8192;
+  hipLaunchKernelGGL(k_refresh_tail, dim3(G + nb((size_t)ctx->ne + ctx->nf + ctx->nv, RB)), dim3(RB), 0, s, A, x, ctx->d_body,
+                     ctx->h_body.e_thresh, ctx->cfg.imelt == 1 ? 1 : 0, G, part);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(RB), 0, s, part, G, 12, ctx->d_red, (int)FIN_FINAL, ctx->d_body, ctx->cfg.dens_ratio,
+                     ctx->d_err);
   MELT_HIP(hipGetLastError());
   return 0;
 }

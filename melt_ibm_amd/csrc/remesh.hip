@@ -105,7 +105,7 @@ class Editor {
     return V / 6.0;
   }
 
-  // update_tri_normal triaux.f90:118-181 on the live faces
+  // update_tri_normal triaux.f90:183-232 on the live faces
   void face_normals() {
     for (int f = 0; f < m_.nf; ++f)
       if (!m_.gf[f]) unit_normal(f, &m_.nor[3 * f]);

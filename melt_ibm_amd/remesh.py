@@ -69,7 +69,7 @@ class TriMesh:
                                                                      self.vmelt)))
 
     def update_normals(self) -> None:
-        """update_tri_normal (triaux.f90:118-181) on the live faces."""
+        """update_tri_normal (triaux.f90:183-232) on the live faces."""
         f = self.vert_of_face - 1
         x = self.xyz
         n = np.cross(x[f[:, 1]] - x[f[:, 0]], x[f[:, 2]] - x[f[:, 0]])

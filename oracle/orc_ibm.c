@@ -863,7 +863,7 @@ static int struct_loads(orc_ctx* o) {  /* mls_structLoads.f90:1-260 */
 }
 
 /* ================================================================ geometry */
-static double calc_volume(orc_ctx* o) {  /* triaux.f90:25-51 */
+static double calc_volume(orc_ctx* o) {  /* calculate_volume triaux.f90:53-80 */
   const double* x = LAG(MELT_L_XYZV);
   double V = 0.0;
   for (int f = 0; f < o->nf; ++f) {
@@ -877,7 +877,7 @@ static double calc_volume(orc_ctx* o) {  /* triaux.f90:25-51 */
   return V / 6.;
 }
 
-static void calc_area(orc_ctx* o, const double* x) {  /* triaux.f90:53-91 */
+static void calc_area(orc_ctx* o, const double* x) {  /* calculate_area triaux.f90:82-119 */
   double S = 0.0;
   for (int f = 0; f < o->nf; ++f) {
     if (o->ghost_f[f]) continue;
@@ -895,7 +895,7 @@ static void calc_area(orc_ctx* o, const double* x) {  /* triaux.f90:53-91 */
   o->body.surface = S;
 }
 
-static void calc_vert_area(orc_ctx* o) {  /* triaux.f90:93-116 */
+static void calc_vert_area(orc_ctx* o) {  /* calculate_vert_area triaux.f90:121-150 */
   double* A = LAG(MELT_L_AVERT);
   for (int v = 0; v < o->nv; ++v) A[v] = 0.;
   for (int f = 0; f < o->nf; ++f) {
@@ -904,7 +904,7 @@ static void calc_vert_area(orc_ctx* o) {  /* triaux.f90:93-116 */
   }
 }
 
-static void tri_normal(orc_ctx* o, const double* x, int check_ghost) {  /* triaux.f90:118-181 */
+static void tri_normal(orc_ctx* o, const double* x, int check_ghost) {  /* calculate_normal / update_tri_normal triaux.f90:154-232 */
   for (int f = 0; f < o->nf; ++f) {
     if (check_ghost && o->ghost_f[f]) continue;
     int v1 = o->vert_of_face[3 * f] - 1, v2 = o->vert_of_face[3 * f + 1] - 1, v3 = o->vert_of_face[3 * f + 2] - 1;
@@ -918,7 +918,7 @@ static void tri_normal(orc_ctx* o, const double* x, int check_ghost) {  /* triau
   }
 }
 
-static void vert_normal(orc_ctx* o) {  /* triaux.f90:183-215 */
+static void vert_normal(orc_ctx* o) {  /* calculate_areaWeighted_vert_normal triaux.f90:237-274 */
   double* vn = LAG(MELT_L_VERT_NOR);
   for (int i = 0; i < 3 * o->nv; ++i) vn[i] = 0.;
   for (int f = 0; f < o->nf; ++f) {
@@ -935,7 +935,7 @@ static void vert_normal(orc_ctx* o) {  /* triaux.f90:183-215 */
   }
 }
 
-static void edge_lengths(orc_ctx* o, const double* x) {  /* triaux.f90:217-245 */
+static void edge_lengths(orc_ctx* o, const double* x) {  /* calculate_eLengths triaux.f90:277-305 */
   o->body.rm_flag = 0;
   for (int e = 0; e < o->ne; ++e) {
     if (o->ghost_e[e]) continue;
@@ -946,7 +946,7 @@ static void edge_lengths(orc_ctx* o, const double* x) {  /* triaux.f90:217-245 *
   }
 }
 
-static void centroids(orc_ctx* o) {  /* triaux.f90:247-262 */
+static void centroids(orc_ctx* o) {  /* calc_centroids_from_vert triaux.f90:308-323 */
   const double* x = LAG(MELT_L_XYZV);
   for (int f = 0; f < o->nf; ++f) {
     if (o->ghost_f[f]) continue;
@@ -957,7 +957,7 @@ static void centroids(orc_ctx* o) {  /* triaux.f90:247-262 */
   }
 }
 
-static void skewness(orc_ctx* o) {  /* triaux.f90:264-290 */
+static void skewness(orc_ctx* o) {  /* calculate_skewness triaux.f90:325-358 */
   /* edge_of_face is 1-based */
   for (int f = 0; f < o->nf; ++f) {
     if (o->ghost_f[f]) continue;
@@ -1243,11 +1243,11 @@ int orc_set_geometry(orc_ctx* o, int32_t nv, int32_t ne, int32_t nf, const doubl
 int orc_geometry_init(orc_ctx* o, double rad_p) {
   melt_body* b = &o->body;
   memset(b, 0, sizeof *b);
-  /* setup_particles create_geo.f90:1-70 */
+  /* setup_particles create_geo.f90:1-104 */
   for (int i = 0; i < 3 * o->nv; ++i) o->xyz0[i] = o->xyz0[i] * rad_p;
   rigid_params(o, o->xyz0, b->pos_cm, &b->volume, b->inert_tensor);
   b->pos_cm[0] = 0.5 * o->cfg.xlen; b->pos_cm[1] = 0.5 * o->cfg.ylen; b->pos_cm[2] = 0.5 * o->cfg.zlen;
-  /* init_geomCoords create_geo.f90:72-90 */
+  /* init_geomCoords create_geo.f90:106-132 */
   for (int v = 0; v < o->nv; ++v)
     for (int d = 0; d < 3; ++d) LAG(MELT_L_XYZV)[3 * v + d] = o->xyz0[3 * v + d] + b->pos_cm[d];
   centroids(o);

@@ -571,7 +571,7 @@ int orc_remesh_step(orc_ctx* o, double vol_melt, int* did) {
   o->body.rm_flag = flag;
   if (!flag) return 0;
   *did = 1;
-  /* update_tri_normal triaux.f90:118-181 */
+  /* update_tri_normal triaux.f90:183-232 */
   for (int f = 1; f <= o->nf; ++f) {
     if (o->ghost_f[f - 1]) continue;
     int v1 = VOF(1, f), v2 = VOF(2, f), v3 = VOF(3, f);
