@@ -841,7 +841,7 @@ static int substep_impl(melt_ctx* ctx, int32_t ns, bool tag_next, bool coef_next
   };
   if (ibm) {
     timing_begin(ctx, SLOT_IBM, &e);
-    if ((rc = ibm_find_centroid_indices(ctx)) || (rc = ibm_weights(ctx))) return rc;
+    if ((rc = ibm_weights(ctx, true))) return rc;   // cell indices and shape functions
     // the gradient probes read the same pre-force fields as the spreading and
     // write only their own outputs (their cell indices into d_pind2, since the
     // spreading still reads d_pind): side stream, alongside the spreading
@@ -900,6 +900,9 @@ static int substep_impl(melt_ctx* ctx, int32_t ns, bool tag_next, bool coef_next
               })))
     return rc;
   if (ibm && coef_after_join) {   // on the side stream after its join point
+    // (measured: waiting for the next tagging first, so that the two do not
+    // share the CUs, moves the stage beside the next explicit pass, which it
+    // then slows by as much as the tagging gains)
     hipStream_t main_stream = ctx->stream;
     ctx->stream = ctx->side;
     rc = queue_coef();

@@ -508,7 +508,7 @@ int poisson_solve(melt_ctx* ctx, int fused_divg, const int* gate = nullptr, unsi
 int euler_fold_slots(melt_ctx* ctx, unsigned long long* slots, unsigned long long* out);
 int ibm_tag(melt_ctx* ctx);
 int ibm_find_centroid_indices(melt_ctx* ctx);
-int ibm_weights(melt_ctx* ctx);
+int ibm_weights(melt_ctx* ctx, bool find = false);   // find: form the cell indices too
 int ibm_force_scatter(melt_ctx* ctx);
 int ibm_force_drain(melt_ctx* ctx, int apply_now);
 int ibm_sub_mean(melt_ctx* ctx);

@@ -665,16 +665,28 @@ __global__ __launch_bounds__(256) void k_centroid_idx(Grid g, const double* __re
 
 struct PhiOut { double* q[4]; };  // ptxAB_q1, ptxAB_q2, ptxAB_temp, ptxAB_q3
 
-// component c: 0 = wght1 (u), 1 = wght2 (v), 2 = wghttemp (T), 3 = wght3 (w)
+// component c: 0 = wght1 (u), 1 = wght2 (v), 2 = wghttemp (T), 3 = wght3 (w).
+// With find, the face's cell indices are formed here as k_centroid_idx
+// forms them (findCentroidIndices.f90:16-48) and component 0 stores them;
+// otherwise they are read from pind.
 __global__ __launch_bounds__(64) void k_mls_weights(Grid g, const double* __restrict__ bar, const unsigned char* __restrict__ gf, int nf,
-                                                    const int* __restrict__ pind, PhiOut out, int* err) {
+                                                    int* __restrict__ pind, int find, PhiOut out, int* err) {
   int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= nf * 4) return;
   int f = id >> 2, comp = id & 3;
   if (gf[f]) return;
   const double* pos = &bar[3 * f];
-  const int* p = &pind[6 * f];
   int k1 = (int)floor(pos[2] * g.dx3) + 1, kst = fnint(pos[2] * g.dx3) + 1;
+  int p[6];
+  if (find) {
+    p[0] = (int)floor(pos[0] * g.dx1) + 1; p[3] = fnint(pos[0] * g.dx1) + 1;
+    p[1] = (int)floor(pos[1] * g.dx2) + 1; p[4] = fnint(pos[1] * g.dx2) + 1;
+    p[2] = pmod(k1 - 1, g.n3) + 1; p[5] = pmod(kst - 1, g.n3) + 1;
+    if (comp == 0)
+      for (int q = 0; q < 6; ++q) pind[6 * f + q] = p[q];
+  } else {
+    for (int q = 0; q < 6; ++q) p[q] = pind[6 * f + q];
+  }
   int i0, j0, k0, c[3];
   if (comp == 0) { i0 = p[3] - 1; j0 = p[1] - 1; k0 = k1 - 1; c[0] = 0; c[1] = 1; c[2] = 1; }
   else if (comp == 1) { i0 = p[0] - 1; j0 = p[4] - 1; k0 = k1 - 1; c[0] = 1; c[1] = 0; c[2] = 1; }
@@ -1206,10 +1218,10 @@ int ibm_find_centroid_indices(melt_ctx* ctx) {
   return 0;
 }
 
-int ibm_weights(melt_ctx* ctx) {
+int ibm_weights(melt_ctx* ctx, bool find) {
   PhiOut o{{ctx->d_lag[MELT_L_PTXAB_Q1], ctx->d_lag[MELT_L_PTXAB_Q2], ctx->d_lag[MELT_L_PTXAB_TEMP], ctx->d_lag[MELT_L_PTXAB_Q3]}};
   hipLaunchKernelGGL(k_mls_weights, dim3(nb((size_t)ctx->nf * 4, 64)), dim3(64), 0, ctx->stream, make_grid(ctx), ctx->d_lag[MELT_L_TRI_BAR],
-                     ctx->d_gf, ctx->nf, ctx->d_pind, o, ctx->d_err);
+                     ctx->d_gf, ctx->nf, ctx->d_pind, find ? 1 : 0, o, ctx->d_err);
   MELT_HIP(hipGetLastError());
   return 0;
 }
