@@ -1344,8 +1344,11 @@ int launch_explicit_xsolve(melt_ctx* ctx, double bxu, double bxt) {
     int D = ctx->tune[TUNE_XF_Z];
     if (D <= 0 || ctx->n3 % D) {
       D = 1;
-      for (int d : {8, 4, 2})   // 8 planes per block measured best (FP32 1-2%, FP64 level or better)
-        if (ctx->n3 % d == 0) { D = d; break; }
+      // 8 planes per block measured best for one cell per thread (FP32 1-2%,
+      // FP64 level or better), 4 for the two-cell form (1%)
+      const bool pair = sizeof(R) == 4 && ctx->tune[TUNE_XF_ONE] == 0;
+      for (int d : {8, 4, 2})
+        if (!(pair && d == 8) && ctx->n3 % d == 0) { D = d; break; }
     }
     const unsigned blocks = (unsigned)((size_t)ctx->n2 * (ctx->n3 / D));
     if constexpr (std::is_same_v<R, float>) {

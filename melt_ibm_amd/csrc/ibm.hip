@@ -139,18 +139,41 @@ __device__ __forceinline__ int mls_grad_lanes(const Grid& g, const R* __restrict
 #pragma unroll
   for (int q = 0; q < 16; ++q) M[q] = 0.0;
   if (active) {
-    // separable Gaussian factors and derivatives, evaluated once per axis
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       xs[q] = gco(g, 0, c[0], ic - 1 + q);
-      double n1 = fabs(pos[0] - xs[q]) / sx;
-      w1[q] = mls_gaussian(n1); dw1[q] = dweight(pos[0], xs[q], n1, sx);
       ys[q] = gco(g, 1, c[1], jc - 1 + q);
-      double n2 = fabs(pos[1] - ys[q]) / sy;
-      w2[q] = mls_gaussian(n2); dw2[q] = dweight(pos[1], ys[q], n2, sy);
       zs[q] = gco(g, 2, c[2], kz[q]);
-      double n3 = fabs(pos[2] - zs[q]) / sz;
-      w3[q] = mls_gaussian(n3); dw3[q] = dweight(pos[2], zs[q], n3, sz);
+    }
+    // separable Gaussian factors and derivatives of the nine (axis, point)
+    // pairs: pair p = 3 axis + point is evaluated on lane p % 4 of the
+    // probe's 4-lane group only (one exponential per pair instead of one per
+    // pair and lane) and read by the others from there
+    double mw[3], md[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const int pp = lane + 4 * r, ax = pp / 3, q = pp - 3 * ax;
+      // selects, not an indexed pick (which would put the arrays in scratch)
+      const double cx = q == 0 ? xs[0] : q == 1 ? xs[1] : xs[2], cy = q == 0 ? ys[0] : q == 1 ? ys[1] : ys[2];
+      const double cz = q == 0 ? zs[0] : q == 1 ? zs[1] : zs[2];
+      const double crd = ax == 0 ? cx : ax == 1 ? cy : cz;
+      const double pc = ax == 0 ? pos[0] : ax == 1 ? pos[1] : pos[2];
+      const double sc = ax == 0 ? sx : ax == 1 ? sy : sz;
+      const double n = fabs(pc - crd) / sc;
+      mw[r] = pp < 9 ? mls_gaussian(n) : 0.0;
+      md[r] = pp < 9 ? dweight(pc, crd, n, sc) : 0.0;
+    }
+    double wv[9], dv[9];
+#pragma unroll
+    for (int pp = 0; pp < 9; ++pp) {
+      wv[pp] = __shfl(mw[pp / 4], pp % 4, 4);
+      dv[pp] = __shfl(md[pp / 4], pp % 4, 4);
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      w1[q] = wv[q]; dw1[q] = dv[q];
+      w2[q] = wv[3 + q]; dw2[q] = dv[3 + q];
+      w3[q] = wv[6 + q]; dw3[q] = dv[6 + q];
     }
 #pragma unroll 1
     for (int nk = 0; nk < 3; ++nk) {
