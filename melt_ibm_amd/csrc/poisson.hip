@@ -361,6 +361,7 @@ __device__ __forceinline__ void dft_reg(cx<T> (&v)[P], bool inv) {
 }
 
 constexpr int RF_NC = 8;   // kx columns per register-FFT block (8 x 16 B = one 128-byte line per row)
+constexpr int RF_NCZ = 8;  // the same for the z pass (16 and 32 measured slower)
 
 // LDS image of one block of NC lines: the exchange tile (NC x P rows of
 // pitch P+1) and the N twiddles.
@@ -453,26 +454,27 @@ __global__ __launch_bounds__(RF_NC * P) void k_fft_y_reg(SpecArgs<T> S, const cx
 
 // z pass with register FFTs: forward FFT, 1/(N1 N2 N3), division by
 // -(ak3+ak1+ak2) with the zero mode set to 0 (phcalc.f90:40-73), inverse FFT.
-template <typename T, int P>
-__global__ __launch_bounds__(RF_NC * P) void k_fft_z_solve_reg(SpecArgs<T> S, const cx<T>* __restrict__ tw, const double* __restrict__ ak1,
+template <typename T, int P, int NC = RF_NCZ>
+__global__ __launch_bounds__(NC * P) void k_fft_z_solve_reg(SpecArgs<T> S, const cx<T>* __restrict__ tw, const double* __restrict__ ak1,
                                                                const double* __restrict__ ak2, const double* __restrict__ ak3, double nrm,
                                                                const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
-  __shared__ RegFftLds<T, P, RF_NC> L;
+  __shared__ RegFftLds<T, P, NC> L;
   constexpr int N = P * P;
-  const int t = threadIdx.x, c = t % RF_NC, b = t / RF_NC;
+  const int t = threadIdx.x, c = t % NC, b = t / NC;
   const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
-  const int ntm = S.n1hp / RF_NC;
-  const int m0 = (int)(blk % (unsigned)ntm) * RF_NC, j = (int)(blk / (unsigned)ntm);
-  for (int q = t; q < N; q += RF_NC * P) L.tw[q] = tw[q];
-  cx<T>* base = S.spec + (size_t)m0 + c + (size_t)S.n1hp * j;
+  const int ntm = (S.n1hp + NC - 1) / NC;   // the last block may overhang the row
+  const int m0 = (int)(blk % (unsigned)ntm) * NC, j = (int)(blk / (unsigned)ntm);
+  for (int q = t; q < N; q += NC * P) L.tw[q] = tw[q];
+  const int m = m0 + c;
+  const bool col = m < S.n1hp;   // columns past the row transform zeros and store nothing
+  cx<T>* base = S.spec + (size_t)(col ? m : 0) + (size_t)S.n1hp * j;
   const size_t rs = (size_t)S.n1hp * S.g.n2;
   cx<T> v[P];
 #pragma unroll
-  for (int a = 0; a < P; ++a) v[a] = base[rs * (b + P * a)];
+  for (int a = 0; a < P; ++a) v[a] = col ? base[rs * (b + P * a)] : mkc<T>(0, 0);
   __syncthreads();
   reg_fft_fwd<T, P>(v, L, c, b, false);
-  const int m = m0 + c;
 #pragma unroll
   for (int k2 = 0; k2 < P; ++k2) {
     const int p = b + P * k2;
@@ -488,7 +490,8 @@ __global__ __launch_bounds__(RF_NC * P) void k_fft_z_solve_reg(SpecArgs<T> S, co
   }
   reg_fft_back<T, P>(v, L, c, b, true);
 #pragma unroll
-  for (int n2 = 0; n2 < P; ++n2) base[rs * (b + P * n2)] = v[n2];
+  for (int n2 = 0; n2 < P; ++n2)
+    if (col) base[rs * (b + P * n2)] = v[n2];
 }
 
 // x pass, forward, with register FFTs (N1 = P*P): RF_XT threads per block,
@@ -722,7 +725,7 @@ int solve_t(melt_ctx* ctx, int fused_divg, const int* gate, unsigned long long* 
   unsigned gy = (unsigned)(((ctx->n1hp + nly - 1) / nly) * ctx->n3);
   // y and z lines of 64 or 256 points: register FFTs (RF_NC columns per block)
   const int py = reg_fft_p(ctx->n2), pz = reg_fft_p(ctx->n3);
-  const unsigned gry = (unsigned)((ctx->n1hp / RF_NC) * ctx->n3), grz = (unsigned)((ctx->n1hp / RF_NC) * ctx->n2);
+  const unsigned gry = (unsigned)((ctx->n1hp / RF_NC) * ctx->n3), grz = (unsigned)(((ctx->n1hp + RF_NCZ - 1) / RF_NCZ) * ctx->n2);
   auto fft_y = [&](int inv) {
     if (py == 16) hipLaunchKernelGGL((k_fft_y_reg<T, 16>), dim3(gry), dim3(RF_NC * 16), 0, ctx->stream, S, twy, inv, gate);
     else if (py == 8) hipLaunchKernelGGL((k_fft_y_reg<T, 8>), dim3(gry), dim3(RF_NC * 8), 0, ctx->stream, S, twy, inv, gate);
@@ -739,9 +742,9 @@ int solve_t(melt_ctx* ctx, int fused_divg, const int* gate, unsigned long long* 
   double nrm = (double)((long long)ctx->n1 * ctx->n2 * ctx->n3);
   if (!gate) timing_begin(ctx, SLOT_FFTZ_SOLVE, &ev);
   if (pz == 16)
-    hipLaunchKernelGGL((k_fft_z_solve_reg<T, 16>), dim3(grz), dim3(RF_NC * 16), 0, ctx->stream, S, twz, ctx->ak1, ctx->ak2, ctx->ak3, nrm, gate);
+    hipLaunchKernelGGL((k_fft_z_solve_reg<T, 16>), dim3(grz), dim3(RF_NCZ * 16), 0, ctx->stream, S, twz, ctx->ak1, ctx->ak2, ctx->ak3, nrm, gate);
   else if (pz == 8)
-    hipLaunchKernelGGL((k_fft_z_solve_reg<T, 8>), dim3(grz), dim3(RF_NC * 8), 0, ctx->stream, S, twz, ctx->ak1, ctx->ak2, ctx->ak3, nrm, gate);
+    hipLaunchKernelGGL((k_fft_z_solve_reg<T, 8>), dim3(grz), dim3(RF_NCZ * 8), 0, ctx->stream, S, twz, ctx->ak1, ctx->ak2, ctx->ak3, nrm, gate);
   else
     hipLaunchKernelGGL((k_fft_z_solve<T>), dim3(gz), dim3(FFT_NT), shz, ctx->stream, S, Pz, twz, nlz, ctx->ak1, ctx->ak2, ctx->ak3, nrm, gate);
   MELT_HIP(hipGetLastError());
